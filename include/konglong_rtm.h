@@ -1,0 +1,197 @@
+/*
+ * konglong_rtm.h -- C ABI of the MI355X-native RTM propagation-and-imaging
+ * library (libkonglong_rtm.so).
+ *
+ * Every entry point returns an int status (KLRTM_OK == 0) and leaves a
+ * message in klrtm_last_error() on failure.  No C++ or torch types cross the
+ * boundary: volumes are plain pointers with sizes, indexed like the
+ * reference, idx = (iy*nx + ix)*nz + iz (bgprtm/FdEngine.h:42).
+ *
+ * Pointers documented as "device" were returned by klrtm_malloc (or any
+ * hipMalloc'd memory of the current device); pointers documented as "host"
+ * are ordinary process memory.  All device work is enqueued on the library
+ * stream of the calling thread (klrtm_set_stream) and is asynchronous unless
+ * stated otherwise.
+ *
+ * The C++ host classes under include/bgprtm/ (KernelCPU, PML, WaveFieldCompress,
+ * ImagingCorrelator, Receiver, Propagator) keep the reference's class names
+ * and signatures and are implemented on top of these entry points.
+ */
+#ifndef KONGLONG_RTM_H_
+#define KONGLONG_RTM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+enum { KLRTM_OK = 0, KLRTM_EINVAL = 1, KLRTM_EDEVICE = 2, KLRTM_ENOMEM = 3, KLRTM_ESTATE = 4 };
+
+/* engine kinds use the reference's FD::Engine values (bgprtm/FdEngine.h:25-27) */
+enum { KLRTM_ENGINE_CFD = 4, KLRTM_ENGINE_FD = 7 };
+/* model kinds use the reference's ModelType values (bgprtm/Model.h:13) */
+enum { KLRTM_ISO = 1, KLRTM_VTI = 2, KLRTM_TTI = 3 };
+/* snapshot storage formats (wf_compress, bgprtm/WaveFieldCompress.h:6, plus block-scaled half types) */
+enum { KLRTM_SNAP_FP32 = 0, KLRTM_SNAP_C16 = 1, KLRTM_SNAP_FP16 = 16, KLRTM_SNAP_BF16 = 17 };
+
+/* ---- errors, device, memory ------------------------------------------------ */
+const char *klrtm_last_error(void);
+int klrtm_version(int *major, int *minor);
+int klrtm_device_count(int *n);
+int klrtm_set_device(int device);
+/* stream used by this thread's calls (NULL = library default stream of the device) */
+int klrtm_set_stream(void *hip_stream);
+int klrtm_malloc(void **ptr, size_t bytes);
+int klrtm_free(void *ptr);
+int klrtm_memcpy_h2d(void *dst_device, const void *src_host, size_t bytes);
+int klrtm_memcpy_d2h(void *dst_host, const void *src_device, size_t bytes);
+int klrtm_memcpy_d2d(void *dst_device, const void *src_device, size_t bytes);
+int klrtm_memset(void *dst_device, int value, size_t bytes);
+int klrtm_synchronize(void);
+
+/* ---- kernel timing (HIP events recorded around each launch of a kernel) ---- */
+int klrtm_timer_enable(int on);
+/* total milliseconds and launch count of a named kernel since the last reset */
+int klrtm_timer_read(const char *kernel_name, double *total_ms, long *count);
+int klrtm_timer_reset(void);
+
+/* ---- Propagator::Vel2 / update2nd (bgprtm/Propagator.cpp:711-742, 1767-1788) */
+/* vel[i] <- vel[i]^2 * dt^2 (device) */
+int klrtm_vel2(float *vel, size_t n, float dt);
+/* w0[i] <- 2*w1[i] - w0[i] (device) */
+int klrtm_update2nd(float *w0, const float *w1, size_t n);
+
+/* ---- KernelCPU + FdEngine_fd / FdEngine_cfd + PML (bgprtm/KernelCPU.h:19-35) */
+typedef struct klrtm_kernel klrtm_kernel;
+/* KernelCPU(nx,ny,nz,dx,dy,dz,engine,nThreads): engine KLRTM_ENGINE_FD (order 8..16,
+ * even) or KLRTM_ENGINE_CFD (order ignored, 10th-order compact).  Requires nx,nz > 2. */
+int klrtm_kernel_create(klrtm_kernel **k, int nx, int ny, int nz, float dx, float dy, float dz, int engine, int order);
+int klrtm_kernel_destroy(klrtm_kernel *k);
+/* KernelCPU::setModel: device pointers, borrowed; vel holds v^2 dt^2; rho must be NULL
+ * (density is outside this build); del/eps needed for VTI/TTI, pjx/pjy for TTI */
+int klrtm_kernel_set_model(klrtm_kernel *k, const float *vel, const float *rho, const float *del, const float *eps,
+                           const float *pjx, const float *pjy);
+/* global_pars keys pml_eps (1e-3), pml3d (1), pml3d_scaler (0.3), top_pml_flag (1);
+ * call before klrtm_kernel_set_boundary */
+int klrtm_kernel_set_pml_options(klrtm_kernel *k, float pml_eps, int pml3d, float pml3d_scaler, int top_pml_flag);
+/* KernelCPU::setBoundary (bgprtm/FdEngine.cpp:108-182): allocates and zeroes PML state */
+int klrtm_kernel_set_boundary(klrtm_kernel *k, int nxbnd1, int nxbnd2, int nybnd1, int nybnd2, int nzbnd1, int nzbnd2,
+                              float dt, float vmax);
+/* KernelCPU::cleanPMLMemory */
+int klrtm_kernel_clean_pml(klrtm_kernel *k);
+/* KernelCPU::ISO/VTI/TTI(p0, pb, pr): pb <- pb - vel * Op(p0); pr is scratch (holds the
+ * scaled y-gradient for VTI/TTI).  All device pointers of nx*ny*nz floats. */
+int klrtm_kernel_iso(klrtm_kernel *k, const float *p0, float *pb, float *pr);
+int klrtm_kernel_vti(klrtm_kernel *k, const float *p0, float *pb, float *pr);
+int klrtm_kernel_tti(klrtm_kernel *k, const float *p0, float *pb, float *pr);
+/* fused Propagator::kernel() + update2nd(): w0 <- 2*w1 - (w0 - vel*Op(w1)) in one pass
+ * where the engine allows it (model = KLRTM_ISO/VTI/TTI) */
+int klrtm_kernel_step(klrtm_kernel *k, int model, const float *w1, float *w0, float *wr);
+/* tile shape of the stencil kernels: tz (z lanes), tx (x rows), ychunk (y planes per
+ * block); 0 keeps the current value.  Used by the autotuner. */
+int klrtm_kernel_set_tile(klrtm_kernel *k, int tz, int tx, int ychunk);
+int klrtm_kernel_get_tile(klrtm_kernel *k, int *tz, int *tx, int *ychunk);
+
+/* CfdPlan(n, d) (bgprtm/Cfd_plan.cpp:26-262), host: bm[2][n*5] (diagonal + 4 upper
+ * bands per row), bl[2][n*4] (elimination multipliers per row), am[2][60] (RHS
+ * boundary table, interior weights at 55..59); [0] gradient plan, [1] divergence */
+int klrtm_cfd_plan(int n, float d, float *bm, float *bl, float *am);
+
+/* ---- Util::CompactOrmsbySpreadCoeff (bgprtm/Util.cpp:317-397), host ---------- */
+/* coeff receives *nlimit taps (input: max taps, output: taps used); returns the
+ * first grid index in *offset */
+int klrtm_spread_coeff(int n, float *coeff, float x, float k1, float k2, int *nlimit, int *offset);
+
+/* ---- Receiver::apply / ExtractReceiverData::extract (bgprtm/Receiver.cpp:394-507,1204-1279) */
+typedef struct klrtm_receiver klrtm_receiver;
+/* nr receivers at float grid coordinates xyz[3*nr] (x,y,z index units), spread size
+ * (taps = 2*spread), band k1<=k2 in [0,0.5], nt samples per trace, dt_dxyz =
+ * 1/(dx*dy*dz).  Kernels are built on the host and uploaded once. */
+int klrtm_receiver_create(klrtm_receiver **r, int nx, int ny, int nz, int nr, const float *xyz_host, int spread,
+                          float k1, float k2, int nt, float dt_dxyz);
+int klrtm_receiver_destroy(klrtm_receiver *r);
+/* traces_host[nr][nt] (Traces::data layout) */
+int klrtm_receiver_set_traces(klrtm_receiver *r, const float *traces_host);
+/* w0 += traces[:, it] * scaler0 * dt_dxyz * kx*ky*kz * vel  (no-op when it outside [0,nt)) */
+int klrtm_receiver_apply(klrtm_receiver *r, float *w0, int it, const float *vel, float scaler0);
+/* record[:, jt] = sum kx*ky*kz * w   (device record kept by the receiver object) */
+int klrtm_receiver_extract(klrtm_receiver *r, const float *w, int jt);
+/* copy the record out as [nr][nt] */
+int klrtm_receiver_get_record(klrtm_receiver *r, float *record_host);
+/* the spread tables as built (for inspection): lengths/offsets per axis */
+int klrtm_receiver_get_tables(klrtm_receiver *r, int *lx, int *ly, int *lz, int *offx, int *offy, int *offz, float *kx,
+                              float *ky, float *kz, int *maxlen);
+
+/* ---- WaveFieldCompress COMPRESS_16 (bgprtm/WaveFieldCompress.cpp:101-207) --------- */
+size_t klrtm_nshort_volume(int nz, int nx, int ny);
+/* device in/out; out holds nshort_volume(nz,nx,ny) uint16 */
+int klrtm_compress16(const float *in, uint16_t *out, int nx, int ny, int nz);
+/* decode with the reference's edge taper (20 cells x/y, 10 cells z) */
+int klrtm_uncompress16(const uint16_t *in, float *out, int nx, int ny, int nz);
+
+/* ---- block-scaled half snapshots (one FP32 scale per 8x8x8 block) ------------ */
+/* fmt KLRTM_SNAP_FP16 or KLRTM_SNAP_BF16; payload nx*ny*nz uint16, scales
+ * ceil(nz/8)*ceil(nx/8)*ceil(ny/8) floats (device) */
+size_t klrtm_blockq_nscales(int nx, int ny, int nz);
+int klrtm_blockq_encode(const float *in, uint16_t *payload, float *scales, int nx, int ny, int nz, int fmt);
+int klrtm_blockq_decode(const uint16_t *payload, const float *scales, float *out, int nx, int ny, int nz, int fmt);
+
+/* ---- ImagingCorrelator::crossone + ImagingCondition::calIllum (zero lag) ----- */
+/* image += S*R, illum += S*S with S = src*taper (taper = wtaperT or z taper as in
+ * ImagingCondition::applyTape); any output pointer may be NULL.  FP64 outputs
+ * accumulate the same products in double. */
+int klrtm_ic_correlate(const float *src, const float *rec, int nx, int ny, int nz, float taper_t, const float *taper_z,
+                       int ntaper_z, float *image, float *illum, double *image64, double *illum64);
+
+/* ---- whole-shot driver (Propagator::rtm / Propagator::mod) ------------------ */
+typedef struct {
+  int nx, ny, nz;
+  float dx, dy, dz, dt;
+  int nt;                 /* propagation steps */
+  int engine, order, model;
+  int nbnd[6];            /* nxbnd1 nxbnd2 nybnd1 nybnd2 nzbnd1 nzbnd2 */
+  float vmax;             /* PML vmax (pml_vel_max) */
+  float pml_eps, pml3d_scaler;
+  int pml3d, top_pml;
+  float src_xyz[3];       /* source grid coordinates */
+  float f_peak;           /* Ricker peak frequency (Hz) */
+  int src_spread, rcv_spread;
+  float khpass, khcut;
+  int nrcv;               /* receivers; coordinates supplied separately */
+  int snap_every;         /* store every k-th step (nprop_per_ic) */
+  int snap_format;        /* KLRTM_SNAP_* */
+  int rec_through_codec;  /* 1: receiver field passes COMPRESS_16 like the reference cache */
+  float tmute_frac;       /* IC time taper length as a fraction of the record (tmutePerc) */
+  int ntaper_z;           /* IC water-depth taper cells */
+  int fuse_step;          /* 1: fused kernel+update2nd */
+} klrtm_shot_config;
+
+typedef struct klrtm_propagator klrtm_propagator;
+int klrtm_propagator_create(klrtm_propagator **p, const klrtm_shot_config *cfg);
+int klrtm_propagator_destroy(klrtm_propagator *p);
+/* host model volumes: velocity (m/s) and optional eps, del, pjx, pjy (NULL allowed) */
+int klrtm_propagator_set_model(klrtm_propagator *p, const float *vel, const float *eps, const float *del,
+                               const float *pjx, const float *pjy);
+/* receivers: rcv_xyz[3*nrcv] grid coordinates */
+int klrtm_propagator_set_receivers(klrtm_propagator *p, const float *rcv_xyz);
+/* forward modelling with extraction every step: record_host [nrcv][nt] (may be NULL) */
+int klrtm_propagator_model(klrtm_propagator *p, float *record_host);
+/* full RTM with the given record [nrcv][nt]; image/illum outputs (host, may be NULL) */
+int klrtm_propagator_rtm(klrtm_propagator *p, const float *record_host, float *image_host, float *illum_host,
+                         double *image64_host, double *illum64_host);
+/* timings of the last call in ms: forward, backward, imaging (inside backward) */
+int klrtm_propagator_timings(klrtm_propagator *p, double *fwd_ms, double *bwd_ms, double *ic_ms);
+/* device pointers of the current wavefields (w0, w1) for inspection */
+int klrtm_propagator_wavefields(klrtm_propagator *p, float **w0, float **w1);
+/* run n plain forward steps with source injection only (benchmark helper); it0 = first step index */
+int klrtm_propagator_forward_steps(klrtm_propagator *p, int it0, int nsteps, int store_snapshots);
+/* reset wavefields and PML state */
+int klrtm_propagator_reset(klrtm_propagator *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KONGLONG_RTM_H_ */

@@ -1,0 +1,208 @@
+"""ctypes binding of libkonglong_rtm.so (the C ABI in include/konglong_rtm.h).
+
+The library is built in-tree by ``konglong_amd/build.py``; importing a wrapper
+without it raises immediately -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkonglong_rtm.so")
+
+KLRTM_OK = 0
+ENGINE_CFD, ENGINE_FD = 4, 7
+ISO, VTI, TTI = 1, 2, 3
+SNAP_FP32, SNAP_C16, SNAP_FP16, SNAP_BF16 = 0, 1, 16, 17
+
+_vp, _i, _f, _d, _sz, _l = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t, C.c_long
+_pi, _pf, _pd, _pl = C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_long)
+
+
+class ShotConfig(C.Structure):
+    """klrtm_shot_config."""
+    _fields_ = [
+        ("nx", _i), ("ny", _i), ("nz", _i),
+        ("dx", _f), ("dy", _f), ("dz", _f), ("dt", _f),
+        ("nt", _i),
+        ("engine", _i), ("order", _i), ("model", _i),
+        ("nbnd", _i * 6),
+        ("vmax", _f), ("pml_eps", _f), ("pml3d_scaler", _f),
+        ("pml3d", _i), ("top_pml", _i),
+        ("src_xyz", _f * 3),
+        ("f_peak", _f),
+        ("src_spread", _i), ("rcv_spread", _i),
+        ("khpass", _f), ("khcut", _f),
+        ("nrcv", _i),
+        ("snap_every", _i), ("snap_format", _i), ("rec_through_codec", _i),
+        ("tmute_frac", _f), ("ntaper_z", _i), ("fuse_step", _i),
+    ]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "klrtm_last_error": (C.c_char_p, []),
+    "klrtm_version": (_i, [_pi, _pi]),
+    "klrtm_device_count": (_i, [_pi]),
+    "klrtm_set_device": (_i, [_i]),
+    "klrtm_set_stream": (_i, [_vp]),
+    "klrtm_malloc": (_i, [C.POINTER(_vp), _sz]),
+    "klrtm_free": (_i, [_vp]),
+    "klrtm_memcpy_h2d": (_i, [_vp, _vp, _sz]),
+    "klrtm_memcpy_d2h": (_i, [_vp, _vp, _sz]),
+    "klrtm_memcpy_d2d": (_i, [_vp, _vp, _sz]),
+    "klrtm_memset": (_i, [_vp, _i, _sz]),
+    "klrtm_synchronize": (_i, []),
+    "klrtm_timer_enable": (_i, [_i]),
+    "klrtm_timer_read": (_i, [C.c_char_p, _pd, _pl]),
+    "klrtm_timer_reset": (_i, []),
+    "klrtm_vel2": (_i, [_vp, _sz, _f]),
+    "klrtm_update2nd": (_i, [_vp, _vp, _sz]),
+    "klrtm_kernel_create": (_i, [C.POINTER(_vp), _i, _i, _i, _f, _f, _f, _i, _i]),
+    "klrtm_kernel_destroy": (_i, [_vp]),
+    "klrtm_kernel_set_model": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "klrtm_kernel_set_pml_options": (_i, [_vp, _f, _i, _f, _i]),
+    "klrtm_kernel_set_boundary": (_i, [_vp, _i, _i, _i, _i, _i, _i, _f, _f]),
+    "klrtm_kernel_clean_pml": (_i, [_vp]),
+    "klrtm_kernel_iso": (_i, [_vp, _vp, _vp, _vp]),
+    "klrtm_kernel_vti": (_i, [_vp, _vp, _vp, _vp]),
+    "klrtm_kernel_tti": (_i, [_vp, _vp, _vp, _vp]),
+    "klrtm_kernel_step": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "klrtm_kernel_set_tile": (_i, [_vp, _i, _i, _i]),
+    "klrtm_kernel_get_tile": (_i, [_vp, _pi, _pi, _pi]),
+    "klrtm_cfd_plan": (_i, [_i, _f, _pf, _pf, _pf]),
+    "klrtm_spread_coeff": (_i, [_i, _pf, _f, _f, _f, _pi, _pi]),
+    "klrtm_receiver_create": (_i, [C.POINTER(_vp), _i, _i, _i, _i, _pf, _i, _f, _f, _i, _f]),
+    "klrtm_receiver_destroy": (_i, [_vp]),
+    "klrtm_receiver_set_traces": (_i, [_vp, _pf]),
+    "klrtm_receiver_apply": (_i, [_vp, _vp, _i, _vp, _f]),
+    "klrtm_receiver_extract": (_i, [_vp, _vp, _i]),
+    "klrtm_receiver_get_record": (_i, [_vp, _pf]),
+    "klrtm_receiver_get_tables": (_i, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pf, _pf, _pf, _pi]),
+    "klrtm_nshort_volume": (_sz, [_i, _i, _i]),
+    "klrtm_compress16": (_i, [_vp, _vp, _i, _i, _i]),
+    "klrtm_uncompress16": (_i, [_vp, _vp, _i, _i, _i]),
+    "klrtm_blockq_nscales": (_sz, [_i, _i, _i]),
+    "klrtm_blockq_encode": (_i, [_vp, _vp, _vp, _i, _i, _i, _i]),
+    "klrtm_blockq_decode": (_i, [_vp, _vp, _vp, _i, _i, _i, _i]),
+    "klrtm_ic_correlate": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _vp, _vp, _vp]),
+    "klrtm_propagator_create": (_i, [C.POINTER(_vp), C.POINTER(ShotConfig)]),
+    "klrtm_propagator_destroy": (_i, [_vp]),
+    "klrtm_propagator_set_model": (_i, [_vp, _pf, _pf, _pf, _pf, _pf]),
+    "klrtm_propagator_set_receivers": (_i, [_vp, _pf]),
+    "klrtm_propagator_model": (_i, [_vp, _pf]),
+    "klrtm_propagator_rtm": (_i, [_vp, _pf, _pf, _pf, _pd, _pd]),
+    "klrtm_propagator_timings": (_i, [_vp, _pd, _pd, _pd]),
+    "klrtm_propagator_wavefields": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    "klrtm_propagator_forward_steps": (_i, [_vp, _i, _i, _i]),
+    "klrtm_propagator_reset": (_i, [_vp]),
+}
+
+_lib = None
+
+
+class KlrtmError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libkonglong_rtm.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python konglong_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return list(_SIGS)
+
+
+def check(rc):
+    if rc != KLRTM_OK:
+        msg = lib().klrtm_last_error()
+        raise KlrtmError(f"klrtm error {rc}: {msg.decode() if msg else ''}")
+
+
+def fptr(a):
+    """float* of a contiguous float32 numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_pf)
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_pd)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_pi)
+
+
+def device_count():
+    n = C.c_int(0)
+    check(lib().klrtm_device_count(C.byref(n)))
+    return n.value
+
+
+class DeviceArray:
+    """A device allocation made through klrtm_malloc, with host transfers."""
+
+    def __init__(self, shape, dtype=np.float32, zero=True):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        check(lib().klrtm_malloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+        if zero:
+            self.zero()
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype, zero=False)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib().klrtm_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().klrtm_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        check(lib().klrtm_memset(self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().klrtm_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(lib().klrtm_synchronize())

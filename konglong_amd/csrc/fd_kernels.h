@@ -1,0 +1,28 @@
+// fd_kernels.h -- launch interface of the FD stencil kernels.
+#pragma once
+#include "klrtm_internal.h"
+
+namespace klrtm {
+
+enum { MODE_KERNEL = 0, MODE_STEP = 1 };
+
+// Arguments of the y-streaming stencil kernels (passed by value).
+struct IsoArgs {
+  int nx, ny, nz;
+  int tz, tx, ychunk;   // tile: z lanes, x rows, y planes per block
+  int nbz, nbx, nby;    // filled by the launcher
+  int do_y;             // Laplacian: y term only when ny > 2 (bgprtm/FdEngine_fd.cpp:207)
+  int has_pml;
+  float cz[9], cx[9], cy[9];  // [0..R]: 2nd-derivative (Laplacian) or 1st-derivative (grad/div) weights
+  PmlDesc pml;
+};
+
+int launch_iso(int R, int mode, const float *u, float *w0, const float *vel, float *pr, const IsoArgs &a);
+int launch_grad(int R, int model, const float *u, float *px, float *py, float *pz, const float *eps, const float *del,
+                const float *pjx, const float *pjy, const IsoArgs &a);
+int launch_div(int R, int mode, const float *px, const float *py, const float *pz, const float *w1, float *w0,
+               const float *vel, const IsoArgs &a);
+int launch_vel2(float *v, size_t n, float dt);
+int launch_update2nd(float *w0, const float *w1, size_t n);
+
+}  // namespace klrtm

@@ -1,0 +1,202 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker / CPU baseline.  The product
+(konglong_amd/) never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_CFD_PATH = os.path.join(HERE, "_ref", "libcfdplan_ref.so")
+REFERENCE_SRC = "/root/reference/bgprtm"
+
+_lib = None
+_pf, _pd, _pi = C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int)
+
+
+def build(ref=True):
+    """Compile liboracle.so (and _ref/ when the reference sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if ref and os.path.isdir(REFERENCE_SRC):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build(ref=False)
+        L = C.CDLL(LIB_PATH)
+        for s, T in (("f", C.c_float), ("d", C.c_double)):
+            getattr(L, f"orc_engine_create_{s}").restype = C.c_void_p
+            getattr(L, f"orc_engine_create_{s}").argtypes = [C.c_int] * 3 + [T] * 3 + [C.c_int] * 2
+            for n in ("destroy", "clean_pml"):
+                getattr(L, f"orc_engine_{n}_{s}").argtypes = [C.c_void_p]
+            getattr(L, f"orc_engine_set_threads_{s}").argtypes = [C.c_void_p, C.c_int]
+            getattr(L, f"orc_engine_set_model_{s}").argtypes = [C.c_void_p] + [C.c_void_p] * 5
+            getattr(L, f"orc_engine_set_boundary_{s}").argtypes = [C.c_void_p, _pi, T, T, T, C.c_int, T, C.c_int, C.c_int]
+            getattr(L, f"orc_engine_kernel_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+            getattr(L, f"orc_engine_step_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+            getattr(L, f"orc_engine_laplacian_{s}").argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+            getattr(L, f"orc_update2nd_{s}").argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.orc_taylor_coef2.argtypes = [C.c_int, _pd]
+        L.orc_taylor_coef1.argtypes = [C.c_int, _pd]
+        L.orc_pml_profile.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _pf, _pf]
+        L.orc_pml_round.argtypes = [C.c_float, _pf, _pf, C.c_int]
+        L.orc_cfd_plan_f.argtypes = [C.c_int, C.c_float, _pf, _pf, _pf]
+        L.orc_cfd_derive_f.argtypes = [C.c_int, C.c_float, C.c_int, _pf, _pf, C.c_int]
+        L.orc_cfd_derive_d.argtypes = [C.c_int, C.c_double, C.c_int, _pd, _pd, C.c_int]
+        L.orc_spread_coeff.restype = C.c_int
+        L.orc_spread_coeff.argtypes = [C.c_int, _pf, C.c_float, C.c_float, C.c_float, _pi]
+        L.orc_taper_kernel.argtypes = [_pf, C.c_int, C.c_int, _pf, C.c_int, C.c_int]
+        L.orc_inject.argtypes = [_pf, _pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, _pf, _pf, _pf, C.c_int, C.c_int,
+                                 C.c_int, _pi, _pi, _pi, _pi, _pi, _pi]
+        L.orc_extract.argtypes = [_pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, _pf, _pf, _pf, C.c_int, C.c_int,
+                                  C.c_int, _pi, _pi, _pi, _pi, _pi, _pi]
+        L.orc_compress16.argtypes = [_pf, C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.orc_uncompress16.argtypes = [C.c_void_p, _pf, C.c_int, C.c_int, C.c_int]
+        L.orc_codec16_table.argtypes = [_pf]
+        L.orc_crossone.argtypes = [_pf, _pf, _pf, C.c_size_t]
+        L.orc_illum.argtypes = [_pf, _pf, C.c_size_t]
+        L.orc_apply_tape.argtypes = [_pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, C.c_int, _pf, C.c_int]
+        _lib = L
+    return _lib
+
+
+def ref_cfd_lib():
+    """The reference's own CfdPlan compiled from bgprtm/Cfd_plan.cpp (None if absent)."""
+    if not os.path.exists(REF_CFD_PATH):
+        return None
+    L = C.CDLL(REF_CFD_PATH)
+    L.ref_cfd_plan.argtypes = [C.c_int, C.c_float, _pf, _pf, _pf]
+    return L
+
+
+class Engine:
+    """Oracle KernelCPU+FdEngine+PML in FP32 (dtype=np.float32) or FP64."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, engine=7, order=8, dtype=np.float32, threads=1):
+        self.s = "f" if dtype == np.float32 else "d"
+        self.dtype = dtype
+        L = lib()
+        self.h = getattr(L, f"orc_engine_create_{self.s}")(nx, ny, nz, dx, dy, dz, engine, order)
+        getattr(L, f"orc_engine_set_threads_{self.s}")(self.h, threads)
+        self.n = nx * ny * nz
+        self._keep = ()
+
+    def _c(self, a):
+        return None if a is None else np.ascontiguousarray(a, self.dtype)
+
+    def set_model(self, vel, eps=None, del_=None, pjx=None, pjy=None):
+        arrs = [self._c(a) for a in (vel, eps, del_, pjx, pjy)]
+        self._keep = arrs
+        getattr(lib(), f"orc_engine_set_model_{self.s}")(self.h, *[None if a is None else a.ctypes.data for a in arrs])
+
+    def set_boundary(self, nb, dt, vmax, pml_eps=1e-3, pml3d=True, scaler=0.3, top=True, dim_pml=3):
+        nb = np.ascontiguousarray(nb, np.int32)
+        getattr(lib(), f"orc_engine_set_boundary_{self.s}")(self.h, nb.ctypes.data_as(_pi), dt, vmax, pml_eps,
+                                                           int(pml3d), scaler, int(top), dim_pml)
+
+    def clean_pml(self):
+        getattr(lib(), f"orc_engine_clean_pml_{self.s}")(self.h)
+
+    def kernel(self, model, p0, pb, pr):
+        getattr(lib(), f"orc_engine_kernel_{self.s}")(self.h, model, p0.ctypes.data, pb.ctypes.data, pr.ctypes.data)
+
+    def step(self, model, w1, w0, wr):
+        getattr(lib(), f"orc_engine_step_{self.s}")(self.h, model, w1.ctypes.data, w0.ctypes.data, wr.ctypes.data)
+
+    def close(self):
+        if getattr(self, "h", None):
+            getattr(lib(), f"orc_engine_destroy_{self.s}")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def taylor_coef2(M):
+    c = np.zeros(M + 1, np.float64)
+    lib().orc_taylor_coef2(M, c.ctypes.data_as(_pd))
+    return c
+
+
+def taylor_coef1(M):
+    c = np.zeros(M + 1, np.float64)
+    lib().orc_taylor_coef1(M, c.ctypes.data_as(_pd))
+    return c
+
+
+def cfd_plan(n, d):
+    bm, bl, am = np.zeros(10 * n, np.float32), np.zeros(8 * n, np.float32), np.zeros(120, np.float32)
+    lib().orc_cfd_plan_f(n, d, bm.ctypes.data_as(_pf), bl.ctypes.data_as(_pf), am.ctypes.data_as(_pf))
+    return bm, bl, am
+
+
+def cfd_derive(x, d, sgn, dtype=np.float32):
+    """Compact derivative of each row of x (lines along the last axis)."""
+    x = np.ascontiguousarray(x, dtype)
+    n = x.shape[-1]
+    out = np.zeros_like(x)
+    nl = x.size // n
+    if dtype == np.float32:
+        lib().orc_cfd_derive_f(n, d, sgn, x.ctypes.data_as(_pf), out.ctypes.data_as(_pf), nl)
+    else:
+        lib().orc_cfd_derive_d(n, d, sgn, x.ctypes.data_as(_pd), out.ctypes.data_as(_pd), nl)
+    return out
+
+
+def pml_profile(npml, dx, dt, vmax, pml_eps=1e-3):
+    coef, feoc = np.zeros(npml, np.float32), np.zeros(npml, np.float32)
+    lib().orc_pml_profile(npml, dx, dt, vmax, pml_eps, coef.ctypes.data_as(_pf), feoc.ctypes.data_as(_pf))
+    return coef, feoc
+
+
+def pml_round(g, p, q):
+    p = np.ascontiguousarray(p, np.float32).copy()
+    q = np.ascontiguousarray(q, np.float32).copy()
+    lib().orc_pml_round(g, p.ctypes.data_as(_pf), q.ctypes.data_as(_pf), p.size)
+    return p, q
+
+
+def spread_coeff(n, x, k1, k2, nlimit):
+    coeff = np.zeros(max(nlimit, 1), np.float32)
+    nl = C.c_int(nlimit)
+    off = lib().orc_spread_coeff(n, coeff.ctypes.data_as(_pf), x, k1, k2, C.byref(nl))
+    return off, coeff[: nl.value].copy()
+
+
+def taper_kernel(kernel, off, taper, n):
+    k = np.ascontiguousarray(kernel, np.float32).copy()
+    t = np.ascontiguousarray(taper, np.float32)
+    lib().orc_taper_kernel(k.ctypes.data_as(_pf), k.size, off, t.ctypes.data_as(_pf), t.size, n)
+    return k
+
+
+def compress16(vol):
+    ny, nx, nz = vol.shape
+    out = np.zeros(ny * (nx * nz + 2), np.uint16)
+    v = np.ascontiguousarray(vol, np.float32)
+    lib().orc_compress16(v.ctypes.data_as(_pf), out.ctypes.data, nx, ny, nz)
+    return out
+
+
+def uncompress16(code, ny, nx, nz):
+    out = np.zeros((ny, nx, nz), np.float32)
+    c = np.ascontiguousarray(code, np.uint16)
+    lib().orc_uncompress16(c.ctypes.data, out.ctypes.data_as(_pf), nx, ny, nz)
+    return out
+
+
+def codec16_table():
+    t = np.zeros(65536, np.float32)
+    lib().orc_codec16_table(t.ctypes.data_as(_pf))
+    return t

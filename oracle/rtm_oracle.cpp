@@ -1,0 +1,942 @@
+// rtm_oracle.cpp -- CPU oracle for the RTM propagation-and-imaging hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is a plain C++ restatement of the
+// reference algorithm (UPC-HPC/konglong, bgprtm/) used as the checker for the
+// HIP product library and as the timed CPU baseline.  Nothing in the product
+// (konglong_amd/) links, loads or calls it.
+//
+// Every routine cites the reference file:line it restates.  The FP32
+// instantiation keeps the reference's operation order (compile with
+// -ffp-contract=off); the FP64 instantiation is the "FP64 oracle" used for
+// NRMSE statements.
+//
+// Index convention (bgprtm/FdEngine.h:42): idx = (iy*nx + ix)*nz + iz.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <vector>
+#include <memory>
+#include <algorithm>
+#include <omp.h>
+
+namespace orc {
+
+enum { ENGINE_CFD = 4, ENGINE_FD = 7 };      // bgprtm/FdEngine.h:25-27
+enum { ISO = 1, VTI = 2, TTI = 3 };          // bgprtm/Model.h:13
+enum { AX_Z = 0, AX_X = 1, AX_Y = 2 };       // bgprtm/PML.h:13-15
+enum { TOP = 0, BOT = 1 };
+enum { ROUND1 = 0, ROUND2 = 1 };
+// component slots of the per-face state: bgprtm/PML.h:22-24
+enum { ZZ = 0, ZX = 1, ZY = 2, XX = 0, XZ = 1, XY = 2, YY = 0, YZ = 1, YX = 2 };
+
+// ---------------------------------------------------------------------------
+// Taylor (maximal-order) central FD coefficients.  The reference reads them
+// from the unvendored libCommon::Taylor tables (bgprtm/FdEngine_fd.cpp:37-47);
+// with fd_use_taylor they are the classical closed forms below.
+//   c2_j = 2(-1)^{j+1} (M!)^2 / (j^2 (M-j)! (M+j)!),  c2_0 = -2 sum c2_j
+//   c1_j =  (-1)^{j+1} (M!)^2 / (j   (M-j)! (M+j)!)
+static double ratio_fact(int M, int j) {  // (M!)^2 / ((M-j)! (M+j)!)
+  double r = 1.0;
+  for(int k = 1; k <= j; k++) r *= double(M - j + k) / double(M + k);
+  return r;
+}
+void taylor_coef2(int M, double *c) {  // c[0..M]
+  double s = 0;
+  for(int j = 1; j <= M; j++) {
+    double sg = (j % 2) ? 1.0 : -1.0;
+    c[j] = 2.0 * sg * ratio_fact(M, j) / (double(j) * j);
+    s += c[j];
+  }
+  c[0] = -2.0 * s;
+}
+void taylor_coef1(int M, double *c) {  // c[1..M]
+  c[0] = 0;
+  for(int j = 1; j <= M; j++) {
+    double sg = (j % 2) ? 1.0 : -1.0;
+    c[j] = sg * ratio_fact(M, j) / double(j);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PML profile: bgprtm/PML.cpp:23-60 (non limit_slope branch).
+template<class T> struct Pml {
+  int npml;
+  std::vector<T> coef, feoc;
+  Pml(int npml_, T dx, T dt, T vmax, T pml_eps) : npml(npml_), coef(npml_), feoc(npml_) {
+    T beta = dx / (dt * vmax);
+    T dg1 = std::log(T(1) / pml_eps) / (T(npml) * beta);
+    for(int i = 0; i < npml; i++) {
+      T temp = T(npml - i) / T(npml);
+      coef[i] = T(1.5) / T(2) * temp * temp * dg1;
+      feoc[npml - 1 - i] = coef[i];
+    }
+  }
+};
+
+// one damping round on one sample: bgprtm/PML.cpp:62-70
+template<class T> static inline void pml_round(T g, T &p, T &q) {
+  T qi = (T(1) - g) / (T(1) + g) * q - g * T(2) / (T(1) - g * g) * p;
+  q = qi;
+  p = qi + p / (T(1) - g);
+}
+
+// ---------------------------------------------------------------------------
+// Compact FD plan: bgprtm/Cfd_plan.cpp:26-262 (10th-order, 9-band LHS).
+template<class T> struct CfdPlan {
+  int n = 0;
+  std::vector<T> bm[2], bl[2], am[2];  // [0]: isgn=1 (gradient), [1]: isgn=2 (divergence)
+
+  void init(int n_, T d) {
+    n = n_;
+    for(int s = 0; s < 2; s++) bm[s].assign(5 * n, 0), bl[s].assign(4 * n, 0), am[s].assign(60, 0);
+    if(n <= 1) return;
+    std::vector<T> b(9 * (size_t)n);
+    for(int s = 0; s < 2; s++) {
+      build(b.data(), bl[s].data(), am[s].data(), d, s + 1);
+      bandec(b.data(), bl[s].data());
+      // keep the diagonal and 4 upper bands, then store row-major [i][5] / [i][4]
+      std::vector<T> u(5 * (size_t)n), l(4 * (size_t)n);
+      for(int j = 0; j < 5; j++)
+        for(int i = 0; i < n; i++) u[(size_t)i * 5 + j] = b[(size_t)j * n + i];
+      for(int j = 0; j < 4; j++)
+        for(int i = 0; i < n; i++) l[(size_t)i * 4 + j] = bl[s][(size_t)j * n + i];
+      bm[s] = u, bl[s] = l;
+    }
+  }
+
+  // LHS bands and RHS table with boundary folding: bgprtm/Cfd_plan.cpp:127-290
+  void build(T *b, T *bl_, T *a, T dx, int isgn) {
+    const T ca5[5] = {T(0.861085998189822), T(0.753782983214427), T(0.257072513289755), T(0.031326722562826),
+                      T(0.000410456919800)};
+    const T cb5[5] = {T(1.0), T(0.753205475977546), T(0.312203852849021), T(0.063580085296350),
+                      T(0.004597763362256)};
+    for(size_t i = 0; i < 4 * (size_t)n; i++) bl_[i] = 0;
+    for(size_t i = 0; i < 9 * (size_t)n; i++) b[i] = 0;
+    for(int i = 0; i < 60; i++) a[i] = 0;
+    T cb[9] = {cb5[4], cb5[3], cb5[2], cb5[1], cb5[0], cb5[1], cb5[2], cb5[3], cb5[4]};
+    // ghost rows are folded back with - for the gradient plan, + for the divergence plan
+    for(int i = 0; i < 4; i++)
+      for(int j = 0; j < 9; j++) {
+        int j1 = i - 4 + j;
+        if(j1 < 0) {
+          int j2 = 4 - i - j1;
+          b[(size_t)j2 * n + i] = (isgn == 1) ? b[(size_t)j2 * n + i] - cb[j] : b[(size_t)j2 * n + i] + cb[j];
+        } else b[(size_t)j * n + i] = b[(size_t)j * n + i] + cb[j];
+      }
+    for(int i = 4; i < n - 4; i++)
+      for(int j = 0; j < 9; j++) b[(size_t)j * n + i] = cb[j];
+    for(int i = n - 4; i < n; i++)
+      for(int j = 0; j < 9; j++) {
+        int j1 = i - 4 + j;
+        if(j1 > n - 1) {
+          int j2 = (n + 3 - i) - (j1 - n + 1);
+          b[(size_t)j2 * n + i] = (isgn == 1) ? b[(size_t)j2 * n + i] - cb[j] : b[(size_t)j2 * n + i] + cb[j];
+        } else b[(size_t)j * n + i] = b[(size_t)j * n + i] + cb[j];
+      }
+    T ca[11] = {-ca5[4], -ca5[3], -ca5[2], -ca5[1], -ca5[0], T(0), ca5[0], ca5[1], ca5[2], ca5[3], ca5[4]};
+    for(int i = 0; i < 5; i++)
+      for(int j = 0; j < 11; j++) {
+        int j1 = i - 5 + j;
+        if(j1 < 0) {
+          int j2 = 5 - i - j1;
+          a[j2 * 5 + i] = (isgn == 1) ? a[j2 * 5 + i] + ca[j] : a[j2 * 5 + i] - ca[j];
+        } else a[j * 5 + i] = a[j * 5 + i] + ca[j];
+      }
+    for(int i = 0; i < 5; i++) a[11 * 5 + i] = ca5[i];
+    for(int i = 0; i < 5; i++)
+      for(int j = 0; j < 12; j++) a[j * 5 + i] = a[j * 5 + i] / dx * T(0.5);
+  }
+
+  // banded LU without pivoting (band storage a[j*n+i]): bgprtm/Cfd_plan.cpp:98-126
+  void bandec(T *a, T *al) {
+    const int m1 = 4, mm = 2 * m1 + 1;
+    int l = m1;
+    for(int i = 0; i < m1; i++) {
+      for(int j = m1 - i; j < mm; j++) a[(size_t)(j - l) * n + i] = a[(size_t)j * n + i];
+      l = l - 1;
+      for(int j = mm - l - 1; j < mm; j++) a[(size_t)j * n + i] = 0;
+    }
+    l = m1;
+    for(int k = 1; k <= n; k++) {
+      T dum = a[k - 1];
+      if(l < n) l++;
+      if(dum == T(0)) a[k - 1] = T(1e-20);
+      for(int i = k + 1; i <= l; i++) {
+        dum = a[i - 1] / a[k - 1];
+        al[(size_t)(i - k - 1) * n + k - 1] = dum;
+        for(int j = 2; j <= mm; j++)
+          a[(size_t)(j - 2) * n + i - 1] = a[(size_t)(j - 1) * n + i - 1] - dum * a[(size_t)(j - 1) * n + k - 1];
+        a[(size_t)(mm - 1) * n + i - 1] = 0;
+      }
+    }
+  }
+
+  // derivative of one line (stride s): RHS band multiply, then forward/back
+  // substitution.  bgprtm/FdEngine_cfd.cpp:170-201 (banmul), 215-254 (banbks)
+  void derive(int sgn, const T *x, T *b, size_t s) const { derive2(sgn, x, s, b, s); }
+  void derive2(int sgn, const T *x, size_t sx, T *b, size_t s) const {
+    const T *a = am[sgn].data(), *u = bm[sgn].data(), *al = bl[sgn].data();
+    for(int i = 0; i < 5; i++) {
+      T acc = 0;
+      int k = i - 5;
+      for(int j = std::max(0, -k); j < 11; j++) acc = acc + a[j * 5 + i] * x[(j + k) * sx];
+      b[i * s] = acc;
+    }
+    for(int i = 5; i < n - 5; i++) {
+      b[i * s] = a[55] * (x[(i + 1) * sx] - x[(i - 1) * sx]) + a[56] * (x[(i + 2) * sx] - x[(i - 2) * sx]) +
+                 a[57] * (x[(i + 3) * sx] - x[(i - 3) * sx]) + a[58] * (x[(i + 4) * sx] - x[(i - 4) * sx]) +
+                 a[59] * (x[(i + 5) * sx] - x[(i - 5) * sx]);
+    }
+    for(int i = n - 5; i < n; i++) {
+      T acc = 0;
+      int k = i - 5;
+      for(int j = 0; j < std::min(11, n - k); j++) acc = acc - a[n - i + 49 - 5 * j] * x[(j + k) * sx];
+      b[i * s] = acc;
+    }
+    for(int k = 0; k < n - 4; k++)
+      for(int j = 0; j < 4; j++) b[(j + k + 1) * s] = b[(j + k + 1) * s] - al[j + k * 4] * b[k * s];
+    for(int k = n - 4; k < n; k++)
+      for(int j = 0; j < n - k - 1; j++) b[(j + k + 1) * s] = b[(j + k + 1) * s] - al[j + k * 4] * b[k * s];
+    int l = 1;
+    for(int i = n - 1; i >= n - 4; i--) {
+      for(int k = 1; k < l; k++) b[i * s] += -u[i * 5 + k] * b[(k + i) * s];
+      b[i * s] = b[i * s] / u[i * 5];
+      if(l < 5) l++;
+    }
+    for(int i = n - 5; i >= 0; i--)
+      b[i * s] = (b[i * s] - u[i * 5 + 1] * b[(i + 1) * s] - u[i * 5 + 2] * b[(i + 2) * s] - u[i * 5 + 3] * b[(i + 3) * s] -
+                  u[i * 5 + 4] * b[(i + 4) * s]) / u[i * 5];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Propagation engine: KernelCPU + FdEngine(_fd|_cfd) + PML state.
+//   bgprtm/KernelCPU.cpp:93-174, bgprtm/FdEngine.cpp:40-53,108-215,301-395,
+//   bgprtm/FdEngine_fd.cpp:16-131,204-470, bgprtm/FdEngine_cfd.cpp:72-153
+template<class T> struct Engine {
+  int nx, ny, nz;
+  T dx, dy, dz;
+  int engine, order, ne;
+  size_t nxz, nxy, nxyz;
+  std::vector<T> c1, c2;              // c1[1..ne], c2[0..ne]
+  T zpi, zpi2, dxi, dyi, dx2i, dy2i;  // derived scales (bgprtm/FdEngine_fd.cpp:110-125,208-209)
+  int nb[6] = {0, 0, 0, 0, 0, 0};     // nxbnd1 nxbnd2 nybnd1 nybnd2 nzbnd1 nzbnd2
+  bool pml3d = true, topPML = true, allPML = true;
+  T scaler = T(0.3);
+  std::unique_ptr<Pml<T>> pml[3][2];
+  std::vector<T> buf[3][2][2][3];
+  const T *vel = nullptr, *eps = nullptr, *del = nullptr, *pjx = nullptr, *pjy = nullptr;
+  CfdPlan<T> cfd[3];  // [AX_Z, AX_X, AX_Y]
+  std::vector<T> px2d, pz2d, wrk;
+  int nthreads = 1;
+
+  Engine(int nx_, int ny_, int nz_, T dx_, T dy_, T dz_, int engine_, int order_)
+      : nx(nx_), ny(ny_), nz(nz_), dx(dx_), dy(dy_), dz(dz_), engine(engine_), order(order_) {
+    nxz = (size_t)nx * nz, nxy = (size_t)nx * ny, nxyz = nxy * nz;
+    ne = order / 2;
+    std::vector<double> d2(ne + 1), d1(ne + 1);
+    taylor_coef2(ne, d2.data());
+    taylor_coef1(ne, d1.data());
+    c1.resize(ne + 1), c2.resize(ne + 1);
+    for(int j = 0; j <= ne; j++) c1[j] = T(d1[j]), c2[j] = T(d2[j]);
+    zpi = T(1) / dz;
+    zpi2 = zpi * zpi;
+    dxi = T(1) / dx;
+    dyi = T(1) / dy;
+    dx2i = T(1.0 / double(dx * dx));
+    dy2i = T(1.0 / double(dy * dy));
+    if(engine == ENGINE_CFD) {
+      cfd[AX_Z].init(nz, dz);
+      cfd[AX_X].init(nx, dx);
+      cfd[AX_Y].init(ny, dy);
+    }
+    px2d.assign(nxz, 0), pz2d.assign(nxz, 0), wrk.assign(std::max(nxz, (size_t)ny * nz), 0);
+  }
+
+  // bgprtm/FdEngine.cpp:108-182
+  void setBoundary(const int *nbnd, T dt, T vmax, T pml_eps, int dimPML) {
+    for(int i = 0; i < 6; i++) nb[i] = nbnd[i];
+    int n_of[3] = {nz, nx, ny};
+    T d_of[3] = {dz, dx, dy};
+    int w[3][2] = {{nb[4], nb[5]}, {nb[0], nb[1]}, {nb[2], nb[3]}};
+    for(int a = 0; a < 3; a++)
+      for(int t = 0; t < 2; t++) {
+        pml[a][t].reset();
+        bool on = w[a][t] > 1 && allPML && !(a == AX_Z && t == TOP && !topPML);
+        if(on) pml[a][t].reset(new Pml<T>(w[a][t], d_of[a], dt, vmax, pml_eps));
+        size_t face = (a == AX_Z) ? nxy : (a == AX_X) ? (size_t)ny * nz : nxz;
+        for(int r = 0; r < 2; r++)
+          for(int c = 0; c < 3; c++)
+            buf[a][t][r][c].assign(c < dimPML && w[a][t] > 0 ? face * w[a][t] : 0, T(0));
+      }
+    (void)n_of;
+  }
+  void cleanPML() {
+    for(auto &a : buf)
+      for(auto &t : a)
+        for(auto &r : t)
+          for(auto &c : r) std::fill(c.begin(), c.end(), T(0));
+  }
+
+  // --- PML application on one column (z-fastest) --------------------------
+  // z-faces: first/last nzbnd entries of a column (bgprtm/FdEngine.cpp:324-328, 352-357)
+  void pml1_z(T *col, int ix, int iy, int r, int comp, T sc) {
+    size_t ixy = (size_t)iy * nx + ix;
+    if(pml[AX_Z][TOP]) {
+      Pml<T> &p = *pml[AX_Z][TOP];
+      T *q = &buf[AX_Z][TOP][r][comp][ixy * nb[4]];
+      for(int i = 0; i < p.npml; i++) pml_round(p.coef[i] * sc, col[i], q[i]);
+    }
+    if(pml[AX_Z][BOT]) {
+      Pml<T> &p = *pml[AX_Z][BOT];
+      T *q = &buf[AX_Z][BOT][r][comp][ixy * nb[5]];
+      T *c = col + nz - nb[5];
+      for(int i = 0; i < p.npml; i++) pml_round(p.feoc[i] * sc, c[i], q[i]);
+    }
+  }
+  // x-faces: whole column with g=coef[ix] (bgprtm/FdEngine.cpp:330-339)
+  void pml1_x(T *col, int ix, int iy, int r, int comp, T sc) {
+    if(ix < nb[0] && pml[AX_X][TOP]) {
+      T g = pml[AX_X][TOP]->coef[ix] * sc;
+      T *q = &buf[AX_X][TOP][r][comp][((size_t)iy * nb[0] + ix) * nz];
+      for(int j = 0; j < nz; j++) pml_round(g, col[j], q[j]);
+    }
+    if(ix >= nx - nb[1] && pml[AX_X][BOT]) {
+      T g = pml[AX_X][BOT]->coef[nx - 1 - ix] * sc;
+      T *q = &buf[AX_X][BOT][r][comp][((size_t)iy * nb[1] + ix - nx + nb[1]) * nz];
+      for(int j = 0; j < nz; j++) pml_round(g, col[j], q[j]);
+    }
+  }
+  // y-faces (bgprtm/FdEngine.cpp:341-350)
+  void pml1_y(T *col, int ix, int iy, int r, int comp, T sc) {
+    if(iy < nb[2] && pml[AX_Y][TOP]) {
+      T g = pml[AX_Y][TOP]->coef[iy] * sc;
+      T *q = &buf[AX_Y][TOP][r][comp][((size_t)iy * nx + ix) * nz];
+      for(int j = 0; j < nz; j++) pml_round(g, col[j], q[j]);
+    }
+    if(iy >= ny - nb[3] && pml[AX_Y][BOT]) {
+      T g = pml[AX_Y][BOT]->coef[ny - 1 - iy] * sc;
+      T *q = &buf[AX_Y][BOT][r][comp][((size_t)(iy - ny + nb[3]) * nx + ix) * nz];
+      for(int j = 0; j < nz; j++) pml_round(g, col[j], q[j]);
+    }
+  }
+  // first-derivative PML with cross-axis damping (bgprtm/FdEngine.cpp:301-323)
+  void pml1_zbnd(T *col, int ix, int iy, int isign) {
+    int r = isign > 0 ? ROUND1 : ROUND2;
+    pml1_z(col, ix, iy, r, ZZ, T(1));
+    if(pml3d) pml1_x(col, ix, iy, r, XZ, scaler), pml1_y(col, ix, iy, r, YZ, scaler);
+  }
+  void pml1_xbnd(T *col, int ix, int iy, int isign) {
+    int r = isign > 0 ? ROUND1 : ROUND2;
+    pml1_x(col, ix, iy, r, XX, T(1));
+    if(pml3d) pml1_z(col, ix, iy, r, ZX, scaler), pml1_y(col, ix, iy, r, YX, scaler);
+  }
+  void pml1_ybnd(T *col, int ix, int iy, int isign) {
+    int r = isign > 0 ? ROUND1 : ROUND2;
+    pml1_y(col, ix, iy, r, YY, T(1));
+    if(pml3d && ny > 1) pml1_z(col, ix, iy, r, ZY, scaler), pml1_x(col, ix, iy, r, XY, scaler);
+  }
+  // second-order (double round) PML on the summed Laplacian, order z, x, y
+  // (bgprtm/FdEngine.cpp:352-395, bgprtm/PML.cpp:72-85,121-141)
+  void pml2_bnd(T *col, int ix, int iy) {
+    size_t ixy = (size_t)iy * nx + ix;
+    if(pml[AX_Z][TOP]) {
+      Pml<T> &p = *pml[AX_Z][TOP];
+      T *q1 = &buf[AX_Z][TOP][ROUND1][ZZ][ixy * nb[4]], *q2 = &buf[AX_Z][TOP][ROUND2][ZZ][ixy * nb[4]];
+      for(int i = 0; i < p.npml; i++) pml_round(p.coef[i], col[i], q1[i]), pml_round(p.coef[i], col[i], q2[i]);
+    }
+    if(pml[AX_Z][BOT]) {
+      Pml<T> &p = *pml[AX_Z][BOT];
+      T *q1 = &buf[AX_Z][BOT][ROUND1][ZZ][ixy * nb[5]], *q2 = &buf[AX_Z][BOT][ROUND2][ZZ][ixy * nb[5]];
+      T *c = col + nz - nb[5];
+      for(int i = 0; i < p.npml; i++) pml_round(p.feoc[i], c[i], q1[i]), pml_round(p.feoc[i], c[i], q2[i]);
+    }
+    if(ix < nb[0] && pml[AX_X][TOP]) {
+      T g = pml[AX_X][TOP]->coef[ix];
+      size_t o = ((size_t)iy * nb[0] + ix) * nz;
+      T *q1 = &buf[AX_X][TOP][ROUND1][XX][o], *q2 = &buf[AX_X][TOP][ROUND2][XX][o];
+      for(int j = 0; j < nz; j++) pml_round(g, col[j], q1[j]), pml_round(g, col[j], q2[j]);
+    }
+    if(ix >= nx - nb[1] && pml[AX_X][BOT]) {
+      T g = pml[AX_X][BOT]->coef[nx - 1 - ix];
+      size_t o = ((size_t)iy * nb[1] + ix - nx + nb[1]) * nz;
+      T *q1 = &buf[AX_X][BOT][ROUND1][XX][o], *q2 = &buf[AX_X][BOT][ROUND2][XX][o];
+      for(int j = 0; j < nz; j++) pml_round(g, col[j], q1[j]), pml_round(g, col[j], q2[j]);
+    }
+    if(iy < nb[2] && pml[AX_Y][TOP]) {
+      T g = pml[AX_Y][TOP]->coef[iy];
+      size_t o = ((size_t)iy * nx + ix) * nz;
+      T *q1 = &buf[AX_Y][TOP][ROUND1][YY][o], *q2 = &buf[AX_Y][TOP][ROUND2][YY][o];
+      for(int j = 0; j < nz; j++) pml_round(g, col[j], q1[j]), pml_round(g, col[j], q2[j]);
+    }
+    if(iy >= ny - nb[3] && pml[AX_Y][BOT]) {
+      T g = pml[AX_Y][BOT]->coef[ny - 1 - iy];
+      size_t o = ((size_t)(iy - ny + nb[3]) * nx + ix) * nz;
+      T *q1 = &buf[AX_Y][BOT][ROUND1][YY][o], *q2 = &buf[AX_Y][BOT][ROUND2][YY][o];
+      for(int j = 0; j < nz; j++) pml_round(g, col[j], q1[j]), pml_round(g, col[j], q2[j]);
+    }
+  }
+
+  // --- Laplacian with zero halo + PML: bgprtm/FdEngine_fd.cpp:204-325 -------
+  void laplacian(const T *in, T *out) {
+    const T cz0 = zpi2 * c2[0];
+    std::vector<T> czm(ne + 1);
+    for(int j = 1; j <= ne; j++) czm[j] = zpi2 * c2[j];
+    bool do_y = ny > 2;
+#pragma omp parallel for num_threads(nthreads) collapse(2) schedule(static)
+    for(int iy = 0; iy < ny; iy++)
+      for(int ix = 0; ix < nx; ix++) {
+        thread_local std::vector<T> bufc;
+        bufc.resize(nz);
+        const T *u = in + (size_t)iy * nxz + (size_t)ix * nz;
+        T *v = out + (size_t)iy * nxz + (size_t)ix * nz;
+        for(int iz = 0; iz < nz; iz++) v[iz] = cz0 * u[iz];
+        for(int m = -ne; m <= ne; m++) {
+          if(m == 0) continue;
+          T c = czm[m < 0 ? -m : m];
+          int iz0 = std::max(0, -m), iz1 = std::min(nz, nz - m);
+          for(int iz = iz0; iz < iz1; iz++) v[iz] += c * u[iz + m];
+        }
+        {
+          T c = c2[0] * dx2i;
+          for(int iz = 0; iz < nz; iz++) bufc[iz] = c * u[iz];
+          int nev0 = std::min(ne, ix), nev1 = std::min(ne, nx - 1 - ix);
+          for(int m = 1; m <= ne; m++) {
+            c = c2[m] * dx2i;
+            if(m <= nev0)
+              for(int iz = 0; iz < nz; iz++) bufc[iz] += c * u[iz - (ptrdiff_t)m * nz];
+            if(m <= nev1)
+              for(int iz = 0; iz < nz; iz++) bufc[iz] += c * u[iz + (ptrdiff_t)m * nz];
+          }
+          for(int iz = 0; iz < nz; iz++) v[iz] += bufc[iz];
+        }
+        if(do_y) {
+          T c = c2[0] * dy2i;
+          for(int iz = 0; iz < nz; iz++) bufc[iz] = c * u[iz];
+          int nev0 = std::min(ne, iy), nev1 = std::min(ne, ny - 1 - iy);
+          for(int m = 1; m <= ne; m++) {
+            c = c2[m] * dy2i;
+            if(m <= nev0)
+              for(int iz = 0; iz < nz; iz++) bufc[iz] += c * u[iz - (ptrdiff_t)m * nxz];
+            if(m <= nev1)
+              for(int iz = 0; iz < nz; iz++) bufc[iz] += c * u[iz + (ptrdiff_t)m * nxz];
+          }
+          for(int iz = 0; iz < nz; iz++) v[iz] += bufc[iz];
+        }
+        if(pml3d) pml2_bnd(v, ix, iy);
+      }
+  }
+
+  // applyVel_minus: bgprtm/KernelCPU.cpp:328-338
+  void applyVel_minus(const T *pr, T *pb) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for(int iy = 0; iy < ny; iy++)
+      for(size_t i = (size_t)iy * nxz; i < (size_t)(iy + 1) * nxz; i++) pb[i] = pb[i] - pr[i] * vel[i];
+  }
+
+  // --- FD first derivatives, one column: bgprtm/FdEngine_fd.cpp:388-470 ----
+  // iacc=0: out = PML(D u); iacc=1: out -= PML(D u) * vel
+  void fd_dz_col(const T *u, T *out, int ix, int iy, int iacc, int isign, const T *vcol) {
+    thread_local std::vector<T> tmp;
+    tmp.assign(nz, T(0));
+    T *vz = iacc ? tmp.data() : out;
+    if(!iacc) std::fill(out, out + nz, T(0));
+    for(int m = -ne; m <= ne; m++) {
+      if(m == 0) continue;
+      T c = (m < 0) ? -(zpi * c1[-m]) : zpi * c1[m];
+      int iz0 = std::max(0, -m), iz1 = std::min(nz, nz - m);
+      for(int iz = iz0; iz < iz1; iz++) vz[iz] += c * u[iz + m];
+    }
+    pml1_zbnd(vz, ix, iy, isign);
+    if(iacc)
+      for(int iz = 0; iz < nz; iz++) out[iz] -= vz[iz] * vcol[iz];
+  }
+  void fd_dx_col(const T *u, T *out, int ix, int iy, int iacc, int isign, const T *vcol) {
+    thread_local std::vector<T> tmp;
+    tmp.assign(nz, T(0));
+    T *vx = iacc ? tmp.data() : out;
+    if(!iacc) std::fill(out, out + nz, T(0));
+    int nev0 = std::min(ne, ix), nev1 = std::min(ne, nx - 1 - ix);
+    for(int m = 1; m <= ne; m++) {
+      T c = c1[m] * dxi;
+      if(m <= nev0)
+        for(int iz = 0; iz < nz; iz++) vx[iz] -= c * u[iz - (ptrdiff_t)m * nz];
+      if(m <= nev1)
+        for(int iz = 0; iz < nz; iz++) vx[iz] += c * u[iz + (ptrdiff_t)m * nz];
+    }
+    pml1_xbnd(vx, ix, iy, isign);
+    if(iacc)
+      for(int iz = 0; iz < nz; iz++) out[iz] -= vx[iz] * vcol[iz];
+  }
+  void fd_dy_col(const T *u, T *out, int ix, int iy, int iacc, int isign, const T *vcol) {
+    thread_local std::vector<T> tmp;
+    tmp.assign(nz, T(0));
+    T *vy = iacc ? tmp.data() : out;
+    if(!iacc) std::fill(out, out + nz, T(0));
+    int nev0 = std::min(ne, iy), nev1 = std::min(ne, ny - 1 - iy);
+    for(int m = 1; m <= ne; m++) {
+      T c = c1[m] * dyi;
+      if(m <= nev0)
+        for(int iz = 0; iz < nz; iz++) vy[iz] -= c * u[iz - (ptrdiff_t)m * nxz];
+      if(m <= nev1)
+        for(int iz = 0; iz < nz; iz++) vy[iz] += c * u[iz + (ptrdiff_t)m * nxz];
+    }
+    pml1_ybnd(vy, ix, iy, isign);
+    if(iacc)
+      for(int iz = 0; iz < nz; iz++) out[iz] -= vy[iz] * vcol[iz];
+  }
+
+  // --- slice / volume derivative drivers (FD or CFD) ------------------------
+  // z-derivative of one y-slice: FdEngine_fd dxz_2D / FdEngine_cfd::dz1_2D (bgprtm/FdEngine_cfd.cpp:131-153)
+  void dz_slice(const T *in, T *out, int iy, int iacc, int isign, const T *vslice) {
+    if(engine == ENGINE_FD) {
+      for(int ix = 0; ix < nx; ix++)
+        fd_dz_col(in + (size_t)ix * nz, out + (size_t)ix * nz, ix, iy, iacc, isign, vslice ? vslice + (size_t)ix * nz : nullptr);
+      return;
+    }
+    int sg = isign == 1 ? 0 : 1;
+    thread_local std::vector<T> w;
+    w.resize(nxz);
+    T *dst = iacc ? w.data() : out;
+    for(int ix = 0; ix < nx; ix++) cfd[AX_Z].derive(sg, in + (size_t)ix * nz, dst + (size_t)ix * nz, 1);
+    for(int ix = 0; ix < nx; ix++) pml1_zbnd(dst + (size_t)ix * nz, ix, iy, isign);
+    if(iacc)
+      for(size_t i = 0; i < nxz; i++) out[i] = out[i] - dst[i] * vslice[i];
+  }
+  // x-derivative of one y-slice: bgprtm/FdEngine_cfd.cpp:83-105
+  void dx_slice(const T *in, T *out, int iy, int iacc, int isign, const T *vslice) {
+    if(engine == ENGINE_FD) {
+      for(int ix = 0; ix < nx; ix++)
+        fd_dx_col(in + (size_t)ix * nz, out + (size_t)ix * nz, ix, iy, iacc, isign, vslice ? vslice + (size_t)ix * nz : nullptr);
+      return;
+    }
+    int sg = isign == 1 ? 0 : 1;
+    thread_local std::vector<T> w;
+    w.resize(nxz);
+    T *dst = iacc ? w.data() : out;
+    for(int iz = 0; iz < nz; iz++) cfd[AX_X].derive(sg, in + iz, dst + iz, (size_t)nz);
+    for(int ix = 0; ix < nx; ix++) pml1_xbnd(dst + (size_t)ix * nz, ix, iy, isign);
+    if(iacc)
+      for(size_t i = 0; i < nxz; i++) out[i] = out[i] - dst[i] * vslice[i];
+  }
+  // y-derivative of the volume: FdEngine_fd::dy_3D / FdEngine::dy_3D + FdEngine_cfd::dy1_2D
+  void dy_vol(const T *in, T *out, int iacc, int isign, const T *v) {
+    if(ny <= 1) return;
+    if(engine == ENGINE_FD) {
+#pragma omp parallel for num_threads(nthreads) collapse(2) schedule(static)
+      for(int iy = 0; iy < ny; iy++)
+        for(int ix = 0; ix < nx; ix++) {
+          size_t o = (size_t)iy * nxz + (size_t)ix * nz;
+          fd_dy_col(in + o, out + o, ix, iy, iacc, isign, v ? v + o : nullptr);
+        }
+      return;
+    }
+    int sg = isign == 1 ? 0 : 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for(int ix = 0; ix < nx; ix++) {
+      thread_local std::vector<T> w;
+      w.resize((size_t)ny * nz);
+      for(int iz = 0; iz < nz; iz++) {
+        // gather the (y,z) plane at this ix into a contiguous line set
+        if(iacc) {
+          cfd[AX_Y].derive2(sg, in + (size_t)ix * nz + iz, nxz, &w[iz], (size_t)nz);
+        } else {
+          cfd[AX_Y].derive(sg, in + (size_t)ix * nz + iz, out + (size_t)ix * nz + iz, nxz);
+        }
+      }
+      for(int iy = 0; iy < ny; iy++) {
+        if(iacc) {
+          T *col = &w[(size_t)iy * nz];
+          pml1_ybnd(col, ix, iy, isign);
+          T *o = out + (size_t)iy * nxz + (size_t)ix * nz;
+          const T *vc = v + (size_t)iy * nxz + (size_t)ix * nz;
+          for(int iz = 0; iz < nz; iz++) o[iz] = o[iz] - col[iz] * vc[iz];
+        } else {
+          pml1_ybnd(out + (size_t)iy * nxz + (size_t)ix * nz, ix, iy, isign);
+        }
+      }
+    }
+  }
+
+  // ScalarVTI / ScalarTTI on one slice: bgprtm/KernelCPU.cpp:176-297
+  void scalar(int modelType, T *wx, T *wy, T *wz, int iy) {
+    size_t off = (size_t)iy * nxz;
+    const T one = 1, two = 2, eight = 8, half = T(0.5), small = T(1e-32);
+    for(size_t i = 0; i < nxz; i++) {
+      T myeps = eps[off + i];
+      T mydel = myeps - del[off + i];
+      myeps = one + two * myeps;
+      T vx = wx[i], vy = wy ? wy[i] : T(0), vz = wz[i];
+      if(modelType == VTI) {
+        T vecx2 = vx * vx + vy * vy;
+        T vecz2 = vz * vz;
+        T eclip = myeps * vecx2 + vecz2;
+        T parts = std::sqrt(one - (eight * (mydel * (vecx2 * vecz2))) / (eclip * eclip + small));
+        T scale = half * (one + parts);
+        T scale2 = myeps * std::min(scale, one);
+        wx[i] = vx * scale2;
+        if(wy) wy[i] = vy * scale2;
+      } else {
+        T ax = pjx[off + i], ay = wy ? pjy[off + i] : T(0);
+        T az = std::sqrt(one - (ax * ax + ay * ay));
+        T vect2 = (vx * vx + vy * vy) + vz * vz;
+        T veczz = (vx * ax + vy * ay) + vz * az;
+        T vecz2 = veczz * veczz;
+        T vech2 = vect2 - vecz2;
+        T eclip = myeps * vech2 + vecz2;
+        T parts = std::sqrt(one - (eight * (mydel * (vech2 * vecz2))) / (eclip * eclip + small));
+        T scale = half * (one + parts);
+        scale = std::min(scale, one);
+        ax = veczz * ax, ay = veczz * ay, az = veczz * az;
+        wx[i] = (ax + (vx - ax) * myeps) * scale;
+        if(wy) wy[i] = (ay + (vy - ay) * myeps) * scale;
+        wz[i] = (az + (vz - az) * myeps) * scale;
+      }
+    }
+  }
+
+  // KernelCPU::TTI: bgprtm/KernelCPU.cpp:93-119
+  void tti(const T *p0, T *pb, T *pr, int modelType) {
+    if(ny > 1) dy_vol(p0, pr, 0, 1, nullptr);
+#pragma omp parallel num_threads(nthreads)
+    {
+      std::vector<T> px(nxz), pz(nxz);
+#pragma omp for schedule(static)
+      for(int iy = 0; iy < ny; iy++) {
+        const T *ps = p0 + (size_t)iy * nxz;
+        dz_slice(ps, pz.data(), iy, 0, 1, nullptr);
+        dx_slice(ps, px.data(), iy, 0, 1, nullptr);
+        if(modelType == TTI || modelType == VTI) scalar(modelType, px.data(), ny > 1 ? pr + (size_t)iy * nxz : nullptr, pz.data(), iy);
+        dx_slice(px.data(), pb + (size_t)iy * nxz, iy, 1, -1, vel + (size_t)iy * nxz);
+        dz_slice(pz.data(), pb + (size_t)iy * nxz, iy, 1, -1, vel + (size_t)iy * nxz);
+      }
+    }
+    if(ny > 1) dy_vol(pr, pb, 1, -1, vel);
+  }
+
+  // KernelCPU::ISO (Laplacian branch) / TTI-alike: bgprtm/KernelCPU.cpp:125-174
+  void kernel(int modelType, const T *p0, T *pb, T *pr) {
+    if(modelType == ISO && engine == ENGINE_FD) {
+      laplacian(p0, pr);
+      applyVel_minus(pr, pb);
+    } else tti(p0, pb, pr, modelType);
+  }
+};
+
+// leapfrog: bgprtm/Propagator.cpp:1767-1788
+template<class T> void update2nd(T *w0, const T *w1, size_t n, int nthreads) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for(size_t i = 0; i < n; i++) w0[i] = (w1[i] + w1[i]) - w0[i];
+}
+
+// ---------------------------------------------------------------------------
+// Spread kernels: bgprtm/Util.cpp:317-397 (CompactOrmsbySpreadCoeff).  The
+// reference takes sincf_unnorm/blackman from unvendored libCommon; this build
+// defines them as sinc_unnorm(b) = sin(b)/b and the 3-term Blackman window
+// 0.42 - 0.5 cos(t) + 0.08 cos(2t) on t in [0, 2pi].
+static inline float sinc_unnorm(float b) { return std::fabs(b) < 1e-20f ? 1.0f : sinf(b) / b; }
+static inline float blackman(float t) { return 0.42f - 0.5f * cosf(t) + 0.08f * cosf(2.0f * t); }
+const float FLT_PI_ = 3.14159265358979f;
+
+int spread_coeff(int n, float *coeff0, float x, float k1, float k2, int *nlimit) {
+  if(n == 1 || *nlimit < 2) {
+    coeff0[0] = 1.0f;
+    *nlimit = 1;
+    return int(nearbyintf(x));
+  }
+  bool use_sinc = (k1 == k2);
+  float a1 = k1 * FLT_PI_, a2 = k2 * FLT_PI_;
+  if(use_sinc) a1 *= 2;
+  int ix = int(floorf(x));
+  int m = std::min(ix, n - 1 - ix);
+  m = std::min(m, *nlimit / 2);
+  *nlimit = 2 * m;
+  int ix0 = ix - m + 1;
+  float *coeff = coeff0 - ix0;
+  int ntaper = m / 2;
+  int i0 = ix - m + 1, i1 = ix + m;
+  float halfwin = std::max(i1 - x, x - i0);
+  for(int i = i0; i < i1 + 1; i++) {
+    float shift = i - x;
+    float b1 = shift * a1;
+    float s1 = sinc_unnorm(b1);
+    if(!use_sinc) {
+      float s2 = sinc_unnorm(shift * a2);
+      float val = (a2 * a2 * s2 * s2 - a1 * a1 * s1 * s1) / FLT_PI_ / (a2 - a1);
+      int i_edge = std::min(i - i0, i1 - i);
+      if(i_edge < ntaper && m > 1) val *= 0.5f * (1 - cosf(FLT_PI_ * i_edge / ntaper));
+      coeff[i] = val;
+    } else {
+      float val = s1 * 2 * k1;
+      if(m > 1) val *= blackman(FLT_PI_ * (halfwin - x + i) / halfwin);
+      coeff[i] = val;
+    }
+  }
+  return ix0;
+}
+// edge taper of xy kernels: bgprtm/Receiver.cpp:291-303
+void taper_kernel(float *kernel, int l, int off, const float *taper, int ntap, int n) {
+  if(n <= ntap * 2) return;
+  if(off < ntap) {
+    int nlim = std::min(ntap, l + off), i0 = std::max(0, off);
+    for(int i = i0; i < nlim; i++) kernel[i - off] *= taper[i];
+  }
+  if(off + l > n - ntap) {
+    int nlim = std::min(ntap, n - off), i0 = std::max(0, n - off - l);
+    for(int i = i0; i < nlim; i++) kernel[n - (off + i + 1)] *= taper[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// WaveFieldCompress COMPRESS_16: bgprtm/WaveFieldCompress.cpp:52-207
+struct Codec16 {
+  std::vector<float> table, wtaper, wztaper;
+  int ntaper = 20, nztaper = 10;
+  Codec16() {
+    table.assign(65536, 0.f);
+    table[0] = 0.f;
+    table[1] = 1.f;
+    uint32_t nanbits = 0x7fffffffu;
+    memcpy(&table[32768], &nanbits, 4);
+    float f = 1.f / ((float)(1 << 30));
+    for(int i = 2; i < 1024; i++) table[i] = f * i;
+    f = 1.f / ((float)(1 << 14));
+    for(int i = 1024; i < 16384; i++) table[i] = f * i;
+    float octave[1024];
+    f = 1.f / 1024.f;
+    for(int i = 0; i < 1024; i++) octave[i] = 1.f + i * f;
+    for(int j = 0; j < 16; j++) {
+      f = 1.f / ((float)(1 << (20 - j)));
+      for(int i = 0; i < 1024; i++) table[16384 + j * 1024 + i] = f * octave[i];
+    }
+    for(int i = 1; i < 32768; i++) table[i + 32768] = -table[i];
+    wtaper.resize(ntaper), wztaper.resize(nztaper);
+    for(int i = 0; i < ntaper; i++) wtaper[i] = 0.5f * (1.0f - cosf((float)M_PI * float(i) / float(ntaper)));
+    for(int i = 0; i < nztaper; i++) wztaper[i] = 0.5f * (1.0f - cosf((float)M_PI * float(i) / float(nztaper)));
+  }
+  // bgprtm/WaveFieldCompress.cpp:107-133
+  static uint16_t shift1(int32_t idat) {
+    uint16_t sgn = (uint16_t)(((uint32_t)idat & 0x80000000u) >> 16);
+    int exp = (int)(((uint32_t)idat & 0x7f800000u) >> 23) - 127;
+    int frc;
+    if(exp > 0) return 0x8000;
+    if(exp == 0) return sgn | 0x1;
+    if(exp > -5) {
+      frc = (idat & 0x7fffff) | 0x800000;
+      return (uint16_t)(sgn | (frc >> (9 - exp)));
+    }
+    if(exp > -21) {
+      exp += 20;
+      frc = (idat & 0x7fffff);
+      return (uint16_t)(sgn | 0x4000 | (exp << 10) | (frc >> 13));
+    }
+    if(exp > -30) {
+      frc = (idat & 0x7fffff) | 0x800000;
+      return (uint16_t)(sgn | (frc >> (-7 - exp)));
+    }
+    return 0;
+  }
+  void compress(const float *in, uint16_t *out, int nx, int ny, int nz) const {
+    size_t nxz = (size_t)nx * nz;
+    for(int iy = 0; iy < ny; iy++) {
+      const float *pin = in + (size_t)iy * nxz;
+      uint16_t *pout = out + (size_t)iy * (nxz + 2);
+      float vmax = fabsf(pin[0]);
+      for(size_t i = 1; i < nxz; i++) vmax = std::max(vmax, fabsf(pin[i]));
+      if(vmax > 1e-25f) {
+        float scale = 1.0f / vmax;
+        for(size_t i = 0; i < nxz; i++) {
+          float w = pin[i] * scale;
+          int32_t bits;
+          memcpy(&bits, &w, 4);
+          pout[2 + i] = shift1(bits);
+        }
+        memcpy(pout, &vmax, 4);
+      } else memset(pout, 0, (nxz + 2) * 2);
+    }
+  }
+  void uncompress(const uint16_t *in, float *out, int nx, int ny, int nz) const {
+    size_t nxz = (size_t)nx * nz;
+    for(int iy = 0; iy < ny; iy++) {
+      const uint16_t *pin = in + (size_t)iy * (nxz + 2);
+      float *pout = out + (size_t)iy * nxz;
+      float scale;
+      memcpy(&scale, pin, 4);
+      if(scale > 1e-25f) {
+        for(int ix = 0; ix < nx; ix++)
+          for(int iz = 0; iz < nz; iz++) {
+            size_t id = (size_t)ix * nz + iz;
+            pout[id] = scale * table[pin[id + 2]];
+            if(ny > ntaper * 2) {
+              if(iy < ntaper) pout[id] *= wtaper[iy];
+              if(iy > ny - ntaper - 1) pout[id] *= wtaper[ny - iy - 1];
+            }
+            if(nx > ntaper * 2) {
+              if(ix < ntaper) pout[id] *= wtaper[ix];
+              if(ix > nx - ntaper - 1) pout[id] *= wtaper[nx - ix - 1];
+            }
+            if(nz > nztaper * 2) {
+              if(iz < nztaper) pout[id] *= wztaper[iz];
+              if(iz > nz - nztaper - 1) pout[id] *= wztaper[nz - iz - 1];
+            }
+          }
+      } else memset(pout, 0, nxz * 4);
+    }
+  }
+};
+
+}  // namespace orc
+
+// ===========================================================================
+// C ABI for the Python test harness (ctypes).  Suffix _f / _d selects FP32 /
+// FP64 instantiations.
+using namespace orc;
+
+extern "C" {
+
+void orc_taylor_coef2(int M, double *c) { taylor_coef2(M, c); }
+void orc_taylor_coef1(int M, double *c) { taylor_coef1(M, c); }
+
+void orc_pml_profile(int npml, float dx, float dt, float vmax, float pml_eps, float *coef, float *feoc) {
+  Pml<float> p(npml, dx, dt, vmax, pml_eps);
+  memcpy(coef, p.coef.data(), npml * 4);
+  memcpy(feoc, p.feoc.data(), npml * 4);
+}
+void orc_pml_round(float g, float *p, float *q, int n) {
+  for(int i = 0; i < n; i++) pml_round(g, p[i], q[i]);
+}
+
+// CFD plan export: bm[2][n*5], bl[2][n*4], am[2][60] (row-major per line index)
+void orc_cfd_plan_f(int n, float d, float *bm, float *bl, float *am) {
+  CfdPlan<float> p;
+  p.init(n, d);
+  for(int s = 0; s < 2; s++) {
+    memcpy(bm + (size_t)s * 5 * n, p.bm[s].data(), 5 * n * 4);
+    memcpy(bl + (size_t)s * 4 * n, p.bl[s].data(), 4 * n * 4);
+    memcpy(am + s * 60, p.am[s].data(), 60 * 4);
+  }
+}
+void orc_cfd_derive_f(int n, float d, int sgn, const float *x, float *b, int nlines) {
+  CfdPlan<float> p;
+  p.init(n, d);
+  for(int l = 0; l < nlines; l++) p.derive(sgn, x + (size_t)l * n, b + (size_t)l * n, 1);
+}
+void orc_cfd_derive_d(int n, double d, int sgn, const double *x, double *b, int nlines) {
+  CfdPlan<double> p;
+  p.init(n, d);
+  for(int l = 0; l < nlines; l++) p.derive(sgn, x + (size_t)l * n, b + (size_t)l * n, 1);
+}
+
+#define ENGINE_API(T, S)                                                                                              \
+  void *orc_engine_create_##S(int nx, int ny, int nz, T dx, T dy, T dz, int engine, int order) {                     \
+    return new Engine<T>(nx, ny, nz, dx, dy, dz, engine, order);                                                      \
+  }                                                                                                                   \
+  void orc_engine_destroy_##S(void *h) { delete(Engine<T> *)h; }                                                      \
+  void orc_engine_set_threads_##S(void *h, int n) { ((Engine<T> *)h)->nthreads = n > 0 ? n : 1; }                     \
+  void orc_engine_set_model_##S(void *h, const T *vel, const T *eps, const T *del, const T *pjx, const T *pjy) {      \
+    Engine<T> *e = (Engine<T> *)h;                                                                                    \
+    e->vel = vel, e->eps = eps, e->del = del, e->pjx = pjx, e->pjy = pjy;                                             \
+  }                                                                                                                   \
+  void orc_engine_set_boundary_##S(void *h, const int *nb, T dt, T vmax, T pml_eps, int pml3d, T scaler, int top,     \
+                                   int dimPML) {                                                                      \
+    Engine<T> *e = (Engine<T> *)h;                                                                                    \
+    e->pml3d = pml3d != 0, e->scaler = scaler, e->topPML = top != 0;                                                  \
+    e->setBoundary(nb, dt, vmax, pml_eps, dimPML);                                                                    \
+  }                                                                                                                   \
+  void orc_engine_clean_pml_##S(void *h) { ((Engine<T> *)h)->cleanPML(); }                                            \
+  void orc_engine_kernel_##S(void *h, int model, const T *p0, T *pb, T *pr) {                                         \
+    ((Engine<T> *)h)->kernel(model, p0, pb, pr);                                                                      \
+  }                                                                                                                   \
+  void orc_engine_laplacian_##S(void *h, const T *in, T *out) { ((Engine<T> *)h)->laplacian(in, out); }             \
+  void orc_update2nd_##S(T *w0, const T *w1, size_t n) { update2nd(w0, w1, n, 1); }                                   \
+  void orc_engine_step_##S(void *h, int model, const T *w1, T *w0, T *wr) {                                           \
+    Engine<T> *e = (Engine<T> *)h;                                                                                    \
+    e->kernel(model, w1, w0, wr);                                                                                     \
+    update2nd(w0, w1, e->nxyz, e->nthreads);                                                                          \
+  }
+
+ENGINE_API(float, f)
+ENGINE_API(double, d)
+
+int orc_spread_coeff(int n, float *coeff, float x, float k1, float k2, int *nlimit) {
+  return spread_coeff(n, coeff, x, k1, k2, nlimit);
+}
+void orc_taper_kernel(float *kernel, int l, int off, const float *taper, int ntap, int n) {
+  taper_kernel(kernel, l, off, taper, ntap, n);
+}
+
+// Receiver injection (omp-buffer form, bgprtm/Receiver.cpp:428-475) and
+// extraction (bgprtm/Receiver.cpp:394-426).  Receivers are given as flat
+// tables: kernels kx[nr*Lx], ky[nr*Ly], kz[nr*Lz] with lengths lx/ly/lz and
+// offsets offx/offy/offz; values val[nr] already scaled by dt_dxyz.
+void orc_inject(float *w0, const float *vel, int nx, int ny, int nz, int nr, const float *val, const float *kx, const float *ky,
+                const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz, const int *offx,
+                const int *offy, const int *offz) {
+  size_t nxz = (size_t)nx * nz;
+  std::vector<float> b((size_t)nx * ny * nz, 0.f);
+  std::vector<char> touched((size_t)nx * ny * nz, 0);
+  for(int ir = 0; ir < nr; ir++) {
+    for(int iy = 0; iy < ly[ir]; iy++) {
+      float cy = val[ir] * ky[(size_t)ir * Ly + iy];
+      for(int ix = 0; ix < lx[ir]; ix++) {
+        float cx = cy * kx[(size_t)ir * Lx + ix];
+        size_t off = ((size_t)(iy + offy[ir]) * nx + ix + offx[ir]) * nz + offz[ir];
+        for(int iz = 0; iz < lz[ir]; iz++) b[off + iz] += cx * kz[(size_t)ir * Lz + iz], touched[off + iz] = 1;
+      }
+    }
+  }
+  for(size_t i = 0; i < (size_t)ny * nxz; i++)
+    if(touched[i]) w0[i] += b[i] * vel[i];
+}
+void orc_extract(const float *w, int nx, int ny, int nz, int nr, float *out, const float *kx, const float *ky, const float *kz,
+                 int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz, const int *offx, const int *offy,
+                 const int *offz) {
+  for(int ir = 0; ir < nr; ir++) {
+    float val = 0;
+    for(int iy = 0; iy < ly[ir]; iy++) {
+      float cy = ky[(size_t)ir * Ly + iy];
+      for(int ix = 0; ix < lx[ir]; ix++) {
+        float cx = cy * kx[(size_t)ir * Lx + ix];
+        size_t off = ((size_t)(iy + offy[ir]) * nx + ix + offx[ir]) * nz + offz[ir];
+        for(int iz = 0; iz < lz[ir]; iz++) val += cx * kz[(size_t)ir * Lz + iz] * w[off + iz];
+      }
+    }
+    out[ir] = val;
+  }
+}
+
+void orc_compress16(const float *in, uint16_t *out, int nx, int ny, int nz) {
+  static Codec16 c;
+  c.compress(in, out, nx, ny, nz);
+}
+void orc_uncompress16(const uint16_t *in, float *out, int nx, int ny, int nz) {
+  static Codec16 c;
+  c.uncompress(in, out, nx, ny, nz);
+}
+void orc_codec16_table(float *t) {
+  static Codec16 c;
+  memcpy(t, c.table.data(), 65536 * 4);
+}
+
+// zero-lag correlation: bgprtm/ImagingCorrelator.cpp:180-194 and illumination
+// bgprtm/ImagingCondition.cpp:143-149
+void orc_crossone(float *image, const float *src, const float *rec, size_t n) {
+  for(size_t i = 0; i < n; i++) image[i] += src[i] * rec[i];
+}
+void orc_illum(float *amp, const float *src, size_t n) {
+  for(size_t i = 0; i < n; i++) amp[i] += src[i] * src[i];
+}
+// time / water-depth taper on the source snapshot: bgprtm/ImagingCondition.cpp:200-226
+void orc_apply_tape(float *wf, int nx, int ny, int nz, int it, const float *wtaperT, int ntaperT, const float *wtaperZ,
+                    int ntaperZ) {
+  size_t nxy = (size_t)nx * ny;
+  if(it < ntaperT) {
+    for(size_t i = 0; i < nxy * nz; i++) wf[i] *= wtaperT[it];
+  } else {
+    for(size_t ixy = 0; ixy < nxy; ixy++)
+      for(int iz = 0; iz < ntaperZ; iz++) wf[ixy * nz + iz] *= wtaperZ[iz];
+  }
+}
+
+}  // extern "C"

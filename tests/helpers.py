@@ -1,0 +1,34 @@
+"""Shared input generators and comparison helpers for the tests."""
+import numpy as np
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def smooth_field(shape, lo, hi, seed, passes=3):
+    """Seeded smooth random field in [lo, hi] (box-filtered noise)."""
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal(shape)
+    for _ in range(passes):
+        for ax in range(3):
+            f = (np.roll(f, 1, ax) + f + np.roll(f, -1, ax)) / 3.0
+    f = (f - f.min()) / (f.max() - f.min() + 1e-30)
+    return (lo + (hi - lo) * f).astype(np.float32)
+
+
+def gaussian_blob(ny, nx, nz, cy, cx, cz, sigma):
+    y, x, z = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
+    r2 = (y - cy) ** 2 + (x - cx) ** 2 + (z - cz) ** 2
+    return np.exp(-r2 / (2.0 * sigma * sigma)).astype(np.float32)
+
+
+def tilt_axis(shape, seed):
+    """pjx, pjy from smooth tilt (0..45 deg) and azimuth (0..360 deg), bgprtm/ModelLoader.cpp:303-321."""
+    th = smooth_field(shape, 0.0, 45.0, seed) * np.float32(np.pi / 180)
+    ph = smooth_field(shape, 0.0, 360.0, seed + 1) * np.float32(np.pi / 180)
+    st = np.sin(th)
+    return (-st * np.cos(ph)).astype(np.float32), (-st * np.sin(ph)).astype(np.float32)

@@ -1,0 +1,113 @@
+"""GPU parity of the per-step engines (KernelCPU + FdEngine + PML) against the
+FP32 oracle on the same seeded inputs.
+
+FD kernels contract multiply-adds (FMA), so they are compared at 1e-4
+relative L2 after several steps; the compact-FD kernels keep the reference's
+operation order without contraction and must match bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from konglong_amd._lib import DeviceArray, ENGINE_FD, ENGINE_CFD, ISO, VTI, TTI
+from konglong_amd.kernel import KernelCPU
+from tests.helpers import rel_l2, smooth_field, gaussian_blob, tilt_axis
+
+pytestmark = pytest.mark.gpu
+
+NY, NX, NZ = 34, 41, 47
+D, DT = 10.0, 1.0e-3
+NB = (6, 7, 5, 8, 6, 9)  # x1 x2 y1 y2 z1 z2
+
+
+def _inputs(model, seed=7):
+    shape = (NY, NX, NZ)
+    v = smooth_field(shape, 1500.0, 2500.0, seed)
+    m = (v * v * np.float32(DT * DT)).astype(np.float32)
+    eps = smooth_field(shape, 0.0, 0.3, seed + 10) if model != ISO else None
+    dl = smooth_field(shape, 0.0, 0.15, seed + 20) if model != ISO else None
+    pjx = pjy = None
+    if model == TTI:
+        pjx, pjy = tilt_axis(shape, seed + 30)
+    w1 = gaussian_blob(NY, NX, NZ, NY / 2, NX / 3, NZ / 2, 3.0)
+    rng = np.random.default_rng(seed + 40)
+    w1 += np.float32(1e-3) * rng.standard_normal(shape).astype(np.float32)
+    w0 = (0.98 * w1).astype(np.float32)
+    return m, eps, dl, pjx, pjy, w1.astype(np.float32), w0
+
+
+CASES = [
+    (ENGINE_FD, 8, ISO), (ENGINE_FD, 16, ISO), (ENGINE_FD, 12, ISO),
+    (ENGINE_FD, 8, VTI), (ENGINE_FD, 16, VTI), (ENGINE_FD, 16, TTI), (ENGINE_FD, 8, TTI),
+    (ENGINE_CFD, 10, ISO), (ENGINE_CFD, 10, VTI),
+]
+
+
+def _ids(c):
+    return f"{'FD' if c[0] == ENGINE_FD else 'CFD'}{c[1]}-{ {1: 'ISO', 2: 'VTI', 3: 'TTI'}[c[2]]}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
+def test_step_parity(klrtm, case):
+    engine, order, model = case
+    print(f"[engine-parity] {_ids(case)} grid={NY}x{NX}x{NZ}", flush=True)
+    m, eps, dl, pjx, pjy, w1, w0 = _inputs(model)
+    vmax = 2500.0
+    # oracle
+    o = oracle.Engine(NX, NY, NZ, D, D, D, engine, order, np.float32)
+    o.set_model(m, eps, dl, pjx, pjy)
+    o.set_boundary(NB, DT, vmax, dim_pml=3)
+    ow1, ow0, owr = w1.copy(), w0.copy(), np.zeros_like(w1)
+    # gpu
+    k = KernelCPU(NX, NY, NZ, D, D, D, engine, order)
+    dm = DeviceArray.from_host(m)
+    de = DeviceArray.from_host(eps) if eps is not None else None
+    dd = DeviceArray.from_host(dl) if dl is not None else None
+    dpx = DeviceArray.from_host(pjx) if pjx is not None else None
+    dpy = DeviceArray.from_host(pjy) if pjy is not None else None
+    k.setModel(dm, None, dd, de, dpx, dpy)
+    k.setBoundary(*NB, DT, vmax)
+    g1, g0, gr = DeviceArray.from_host(w1), DeviceArray.from_host(w0), DeviceArray((NY, NX, NZ))
+    nsteps = 12
+    for it in range(nsteps):
+        o.step(model, ow1, ow0, owr)
+        ow0, ow1 = ow1, ow0
+        k.step(model, g1, g0, gr)
+        g0, g1 = g1, g0
+    got = g1.download()
+    err = rel_l2(got, ow1)
+    print(f"  rel_l2 after {nsteps} steps = {err:.3e}", flush=True)
+    assert np.isfinite(got).all()
+    if engine == ENGINE_CFD:
+        assert np.array_equal(got.view(np.uint32), ow1.view(np.uint32)), f"CFD not bit-exact (rel_l2={err:.3e})"
+    else:
+        assert err <= 1e-4
+
+
+@pytest.mark.parametrize("case", [(ENGINE_FD, 8, ISO), (ENGINE_FD, 16, VTI), (ENGINE_CFD, 10, VTI)],
+                         ids=["FD8-ISO", "FD16-VTI", "CFD10-VTI"])
+def test_kernel_api_parity(klrtm, case):
+    """KernelCPU::ISO/VTI(p0, pb, pr) without the fused update: pb and pr outputs."""
+    engine, order, model = case
+    print(f"[kernel-api] {_ids(case)}", flush=True)
+    m, eps, dl, pjx, pjy, w1, w0 = _inputs(model, seed=11)
+    o = oracle.Engine(NX, NY, NZ, D, D, D, engine, order, np.float32)
+    o.set_model(m, eps, dl, pjx, pjy)
+    o.set_boundary(NB, DT, 2500.0, dim_pml=3)
+    opb, opr = w0.copy(), np.zeros_like(w1)
+    o.kernel(model, w1, opb, opr)
+    k = KernelCPU(NX, NY, NZ, D, D, D, engine, order)
+    dm = DeviceArray.from_host(m)
+    de = DeviceArray.from_host(eps) if eps is not None else None
+    dd = DeviceArray.from_host(dl) if dl is not None else None
+    k.setModel(dm, None, dd, de, None, None)
+    k.setBoundary(*NB, DT, 2500.0)
+    g1, gb, gr = DeviceArray.from_host(w1), DeviceArray.from_host(w0), DeviceArray((NY, NX, NZ))
+    (k.ISO if model == ISO else k.VTI)(g1, gb, gr)
+    pb, pr = gb.download(), gr.download()
+    if engine == ENGINE_CFD:
+        assert np.array_equal(pb.view(np.uint32), opb.view(np.uint32))
+        assert np.array_equal(pr.view(np.uint32), opr.view(np.uint32))
+    else:
+        assert rel_l2(pb, opb) <= 1e-5
+        assert rel_l2(pr, opr) <= 1e-5
