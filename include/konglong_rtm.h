@@ -175,6 +175,8 @@ int klrtm_propagator_destroy(klrtm_propagator *p);
 /* host model volumes: velocity (m/s) and optional eps, del, pjx, pjy (NULL allowed) */
 int klrtm_propagator_set_model(klrtm_propagator *p, const float *vel, const float *eps, const float *del,
                                const float *pjx, const float *pjy);
+/* source wavelet samples (host, nt floats); replaces the built-in Ricker (Source::setupWavelet) */
+int klrtm_propagator_set_source_wavelet(klrtm_propagator *p, const float *wavelet_host);
 /* receivers: rcv_xyz[3*nrcv] grid coordinates */
 int klrtm_propagator_set_receivers(klrtm_propagator *p, const float *rcv_xyz);
 /* forward modelling with extraction every step: record_host [nrcv][nt] (may be NULL) */

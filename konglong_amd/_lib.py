@@ -91,6 +91,7 @@ _SIGS = {
     "klrtm_propagator_destroy": (_i, [_vp]),
     "klrtm_propagator_set_model": (_i, [_vp, _pf, _pf, _pf, _pf, _pf]),
     "klrtm_propagator_set_receivers": (_i, [_vp, _pf]),
+    "klrtm_propagator_set_source_wavelet": (_i, [_vp, _pf]),
     "klrtm_propagator_model": (_i, [_vp, _pf]),
     "klrtm_propagator_rtm": (_i, [_vp, _pf, _pf, _pf, _pd, _pd]),
     "klrtm_propagator_timings": (_i, [_vp, _pd, _pd, _pd]),

@@ -1,0 +1,386 @@
+// propagator.cpp -- one-shot RTM driver on the device (Propagator::rtm / mod).
+//
+// Follows bgprtm/Propagator.cpp:1157-1260 (migration time loop),
+// 1054-1155 (modelling with receiver extraction), 2157-2280 (rtm: forward
+// with snapshot cache, backward, imaging condition) and
+// bgprtm/ImagingCondition.cpp:180-226 (tapers).  Every per-step operation
+// stays on the device: the leapfrog step (fused with the kernel), injection,
+// extraction, snapshot encode, and -- unlike the reference, which caches the
+// receiver wavefield too and correlates afterwards -- the imaging condition
+// runs inside the backward loop on the live receiver wavefield.
+#include "klrtm_internal.h"
+#include "engine.h"
+#include "receiver.h"
+#include "codec.h"
+#include "ic.h"
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+
+namespace klrtm {
+
+struct Propagator {
+  klrtm_shot_config c{};
+  Engine *eng = nullptr;
+  size_t n = 0;
+  float *d_vel = nullptr, *d_eps = nullptr, *d_del = nullptr, *d_pjx = nullptr, *d_pjy = nullptr;
+  float *w0 = nullptr, *w1 = nullptr, *wr = nullptr;
+  Receiver *src = nullptr, *rcv = nullptr;
+  int ntsav = 0;
+  size_t snap_stride_bytes = 0;  // per snapshot payload
+  size_t nscales = 0;
+  char *d_snap = nullptr;
+  float *d_scales = nullptr;
+  unsigned *d_smax = nullptr;
+  float *d_img = nullptr, *d_ill = nullptr;
+  double *d_img64 = nullptr, *d_ill64 = nullptr;
+  float *d_taper_z = nullptr;
+  std::vector<float> wtaperT;
+  double t_fwd = 0, t_bwd = 0, t_ic = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  ~Propagator();
+  int alloc_snapshots();
+  int reset();
+  int step(int it);
+  int store(int itsav);
+  int image(int itsav);
+};
+
+Propagator::~Propagator() {
+  delete eng;
+  if(src) receiver_destroy(src);
+  if(rcv) receiver_destroy(rcv);
+  for(void *p : {(void *)d_vel, (void *)d_eps, (void *)d_del, (void *)d_pjx, (void *)d_pjy, (void *)w0, (void *)w1,
+                 (void *)wr, (void *)d_snap, (void *)d_scales, (void *)d_smax, (void *)d_img, (void *)d_ill,
+                 (void *)d_img64, (void *)d_ill64, (void *)d_taper_z})
+    (void)hipFree(p);
+  for(auto &e : ev)
+    if(e) (void)hipEventDestroy(e);
+}
+
+static int dmalloc(float **p, size_t n) {
+  hipError_t e = hipMalloc(p, n * sizeof(float));
+  if(e != hipSuccess) {
+    set_error("device allocation of %zu floats failed: %s", n, hipGetErrorString(e));
+    return KLRTM_ENOMEM;
+  }
+  return KLRTM_OK;
+}
+
+int Propagator::alloc_snapshots() {
+  if(d_snap) return KLRTM_OK;
+  const int k = std::max(1, c.snap_every);
+  ntsav = (c.nt + k - 1) / k;
+  switch(c.snap_format) {
+  case KLRTM_SNAP_FP32: snap_stride_bytes = n * sizeof(float); break;
+  case KLRTM_SNAP_C16: snap_stride_bytes = klrtm_nshort_volume(c.nz, c.nx, c.ny) * sizeof(uint16_t); break;
+  case KLRTM_SNAP_FP16:
+  case KLRTM_SNAP_BF16:
+    snap_stride_bytes = n * sizeof(uint16_t);
+    nscales = blockq_nscales(c.nx, c.ny, c.nz);
+    break;
+  default: set_error("unknown snapshot format %d", c.snap_format); return KLRTM_EINVAL;
+  }
+  snap_stride_bytes = (snap_stride_bytes + 255) & ~(size_t)255;
+  hipError_t e = hipMalloc(&d_snap, snap_stride_bytes * ntsav);
+  if(e != hipSuccess) {
+    set_error("snapshot store of %d x %zu bytes does not fit: %s", ntsav, snap_stride_bytes, hipGetErrorString(e));
+    d_snap = nullptr;
+    return KLRTM_ENOMEM;
+  }
+  if(nscales) KL_HIP(hipMalloc(&d_scales, sizeof(float) * nscales * ntsav));
+  return KLRTM_OK;
+}
+
+int Propagator::reset() {
+  KL_HIP(hipMemsetAsync(w0, 0, n * sizeof(float), stream()));
+  KL_HIP(hipMemsetAsync(w1, 0, n * sizeof(float), stream()));
+  return eng->clean();  // KernelCPU::cleanPMLMemory (bgprtm/Propagator.cpp:1165-1166)
+}
+
+// Propagator::kernel() + update2nd (bgprtm/Propagator.cpp:1217-1223)
+int Propagator::step(int) {
+  if(c.fuse_step) return eng->run(c.model, MODE_STEP, w1, w0, wr);
+  int rc = eng->run(c.model, MODE_KERNEL, w1, w0, wr);
+  if(rc) return rc;
+  return launch_update2nd(w0, w1, n);
+}
+
+int Propagator::store(int itsav) {
+  char *dst = d_snap + snap_stride_bytes * (size_t)itsav;
+  switch(c.snap_format) {
+  case KLRTM_SNAP_FP32:
+    KL_HIP(hipMemcpyAsync(dst, w0, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    return KLRTM_OK;
+  case KLRTM_SNAP_C16: return c16_encode(w0, (uint16_t *)dst, c.nx, c.ny, c.nz, d_smax);
+  default: return blockq_encode(w0, (uint16_t *)dst, d_scales + nscales * (size_t)itsav, c.nx, c.ny, c.nz, c.snap_format);
+  }
+}
+
+int Propagator::image(int itsav) {
+  IcArgs a;
+  a.nx = c.nx, a.ny = c.ny, a.nz = c.nz;
+  a.T = make_edge_taper();
+  int ntaperT = (int)wtaperT.size();
+  a.taper_t = itsav < ntaperT ? wtaperT[itsav] : -1.f;
+  a.taper_z = d_taper_z;
+  a.ntaper_z = d_taper_z ? c.ntaper_z : 0;
+  const bool rq = c.rec_through_codec && c.snap_format == KLRTM_SNAP_C16;
+  if(rq) {
+    int rc = c16_slice_max(w0, c.nx, c.ny, c.nz, d_smax);
+    if(rc) return rc;
+  }
+  const char *snap = d_snap + snap_stride_bytes * (size_t)itsav;
+  const float *sc = nscales ? d_scales + nscales * (size_t)itsav : nullptr;
+  return launch_ic(c.snap_format, rq, a, snap, sc, w0, d_smax, d_img, d_ill, d_img64, d_ill64);
+}
+
+static float ricker(float t, float fp) {
+  float tau = t - 1.5f / fp;
+  float a = (float)(M_PI * M_PI) * fp * fp * tau * tau;
+  return (1.f - 2.f * a) * expf(-a);
+}
+
+int propagator_create(Propagator **out, const klrtm_shot_config *cfg) {
+  const klrtm_shot_config &c = *cfg;
+  KL_REQUIRE(c.nx > 2 && c.ny >= 1 && c.nz > 2, KLRTM_EINVAL, "bad grid");
+  KL_REQUIRE(c.nt > 0 && c.dt > 0, KLRTM_EINVAL, "need nt > 0 and dt > 0");
+  KL_REQUIRE(c.model >= KLRTM_ISO && c.model <= KLRTM_TTI, KLRTM_EINVAL, "bad model type");
+  KL_REQUIRE(c.snap_every >= 1, KLRTM_EINVAL, "snap_every must be >= 1");
+  Propagator *P = new Propagator();
+  P->c = c;
+  P->n = (size_t)c.nx * c.ny * c.nz;
+  klrtm_kernel *kh = nullptr;
+  int rc = klrtm_kernel_create(&kh, c.nx, c.ny, c.nz, c.dx, c.dy, c.dz, c.engine, c.order);
+  if(rc) {
+    delete P;
+    return rc;
+  }
+  P->eng = reinterpret_cast<Engine *>(kh);
+  if((rc = klrtm_kernel_set_pml_options(kh, c.pml_eps > 0 ? c.pml_eps : 1e-3f, c.pml3d, c.pml3d_scaler, c.top_pml)) ||
+     (rc = klrtm_kernel_set_boundary(kh, c.nbnd[0], c.nbnd[1], c.nbnd[2], c.nbnd[3], c.nbnd[4], c.nbnd[5], c.dt, c.vmax))) {
+    delete P;
+    return rc;
+  }
+  if((rc = dmalloc(&P->d_vel, P->n)) || (rc = dmalloc(&P->w0, P->n)) || (rc = dmalloc(&P->w1, P->n)) ||
+     (rc = dmalloc(&P->wr, P->n)) || (rc = dmalloc(&P->d_img, P->n)) || (rc = dmalloc(&P->d_ill, P->n))) {
+    delete P;
+    return rc;
+  }
+  hipError_t e = hipMalloc(&P->d_smax, sizeof(unsigned) * c.ny);
+  if(e == hipSuccess) e = hipMalloc(&P->d_img64, sizeof(double) * P->n);
+  if(e == hipSuccess) e = hipMalloc(&P->d_ill64, sizeof(double) * P->n);
+  if(e != hipSuccess) {
+    set_error("allocation failed: %s", hipGetErrorString(e));
+    delete P;
+    return KLRTM_ENOMEM;
+  }
+  // source: one trace, Ricker wavelet (fpeak), spread kernel like the receivers
+  float dxyz = c.dx * c.dy * c.dz;
+  rc = receiver_create(&P->src, c.nx, c.ny, c.nz, 1, c.src_xyz, c.src_spread, c.khpass, c.khcut, c.nt, 1.0f / dxyz);
+  if(rc) {
+    delete P;
+    return rc;
+  }
+  std::vector<float> wav(c.nt);
+  for(int it = 0; it < c.nt; it++) wav[it] = ricker(it * c.dt, c.f_peak);
+  if((rc = receiver_set_traces(*P->src, wav.data()))) {
+    delete P;
+    return rc;
+  }
+  // IC tapers (bgprtm/ImagingCondition.cpp:180-198, bgprtm/Propagator.cpp:1952-1974)
+  int k = std::max(1, c.snap_every);
+  float tmax = c.nt * c.dt, dtsav = k * c.dt;
+  int ntaperT = (int)nearbyintf(c.tmute_frac * tmax / dtsav);
+  for(int i = 0; i < ntaperT; i++) P->wtaperT.push_back(0.5f * (1.0f - cosf((float)M_PI * float(i) / float(ntaperT))));
+  if(c.ntaper_z > 0) {
+    std::vector<float> tz(c.ntaper_z);
+    for(int i = 0; i < c.ntaper_z; i++) tz[i] = 0.5f * (1.0f - cosf((float)M_PI * float(i) / float(c.ntaper_z)));
+    if((rc = dmalloc(&P->d_taper_z, c.ntaper_z))) {
+      delete P;
+      return rc;
+    }
+    (void)hipMemcpy(P->d_taper_z, tz.data(), sizeof(float) * c.ntaper_z, hipMemcpyHostToDevice);
+  }
+  for(auto &ev : P->ev) (void)hipEventCreate(&ev);
+  *out = P;
+  return KLRTM_OK;
+}
+
+static int upload_opt(float **d, const float *h, size_t n) {
+  if(!h) return KLRTM_OK;
+  if(!*d) {
+    int rc = dmalloc(d, n);
+    if(rc) return rc;
+  }
+  KL_HIP(hipMemcpy(*d, h, n * sizeof(float), hipMemcpyHostToDevice));
+  return KLRTM_OK;
+}
+
+int propagator_set_model(Propagator &P, const float *vel, const float *eps, const float *del, const float *pjx,
+                         const float *pjy) {
+  KL_REQUIRE(vel, KLRTM_EINVAL, "velocity is required");
+  int rc;
+  if((rc = upload_opt(&P.d_vel, vel, P.n)) || (rc = upload_opt(&P.d_eps, eps, P.n)) || (rc = upload_opt(&P.d_del, del, P.n)) ||
+     (rc = upload_opt(&P.d_pjx, pjx, P.n)) || (rc = upload_opt(&P.d_pjy, pjy, P.n)))
+    return rc;
+  if((rc = launch_vel2(P.d_vel, P.n, P.c.dt))) return rc;  // Propagator::Vel2
+  return klrtm_kernel_set_model(reinterpret_cast<klrtm_kernel *>(P.eng), P.d_vel, nullptr, P.d_del, P.d_eps, P.d_pjx,
+                                P.d_pjy);
+}
+
+int propagator_set_receivers(Propagator &P, const float *xyz) {
+  if(P.rcv) receiver_destroy(P.rcv), P.rcv = nullptr;
+  const klrtm_shot_config &c = P.c;
+  return receiver_create(&P.rcv, c.nx, c.ny, c.nz, c.nrcv, xyz, c.rcv_spread, c.khpass, c.khcut, c.nt,
+                         1.0f / (c.dx * c.dy * c.dz));
+}
+
+// forward modelling with extraction every step (bgprtm/Propagator.cpp:1101-1153)
+int propagator_model(Propagator &P, float *record) {
+  KL_REQUIRE(P.rcv, KLRTM_ESTATE, "receivers not set");
+  int rc;
+  if((rc = P.reset()) || (rc = receiver_clear_record(*P.rcv))) return rc;
+  KL_HIP(hipEventRecord(P.ev[0], stream()));
+  for(int it = 0; it < P.c.nt; it++) {
+    if((rc = P.step(it))) return rc;
+    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f))) return rc;  // it_shift_force = -1
+    if((rc = receiver_extract(*P.rcv, P.w0, it))) return rc;
+    std::swap(P.w0, P.w1);
+  }
+  KL_HIP(hipEventRecord(P.ev[1], stream()));
+  KL_HIP(hipEventSynchronize(P.ev[1]));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, P.ev[0], P.ev[1]);
+  P.t_fwd = ms, P.t_bwd = 0, P.t_ic = 0;
+  if(record) return receiver_get_record(*P.rcv, record);
+  return KLRTM_OK;
+}
+
+// forward with snapshots + backward with imaging (bgprtm/Propagator.cpp:1210-1255, 2226-2275)
+int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, double *img64, double *ill64) {
+  KL_REQUIRE(P.rcv, KLRTM_ESTATE, "receivers not set");
+  int rc;
+  if((rc = P.alloc_snapshots())) return rc;
+  const int k = std::max(1, P.c.snap_every), nt = P.c.nt;
+  // forward: source wavefield, stored every k steps
+  if((rc = P.reset())) return rc;
+  KL_HIP(hipEventRecord(P.ev[0], stream()));
+  for(int it = 0; it < nt; it++) {
+    if((rc = P.step(it))) return rc;
+    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f))) return rc;
+    if(it % k == 0 && it / k < P.ntsav && (rc = P.store(it / k))) return rc;
+    std::swap(P.w0, P.w1);
+  }
+  KL_HIP(hipEventRecord(P.ev[1], stream()));
+  // backward: receiver wavefield from the record, imaging at the stored steps
+  if(record && (rc = receiver_set_traces(*P.rcv, record))) return rc;
+  if(!record && (rc = receiver_set_traces_device(*P.rcv, receiver_record_device(*P.rcv)))) return rc;
+  KL_HIP(hipMemsetAsync(P.d_img, 0, P.n * sizeof(float), stream()));
+  KL_HIP(hipMemsetAsync(P.d_ill, 0, P.n * sizeof(float), stream()));
+  KL_HIP(hipMemsetAsync(P.d_img64, 0, P.n * sizeof(double), stream()));
+  KL_HIP(hipMemsetAsync(P.d_ill64, 0, P.n * sizeof(double), stream()));
+  if((rc = P.reset())) return rc;
+  KL_HIP(hipEventRecord(P.ev[2], stream()));
+  for(int it = nt - 1; it >= 0; it--) {
+    if((rc = P.step(it))) return rc;
+    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f))) return rc;
+    if(it % k == 0 && it / k < P.ntsav && (rc = P.image(it / k))) return rc;
+    std::swap(P.w0, P.w1);
+  }
+  KL_HIP(hipEventRecord(P.ev[3], stream()));
+  KL_HIP(hipEventSynchronize(P.ev[3]));
+  float f = 0, b = 0;
+  (void)hipEventElapsedTime(&f, P.ev[0], P.ev[1]);
+  (void)hipEventElapsedTime(&b, P.ev[2], P.ev[3]);
+  P.t_fwd = f, P.t_bwd = b;
+  double icms = 0;
+  long cnt = 0;
+  klrtm_timer_read("ic_correlate", &icms, &cnt);
+  P.t_ic = icms;
+  if(img) KL_HIP(hipMemcpy(img, P.d_img, P.n * sizeof(float), hipMemcpyDeviceToHost));
+  if(ill) KL_HIP(hipMemcpy(ill, P.d_ill, P.n * sizeof(float), hipMemcpyDeviceToHost));
+  if(img64) KL_HIP(hipMemcpy(img64, P.d_img64, P.n * sizeof(double), hipMemcpyDeviceToHost));
+  if(ill64) KL_HIP(hipMemcpy(ill64, P.d_ill64, P.n * sizeof(double), hipMemcpyDeviceToHost));
+  return KLRTM_OK;
+}
+
+int propagator_forward_steps(Propagator &P, int it0, int nsteps, int store_snapshots) {
+  int rc;
+  if(store_snapshots && (rc = P.alloc_snapshots())) return rc;
+  const int k = std::max(1, P.c.snap_every);
+  for(int it = it0; it < it0 + nsteps; it++) {
+    if((rc = P.step(it))) return rc;
+    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f))) return rc;
+    if(store_snapshots && it % k == 0) {
+      int s = (it / k) % std::max(1, P.ntsav);
+      if((rc = P.store(s))) return rc;
+    }
+    std::swap(P.w0, P.w1);
+  }
+  return KLRTM_OK;
+}
+
+}  // namespace klrtm
+
+using namespace klrtm;
+
+extern "C" {
+
+int klrtm_propagator_create(klrtm_propagator **p, const klrtm_shot_config *cfg) {
+  KL_REQUIRE(p && cfg, KLRTM_EINVAL, "null argument");
+  Propagator *P = nullptr;
+  int rc = propagator_create(&P, cfg);
+  *p = reinterpret_cast<klrtm_propagator *>(P);
+  return rc;
+}
+int klrtm_propagator_destroy(klrtm_propagator *p) {
+  delete reinterpret_cast<Propagator *>(p);
+  return KLRTM_OK;
+}
+int klrtm_propagator_set_model(klrtm_propagator *p, const float *vel, const float *eps, const float *del, const float *pjx,
+                               const float *pjy) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  return propagator_set_model(*reinterpret_cast<Propagator *>(p), vel, eps, del, pjx, pjy);
+}
+int klrtm_propagator_set_source_wavelet(klrtm_propagator *p, const float *wav) {
+  KL_REQUIRE(p && wav, KLRTM_EINVAL, "null argument");
+  return receiver_set_traces(*reinterpret_cast<Propagator *>(p)->src, wav);
+}
+int klrtm_propagator_set_receivers(klrtm_propagator *p, const float *xyz) {
+  KL_REQUIRE(p && xyz, KLRTM_EINVAL, "null argument");
+  return propagator_set_receivers(*reinterpret_cast<Propagator *>(p), xyz);
+}
+int klrtm_propagator_model(klrtm_propagator *p, float *record) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  return propagator_model(*reinterpret_cast<Propagator *>(p), record);
+}
+int klrtm_propagator_rtm(klrtm_propagator *p, const float *record, float *img, float *ill, double *img64, double *ill64) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  return propagator_rtm(*reinterpret_cast<Propagator *>(p), record, img, ill, img64, ill64);
+}
+int klrtm_propagator_timings(klrtm_propagator *p, double *f, double *b, double *ic) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  if(f) *f = P.t_fwd;
+  if(b) *b = P.t_bwd;
+  if(ic) *ic = P.t_ic;
+  return KLRTM_OK;
+}
+int klrtm_propagator_wavefields(klrtm_propagator *p, float **w0, float **w1) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  if(w0) *w0 = P.w0;
+  if(w1) *w1 = P.w1;
+  return KLRTM_OK;
+}
+int klrtm_propagator_forward_steps(klrtm_propagator *p, int it0, int nsteps, int store) {
+  KL_REQUIRE(p && nsteps >= 0, KLRTM_EINVAL, "bad argument");
+  return propagator_forward_steps(*reinterpret_cast<Propagator *>(p), it0, nsteps, store);
+}
+int klrtm_propagator_reset(klrtm_propagator *p) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  return reinterpret_cast<Propagator *>(p)->reset();
+}
+
+}  // extern "C"

@@ -1,0 +1,18 @@
+// receiver.h -- device receiver object (Receiver / ExtractReceiverData).
+#pragma once
+#include "klrtm_internal.h"
+
+namespace klrtm {
+struct Receiver;
+int spread_coeff(int n, float *coeff0, float x, float k1, float k2, int *nlimit);
+int receiver_create(Receiver **out, int nx, int ny, int nz, int nr, const float *xyz, int spread, float k1, float k2,
+                    int nt, float dt_dxyz);
+void receiver_destroy(Receiver *R);
+int receiver_set_traces(Receiver &R, const float *traces_host);
+int receiver_set_traces_device(Receiver &R, const float *tm_device);
+int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scaler0);
+int receiver_extract(Receiver &R, const float *w, int jt);
+int receiver_get_record(Receiver &R, float *out_host);
+int receiver_clear_record(Receiver &R);
+float *receiver_record_device(Receiver &R);
+}  // namespace klrtm

@@ -1,0 +1,441 @@
+// receiver.hip -- source/receiver injection and extraction on the device.
+//
+// Receiver::spreadCoeffs (bgprtm/Receiver.cpp:306-363) builds one separable
+// kernel (kx, ky, kz) per trace; Receiver::apply (bgprtm/Receiver.cpp:428-475,
+// 1204-1279) scatters trace*kx*ky*kz*vel into w0 and Receiver::extract_value
+// (bgprtm/Receiver.cpp:394-426) gathers it back.  The reference scatters into
+// per-thread private volumes and reduces them; here every output cell gathers
+// its contributions instead, so no atomics are needed and the result does not
+// depend on scheduling:
+//   * receivers are grouped by their z kernel (same offz, lz and taps: e.g. a
+//     flat acquisition surface is one group);
+//   * per group, each (x,y) column sums val*ky*kx over the receivers whose x/y
+//     kernels cover it (found through a bin index keyed by kernel origin), then
+//     the z kernel is applied to the column slab;
+//   * extraction first contracts the slab with kz per column, then each
+//     receiver contracts its ky*kx window of that plane.
+// Contraction is off so a single trace (the source) is injected with exactly
+// the reference's products and sums.
+#pragma clang fp contract(off)
+#include "klrtm_internal.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+namespace klrtm {
+
+// ---------------------------------------------------------------------------
+// Util::CompactOrmsbySpreadCoeff (bgprtm/Util.cpp:317-397).  libCommon's
+// sincf_unnorm and blackman are unvendored; this build uses
+// sinc_unnorm(b) = sin(b)/b and the 3-term Blackman window
+// 0.42 - 0.5 cos(t) + 0.08 cos(2t) on t in [0, 2 pi].
+static const float kPi = 3.14159265358979f;
+static inline float sinc_u(float b) { return std::fabs(b) < 1e-20f ? 1.0f : sinf(b) / b; }
+static inline float blackman3(float t) { return 0.42f - 0.5f * cosf(t) + 0.08f * cosf(2.0f * t); }
+
+int spread_coeff(int n, float *coeff0, float x, float k1, float k2, int *nlimit) {
+  if(n == 1 || *nlimit < 2) {
+    coeff0[0] = 1.0f;
+    *nlimit = 1;
+    return int(nearbyintf(x));
+  }
+  const bool use_sinc = (k1 == k2);
+  float a1 = k1 * kPi, a2 = k2 * kPi;
+  if(use_sinc) a1 *= 2;
+  const int ix = int(floorf(x));
+  int m = std::min(std::min(ix, n - 1 - ix), *nlimit / 2);
+  if(m < 0) m = 0;
+  *nlimit = 2 * m;
+  const int i0 = ix - m + 1, i1 = ix + m;
+  const int ntaper = m / 2;
+  const float halfwin = std::max(i1 - x, x - i0);
+  for(int i = i0; i <= i1; i++) {
+    float shift = i - x;
+    float s1 = sinc_u(shift * a1);
+    float val;
+    if(!use_sinc) {
+      float s2 = sinc_u(shift * a2);
+      val = (a2 * a2 * s2 * s2 - a1 * a1 * s1 * s1) / kPi / (a2 - a1);
+      int i_edge = std::min(i - i0, i1 - i);
+      if(i_edge < ntaper && m > 1) val *= 0.5f * (1 - cosf(kPi * i_edge / ntaper));
+    } else {
+      val = s1 * 2 * k1;
+      if(m > 1) val *= blackman3(kPi * (halfwin - x + i) / halfwin);
+    }
+    coeff0[i - i0] = val;
+  }
+  return i0;
+}
+
+// x/y kernel edge taper: bgprtm/Receiver.cpp:291-303
+static void taper_edges(float *k, int l, int off, const float *taper, int ntap, int n) {
+  if(n <= ntap * 2) return;
+  if(off < ntap) {
+    int nlim = std::min(ntap, l + off), i0 = std::max(0, off);
+    for(int i = i0; i < nlim; i++) k[i - off] *= taper[i];
+  }
+  if(off + l > n - ntap) {
+    int nlim = std::min(ntap, n - off), i0 = std::max(0, n - off - l);
+    for(int i = i0; i < nlim; i++) k[n - (off + i + 1)] *= taper[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+struct RcvGroup {
+  int offz, lz;
+  int bx0, by0, bw, bh;  // plane bounding box of the group's x/y kernels
+  int lxmax, lymax;
+  float *kz;             // device, lz taps
+  int *bin_start;        // device, bw*bh+1 (CSR over kernel origins)
+  int *bin_rcv;          // device, receiver ids
+  int nr;
+};
+
+struct Receiver {
+  int nx, ny, nz, nr, nt;
+  float dt_dxyz;
+  int L;  // max taps per axis (x/y)
+  std::vector<int> lx, ly, lz, offx, offy, offz;
+  std::vector<float> kx, ky, kz;  // [nr][L] / [nr][Lz]
+  int Lz;
+  std::vector<RcvGroup> groups;
+  float *d_kx = nullptr, *d_ky = nullptr;
+  int *d_lx = nullptr, *d_ly = nullptr, *d_offx = nullptr, *d_offy = nullptr;
+  float *d_traces = nullptr;  // [nt][nr]
+  float *d_record = nullptr;  // [nt][nr]
+  float *d_plane = nullptr;   // scratch plane (largest group bbox)
+  size_t plane_cells = 0;
+  ~Receiver();
+};
+
+Receiver::~Receiver() {
+  for(auto &g : groups) (void)hipFree(g.kz), (void)hipFree(g.bin_start), (void)hipFree(g.bin_rcv);
+  (void)hipFree(d_kx), (void)hipFree(d_ky), (void)hipFree(d_lx), (void)hipFree(d_ly);
+  (void)hipFree(d_offx), (void)hipFree(d_offy), (void)hipFree(d_traces), (void)hipFree(d_record), (void)hipFree(d_plane);
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+struct GroupArgs {
+  int nx, ny, nz, nr;
+  int offz, lz, bx0, by0, bw, bh, lxmax, lymax, L;
+};
+
+// T[iy][ix] = sum_r (val_r*ky_r)*kx_r over receivers covering (ix,iy)
+__global__ void inject_plane_kernel(const GroupArgs a, const int *__restrict__ bin_start, const int *__restrict__ bin_rcv,
+                                    const float *__restrict__ kx, const float *__restrict__ ky,
+                                    const int *__restrict__ lx, const int *__restrict__ ly,
+                                    const int *__restrict__ offx, const int *__restrict__ offy,
+                                    const float *__restrict__ val, float scaler, float *__restrict__ plane) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if(c >= a.bw * a.bh) return;
+  int px = c % a.bw, py = c / a.bw;
+  int ix = a.bx0 + px, iy = a.by0 + py;
+  float T = 0.f;
+  int qy0 = max(0, py - a.lymax + 1), qx0 = max(0, px - a.lxmax + 1);
+  for(int qy = qy0; qy <= py; qy++)
+    for(int qx = qx0; qx <= px; qx++) {
+      int b = qy * a.bw + qx;
+      for(int k = bin_start[b]; k < bin_start[b + 1]; k++) {
+        int r = bin_rcv[k];
+        int jx = ix - offx[r], jy = iy - offy[r];
+        if(jx >= 0 && jy >= 0 && jx < lx[r] && jy < ly[r]) T += ((val[r] * scaler) * ky[(size_t)r * a.L + jy]) * kx[(size_t)r * a.L + jx];
+      }
+    }
+  plane[c] = T;
+}
+
+// w0[col, offz+k] += T*kz[k]*vel
+__global__ void inject_apply_kernel(const GroupArgs a, const float *__restrict__ plane, const float *__restrict__ kz,
+                                    float *__restrict__ w0, const float *__restrict__ vel) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long tot = (long)a.bw * a.bh * a.lz;
+  if(i >= tot) return;
+  int k = (int)(i % a.lz);
+  int c = (int)(i / a.lz);
+  int ix = a.bx0 + c % a.bw, iy = a.by0 + c / a.bw;
+  float T = plane[c];
+  if(T == 0.f) return;
+  size_t o = ((size_t)iy * a.nx + ix) * a.nz + a.offz + k;
+  w0[o] += (T * kz[k]) * vel[o];
+}
+
+// P[iy][ix] = sum_k kz[k]*w[col, offz+k]
+__global__ void extract_plane_kernel(const GroupArgs a, const float *__restrict__ kz, const float *__restrict__ w,
+                                     float *__restrict__ plane) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if(c >= a.bw * a.bh) return;
+  int ix = a.bx0 + c % a.bw, iy = a.by0 + c / a.bw;
+  const float *col = w + ((size_t)iy * a.nx + ix) * a.nz + a.offz;
+  float s = 0.f;
+  for(int k = 0; k < a.lz; k++) s += kz[k] * col[k];
+  plane[c] = s;
+}
+
+// record[jt][r] = sum_{iy,ix} ky*kx*P
+__global__ void extract_rcv_kernel(const GroupArgs a, const int *__restrict__ ids, const float *__restrict__ kx,
+                                   const float *__restrict__ ky, const int *__restrict__ lx, const int *__restrict__ ly,
+                                   const int *__restrict__ offx, const int *__restrict__ offy,
+                                   const float *__restrict__ plane, float *__restrict__ rec_row) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if(k >= a.nr) return;
+  int r = ids[k];
+  float val = 0.f;
+  for(int jy = 0; jy < ly[r]; jy++) {
+    float cy = ky[(size_t)r * a.L + jy];
+    const float *row = plane + (size_t)(offy[r] + jy - a.by0) * a.bw + (offx[r] - a.bx0);
+    for(int jx = 0; jx < lx[r]; jx++) val += (cy * kx[(size_t)r * a.L + jx]) * row[jx];
+  }
+  rec_row[r] = val;
+}
+
+static GroupArgs gargs(const Receiver &R, const RcvGroup &g) {
+  GroupArgs a;
+  a.nx = R.nx, a.ny = R.ny, a.nz = R.nz, a.nr = g.nr;
+  a.offz = g.offz, a.lz = g.lz, a.bx0 = g.bx0, a.by0 = g.by0, a.bw = g.bw, a.bh = g.bh;
+  a.lxmax = g.lxmax, a.lymax = g.lymax, a.L = R.L;
+  return a;
+}
+
+int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scaler0) {
+  if(it < 0 || it > R.nt - 1) return KLRTM_OK;  // bgprtm/Receiver.cpp:1205
+  KL_REQUIRE(R.d_traces, KLRTM_ESTATE, "receiver traces not set");
+  const float scaler = scaler0 * R.dt_dxyz;
+  const float *val = R.d_traces + (size_t)it * R.nr;
+  LaunchTimer tm("inject");
+  for(auto &g : R.groups) {
+    GroupArgs a = gargs(R, g);
+    int cells = g.bw * g.bh;
+    hipLaunchKernelGGL(inject_plane_kernel, dim3(div_up(cells, 256)), dim3(256), 0, stream(), a, g.bin_start, g.bin_rcv,
+                       R.d_kx, R.d_ky, R.d_lx, R.d_ly, R.d_offx, R.d_offy, val, scaler, R.d_plane);
+    long tot = (long)cells * g.lz;
+    hipLaunchKernelGGL(inject_apply_kernel, dim3(div_up(tot, 256)), dim3(256), 0, stream(), a, R.d_plane, g.kz, w0, vel);
+  }
+  return check_launch("inject");
+}
+
+int receiver_extract(Receiver &R, const float *w, int jt) {
+  KL_REQUIRE(jt >= 0 && jt < R.nt, KLRTM_EINVAL, "extract time index %d outside [0,%d)", jt, R.nt);
+  float *row = R.d_record + (size_t)jt * R.nr;
+  LaunchTimer tm("extract");
+  for(auto &g : R.groups) {
+    GroupArgs a = gargs(R, g);
+    int cells = g.bw * g.bh;
+    hipLaunchKernelGGL(extract_plane_kernel, dim3(div_up(cells, 256)), dim3(256), 0, stream(), a, g.kz, w, R.d_plane);
+    hipLaunchKernelGGL(extract_rcv_kernel, dim3(div_up(g.nr, 256)), dim3(256), 0, stream(), a, g.bin_rcv, R.d_kx, R.d_ky,
+                       R.d_lx, R.d_ly, R.d_offx, R.d_offy, R.d_plane, row);
+  }
+  return check_launch("extract");
+}
+
+template<class T> static int upload(T **dst, const std::vector<T> &src) {
+  if(src.empty()) return KLRTM_OK;
+  KL_HIP(hipMalloc(dst, src.size() * sizeof(T)));
+  KL_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return KLRTM_OK;
+}
+
+// Receiver + spreadCoeffs: bgprtm/Receiver.cpp:129-151 (z range), 306-363
+int receiver_create(Receiver **out, int nx, int ny, int nz, int nr, const float *xyz, int spread, float k1, float k2,
+                    int nt, float dt_dxyz) {
+  KL_REQUIRE(nr > 0 && nt > 0, KLRTM_EINVAL, "need nr > 0 and nt > 0");
+  KL_REQUIRE(spread > 0, KLRTM_EINVAL, "spread size must be > 0");
+  KL_REQUIRE(k1 > 0 && k1 <= k2 && k2 <= 0.5f, KLRTM_EINVAL, "k1,k2 need to be in (0,0.5] with k1 <= k2");
+  Receiver *R = new Receiver();
+  R->nx = nx, R->ny = ny, R->nz = nz, R->nr = nr, R->nt = nt, R->dt_dxyz = dt_dxyz;
+  // z range of the acquisition and the z spread it allows (update_zrange)
+  float zmin = 1e30f, zmax = -1e30f;
+  for(int r = 0; r < nr; r++) {
+    float x = xyz[3 * r], y = xyz[3 * r + 1], z = xyz[3 * r + 2];
+    if(!(x >= 0 && x <= nx - 1 && y >= 0 && y <= ny - 1 && z >= 0 && z <= nz - 1)) {
+      delete R;
+      set_error("receiver %d at (%g,%g,%g) is outside the grid", r, x, y, z);
+      return KLRTM_EINVAL;
+    }
+    zmin = std::min(zmin, z), zmax = std::max(zmax, z);
+  }
+  int izmin = (int)floorf(zmin), izmax = (int)floorf(zmax);
+  int spreadZ = spread;
+  int zlimit = std::min(izmin + 1, nz - 1 - izmax);
+  if(zlimit < spreadZ) spreadZ = std::max(5, zlimit);
+  int iz0 = izmin - spreadZ + 1, iz1 = izmax + spreadZ;
+  if(iz0 < 0 || iz1 >= nz) {
+    delete R;
+    set_error("receiver z range [%d,%d] with spread %d leaves the grid", izmin, izmax, spreadZ);
+    return KLRTM_EINVAL;
+  }
+  int limx = nx == 1 ? 1 : std::max(1, spread * 2);
+  int limy = ny == 1 ? 1 : std::max(1, spread * 2);
+  int limz = std::max(1, iz1 - iz0 + 1);
+  R->L = std::max(limx, limy);
+  R->Lz = limz;
+  R->lx.resize(nr), R->ly.resize(nr), R->lz.resize(nr), R->offx.resize(nr), R->offy.resize(nr), R->offz.resize(nr);
+  R->kx.assign((size_t)nr * R->L, 0.f), R->ky.assign((size_t)nr * R->L, 0.f), R->kz.assign((size_t)nr * limz, 0.f);
+  const int ntaperXY = 10;  // bgprtm/Receiver.cpp:1013
+  std::vector<float> wt(ntaperXY);
+  for(int i = 0; i < ntaperXY; i++) wt[i] = 0.5f * (1.0f - cosf((float)(M_PI * i / ntaperXY)));
+  for(int r = 0; r < nr; r++) {
+    int l;
+    l = limx;
+    R->offx[r] = spread_coeff(nx, &R->kx[(size_t)r * R->L], xyz[3 * r], k1, k2, &l);
+    R->lx[r] = l;
+    l = limy;
+    R->offy[r] = spread_coeff(ny, &R->ky[(size_t)r * R->L], xyz[3 * r + 1], k1, k2, &l);
+    R->ly[r] = l;
+    l = limz;
+    R->offz[r] = spread_coeff(nz, &R->kz[(size_t)r * limz], xyz[3 * r + 2], k1, k2, &l);
+    R->lz[r] = l;
+    taper_edges(&R->kx[(size_t)r * R->L], R->lx[r], R->offx[r], wt.data(), ntaperXY, nx);
+    taper_edges(&R->ky[(size_t)r * R->L], R->ly[r], R->offy[r], wt.data(), ntaperXY, ny);
+  }
+  // group by identical z kernels
+  std::map<std::vector<uint32_t>, std::vector<int>> by_z;
+  for(int r = 0; r < nr; r++) {
+    std::vector<uint32_t> key(2 + R->lz[r]);
+    key[0] = (uint32_t)R->offz[r], key[1] = (uint32_t)R->lz[r];
+    memcpy(key.data() + 2, &R->kz[(size_t)r * limz], sizeof(float) * R->lz[r]);
+    by_z[key].push_back(r);
+  }
+  int rc = KLRTM_OK;
+  for(auto &kv : by_z) {
+    const std::vector<int> &ids = kv.second;
+    RcvGroup g{};
+    int r0 = ids[0];
+    g.offz = R->offz[r0], g.lz = R->lz[r0], g.nr = (int)ids.size();
+    int x0 = 1 << 30, y0 = 1 << 30, x1 = -1, y1 = -1;
+    g.lxmax = 1, g.lymax = 1;
+    for(int r : ids) {
+      x0 = std::min(x0, R->offx[r]), y0 = std::min(y0, R->offy[r]);
+      x1 = std::max(x1, R->offx[r] + std::max(R->lx[r], 1)), y1 = std::max(y1, R->offy[r] + std::max(R->ly[r], 1));
+      g.lxmax = std::max(g.lxmax, R->lx[r]), g.lymax = std::max(g.lymax, R->ly[r]);
+    }
+    g.bx0 = std::max(0, x0), g.by0 = std::max(0, y0);
+    g.bw = std::min(nx, x1) - g.bx0, g.bh = std::min(ny, y1) - g.by0;
+    // CSR bins keyed by kernel origin, receivers in increasing id
+    std::vector<int> cnt((size_t)g.bw * g.bh + 1, 0), rid;
+    std::vector<int> keyof(ids.size());
+    for(size_t i = 0; i < ids.size(); i++) {
+      int r = ids[i];
+      int bx = std::min(std::max(R->offx[r] - g.bx0, 0), g.bw - 1), by = std::min(std::max(R->offy[r] - g.by0, 0), g.bh - 1);
+      keyof[i] = by * g.bw + bx;
+      cnt[keyof[i] + 1]++;
+    }
+    for(size_t b = 1; b < cnt.size(); b++) cnt[b] += cnt[b - 1];
+    rid.assign(ids.size(), 0);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for(size_t i = 0; i < ids.size(); i++) rid[fill[keyof[i]]++] = ids[i];
+    std::vector<float> kzv(R->kz.begin() + (size_t)r0 * limz, R->kz.begin() + (size_t)r0 * limz + std::max(g.lz, 1));
+    if((rc = upload(&g.kz, kzv)) || (rc = upload(&g.bin_start, cnt)) || (rc = upload(&g.bin_rcv, rid))) break;
+    R->plane_cells = std::max(R->plane_cells, (size_t)g.bw * g.bh);
+    R->groups.push_back(g);
+  }
+  if(!rc) {
+    if((rc = upload(&R->d_kx, R->kx)) || (rc = upload(&R->d_ky, R->ky)) || (rc = upload(&R->d_lx, R->lx)) ||
+       (rc = upload(&R->d_ly, R->ly)) || (rc = upload(&R->d_offx, R->offx)) || (rc = upload(&R->d_offy, R->offy))) {
+    }
+  }
+  if(!rc) {
+    hipError_t e = hipMalloc(&R->d_plane, std::max<size_t>(1, R->plane_cells) * sizeof(float));
+    if(e == hipSuccess) e = hipMalloc(&R->d_record, (size_t)nt * nr * sizeof(float));
+    if(e == hipSuccess) e = hipMemset(R->d_record, 0, (size_t)nt * nr * sizeof(float));
+    if(e != hipSuccess) set_error("receiver allocation failed: %s", hipGetErrorString(e)), rc = KLRTM_ENOMEM;
+  }
+  if(rc) {
+    delete R;
+    return rc;
+  }
+  *out = R;
+  return KLRTM_OK;
+}
+
+int receiver_set_traces(Receiver &R, const float *traces) {
+  std::vector<float> t((size_t)R.nt * R.nr);
+  for(int r = 0; r < R.nr; r++)
+    for(int it = 0; it < R.nt; it++) t[(size_t)it * R.nr + r] = traces[(size_t)r * R.nt + it];
+  if(!R.d_traces) KL_HIP(hipMalloc(&R.d_traces, t.size() * sizeof(float)));
+  KL_HIP(hipMemcpy(R.d_traces, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+  return KLRTM_OK;
+}
+
+int receiver_set_traces_device(Receiver &R, const float *tm_device) {
+  if(!R.d_traces) KL_HIP(hipMalloc(&R.d_traces, (size_t)R.nt * R.nr * sizeof(float)));
+  KL_HIP(hipMemcpyAsync(R.d_traces, tm_device, (size_t)R.nt * R.nr * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+  return KLRTM_OK;
+}
+
+int receiver_get_record(Receiver &R, float *out) {
+  std::vector<float> t((size_t)R.nt * R.nr);
+  KL_HIP(hipStreamSynchronize(stream()));
+  KL_HIP(hipMemcpy(t.data(), R.d_record, t.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for(int r = 0; r < R.nr; r++)
+    for(int it = 0; it < R.nt; it++) out[(size_t)r * R.nt + it] = t[(size_t)it * R.nr + r];
+  return KLRTM_OK;
+}
+
+int receiver_clear_record(Receiver &R) {
+  KL_HIP(hipMemsetAsync(R.d_record, 0, (size_t)R.nt * R.nr * sizeof(float), stream()));
+  return KLRTM_OK;
+}
+
+float *receiver_record_device(Receiver &R) { return R.d_record; }
+void receiver_destroy(Receiver *R) { delete R; }
+
+}  // namespace klrtm
+
+using namespace klrtm;
+
+extern "C" {
+
+int klrtm_spread_coeff(int n, float *coeff, float x, float k1, float k2, int *nlimit, int *offset) {
+  KL_REQUIRE(coeff && nlimit && offset, KLRTM_EINVAL, "null argument");
+  KL_REQUIRE(k1 > 0 && k1 <= k2 && k2 <= 0.5f, KLRTM_EINVAL, "k1,k2 need to be in (0,0.5] with k1 <= k2");
+  KL_REQUIRE(n >= 1 && x >= 0 && x <= n - 1, KLRTM_EINVAL, "x=%g outside [0,%d]", x, n - 1);
+  *offset = spread_coeff(n, coeff, x, k1, k2, nlimit);
+  return KLRTM_OK;
+}
+
+int klrtm_receiver_create(klrtm_receiver **r, int nx, int ny, int nz, int nr, const float *xyz, int spread, float k1,
+                          float k2, int nt, float dt_dxyz) {
+  KL_REQUIRE(r && xyz, KLRTM_EINVAL, "null argument");
+  Receiver *R = nullptr;
+  int rc = receiver_create(&R, nx, ny, nz, nr, xyz, spread, k1, k2, nt, dt_dxyz);
+  *r = reinterpret_cast<klrtm_receiver *>(R);
+  return rc;
+}
+int klrtm_receiver_destroy(klrtm_receiver *r) {
+  delete reinterpret_cast<Receiver *>(r);
+  return KLRTM_OK;
+}
+int klrtm_receiver_set_traces(klrtm_receiver *r, const float *traces) {
+  KL_REQUIRE(r && traces, KLRTM_EINVAL, "null argument");
+  return receiver_set_traces(*reinterpret_cast<Receiver *>(r), traces);
+}
+int klrtm_receiver_apply(klrtm_receiver *r, float *w0, int it, const float *vel, float scaler0) {
+  KL_REQUIRE(r && w0 && vel, KLRTM_EINVAL, "null argument");
+  return receiver_apply(*reinterpret_cast<Receiver *>(r), w0, it, vel, scaler0);
+}
+int klrtm_receiver_extract(klrtm_receiver *r, const float *w, int jt) {
+  KL_REQUIRE(r && w, KLRTM_EINVAL, "null argument");
+  return receiver_extract(*reinterpret_cast<Receiver *>(r), w, jt);
+}
+int klrtm_receiver_get_record(klrtm_receiver *r, float *rec) {
+  KL_REQUIRE(r && rec, KLRTM_EINVAL, "null argument");
+  return receiver_get_record(*reinterpret_cast<Receiver *>(r), rec);
+}
+int klrtm_receiver_get_tables(klrtm_receiver *rr, int *lx, int *ly, int *lz, int *offx, int *offy, int *offz, float *kx,
+                              float *ky, float *kz, int *maxlen) {
+  KL_REQUIRE(rr, KLRTM_EINVAL, "null receiver");
+  Receiver &R = *reinterpret_cast<Receiver *>(rr);
+  if(maxlen) maxlen[0] = R.L, maxlen[1] = R.Lz;
+  auto cp = [](int *d, const std::vector<int> &s) {
+    if(d) memcpy(d, s.data(), s.size() * sizeof(int));
+  };
+  cp(lx, R.lx), cp(ly, R.ly), cp(lz, R.lz), cp(offx, R.offx), cp(offy, R.offy), cp(offz, R.offz);
+  if(kx) memcpy(kx, R.kx.data(), R.kx.size() * sizeof(float));
+  if(ky) memcpy(ky, R.ky.data(), R.ky.size() * sizeof(float));
+  if(kz) memcpy(kz, R.kz.data(), R.kz.size() * sizeof(float));
+  return KLRTM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+"""Propagator: one RTM shot on the GPU (bgprtm/Propagator.h:100-104 rtm / mod)."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import lib, check, fptr, dptr, ShotConfig, ENGINE_FD, ISO, SNAP_FP32
+
+
+def shot_config(nx, ny, nz, d, dt, nt, engine=ENGINE_FD, order=8, model=ISO, nbnd=20, vmax=None, src_xyz=None,
+                f_peak=15.0, spread=10, khpass=0.3, khcut=0.3, nrcv=0, snap_every=1, snap_format=SNAP_FP32,
+                rec_through_codec=0, tmute_frac=0.1, ntaper_z=0, fuse_step=1, pml_eps=1e-3, pml3d=1,
+                pml3d_scaler=0.3, top_pml=1):
+    c = ShotConfig()
+    c.nx, c.ny, c.nz = nx, ny, nz
+    dx, dy, dz = (d, d, d) if np.isscalar(d) else d
+    c.dx, c.dy, c.dz, c.dt, c.nt = dx, dy, dz, dt, nt
+    c.engine, c.order, c.model = engine, order, model
+    nb = [nbnd] * 6 if np.isscalar(nbnd) else list(nbnd)
+    for i in range(6):
+        c.nbnd[i] = nb[i]
+    c.vmax = vmax if vmax is not None else 4500.0
+    c.pml_eps, c.pml3d_scaler, c.pml3d, c.top_pml = pml_eps, pml3d_scaler, pml3d, top_pml
+    sx = src_xyz if src_xyz is not None else ((nx - 1) / 2, (ny - 1) / 2, (nz - 1) / 2)
+    for i in range(3):
+        c.src_xyz[i] = sx[i]
+    c.f_peak = f_peak
+    c.src_spread = c.rcv_spread = spread
+    c.khpass, c.khcut = khpass, khcut
+    c.nrcv = nrcv
+    c.snap_every, c.snap_format, c.rec_through_codec = snap_every, snap_format, rec_through_codec
+    c.tmute_frac, c.ntaper_z, c.fuse_step = tmute_frac, ntaper_z, fuse_step
+    return c
+
+
+class Propagator:
+    def __init__(self, cfg):
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(lib().klrtm_propagator_create(C.byref(h), C.byref(cfg)))
+        self.h = h.value
+        self.shape = (cfg.ny, cfg.nx, cfg.nz)
+
+    def set_model(self, vel, eps=None, del_=None, pjx=None, pjy=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (vel, eps, del_, pjx, pjy)]
+        check(lib().klrtm_propagator_set_model(self.h, *[fptr(a) for a in arrs]))
+
+    def set_source_wavelet(self, wavelet):
+        w = np.ascontiguousarray(wavelet, np.float32)
+        assert w.shape == (self.cfg.nt,)
+        check(lib().klrtm_propagator_set_source_wavelet(self.h, fptr(w)))
+
+    def set_receivers(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        assert xyz.shape[0] == self.cfg.nrcv
+        check(lib().klrtm_propagator_set_receivers(self.h, fptr(xyz)))
+
+    def model(self, want_record=True):
+        rec = np.zeros((self.cfg.nrcv, self.cfg.nt), np.float32) if want_record else None
+        check(lib().klrtm_propagator_model(self.h, fptr(rec)))
+        return rec
+
+    def rtm(self, record=None, want=("image", "illum", "image64", "illum64")):
+        n = self.shape
+        img = np.zeros(n, np.float32) if "image" in want else None
+        ill = np.zeros(n, np.float32) if "illum" in want else None
+        img64 = np.zeros(n, np.float64) if "image64" in want else None
+        ill64 = np.zeros(n, np.float64) if "illum64" in want else None
+        r = None if record is None else np.ascontiguousarray(record, np.float32)
+        check(lib().klrtm_propagator_rtm(self.h, fptr(r), fptr(img), fptr(ill), dptr(img64), dptr(ill64)))
+        return dict(image=img, illum=ill, image64=img64, illum64=ill64)
+
+    def timings(self):
+        f, b, i = C.c_double(), C.c_double(), C.c_double()
+        check(lib().klrtm_propagator_timings(self.h, C.byref(f), C.byref(b), C.byref(i)))
+        return dict(forward_ms=f.value, backward_ms=b.value, ic_ms=i.value)
+
+    def forward_steps(self, it0, nsteps, store_snapshots=False):
+        check(lib().klrtm_propagator_forward_steps(self.h, it0, nsteps, int(store_snapshots)))
+
+    def reset(self):
+        check(lib().klrtm_propagator_reset(self.h))
+
+    def wavefield(self):
+        """Current w1 (the latest time level after a step's swap), downloaded."""
+        w0, w1 = C.c_void_p(), C.c_void_p()
+        check(lib().klrtm_propagator_wavefields(self.h, C.byref(w0), C.byref(w1)))
+        out = np.empty(self.shape, np.float32)
+        check(lib().klrtm_memcpy_d2h(out.ctypes.data, w1.value, out.nbytes))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().klrtm_propagator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

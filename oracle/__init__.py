@@ -53,6 +53,7 @@ def lib():
         L.orc_cfd_derive_d.argtypes = [C.c_int, C.c_double, C.c_int, _pd, _pd, C.c_int]
         L.orc_spread_coeff.restype = C.c_int
         L.orc_spread_coeff.argtypes = [C.c_int, _pf, C.c_float, C.c_float, C.c_float, _pi]
+        L.orc_xy_taper.argtypes = [C.c_int, _pf]
         L.orc_taper_kernel.argtypes = [_pf, C.c_int, C.c_int, _pf, C.c_int, C.c_int]
         L.orc_inject.argtypes = [_pf, _pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, _pf, _pf, _pf, C.c_int, C.c_int,
                                  C.c_int, _pi, _pi, _pi, _pi, _pi, _pi]
@@ -172,6 +173,12 @@ def spread_coeff(n, x, k1, k2, nlimit):
     nl = C.c_int(nlimit)
     off = lib().orc_spread_coeff(n, coeff.ctypes.data_as(_pf), x, k1, k2, C.byref(nl))
     return off, coeff[: nl.value].copy()
+
+
+def xy_taper(ntap=10):
+    w = np.zeros(ntap, np.float32)
+    lib().orc_xy_taper(ntap, w.ctypes.data_as(_pf))
+    return w
 
 
 def taper_kernel(kernel, off, taper, n):

@@ -1,0 +1,183 @@
+"""Oracle RTM pipeline: one shot on the CPU, step by step like the reference.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).  Restates
+  * Receiver::update_zrange + spreadCoeffs   bgprtm/Receiver.cpp:129-151, 306-363
+  * Receiver::apply (omp buffer form)        bgprtm/Receiver.cpp:428-475, 1204-1279
+  * Receiver::extract_value                  bgprtm/Receiver.cpp:394-426
+  * Propagator::modelling time loop          bgprtm/Propagator.cpp:1101-1153
+  * Propagator::migration time loop          bgprtm/Propagator.cpp:1210-1255
+  * ImagingCondition tapers + crossone       bgprtm/ImagingCondition.cpp:180-226,
+                                             bgprtm/ImagingCorrelator.cpp:180-194
+with the oracle engine for KernelCPU/FdEngine/PML.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import lib, Engine, spread_coeff, taper_kernel, xy_taper, compress16, uncompress16
+
+_pf, _pi = C.POINTER(C.c_float), C.POINTER(C.c_int)
+
+
+def ricker(nt, dt, fp, dtype=np.float32):
+    """Closed-form Ricker, f(t) = (1 - 2 a) exp(-a), a = (pi fp (t - 1.5/fp))^2."""
+    t = np.arange(nt, dtype=np.float64) * dt
+    tau = t - 1.5 / fp
+    a = (math.pi * fp * tau) ** 2
+    return ((1 - 2 * a) * np.exp(-a)).astype(dtype)
+
+
+def spread_tables(nx, ny, nz, xyz, spread, k1, k2):
+    """Per-receiver separable kernels (lx,ly,lz taps and offsets)."""
+    xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
+    nr = xyz.shape[0]
+    izmin, izmax = int(math.floor(xyz[:, 2].min())), int(math.floor(xyz[:, 2].max()))
+    spz = spread
+    zlimit = min(izmin + 1, nz - 1 - izmax)
+    if zlimit < spz:
+        spz = max(5, zlimit)
+    iz0, iz1 = izmin - spz + 1, izmax + spz
+    assert iz0 >= 0 and iz1 < nz, "receiver z range leaves the grid"
+    limx = 1 if nx == 1 else max(1, 2 * spread)
+    limy = 1 if ny == 1 else max(1, 2 * spread)
+    limz = max(1, iz1 - iz0 + 1)
+    L = max(limx, limy)
+    t = dict(L=L, Lz=limz, lx=np.zeros(nr, np.int32), ly=np.zeros(nr, np.int32), lz=np.zeros(nr, np.int32),
+             offx=np.zeros(nr, np.int32), offy=np.zeros(nr, np.int32), offz=np.zeros(nr, np.int32),
+             kx=np.zeros((nr, L), np.float32), ky=np.zeros((nr, L), np.float32), kz=np.zeros((nr, limz), np.float32))
+    wt = xy_taper(10)
+    for r in range(nr):
+        ox, kx = spread_coeff(nx, float(xyz[r, 0]), k1, k2, limx)
+        oy, ky = spread_coeff(ny, float(xyz[r, 1]), k1, k2, limy)
+        oz, kz = spread_coeff(nz, float(xyz[r, 2]), k1, k2, limz)
+        kx = taper_kernel(kx, ox, wt, nx) if kx.size else kx
+        ky = taper_kernel(ky, oy, wt, ny) if ky.size else ky
+        t["offx"][r], t["offy"][r], t["offz"][r] = ox, oy, oz
+        t["lx"][r], t["ly"][r], t["lz"][r] = kx.size, ky.size, kz.size
+        t["kx"][r, :kx.size], t["ky"][r, :ky.size], t["kz"][r, :kz.size] = kx, ky, kz
+    return t
+
+
+def _tab_args(t):
+    return (t["kx"].ctypes.data_as(_pf), t["ky"].ctypes.data_as(_pf), t["kz"].ctypes.data_as(_pf), t["L"], t["L"],
+            t["Lz"], t["lx"].ctypes.data_as(_pi), t["ly"].ctypes.data_as(_pi), t["lz"].ctypes.data_as(_pi),
+            t["offx"].ctypes.data_as(_pi), t["offy"].ctypes.data_as(_pi), t["offz"].ctypes.data_as(_pi))
+
+
+def inject(w0, vel, shape, t, vals):
+    ny, nx, nz = shape
+    vals = np.ascontiguousarray(vals, np.float32)
+    lib().orc_inject(w0.ctypes.data_as(_pf), vel.ctypes.data_as(_pf), nx, ny, nz, vals.size, vals.ctypes.data_as(_pf),
+                     *_tab_args(t))
+
+
+def extract(w, shape, t):
+    ny, nx, nz = shape
+    nr = t["lx"].size
+    out = np.zeros(nr, np.float32)
+    lib().orc_extract(w.ctypes.data_as(_pf), nx, ny, nz, nr, out.ctypes.data_as(_pf), *_tab_args(t))
+    return out
+
+
+class Shot:
+    """One shot: model (record) and rtm (image) in FP32, reference step order."""
+
+    def __init__(self, nx, ny, nz, d, dt, nt, engine, order, model, nbnd, vmax, src_xyz, f_peak, spread, k1, k2,
+                 rcv_xyz, snap_every=1, snap_format=0, tmute_frac=0.1, threads=4):
+        self.shape = (ny, nx, nz)
+        self.nx, self.ny, self.nz, self.dt, self.nt = nx, ny, nz, dt, nt
+        self.model_type, self.snap_every, self.snap_format = model, snap_every, snap_format
+        self.eng = Engine(nx, ny, nz, d, d, d, engine, order, np.float32, threads=threads)
+        self.nbnd, self.vmax = nbnd, vmax
+        dxyz = np.float32(d) * np.float32(d) * np.float32(d)
+        self.dt_dxyz = np.float32(1.0) / dxyz
+        self.src = spread_tables(nx, ny, nz, np.asarray(src_xyz, np.float32).reshape(1, 3), spread, k1, k2)
+        self.rcv = spread_tables(nx, ny, nz, rcv_xyz, spread, k1, k2)
+        self.wavelet = ricker(nt, dt, f_peak)
+        self.tmute_frac = tmute_frac
+
+    def set_model(self, vel, eps=None, del_=None, pjx=None, pjy=None):
+        v = np.asarray(vel, np.float32)
+        dt = np.float32(self.dt)
+        self.m = ((v * v) * (dt * dt)).astype(np.float32)  # Propagator::Vel2: dt*dt in FP32
+        self.eng.set_model(self.m, eps, del_, pjx, pjy)
+        self.eng.set_boundary(self.nbnd, self.dt, self.vmax, dim_pml=3)
+
+    def _fresh(self):
+        self.eng.clean_pml()
+        z = np.zeros(self.shape, np.float32)
+        return z.copy(), z.copy(), z.copy()
+
+    def model(self):
+        w0, w1, wr = self._fresh()
+        nr = self.rcv["lx"].size
+        rec = np.zeros((nr, self.nt), np.float32)
+        for it in range(self.nt):
+            self.eng.step(self.model_type, w1, w0, wr)
+            if 0 <= it - 1 < self.nt:
+                inject(w0, self.m, self.shape, self.src, self.wavelet[it - 1:it] * self.dt_dxyz)
+            rec[:, it] = extract(w0, self.shape, self.rcv)
+            w0, w1 = w1, w0
+        self.last = w1
+        return rec
+
+    def _decode(self, snap):
+        if self.snap_format == 1:
+            return uncompress16(snap, *self.shape)
+        # the reference's decoder applies the same edge taper to every codec
+        return _edge_taper(snap.reshape(self.shape))
+
+    def rtm(self, record):
+        k = self.snap_every
+        ntsav = (self.nt + k - 1) // k
+        snaps = [None] * ntsav
+        w0, w1, wr = self._fresh()
+        for it in range(self.nt):
+            self.eng.step(self.model_type, w1, w0, wr)
+            if 0 <= it - 1 < self.nt:
+                inject(w0, self.m, self.shape, self.src, self.wavelet[it - 1:it] * self.dt_dxyz)
+            if it % k == 0 and it // k < ntsav:
+                snaps[it // k] = compress16(w0) if self.snap_format == 1 else w0.copy()
+            w0, w1 = w1, w0
+        img = np.zeros(self.shape, np.float32)
+        ill = np.zeros(self.shape, np.float32)
+        tmax = self.nt * self.dt
+        ntaperT = int(np.rint(np.float32(self.tmute_frac * tmax / (k * self.dt))))
+        wtT = [np.float32(0.5) * (np.float32(1) - np.cos(np.float32(np.pi) * np.float32(i) / np.float32(ntaperT)))
+               for i in range(ntaperT)]
+        w0, w1, wr = self._fresh()
+        rec = np.ascontiguousarray(record, np.float32)
+        for it in range(self.nt - 1, -1, -1):
+            self.eng.step(self.model_type, w1, w0, wr)
+            ia = it + 1
+            if 0 <= ia < self.nt:
+                inject(w0, self.m, self.shape, self.rcv, rec[:, ia] * self.dt_dxyz)
+            if it % k == 0 and it // k < ntsav:
+                s = it // k
+                S = self._decode(snaps[s])
+                R = uncompress16(compress16(w0), *self.shape) if self.snap_format == 1 else _edge_taper(w0)
+                if s < ntaperT:
+                    S = (S * wtT[s]).astype(np.float32)
+                img += S * R
+                ill += S * S
+            w0, w1 = w1, w0
+        return img, ill
+
+
+def _edge_taper(v):
+    """Edge taper of WaveFieldCompress::uncompress (20 x/y cells, 10 z cells)."""
+    ny, nx, nz = v.shape
+    out = np.array(v, np.float32, copy=True)
+    wt = (0.5 * (1 - np.cos(np.float32(np.pi) * np.arange(20, dtype=np.float32) / np.float32(20)))).astype(np.float32)
+    wz = (0.5 * (1 - np.cos(np.float32(np.pi) * np.arange(10, dtype=np.float32) / np.float32(10)))).astype(np.float32)
+    if ny > 40:
+        out[:20] *= wt[:, None, None]
+        out[ny - 20:] *= wt[::-1, None, None]
+    if nx > 40:
+        out[:, :20] *= wt[None, :, None]
+        out[:, nx - 20:] *= wt[None, ::-1, None]
+    if nz > 20:
+        out[:, :, :10] *= wz[None, None, :]
+        out[:, :, nz - 10:] *= wz[None, None, ::-1]
+    return out

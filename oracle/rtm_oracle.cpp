@@ -862,6 +862,10 @@ ENGINE_API(double, d)
 int orc_spread_coeff(int n, float *coeff, float x, float k1, float k2, int *nlimit) {
   return spread_coeff(n, coeff, x, k1, k2, nlimit);
 }
+// x/y kernel taper weights of Receiver::spread (bgprtm/Receiver.cpp:1013-1014 via spreadCoeffs:323-325)
+void orc_xy_taper(int ntap, float *w) {
+  for(int i = 0; i < ntap; i++) w[i] = 0.5f * (1.0f - cosf(M_PI * i / ntap));
+}
 void orc_taper_kernel(float *kernel, int l, int off, const float *taper, int ntap, int n) {
   taper_kernel(kernel, l, off, taper, ntap, n);
 }

@@ -1,0 +1,322 @@
+"""Pins the CPU oracle against closed forms, known answers, brute force and the
+reference's own compilable source (bgprtm/Cfd_plan.cpp), so that a dropped
+term, wrong sign or transposed index in the restatement fails a check."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import pipeline
+from tests.helpers import rel_l2
+
+
+# ---- FD coefficients --------------------------------------------------------------
+def test_taylor_fd8_known_answer():
+    c2 = oracle.taylor_coef2(4)
+    assert np.allclose(c2, [-205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560], rtol=0, atol=1e-15)
+    c1 = oracle.taylor_coef1(4)
+    assert np.allclose(c1[1:], [4 / 5, -1 / 5, 4 / 105, -1 / 280], rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("M", [4, 5, 6, 7, 8])
+def test_taylor_exact_on_polynomials(M):
+    # second derivative exact for x^p, p <= 2M+1; first derivative for p <= 2M
+    c2, c1 = oracle.taylor_coef2(M), oracle.taylor_coef1(M)
+    x0 = 0.37
+    for p in range(0, 2 * M + 2):
+        f = lambda x: x ** p
+        d2 = c2[0] * f(x0) + sum(c2[j] * (f(x0 + j) + f(x0 - j)) for j in range(1, M + 1))
+        d1 = sum(c1[j] * (f(x0 + j) - f(x0 - j)) for j in range(1, M + 1))
+        e2 = p * (p - 1) * x0 ** (p - 2) if p >= 2 else 0.0
+        e1 = p * x0 ** (p - 1) if p >= 1 else 0.0
+        assert abs(d2 - e2) <= 1e-8 * max(1, abs(e2)) * 10 ** (p / 4)
+        if p <= 2 * M:
+            assert abs(d1 - e1) <= 1e-8 * max(1, abs(e1)) * 10 ** (p / 4)
+
+
+def test_k1max_fd8_fd16():
+    # k1max = sqrt(max of -symbol) = sqrt(sum_j 2|c2_j| ... evaluated at Nyquist) (bgprtm/FdEngine_fd.cpp:69-81)
+    for M, ref in ((4, 2.54982), (8, 2.72524)):
+        c2 = oracle.taylor_coef2(M)
+        s = -c2[0] + sum(2 * c2[j] * (-1) ** (j + 1) for j in range(1, M + 1))
+        assert abs(math.sqrt(s) - ref) < 5e-5
+
+
+# ---- PML ---------------------------------------------------------------------------
+def test_pml_profile_closed_form():
+    npml, dx, dt, vmax, eps = 20, 10.0, 1e-3, 4500.0, 1e-3
+    coef, feoc = oracle.pml_profile(npml, dx, dt, vmax, eps)
+    beta = dx / (dt * vmax)
+    dg1 = math.log(1 / eps) / (npml * beta)
+    exp = [0.75 * ((npml - i) / npml) ** 2 * dg1 for i in range(npml)]
+    assert np.allclose(coef, exp, rtol=2e-6)
+    assert np.array_equal(feoc, coef[::-1])
+    assert coef[0] == coef.max() and coef[-1] == coef.min() > 0
+
+
+def test_pml_round_identity_and_decay():
+    p0 = np.linspace(-1, 1, 9).astype(np.float32)
+    p, q = oracle.pml_round(0.0, p0, np.zeros_like(p0))
+    assert np.array_equal(p, p0) and not q.any()          # g = 0: identity, no state
+    # constant input through one face: recursive filter converges, output bounded by input
+    g = 0.3
+    pp, qq = np.ones(1, np.float32), np.zeros(1, np.float32)
+    out = []
+    for _ in range(200):
+        o, qq = oracle.pml_round(g, pp, qq)
+        out.append(float(o[0]))
+    assert all(abs(v) <= 1.5 for v in out) and abs(out[-1] - out[-2]) < 1e-6
+
+
+# ---- CFD plan and derivative --------------------------------------------------------
+@pytest.mark.parametrize("n,d", [(10, 1.0), (16, 12.5), (128, 10.0), (400, 12.5), (37, 7.3)])
+def test_cfd_plan_matches_reference_source(n, d):
+    ref = oracle.ref_cfd_lib()
+    if ref is None:
+        pytest.skip("reference CfdPlan not built (oracle/_ref); run `make -C oracle ref`")
+    import ctypes as C
+    P = C.POINTER(C.c_float)
+    a = oracle.cfd_plan(n, d)
+    b = [np.zeros(10 * n, np.float32), np.zeros(8 * n, np.float32), np.zeros(120, np.float32)]
+    ref.ref_cfd_plan(n, d, *[x.ctypes.data_as(P) for x in b])
+    for x, y in zip(a, b):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def _dense_cfd(n, d, sgn):
+    """Assemble B (9-band) and A (11-band) from the literals and folding rules, dense."""
+    ca5 = np.array([0.861085998189822, 0.753782983214427, 0.257072513289755, 0.031326722562826, 0.000410456919800])
+    cb5 = np.array([1.0, 0.753205475977546, 0.312203852849021, 0.063580085296350, 0.004597763362256])
+    fold = -1.0 if sgn == 0 else 1.0  # gradient: odd extension, divergence: even
+    B = np.zeros((n, n))
+    A = np.zeros((n, n))
+    for i in range(n):
+        for k in range(-4, 5):
+            c = cb5[abs(k)]
+            j = i + k
+            if j < 0:
+                B[i, -j] += fold * c
+            elif j > n - 1:
+                B[i, 2 * (n - 1) - j] += fold * c
+            else:
+                B[i, j] += c
+    for i in range(n):
+        for k in range(-5, 6):
+            c = 0.0 if k == 0 else math.copysign(ca5[abs(k) - 1], k) / (2 * d)
+            j = i + k
+            if 5 <= i < n - 5:
+                A[i, j] += c
+            elif j < 0:
+                A[i, -j] += -fold * c
+            elif j > n - 1:
+                A[i, 2 * (n - 1) - j] += -fold * c
+            else:
+                A[i, j] += c
+    return B, A
+
+
+@pytest.mark.parametrize("n", [12, 23, 40])
+@pytest.mark.parametrize("sgn", [0, 1])
+def test_cfd_derivative_equals_dense_solve(n, sgn):
+    d = 3.0
+    rng = np.random.default_rng(n + 7 * sgn)
+    x = rng.standard_normal((5, n))
+    got = oracle.cfd_derive(x, d, sgn, np.float64)
+    B, A = _dense_cfd(n, d, sgn)
+    want = np.linalg.solve(B, A @ x.T).T
+    assert np.allclose(got, want, rtol=1e-9, atol=1e-9)
+
+
+def test_cfd_interior_symbol():
+    ca5 = [0.861085998189822, 0.753782983214427, 0.257072513289755, 0.031326722562826, 0.000410456919800]
+    cb5 = [1.0, 0.753205475977546, 0.312203852849021, 0.063580085296350, 0.004597763362256]
+    ratio = sum((j + 1) * ca5[j] for j in range(5)) / (cb5[0] + 2 * sum(cb5[1:]))
+    assert abs(ratio - 1.0000166) < 1e-6
+    th = np.linspace(0, np.pi, 20001)
+    num = sum(ca5[j] * np.sin((j + 1) * th) for j in range(5))
+    den = cb5[0] + 2 * sum(cb5[j] * np.cos(j * th) for j in range(1, 5))
+    k = num / den
+    assert abs(k.max() - 2.9472) < 5e-4 and abs(th[k.argmax()] / np.pi - 0.946) < 2e-3
+
+
+def test_cfd_derivative_of_boundary_compatible_waves():
+    # gradient plan (sgn 0): the line is extended evenly about both ends, so a
+    # cosine with zero slope at both ends is represented exactly; the divergence
+    # plan (sgn 1) extends oddly, so a sine vanishing at both ends is.
+    n, d = 200, 1.0
+    x = np.arange(n) * d
+    kk = 10 * np.pi / (n - 1)
+    got = oracle.cfd_derive(np.cos(kk * x)[None], d, 0, np.float64)[0]
+    assert np.max(np.abs(got + kk * np.sin(kk * x))) < 1e-4 * kk
+    got = oracle.cfd_derive(np.sin(kk * x)[None], d, 1, np.float64)[0]
+    assert np.max(np.abs(got - kk * np.cos(kk * x))) < 1e-4 * kk
+
+
+# ---- Laplacian / step properties ------------------------------------------------------
+def _engine(nx, ny, nz, order=8, engine=7, dtype=np.float64, nb=(0, 0, 0, 0, 0, 0)):
+    e = oracle.Engine(nx, ny, nz, 10.0, 10.0, 10.0, engine, order, dtype)
+    return e
+
+
+def test_laplacian_constant_field_interior_zero_and_brute_force():
+    nx, ny, nz = 12, 11, 13
+    e = _engine(nx, ny, nz)
+    m = np.ones((ny, nx, nz))
+    e.set_model(m)
+    e.set_boundary([0] * 6, 1e-3, 3000.0, dim_pml=1)
+    u = np.ones((ny, nx, nz))
+    pb, pr = np.zeros_like(u), np.zeros_like(u)
+    e.kernel(1, u, pb, pr)
+    assert np.abs(pr[4:-4, 4:-4, 4:-4]).max() < 1e-12
+    # brute force: dense operator with zero halo on a random field
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((ny, nx, nz))
+    pb, pr = np.zeros_like(u), np.zeros_like(u)
+    e.kernel(1, u, pb, pr)
+    c2 = oracle.taylor_coef2(4) / 100.0
+    want = np.zeros_like(u)
+    up = np.pad(u, 4)
+    for ax in range(3):
+        for j in range(-4, 5):
+            sl = [slice(4, 4 + s) for s in u.shape]
+            sl[ax] = slice(4 + j, 4 + j + u.shape[ax])
+            want += c2[abs(j)] * up[tuple(sl)]
+    assert np.allclose(pr, want, rtol=1e-10, atol=1e-10)
+    assert np.allclose(pb, -want * m, rtol=1e-10, atol=1e-10)
+
+
+def test_step_symmetry_and_time_reversibility():
+    n = 21
+    e = _engine(n, n, n)
+    m = np.full((n, n, n), (2000.0 * 1e-3) ** 2 / 1.0)
+    e.set_model(m)
+    e.set_boundary([0] * 6, 1e-3, 3000.0, dim_pml=1)
+    w1 = np.zeros((n, n, n))
+    w1[10, 10, 10] = 1.0
+    w0 = w1.copy()
+    wr = np.zeros_like(w1)
+    hist = [w0.copy(), w1.copy()]
+    a0, a1 = w0.copy(), w1.copy()
+    for _ in range(6):
+        e.step(1, a1, a0, wr)
+        a0, a1 = a1, a0
+        hist.append(a1.copy())
+    f = a1
+    assert np.allclose(f, f[::-1], atol=1e-12) and np.allclose(f, np.transpose(f, (1, 0, 2)), atol=1e-12)
+    assert np.allclose(f, np.transpose(f, (2, 1, 0)), atol=1e-12)
+    # leapfrog without PML is time reversible: step back from (u_n, u_{n-1})
+    b0, b1 = hist[-1].copy(), hist[-2].copy()
+    for k in range(6):
+        e.step(1, b1, b0, wr)
+        b0, b1 = b1, b0
+    assert np.allclose(b1, hist[1], atol=1e-9) and np.allclose(b0, hist[0], atol=1e-9)
+
+
+def test_pml_absorbs_energy():
+    # same smooth pulse with and without absorbing layers: the zero-halo box keeps
+    # its energy, the PML box loses nearly all of it once the pulse has crossed
+    n = 40
+    m = np.full((n, n, n), (2000.0 * 1e-3) ** 2)
+    from tests.helpers import gaussian_blob
+    blob = gaussian_blob(n, n, n, 20, 20, 20, 2.0).astype(np.float64)
+    final = {}
+    for w in (0, 10):
+        e = _engine(n, n, n)
+        e.set_model(m)
+        e.set_boundary([w] * 6, 1e-3, 2000.0, dim_pml=1)
+        w1, w0, wr = blob.copy(), blob.copy(), np.zeros_like(blob)
+        for it in range(400):
+            e.step(1, w1, w0, wr)
+            w0, w1 = w1, w0
+        final[w] = float((w1 ** 2).sum())
+    assert final[10] < 1e-2 * final[0]
+
+
+def test_fp32_oracle_tracks_fp64():
+    n = 24
+    rng = np.random.default_rng(5)
+    v = 1500 + 1000 * rng.random((n, n, n))
+    m = (v * 1e-3) ** 2
+    res = {}
+    for dt_ in (np.float32, np.float64):
+        e = oracle.Engine(n, n, n, 10.0, 10.0, 10.0, 7, 8, dt_)
+        e.set_model(m)
+        e.set_boundary([5] * 6, 1e-3, 2500.0, dim_pml=1)
+        w1 = np.zeros((n, n, n), dt_)
+        w1[12, 12, 12] = 1
+        w0, wr = w1.copy(), np.zeros_like(w1)
+        for _ in range(20):
+            e.step(1, w1, w0, wr)
+            w0, w1 = w1, w0
+        res[dt_] = w1
+    assert rel_l2(res[np.float32], res[np.float64]) < 1e-5
+
+
+# ---- spread kernels, injection/extraction -------------------------------------------------
+def test_spread_sinc_blackman_dc_gain_and_support():
+    for x in (20.0, 20.3, 64.5):
+        off, k = oracle.spread_coeff(128, x, 0.3, 0.3, 20)
+        assert k.size == 20 and off == int(math.floor(x)) - 9
+        assert abs(k.sum() - 1.0) < 2e-2
+    off, k = oracle.spread_coeff(128, 3.0, 0.3, 0.3, 20)      # near the edge: m = ix
+    assert k.size == 6 and off == 1
+    off, k = oracle.spread_coeff(128, 0.0, 0.3, 0.3, 20)      # on the edge: no taps
+    assert k.size == 0
+    off, k = oracle.spread_coeff(128, 40.0, 0.40, 0.48, 20)   # Ormsby band
+    assert k.size == 20 and abs(k.sum() - 1.0) < 5e-2
+
+
+def test_inject_extract_adjoint():
+    shape = (30, 32, 34)
+    xyz = np.array([[10.0, 12.0, 15.0], [20.5, 7.25, 15.0], [15.0, 15.0, 16.4], [2.0, 28.0, 15.0]], np.float32)
+    t = pipeline.spread_tables(shape[1], shape[0], shape[2], xyz, 5, 0.3, 0.3)
+    rng = np.random.default_rng(1)
+    d = rng.standard_normal(4).astype(np.float32)
+    w = rng.standard_normal(shape).astype(np.float32)
+    one = np.ones(shape, np.float32)
+    z = np.zeros(shape, np.float32)
+    pipeline.inject(z, one, shape, t, d)
+    lhs = float(np.dot(z.ravel().astype(np.float64), w.ravel()))
+    rhs = float(np.dot(d.astype(np.float64), pipeline.extract(w, shape, t)))
+    assert abs(lhs - rhs) < 1e-5 * max(1, abs(lhs))
+
+
+# ---- COMPRESS_16 -------------------------------------------------------------------------
+def test_codec16_table_known_values():
+    t = oracle.codec16_table()
+    assert t[0] == 0 and t[1] == 1 and np.isnan(t[32768])
+    assert t[1023] == np.float32(1023 * 2.0 ** -30) and t[1024] == np.float32(1024 * 2.0 ** -14)
+    assert t[8192] == 0.5 and t[16384] == np.float32(2.0 ** -20)
+    assert t[16384 + 15 * 1024 + 1023] == np.float32(2.0 ** -5 * (1 + 1023 / 1024))
+    assert np.array_equal(t[32769:], -t[1:32768])
+
+
+def test_codec16_roundtrip_bound_and_zero():
+    rng = np.random.default_rng(9)
+    v = rng.standard_normal((45, 44, 30)).astype(np.float32)
+    v[3] = 0.0
+    c = oracle.compress16(v)
+    assert c.size == 45 * (44 * 30 + 2)
+    out = oracle.uncompress16(c, 45, 44, 30)
+    assert not out[3].any()
+    # interior (untapered) samples within the top octave range: relative error <= 2^-10
+    ny, nx, nz = v.shape
+    core = (slice(20, ny - 20), slice(20, nx - 20), slice(10, nz - 10))
+    vm = np.abs(v).max(axis=(1, 2))[:, None, None]
+    big = (np.abs(v) > vm / 32)[core]
+    err = np.abs(out[core] - v[core])[big] / np.abs(v[core])[big]
+    assert err.max() <= 2.0 ** -10 + 1e-7
+    assert out[core][~big].size == 0 or np.abs(out[core] - v[core])[~big].max() <= vm.max() * 2.0 ** -10
+
+
+def test_codec16_shift_known_answers():
+    # 0.5 * vmax -> code 0x2000 (exp=-1 branch), decodes to exactly 0.5*vmax
+    v = np.zeros((1, 4, 4), np.float32)
+    v[0, 0, 0] = 2.0
+    v[0, 1, 1] = 1.0
+    v[0, 2, 2] = -0.5
+    c = oracle.compress16(v)
+    codes = c[2:].reshape(4, 4)
+    assert codes[0, 0] == 0x1 and codes[1, 1] == 0x2000 and codes[2, 2] == 0x8000 | 0x1000
+    assert c[0:2].view(np.float32)[0] == 2.0
