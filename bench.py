@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""bench.py -- RTM propagation-and-imaging throughput on one MI355X.
+
+One benchmark step is one RTM time step of a shot: a forward step of the
+source wavefield (kernel + leapfrog + source injection + snapshot store on
+its cadence) and a backward step of the receiver wavefield (kernel + leapfrog
++ injection of the full surface receiver grid + zero-lag imaging condition on
+the snapshot cadence).  Inputs are resident in HBM when the timed region
+starts.  `value` counts both propagations: 2 * nx*ny*nz cell updates per step.
+
+Default workload (configs[1]-style): VTI pseudo-acoustic RTM with the
+10th-order compact FD engine on a 400^3 grid, dx = 12.5 m, 32-cell PML,
+20 Hz Ricker, 400x400 surface receivers, FP32 snapshots every 4 steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s + RTM shots/hour, 1 GPU, learned/batched vs plain at equal NRMSE"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: grid (ny, nx, nz), spacing, engine, order, model, PML, Ricker fpeak, snapshot cadence, record length
+    "vti-cfd10-400": dict(shape=(400, 400, 400), d=12.5, engine="CFD", order=10, model="VTI", nbnd=32, fpeak=20.0,
+                          snap_every=4, record_s=2.0, seed=202),
+    "iso-fd8-400": dict(shape=(400, 400, 400), d=12.5, engine="FD", order=8, model="ISO", nbnd=32, fpeak=15.0,
+                        snap_every=4, record_s=2.0, seed=505),
+    "iso-fd8-128": dict(shape=(128, 128, 128), d=10.0, engine="FD", order=8, model="ISO", nbnd=20, fpeak=15.0,
+                        snap_every=1, record_s=0.6, seed=101, dt=1e-3),
+}
+# algorithmic HBM bytes per cell update of the step (state + model arrays read once, new field written once)
+VOL_BYTES = {"ISO": 16, "VTI": 24, "TTI": 32}
+
+
+def build_case(name):
+    from konglong_amd import models
+    from konglong_amd._lib import ENGINE_FD, ENGINE_CFD, ISO, VTI, TTI
+    w = dict(WORKLOADS[name])
+    ny, nx, nz = w["shape"]
+    eng = ENGINE_CFD if w["engine"] == "CFD" else ENGINE_FD
+    mdl = {"ISO": ISO, "VTI": VTI, "TTI": TTI}[w["model"]]
+    eps = dl = None
+    if mdl == ISO and nz == 128:
+        vel = models.layered_model(nz, w["d"], w["seed"])
+    elif mdl == ISO:
+        vel = models.smooth_vti_model(w["shape"], 20.0, w["seed"])[0]
+    else:
+        vel, eps, dl = models.smooth_vti_model(w["shape"], 20.0, w["seed"])
+    dt = w.get("dt") or models.cfl_dt(vel, w["d"], eng, w["order"], mdl, eps, dl)
+    return w, eng, mdl, vel, eps, dl, dt
+
+
+def parse_traffic(name):
+    """HBM bytes per step from the committed rocprofv3 summary of this workload (or None)."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except ValueError:
+                pass
+    if "fetch_bytes_per_step" in vals and "write_bytes_per_step" in vals:
+        return vals["fetch_bytes_per_step"] + vals["write_bytes_per_step"]
+    return None
+
+
+def cpu_baseline(vel, eps, dl, w, eng, mdl, dt, budget_s):
+    """The FP32 CPU oracle (reference-structured, OpenMP) timed on this host for a bounded sample."""
+    import oracle
+    ny, nx, nz = w["shape"]
+    threads = max(1, min(16, os.cpu_count() or 1))
+    m = (vel * vel * (np.float32(dt) * np.float32(dt))).astype(np.float32)
+    o = oracle.Engine(nx, ny, nz, w["d"], w["d"], w["d"], eng, w["order"], np.float32, threads=threads)
+    o.set_model(m, eps, dl)
+    o.set_boundary([w["nbnd"]] * 6, dt, float(vel.max()), dim_pml=3)
+    rng = np.random.default_rng(1)
+    w1 = (1e-3 * rng.standard_normal(w["shape"])).astype(np.float32)
+    w0, wr = w1.copy(), np.zeros_like(w1)
+    t = time.perf_counter()
+    o.step(mdl, w1, w0, wr)
+    one = time.perf_counter() - t
+    n = max(1, min(20, int(budget_s / max(one, 1e-3))))
+    t = time.perf_counter()
+    for _ in range(n):
+        o.step(mdl, w1, w0, wr)
+        w0, w1 = w1, w0
+    el = time.perf_counter() - t
+    o.close()
+    return {"value": round(nx * ny * nz * n / el / 1e9, 5), "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} {w['model']}-{w['engine']}{w['order']} time steps (kernel + leapfrog, PML) on the "
+                      f"{ny}x{nx}x{nz} model, FP32, {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--workload", default="vti-cfd10-400", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
+              f"(requested --gpus {args.gpus}, WORLD_SIZE {world}); multi-GPU shot sharding is described "
+              f"in DESIGN.md but not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes as C
+    from konglong_amd import _lib
+    from konglong_amd import models
+    from konglong_amd.propagator import Propagator, shot_config
+    from konglong_amd._lib import SNAP_FP32
+
+    L = _lib.lib()
+    if _lib.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    w, eng, mdl, vel, eps, dl, dt = build_case(args.workload)
+    ny, nx, nz = w["shape"]
+    ncell = nx * ny * nz
+    nb = w["nbnd"]
+    nt_shot = int(round(w["record_s"] / dt))
+    K, W = args.steps, args.warmup
+    nt = W + K + 2
+    iz_rcv = nb + 2 if nb > 0 else 2
+    rcv = models.surface_receivers(nx, ny, iz_rcv)
+    kband = (0.40, 0.48) if eng == _lib.ENGINE_CFD else (0.3, 0.3)
+    cfg = shot_config(nx, ny, nz, w["d"], dt, nt, eng, w["order"], mdl, nbnd=nb, vmax=float(vel.max()),
+                      src_xyz=((nx - 1) / 2, (ny - 1) / 2, float(iz_rcv)), f_peak=w["fpeak"], spread=10,
+                      khpass=kband[0], khcut=kband[1], nrcv=rcv.shape[0], snap_every=w["snap_every"],
+                      snap_format=SNAP_FP32, tmute_frac=0.1, fuse_step=1, snap_slots=16)
+    p = Propagator(cfg)
+    p.set_model(vel, eps, dl)
+    p.set_receivers(rcv)
+    rng = np.random.default_rng(7)
+    p.set_record((1e-3 * rng.standard_normal((rcv.shape[0], nt))).astype(np.float32))
+
+    # warm-up: forward then backward
+    p.forward_steps(0, W, store_snapshots=True)
+    p.backward_steps(W - 1, W, image=True)
+    _lib.synchronize()
+    # timed region: K forward + K backward steps; HIP events around every launch
+    _lib.check(L.klrtm_timer_reset())
+    _lib.check(L.klrtm_timer_enable(1))
+    _lib.synchronize()
+    t0 = time.perf_counter()
+    p.forward_steps(W, K, store_snapshots=True)
+    p.backward_steps(W + K - 1, K, image=True)
+    _lib.synchronize()
+    el = time.perf_counter() - t0
+    _lib.check(L.klrtm_timer_enable(0))
+
+    names = ["iso_step", "iso_kernel", "update2nd", "fd_grad", "fd_div", "fd_div_step", "cfd_grad_y", "cfd_grad_z",
+             "cfd_grad_x", "cfd_div_x", "cfd_div_z", "cfd_div_y", "inject", "extract", "c16_encode", "blockq_encode",
+             "ic_correlate"]
+    kern = {}
+    tot, cnt = C.c_double(), C.c_long()
+    for k in names:
+        L.klrtm_timer_read(k.encode(), C.byref(tot), C.byref(cnt))
+        if cnt.value:
+            kern[k] = {"mean_ms": round(tot.value / cnt.value, 4), "launches": cnt.value}
+
+    ms_step = el / K * 1e3
+    value = 2.0 * ncell * K / el / 1e9
+    # dominant kernel: the propagation step (one fused launch for FD ISO, the line-solve passes for CFD)
+    if eng == _lib.ENGINE_CFD:
+        group = [k for k in kern if k.startswith("cfd_")]
+        kname = "cfd_vti_step" if mdl == _lib.VTI else "cfd_step"
+    elif mdl == _lib.ISO:
+        group = ["iso_step"]
+        kname = "iso_step"
+    else:
+        group = [k for k in ("fd_grad", "fd_div_step") if k in kern]
+        kname = "fd_step"
+    step_ms = sum(kern[k]["mean_ms"] for k in group if k in kern)
+    # PML state bytes per step: 2nd-order form: q1,q2 read+write per face cell (16 B);
+    # 1st-order form: 3 derivative components x 2 rounds x (read+write) = 48 B per face cell
+    face_cells = 2 * nb * (nx * ny + ny * nz + nx * nz) if nb > 1 else 0
+    pml_b = (16 if (eng == _lib.ENGINE_FD and mdl == _lib.ISO) else 48) * face_cells
+    alg_bytes = VOL_BYTES[w["model"]] * ncell + pml_b
+    achieved = alg_bytes / (step_ms * 1e-3) / 1e9 if step_ms > 0 else 0.0
+    traffic = parse_traffic(args.workload)
+    roofline = {"kernel": kname, "launches_per_step": len(group), "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "algorithmic_bytes_per_step": alg_bytes, "step_kernel_ms": round(step_ms, 4)}
+    shots_per_hour = 3600.0 / (nt_shot * el / K)
+    cpu = None
+    if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
+        cpu = cpu_baseline(vel, eps, dl, w, eng, mdl, dt, args.cpu_budget)
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": args.workload, "grid": [nx, ny, nz], "spacing_m": w["d"], "engine": w["engine"],
+                   "order": w["order"], "model": w["model"], "pml": nb, "dt_s": round(dt, 7), "nt_shot": nt_shot,
+                   "receivers": int(rcv.shape[0]), "snap_every": w["snap_every"], "snap_format": "fp32",
+                   "step": "1 forward + 1 backward RTM time step (injection, snapshot, imaging on cadence)"},
+        "shots_per_hour": round(shots_per_hour, 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kern,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -167,6 +167,7 @@ typedef struct {
   float tmute_frac;       /* IC time taper length as a fraction of the record (tmutePerc) */
   int ntaper_z;           /* IC water-depth taper cells */
   int fuse_step;          /* 1: fused kernel+update2nd */
+  int snap_slots;         /* > 0: snapshot slots kept (ring for benchmarks); 0: all */
 } klrtm_shot_config;
 
 typedef struct klrtm_propagator klrtm_propagator;
@@ -188,8 +189,13 @@ int klrtm_propagator_rtm(klrtm_propagator *p, const float *record_host, float *i
 int klrtm_propagator_timings(klrtm_propagator *p, double *fwd_ms, double *bwd_ms, double *ic_ms);
 /* device pointers of the current wavefields (w0, w1) for inspection */
 int klrtm_propagator_wavefields(klrtm_propagator *p, float **w0, float **w1);
-/* run n plain forward steps with source injection only (benchmark helper); it0 = first step index */
+/* forward steps it0..it0+nsteps-1 with source injection; stored steps go to slot (it/k) mod slots */
 int klrtm_propagator_forward_steps(klrtm_propagator *p, int it0, int nsteps, int store_snapshots);
+/* backward steps it0, it0-1, ... (nsteps of them) injecting the receiver traces at it+1 and, when
+ * image != 0, correlating with the stored snapshot of every k-th step (slot (it/k) mod slots) */
+int klrtm_propagator_backward_steps(klrtm_propagator *p, int it0, int nsteps, int image);
+/* receiver traces for the backward run, host [nrcv][nt] */
+int klrtm_propagator_set_record(klrtm_propagator *p, const float *record_host);
 /* reset wavefields and PML state */
 int klrtm_propagator_reset(klrtm_propagator *p);
 

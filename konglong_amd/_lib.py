@@ -37,6 +37,7 @@ class ShotConfig(C.Structure):
         ("nrcv", _i),
         ("snap_every", _i), ("snap_format", _i), ("rec_through_codec", _i),
         ("tmute_frac", _f), ("ntaper_z", _i), ("fuse_step", _i),
+        ("snap_slots", _i),
     ]
 
 
@@ -98,6 +99,8 @@ _SIGS = {
     "klrtm_propagator_wavefields": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
     "klrtm_propagator_forward_steps": (_i, [_vp, _i, _i, _i]),
     "klrtm_propagator_reset": (_i, [_vp]),
+    "klrtm_propagator_backward_steps": (_i, [_vp, _i, _i, _i]),
+    "klrtm_propagator_set_record": (_i, [_vp, _pf]),
 }
 
 _lib = None

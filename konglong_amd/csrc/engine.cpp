@@ -43,6 +43,7 @@ void pml_profile(int npml, float dx, float dt, float vmax, float pml_eps, float 
 
 Engine::~Engine() {
   for(int f = 0; f < 6; f++) (void)hipFree(d_prof[f]);
+  (void)hipFree(d_abc);
   (void)hipFree(d_state);
   (void)hipFree(d_px);
   (void)hipFree(d_pz);
@@ -116,6 +117,31 @@ int Engine::set_boundary(const int *nbnd, float dt_, float vmax_) {
       KL_HIP(hipMemcpyAsync(d_prof[f], host_prof[f].data(), sizeof(float) * nbnd[f], hipMemcpyHostToDevice, stream()));
       desc.g[f] = d_prof[f];
     } else host_prof[f].clear();
+  }
+  // per-position round coefficients, scale 1 and pml3d_scaler (PML.cpp:62-70 factors)
+  (void)hipFree(d_abc);
+  d_abc = nullptr;
+  {
+    std::vector<float> tab;
+    size_t off[6][2];
+    for(int f = 0; f < 6; f++)
+      for(int si = 0; si < 2; si++) {
+        off[f][si] = tab.size();
+        float sc = si ? pml3d_scaler : 1.0f;
+        for(int j = 0; j < wid[f]; j++) {
+          float g = host_prof[f][j] * sc;
+          tab.push_back((1.0f - g) / (1.0f + g));
+          tab.push_back(g * 2.0f / (1.0f - g * g));
+          tab.push_back(1.0f / (1.0f - g));
+        }
+      }
+    if(!tab.empty()) {
+      KL_HIP(hipMalloc(&d_abc, tab.size() * sizeof(float)));
+      KL_HIP(hipMemcpyAsync(d_abc, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, stream()));
+    }
+    for(int f = 0; f < 6; f++)
+      for(int si = 0; si < 2; si++) desc.abc[f][si] = wid[f] ? d_abc + off[f][si] : nullptr;
+    KL_HIP(hipStreamSynchronize(stream()));
   }
   desc.scaler = pml3d_scaler;
   desc.pml3d = pml3d;

@@ -21,6 +21,7 @@ struct Engine {
   int wid[6] = {0, 0, 0, 0, 0, 0};
   std::vector<float> host_prof[6];
   float *d_prof[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  float *d_abc = nullptr;  // precomputed PML round coefficients (see PmlDesc::abc)
   float *d_state = nullptr;
   size_t state_floats = 0;
   int state_comps = 0;

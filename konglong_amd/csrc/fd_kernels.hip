@@ -21,64 +21,101 @@
 namespace klrtm {
 
 // ---------------------------------------------------------------------------
+// Common tile bookkeeping and the one-plane-ahead prefetch of a cross-shaped
+// halo (z halo of the own row, x halo of the own column).
+struct TileCoord {
+  int tz, tx, iz, ix, y0, y1;
+  bool act;
+  size_t col;
+};
+__device__ __forceinline__ TileCoord tile_coord(const IsoArgs &a) {
+  TileCoord c;
+  int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = t % a.nbx;
+  t /= a.nbx;
+  const int bz = t % a.nbz;
+  const int by = t / a.nbz;
+  c.tz = threadIdx.x % a.tz, c.tx = threadIdx.x / a.tz;
+  c.iz = bz * a.tz + c.tz, c.ix = bx * a.tx + c.tx;
+  c.y0 = by * a.ychunk, c.y1 = min(a.ny, c.y0 + a.ychunk);
+  c.act = c.iz < a.nz && c.ix < a.nx;
+  c.col = (size_t)c.ix * a.nz + c.iz;
+  return c;
+}
+struct Halo {
+  float z0, z1, x0, x1;
+};
+template<int R>
+__device__ __forceinline__ Halo load_halo(const float *__restrict__ u, size_t pl, const TileCoord &c, const IsoArgs &a,
+                                          bool zh, bool xh) {
+  Halo h = {0.f, 0.f, 0.f, 0.f};
+  if(zh) {
+    if(c.tz < R && c.ix < a.nx && c.iz - R >= 0) h.z0 = u[pl + c.col - R];
+    if(c.tz >= a.tz - R && c.ix < a.nx && c.iz + R < a.nz) h.z1 = u[pl + c.col + R];
+  }
+  if(xh) {
+    if(c.tx < R && c.iz < a.nz && c.ix - R >= 0) h.x0 = u[pl + c.col - (size_t)R * a.nz];
+    if(c.tx >= a.tx - R && c.iz < a.nz && c.ix + R < a.nx) h.x1 = u[pl + c.col + (size_t)R * a.nz];
+  }
+  return h;
+}
+
+// ---------------------------------------------------------------------------
 // ISO Laplacian step
 template<int R, int MODE>
 __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict__ u, float *__restrict__ w0,
                                                         const float *__restrict__ vel, float *__restrict__ pr,
                                                         const IsoArgs a) {
   extern __shared__ float s[];
-  const int TZ = a.tz, TX = a.tx, S = TZ + 2 * R;
+  const int S = a.tz + 2 * R;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t nxz = (size_t)nx * nz;
-  int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int bx = t % a.nbx;
-  t /= a.nbx;
-  const int bz = t % a.nbz;
-  const int by = t / a.nbz;
-  const int tz = threadIdx.x % TZ, tx = threadIdx.x / TZ;
-  const int iz = bz * TZ + tz, ix = bx * TX + tx;
-  const int y0 = by * a.ychunk, y1 = min(ny, y0 + a.ychunk);
-  const bool act = iz < nz && ix < nx;
-  const size_t col = (size_t)ix * nz + iz;
-
+  const TileCoord c = tile_coord(a);
+  const int ix = c.ix, iz = c.iz;
   float q[2 * R + 1];
 #pragma unroll
   for(int k = 0; k < 2 * R; k++) {
-    int yy = y0 - R + k;
-    q[k] = (act && yy >= 0 && yy < ny) ? u[(size_t)yy * nxz + col] : 0.f;
+    int yy = c.y0 - R + k;
+    q[k] = (c.act && yy >= 0 && yy < ny) ? u[(size_t)yy * nxz + c.col] : 0.f;
   }
-  float *srow = s + (tx + R) * S + tz + R;
-
-  for(int iy = y0; iy < y1; iy++) {
+  float *srow = s + (c.tx + R) * S + c.tz + R;
+  // prefetch of plane y0
+  float qn = 0.f, wn = 0.f, vn = 0.f;
+  Halo hn = load_halo<R>(u, (size_t)c.y0 * nxz, c, a, true, true);
+  if(c.act) {
+    if(c.y0 + R < ny) qn = u[(size_t)(c.y0 + R) * nxz + c.col];
+    wn = w0[(size_t)c.y0 * nxz + c.col], vn = vel[(size_t)c.y0 * nxz + c.col];
+  }
+  for(int iy = c.y0; iy < c.y1; iy++) {
     const size_t pl = (size_t)iy * nxz;
-    {
-      int yy = iy + R;
-      q[2 * R] = (act && yy < ny) ? u[(size_t)yy * nxz + col] : 0.f;
+    q[2 * R] = qn;
+    const Halo h = hn;
+    const float wv = wn, vv = vn;
+    if(iy + 1 < c.y1) {  // issue next plane's loads; they stay in flight across the barriers
+      const size_t pn = pl + nxz;
+      hn = load_halo<R>(u, pn, c, a, true, true);
+      if(c.act) {
+        qn = (iy + 1 + R < ny) ? u[pn + (size_t)R * nxz + c.col] : 0.f;
+        wn = w0[pn + c.col], vn = vel[pn + c.col];
+      }
     }
-    float hz0 = 0.f, hz1 = 0.f, hx0 = 0.f, hx1 = 0.f;
-    if(tz < R && ix < nx && iz - R >= 0) hz0 = u[pl + col - R];
-    if(tz >= TZ - R && ix < nx && iz + R < nz) hz1 = u[pl + (size_t)ix * nz + iz + R];
-    if(tx < R && iz < nz && ix - R >= 0) hx0 = u[pl + (size_t)(ix - R) * nz + iz];
-    if(tx >= TX - R && iz < nz && ix + R < nx) hx1 = u[pl + (size_t)(ix + R) * nz + iz];
-    float wv = 0.f, vv = 0.f;
-    if(act) wv = w0[pl + col], vv = vel[pl + col];
-    __syncthreads();
+    lds_barrier();
     srow[0] = q[R];
-    if(tz < R) srow[-R] = hz0;
-    if(tz >= TZ - R) srow[R] = hz1;
-    if(tx < R) srow[-R * S] = hx0;
-    if(tx >= TX - R) srow[R * S] = hx1;
-    __syncthreads();
-    if(act) {
-      const float c = q[R];
+    if(c.tz < R) srow[-R] = h.z0;
+    if(c.tz >= a.tz - R) srow[R] = h.z1;
+    if(c.tx < R) srow[-R * S] = h.x0;
+    if(c.tx >= a.tx - R) srow[R * S] = h.x1;
+    lds_barrier();
+    if(c.act) {
+      const float ctr = q[R];
       // z part, reference order m = -R..-1, 1..R (bgprtm/FdEngine_fd.cpp:223-236)
-      float lz = a.cz[0] * c;
+      float lz = a.cz[0] * ctr;
 #pragma unroll
       for(int m = R; m >= 1; m--) lz = fmaf(a.cz[m], srow[-m], lz);
 #pragma unroll
       for(int m = 1; m <= R; m++) lz = fmaf(a.cz[m], srow[m], lz);
       // x part (bgprtm/FdEngine_fd.cpp:262-274)
-      float lx = a.cx[0] * c;
+      float lx = a.cx[0] * ctr;
 #pragma unroll
       for(int m = 1; m <= R; m++) {
         lx = fmaf(a.cx[m], srow[-m * S], lx);
@@ -86,7 +123,7 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
       }
       float L = lz + lx;
       if(a.do_y) {
-        float ly = a.cy[0] * c;
+        float ly = a.cy[0] * ctr;
 #pragma unroll
         for(int m = 1; m <= R; m++) {
           ly = fmaf(a.cy[m], q[R - m], ly);
@@ -103,20 +140,20 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
           int f = faces[k];
           int j = face_pos(P, f, ix, iy, iz, nx, ny, nz);
           if(j >= 0) {
-            float g = P.g[f][j];
+            const float *t = P.abc[f][0] + 3 * j;
             size_t st = state_index(f, P.w[f], j, ix, iy, iz, nx, nz);
             float q1 = P.q[f][0][0][st], q2 = P.q[f][1][0][st];
-            L = pml_round(g, L, q1);
-            L = pml_round(g, L, q2);
+            L = pml_round_abc(t, L, q1);
+            L = pml_round_abc(t, L, q2);
             P.q[f][0][0][st] = q1, P.q[f][1][0][st] = q2;
           }
         }
       }
       if(MODE == MODE_STEP) {
-        w0[pl + col] = (c + c) - fmaf(-vv, L, wv);
+        w0[pl + c.col] = (ctr + ctr) - fmaf(-vv, L, wv);
       } else {
-        pr[pl + col] = L;
-        w0[pl + col] = fmaf(-L, vv, wv);
+        pr[pl + c.col] = L;
+        w0[pl + c.col] = fmaf(-L, vv, wv);
       }
     }
 #pragma unroll
@@ -133,48 +170,42 @@ __global__ void __launch_bounds__(1024) grad_kernel(const float *__restrict__ u,
                                                     const float *__restrict__ pjx, const float *__restrict__ pjy,
                                                     const IsoArgs a) {
   extern __shared__ float s[];
-  const int TZ = a.tz, TX = a.tx, S = TZ + 2 * R;
+  const int S = a.tz + 2 * R;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t nxz = (size_t)nx * nz;
-  int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int bx = t % a.nbx;
-  t /= a.nbx;
-  const int bz = t % a.nbz;
-  const int by = t / a.nbz;
-  const int tz = threadIdx.x % TZ, tx = threadIdx.x / TZ;
-  const int iz = bz * TZ + tz, ix = bx * TX + tx;
-  const int y0 = by * a.ychunk, y1 = min(ny, y0 + a.ychunk);
-  const bool act = iz < nz && ix < nx;
-  const size_t col = (size_t)ix * nz + iz;
+  const TileCoord c = tile_coord(a);
+  const int ix = c.ix, iz = c.iz;
   const bool do_y = ny > 1;
-
   float q[2 * R + 1];
 #pragma unroll
   for(int k = 0; k < 2 * R; k++) {
-    int yy = y0 - R + k;
-    q[k] = (act && yy >= 0 && yy < ny) ? u[(size_t)yy * nxz + col] : 0.f;
+    int yy = c.y0 - R + k;
+    q[k] = (c.act && yy >= 0 && yy < ny) ? u[(size_t)yy * nxz + c.col] : 0.f;
   }
-  float *srow = s + (tx + R) * S + tz + R;
-
-  for(int iy = y0; iy < y1; iy++) {
+  float *srow = s + (c.tx + R) * S + c.tz + R;
+  float qn = 0.f;
+  Halo hn = load_halo<R>(u, (size_t)c.y0 * nxz, c, a, true, true);
+  if(c.act && c.y0 + R < ny) qn = u[(size_t)(c.y0 + R) * nxz + c.col];
+  for(int iy = c.y0; iy < c.y1; iy++) {
     const size_t pl = (size_t)iy * nxz;
-    {
-      int yy = iy + R;
-      q[2 * R] = (act && yy < ny) ? u[(size_t)yy * nxz + col] : 0.f;
+    q[2 * R] = qn;
+    const Halo h = hn;
+    float e = 0.f, d = 0.f, ax = 0.f, ay = 0.f;
+    if(c.act && MODEL != KLRTM_ISO) e = eps[pl + c.col], d = del[pl + c.col];
+    if(c.act && MODEL == KLRTM_TTI) ax = pjx[pl + c.col], ay = do_y ? pjy[pl + c.col] : 0.f;
+    if(iy + 1 < c.y1) {
+      const size_t pn = pl + nxz;
+      hn = load_halo<R>(u, pn, c, a, true, true);
+      if(c.act) qn = (iy + 1 + R < ny) ? u[pn + (size_t)R * nxz + c.col] : 0.f;
     }
-    float hz0 = 0.f, hz1 = 0.f, hx0 = 0.f, hx1 = 0.f;
-    if(tz < R && ix < nx && iz - R >= 0) hz0 = u[pl + col - R];
-    if(tz >= TZ - R && ix < nx && iz + R < nz) hz1 = u[pl + (size_t)ix * nz + iz + R];
-    if(tx < R && iz < nz && ix - R >= 0) hx0 = u[pl + (size_t)(ix - R) * nz + iz];
-    if(tx >= TX - R && iz < nz && ix + R < nx) hx1 = u[pl + (size_t)(ix + R) * nz + iz];
-    __syncthreads();
+    lds_barrier();
     srow[0] = q[R];
-    if(tz < R) srow[-R] = hz0;
-    if(tz >= TZ - R) srow[R] = hz1;
-    if(tx < R) srow[-R * S] = hx0;
-    if(tx >= TX - R) srow[R * S] = hx1;
-    __syncthreads();
-    if(act) {
+    if(c.tz < R) srow[-R] = h.z0;
+    if(c.tz >= a.tz - R) srow[R] = h.z1;
+    if(c.tx < R) srow[-R * S] = h.x0;
+    if(c.tx >= a.tx - R) srow[R * S] = h.x1;
+    lds_barrier();
+    if(c.act) {
       // z: vz accumulates (-c_m) u[-m] for m=R..1, then c_m u[+m] (bgprtm/FdEngine_fd.cpp:394-403)
       float gz = 0.f;
 #pragma unroll
@@ -197,13 +228,13 @@ __global__ void __launch_bounds__(1024) grad_kernel(const float *__restrict__ u,
         }
       }
       if(a.has_pml) {
-        gz = pml1_z(a.pml, 0, gz, ix, iy, iz, nx, ny, nz);
-        gx = pml1_x(a.pml, 0, gx, ix, iy, iz, nx, ny, nz);
-        if(do_y) gy = pml1_y(a.pml, 0, gy, ix, iy, iz, nx, ny, nz);
+        gz = pml1_z<true>(a.pml, 0, gz, ix, iy, iz, nx, ny, nz);
+        gx = pml1_x<true>(a.pml, 0, gx, ix, iy, iz, nx, ny, nz);
+        if(do_y) gy = pml1_y<true>(a.pml, 0, gy, ix, iy, iz, nx, ny, nz);
       }
-      const size_t i = pl + col;
-      if(MODEL == KLRTM_VTI) scalar_vti(eps[i], del[i], gx, gy, gz);
-      else if(MODEL == KLRTM_TTI) scalar_tti(eps[i], del[i], pjx[i], do_y ? pjy[i] : 0.f, gx, gy, gz);
+      const size_t i = pl + c.col;
+      if(MODEL == KLRTM_VTI) scalar_vti(e, d, gx, gy, gz);
+      else if(MODEL == KLRTM_TTI) scalar_tti(e, d, ax, ay, gx, gy, gz);
       px[i] = gx;
       if(do_y) py[i] = gy;
       pz[i] = gz;
@@ -225,57 +256,49 @@ __global__ void __launch_bounds__(1024) div_kernel(const float *__restrict__ px,
   extern __shared__ float s[];
   const int TZ = a.tz, TX = a.tx;
   const int SZ = TZ + 2 * R;  // row stride of the z-haloed tile (pz)
-  float *sz = s;                          // TX rows x (TZ+2R)
-  float *sx = s + TX * SZ;                // (TX+2R) rows x TZ
+  float *sz = s;                // TX rows x (TZ+2R)
+  float *sx = s + TX * SZ;      // (TX+2R) rows x TZ
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t nxz = (size_t)nx * nz;
-  int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int bx = t % a.nbx;
-  t /= a.nbx;
-  const int bz = t % a.nbz;
-  const int by = t / a.nbz;
-  const int tz = threadIdx.x % TZ, tx = threadIdx.x / TZ;
-  const int iz = bz * TZ + tz, ix = bx * TX + tx;
-  const int y0 = by * a.ychunk, y1 = min(ny, y0 + a.ychunk);
-  const bool act = iz < nz && ix < nx;
-  const size_t col = (size_t)ix * nz + iz;
+  const TileCoord c = tile_coord(a);
+  const int ix = c.ix, iz = c.iz;
   const bool do_y = ny > 1;
-
   float q[2 * R + 1];
 #pragma unroll
   for(int k = 0; k < 2 * R; k++) {
-    int yy = y0 - R + k;
-    q[k] = (do_y && act && yy >= 0 && yy < ny) ? py[(size_t)yy * nxz + col] : 0.f;
+    int yy = c.y0 - R + k;
+    q[k] = (do_y && c.act && yy >= 0 && yy < ny) ? py[(size_t)yy * nxz + c.col] : 0.f;
   }
-  float *zrow = sz + tx * SZ + tz + R;
-  float *xrow = sx + (tx + R) * TZ + tz;
-
-  for(int iy = y0; iy < y1; iy++) {
+  float *zrow = sz + c.tx * SZ + c.tz + R;
+  float *xrow = sx + (c.tx + R) * TZ + c.tz;
+  auto fetch = [&](size_t p, float &qv, float &cz, float &cx, Halo &hz, Halo &hx, float &wv, float &vv, float &uv, int iyp) {
+    hz = load_halo<R>(pz, p, c, a, true, false);
+    hx = load_halo<R>(px, p, c, a, false, true);
+    if(c.act) {
+      qv = (do_y && iyp + R < ny) ? py[p + (size_t)R * nxz + c.col] : 0.f;
+      cz = pz[p + c.col], cx = px[p + c.col];
+      wv = w0[p + c.col], vv = vel[p + c.col];
+      if(MODE == MODE_STEP) uv = w1[p + c.col];
+    }
+  };
+  float qn = 0.f, czn = 0.f, cxn = 0.f, wn = 0.f, vn = 0.f, un = 0.f;
+  Halo hzn, hxn;
+  fetch((size_t)c.y0 * nxz, qn, czn, cxn, hzn, hxn, wn, vn, un, c.y0);
+  for(int iy = c.y0; iy < c.y1; iy++) {
     const size_t pl = (size_t)iy * nxz;
-    {
-      int yy = iy + R;
-      q[2 * R] = (do_y && act && yy < ny) ? py[(size_t)yy * nxz + col] : 0.f;
-    }
-    float cz = 0.f, cx = 0.f, hz0 = 0.f, hz1 = 0.f, hx0 = 0.f, hx1 = 0.f;
-    if(act) cz = pz[pl + col], cx = px[pl + col];
-    if(tz < R && ix < nx && iz - R >= 0) hz0 = pz[pl + col - R];
-    if(tz >= TZ - R && ix < nx && iz + R < nz) hz1 = pz[pl + (size_t)ix * nz + iz + R];
-    if(tx < R && iz < nz && ix - R >= 0) hx0 = px[pl + (size_t)(ix - R) * nz + iz];
-    if(tx >= TX - R && iz < nz && ix + R < nx) hx1 = px[pl + (size_t)(ix + R) * nz + iz];
-    float wv = 0.f, vv = 0.f, uv = 0.f;
-    if(act) {
-      wv = w0[pl + col], vv = vel[pl + col];
-      if(MODE == MODE_STEP) uv = w1[pl + col];
-    }
-    __syncthreads();
+    q[2 * R] = qn;
+    const float cz = czn, cx = cxn, wv = wn, vv = vn, uv = un;
+    const Halo hz = hzn, hx = hxn;
+    if(iy + 1 < c.y1) fetch(pl + nxz, qn, czn, cxn, hzn, hxn, wn, vn, un, iy + 1);
+    lds_barrier();
     zrow[0] = cz;
-    if(tz < R) zrow[-R] = hz0;
-    if(tz >= TZ - R) zrow[R] = hz1;
+    if(c.tz < R) zrow[-R] = hz.z0;
+    if(c.tz >= TZ - R) zrow[R] = hz.z1;
     xrow[0] = cx;
-    if(tx < R) xrow[-R * TZ] = hx0;
-    if(tx >= TX - R) xrow[R * TZ] = hx1;
-    __syncthreads();
-    if(act) {
+    if(c.tx < R) xrow[-R * TZ] = hx.x0;
+    if(c.tx >= TX - R) xrow[R * TZ] = hx.x1;
+    lds_barrier();
+    if(c.act) {
       float dxv = 0.f;
 #pragma unroll
       for(int m = 1; m <= R; m++) {
@@ -296,15 +319,15 @@ __global__ void __launch_bounds__(1024) div_kernel(const float *__restrict__ px,
         }
       }
       if(a.has_pml) {
-        dxv = pml1_x(a.pml, 1, dxv, ix, iy, iz, nx, ny, nz);
-        dzv = pml1_z(a.pml, 1, dzv, ix, iy, iz, nx, ny, nz);
-        if(do_y) dyv = pml1_y(a.pml, 1, dyv, ix, iy, iz, nx, ny, nz);
+        dxv = pml1_x<true>(a.pml, 1, dxv, ix, iy, iz, nx, ny, nz);
+        dzv = pml1_z<true>(a.pml, 1, dzv, ix, iy, iz, nx, ny, nz);
+        if(do_y) dyv = pml1_y<true>(a.pml, 1, dyv, ix, iy, iz, nx, ny, nz);
       }
       float w = fmaf(-dxv, vv, wv);
       w = fmaf(-dzv, vv, w);
       if(do_y) w = fmaf(-dyv, vv, w);
       if(MODE == MODE_STEP) w = (uv + uv) - w;
-      w0[pl + col] = w;
+      w0[pl + c.col] = w;
     }
 #pragma unroll
     for(int k = 0; k < 2 * R; k++) q[k] = q[k + 1];

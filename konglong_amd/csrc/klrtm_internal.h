@@ -51,6 +51,8 @@ struct PmlDesc {
   const float *g[6];     // damping profile, index = position inside the layer
                          // (top faces: coef[j], bottom faces: feoc[j])
   float *q[6][2][3];     // state [face][round][component], see state_index()
+  const float *abc[6][2];  // per position j: a=(1-g)/(1+g), b=2g/(1-g^2), c=1/(1-g) at
+                           // [j*3..j*3+2] for g scaled by 1 ([0]) or by pml3d_scaler ([1])
   float scaler;          // pml3d_scaler for cross-axis damping
   int pml3d;
 };
@@ -78,6 +80,24 @@ __device__ __forceinline__ float pml_round(float g, float p, float &q) {
   float qi = (1.0f - g) / (1.0f + g) * q - g * 2.0f / (1.0f - g * g) * p;
   q = qi;
   return qi + p / (1.0f - g);
+}
+
+// The same round with the per-position coefficients precomputed on the host
+// (p/(1-g) becomes p*c); used by the FD kernels.
+__device__ __forceinline__ float pml_round_abc(const float *t, float p, float &q) {
+  float qi = t[0] * q - t[1] * p;
+  q = qi;
+  return qi + p * t[2];
+}
+
+// Workgroup barrier for LDS hand-off only: waits for this wave's LDS accesses
+// (lgkmcnt) but not for outstanding global loads, so prefetches issued before
+// the barrier stay in flight (__syncthreads would drain them).
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // gfx9 encoding: lgkmcnt(0), vmcnt/expcnt untouched
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
 }
 
 // State offset of cell (ix,iy,iz) inside face f's layer (reference layout,
@@ -108,6 +128,17 @@ __device__ __forceinline__ int face_pos(const PmlDesc &p, int f, int ix, int iy,
 }
 
 // single-round PML of one face on sample p (first-derivative form, FdEngine::pml1_*)
+// FAST selects the precomputed-coefficient round; si = 0 own axis, 1 cross axis.
+__device__ __forceinline__ float pml1_face_fast(const PmlDesc &P, int f, int round, int comp, int si, float p, int ix,
+                                                int iy, int iz, int nx, int ny, int nz) {
+  int j = face_pos(P, f, ix, iy, iz, nx, ny, nz);
+  if(j < 0) return p;
+  size_t s = state_index(f, P.w[f], j, ix, iy, iz, nx, nz);
+  float q = P.q[f][round][comp][s];
+  p = pml_round_abc(P.abc[f][si] + 3 * j, p, q);
+  P.q[f][round][comp][s] = q;
+  return p;
+}
 __device__ __forceinline__ float pml1_face(const PmlDesc &P, int f, int round, int comp, float sc, float p, int ix, int iy,
                                            int iz, int nx, int ny, int nz) {
   int j = face_pos(P, f, ix, iy, iz, nx, ny, nz);

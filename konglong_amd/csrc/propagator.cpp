@@ -71,6 +71,7 @@ int Propagator::alloc_snapshots() {
   if(d_snap) return KLRTM_OK;
   const int k = std::max(1, c.snap_every);
   ntsav = (c.nt + k - 1) / k;
+  if(c.snap_slots > 0) ntsav = std::min(ntsav, c.snap_slots);
   switch(c.snap_format) {
   case KLRTM_SNAP_FP32: snap_stride_bytes = n * sizeof(float); break;
   case KLRTM_SNAP_C16: snap_stride_bytes = klrtm_nshort_volume(c.nz, c.nx, c.ny) * sizeof(uint16_t); break;
@@ -263,6 +264,8 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
   int rc;
   if((rc = P.alloc_snapshots())) return rc;
   const int k = std::max(1, P.c.snap_every), nt = P.c.nt;
+  KL_REQUIRE(P.ntsav >= (nt + k - 1) / k, KLRTM_ESTATE, "snap_slots (%d) cannot hold the %d snapshots of a shot",
+             P.ntsav, (nt + k - 1) / k);
   // forward: source wavefield, stored every k steps
   if((rc = P.reset())) return rc;
   KL_HIP(hipEventRecord(P.ev[0], stream()));
@@ -316,6 +319,20 @@ int propagator_forward_steps(Propagator &P, int it0, int nsteps, int store_snaps
       int s = (it / k) % std::max(1, P.ntsav);
       if((rc = P.store(s))) return rc;
     }
+    std::swap(P.w0, P.w1);
+  }
+  return KLRTM_OK;
+}
+
+int propagator_backward_steps(Propagator &P, int it0, int nsteps, int image) {
+  KL_REQUIRE(P.rcv, KLRTM_ESTATE, "receivers not set");
+  int rc;
+  if(image && (rc = P.alloc_snapshots())) return rc;
+  const int k = std::max(1, P.c.snap_every);
+  for(int it = it0; it > it0 - nsteps; it--) {
+    if((rc = P.step(it))) return rc;
+    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f))) return rc;
+    if(image && it >= 0 && it % k == 0 && (rc = P.image((it / k) % std::max(1, P.ntsav)))) return rc;
     std::swap(P.w0, P.w1);
   }
   return KLRTM_OK;
@@ -377,6 +394,16 @@ int klrtm_propagator_wavefields(klrtm_propagator *p, float **w0, float **w1) {
 int klrtm_propagator_forward_steps(klrtm_propagator *p, int it0, int nsteps, int store) {
   KL_REQUIRE(p && nsteps >= 0, KLRTM_EINVAL, "bad argument");
   return propagator_forward_steps(*reinterpret_cast<Propagator *>(p), it0, nsteps, store);
+}
+int klrtm_propagator_backward_steps(klrtm_propagator *p, int it0, int nsteps, int image) {
+  KL_REQUIRE(p && nsteps >= 0, KLRTM_EINVAL, "bad argument");
+  return propagator_backward_steps(*reinterpret_cast<Propagator *>(p), it0, nsteps, image);
+}
+int klrtm_propagator_set_record(klrtm_propagator *p, const float *record) {
+  KL_REQUIRE(p && record, KLRTM_EINVAL, "null argument");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  KL_REQUIRE(P.rcv, KLRTM_ESTATE, "receivers not set");
+  return receiver_set_traces(*P.rcv, record);
 }
 int klrtm_propagator_reset(klrtm_propagator *p) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");

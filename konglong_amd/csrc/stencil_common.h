@@ -39,36 +39,46 @@ static __device__ __forceinline__ void scalar_tti(float e, float d, float ax, fl
 
 // first-derivative PML of one component (FdEngine::pml1_{z,x,y}bnd):
 // own axis with scale 1 first, then the two cross axes with pml3d_scaler.
+// FAST uses the precomputed coefficients, otherwise the reference's divisions.
+template<bool FAST>
+__device__ __forceinline__ float pml1_one(const PmlDesc &P, int f, int r, int comp, int si, float p, int ix, int iy, int iz,
+                                          int nx, int ny, int nz) {
+  if(FAST) return pml1_face_fast(P, f, r, comp, si, p, ix, iy, iz, nx, ny, nz);
+  return pml1_face(P, f, r, comp, si ? P.scaler : 1.f, p, ix, iy, iz, nx, ny, nz);
+}
+template<bool FAST = false>
 static __device__ __forceinline__ float pml1_z(const PmlDesc &P, int r, float p, int ix, int iy, int iz, int nx, int ny, int nz) {
-  p = pml1_face(P, FZ1, r, 0, 1.f, p, ix, iy, iz, nx, ny, nz);
-  p = pml1_face(P, FZ2, r, 0, 1.f, p, ix, iy, iz, nx, ny, nz);
+  p = pml1_one<FAST>(P, FZ1, r, 0, 0, p, ix, iy, iz, nx, ny, nz);
+  p = pml1_one<FAST>(P, FZ2, r, 0, 0, p, ix, iy, iz, nx, ny, nz);
   if(P.pml3d) {
-    p = pml1_face(P, FX1, r, 1, P.scaler, p, ix, iy, iz, nx, ny, nz);
-    p = pml1_face(P, FX2, r, 1, P.scaler, p, ix, iy, iz, nx, ny, nz);
-    p = pml1_face(P, FY1, r, 1, P.scaler, p, ix, iy, iz, nx, ny, nz);
-    p = pml1_face(P, FY2, r, 1, P.scaler, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FX1, r, 1, 1, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FX2, r, 1, 1, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FY1, r, 1, 1, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FY2, r, 1, 1, p, ix, iy, iz, nx, ny, nz);
   }
   return p;
 }
+template<bool FAST = false>
 static __device__ __forceinline__ float pml1_x(const PmlDesc &P, int r, float p, int ix, int iy, int iz, int nx, int ny, int nz) {
-  p = pml1_face(P, FX1, r, 0, 1.f, p, ix, iy, iz, nx, ny, nz);
-  p = pml1_face(P, FX2, r, 0, 1.f, p, ix, iy, iz, nx, ny, nz);
+  p = pml1_one<FAST>(P, FX1, r, 0, 0, p, ix, iy, iz, nx, ny, nz);
+  p = pml1_one<FAST>(P, FX2, r, 0, 0, p, ix, iy, iz, nx, ny, nz);
   if(P.pml3d) {
-    p = pml1_face(P, FZ1, r, 1, P.scaler, p, ix, iy, iz, nx, ny, nz);
-    p = pml1_face(P, FZ2, r, 1, P.scaler, p, ix, iy, iz, nx, ny, nz);
-    p = pml1_face(P, FY1, r, 2, P.scaler, p, ix, iy, iz, nx, ny, nz);
-    p = pml1_face(P, FY2, r, 2, P.scaler, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FZ1, r, 1, 1, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FZ2, r, 1, 1, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FY1, r, 2, 1, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FY2, r, 2, 1, p, ix, iy, iz, nx, ny, nz);
   }
   return p;
 }
+template<bool FAST = false>
 static __device__ __forceinline__ float pml1_y(const PmlDesc &P, int r, float p, int ix, int iy, int iz, int nx, int ny, int nz) {
-  p = pml1_face(P, FY1, r, 0, 1.f, p, ix, iy, iz, nx, ny, nz);
-  p = pml1_face(P, FY2, r, 0, 1.f, p, ix, iy, iz, nx, ny, nz);
+  p = pml1_one<FAST>(P, FY1, r, 0, 0, p, ix, iy, iz, nx, ny, nz);
+  p = pml1_one<FAST>(P, FY2, r, 0, 0, p, ix, iy, iz, nx, ny, nz);
   if(P.pml3d && ny > 1) {
-    p = pml1_face(P, FZ1, r, 2, P.scaler, p, ix, iy, iz, nx, ny, nz);
-    p = pml1_face(P, FZ2, r, 2, P.scaler, p, ix, iy, iz, nx, ny, nz);
-    p = pml1_face(P, FX1, r, 2, P.scaler, p, ix, iy, iz, nx, ny, nz);
-    p = pml1_face(P, FX2, r, 2, P.scaler, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FZ1, r, 2, 1, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FZ2, r, 2, 1, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FX1, r, 2, 1, p, ix, iy, iz, nx, ny, nz);
+    p = pml1_one<FAST>(P, FX2, r, 2, 1, p, ix, iy, iz, nx, ny, nz);
   }
   return p;
 }

@@ -1,0 +1,115 @@
+"""Seeded synthetic earth models and the CFL time step.
+
+The reference loads models from files (ModelLoader) and derives dt from the
+CFL rule of ModelPrepare::calcDeltaTime (bgprtm/ModelPrepare.cpp:647-760);
+these generators build the benchmark workloads' models directly.
+"""
+import math
+
+import numpy as np
+
+from ._lib import ENGINE_FD, ISO, VTI, TTI
+
+
+def gaussian_smooth_noise(shape, sigma, seed, dtype=np.float32):
+    """White noise filtered by a 3D Gaussian of std `sigma` cells (spectral), zero mean, unit std."""
+    import scipy.fft as sfft
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal(shape, dtype=np.float32)
+    F = sfft.rfftn(f, workers=-1)
+    for ax, n in enumerate(shape):
+        k = sfft.rfftfreq(n) if ax == len(shape) - 1 else sfft.fftfreq(n)
+        g = np.exp(-0.5 * (2 * np.pi * k * sigma) ** 2).astype(np.float32)
+        sh = [1] * len(shape)
+        sh[ax] = g.size
+        F *= g.reshape(sh)
+    out = sfft.irfftn(F, s=shape, workers=-1).astype(dtype)
+    out -= out.mean()
+    out /= out.std() + 1e-30
+    return out
+
+
+def _to_range(f, lo, hi):
+    a, b = float(f.min()), float(f.max())
+    return (lo + (hi - lo) * (f - a) / (b - a + 1e-30)).astype(np.float32)
+
+
+def layered_model(n=128, d=10.0, seed=101, nlayers=6, vlo=1500.0, vhi=4500.0, anomaly_sigma=8.0, vclip=4400.0):
+    """Flat layers with random depths and velocities plus one +-10% Gaussian anomaly.
+
+    Velocities above `vclip` are clipped so FD8 at dt = 1 ms stays inside its
+    3D CFL limit (v dt/dx <= 0.4529 for dx = 10 m).  Array layout (ny, nx, nz).
+    """
+    rng = np.random.default_rng(seed)
+    depths = np.sort(rng.uniform(0.0, n * d, nlayers - 1))
+    vels = rng.uniform(vlo, vhi, nlayers)
+    z = np.arange(n) * d
+    prof = vels[np.searchsorted(depths, z)]
+    v = np.broadcast_to(prof[None, None, :], (n, n, n)).astype(np.float32).copy()
+    c = rng.uniform(n / 4, 3 * n / 4, 3)
+    amp = 0.1 * (1 if rng.random() < 0.5 else -1)
+    y, x, zz = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    g = np.exp(-((y - c[0]) ** 2 + (x - c[1]) ** 2 + (zz - c[2]) ** 2) / (2 * anomaly_sigma ** 2))
+    v *= (1 + amp * g).astype(np.float32)
+    return np.minimum(v, vclip).astype(np.float32)
+
+
+def smooth_vti_model(shape=(400, 400, 400), sigma=20.0, seed=202):
+    """Vp: Gaussian-filtered noise mapped to 1500-5000 m/s; eps 0-0.3, delta 0-0.15 (same smoothing)."""
+    vp = _to_range(gaussian_smooth_noise(shape, sigma, seed), 1500.0, 5000.0)
+    eps = _to_range(gaussian_smooth_noise(shape, sigma, seed + 1), 0.0, 0.3)
+    dl = _to_range(gaussian_smooth_noise(shape, sigma, seed + 2), 0.0, 0.15)
+    return vp, eps, dl
+
+
+def tilt_axes(shape, sigma, seed):
+    """pjx, pjy from smooth tilt in 0-45 deg and azimuth in 0-360 deg (bgprtm/ModelLoader.cpp:303-321)."""
+    th = _to_range(gaussian_smooth_noise(shape, sigma, seed), 0.0, 45.0) * np.float32(np.pi / 180)
+    ph = _to_range(gaussian_smooth_noise(shape, sigma, seed + 1), 0.0, 360.0) * np.float32(np.pi / 180)
+    st = np.sin(th)
+    return (-st * np.cos(ph)).astype(np.float32), (-st * np.sin(ph)).astype(np.float32)
+
+
+def fd_k1max(order):
+    """sqrt of the Nyquist symbol of the Taylor 2nd-derivative stencil, x1.01 (bgprtm/FdEngine_fd.cpp:69-81)."""
+    M = order // 2
+    c = []
+    s = 0.0
+    for j in range(1, M + 1):
+        r = 1.0
+        for k in range(1, j + 1):
+            r *= (M - j + k) / (M + k)
+        a = 2.0 * (1 if j % 2 else -1) * r / (j * j)
+        c.append(a)
+        s += a
+    c0 = -2.0 * s
+    sym = -c0 + sum(2 * cj * (-1) ** (j + 1) for j, cj in enumerate(c, start=1))
+    return math.sqrt(sym) * 1.01
+
+
+def cfl_dt(vel, d, engine=ENGINE_FD, order=8, model=ISO, eps=None, dl=None):
+    """dt from the CFL rule of ModelPrepare::calcDeltaTime (ISO and VTI branches; TTI uses the VTI bound)."""
+    dx, dy, dz = (d, d, d) if np.isscalar(d) else d
+    k1max = fd_k1max(order) if engine == ENGINE_FD else math.pi
+    rdx2, rdy2, rdz2 = 1.0 / dx ** 2, 1.0 / dy ** 2, 1.0 / dz ** 2
+    v = np.asarray(vel, np.float64)
+    if model == ISO:
+        lam = v * math.sqrt(rdx2 + rdy2 + rdz2)
+        return float(np.min(2.0 / k1max / lam))
+    e = np.asarray(eps, np.float64)
+    dd = np.asarray(dl, np.float64)
+    tmp = (1 + 2 * e) * (rdx2 + rdy2) + rdz2
+    lam = v * np.sqrt((tmp + np.sqrt(np.maximum(tmp * tmp - 8 * (e - dd) * (rdx2 + rdy2) * rdz2, 0))) / 2)
+    return float(np.min(2.0 / k1max / lam / np.sqrt(1 + 2 * e)))
+
+
+def surface_receivers(nx, ny, iz, step=1):
+    """Receivers on every `step`-th (x, y) cell at depth index iz (grid units)."""
+    ys, xs = np.meshgrid(np.arange(0, ny, step), np.arange(0, nx, step), indexing="ij")
+    return np.stack([xs.ravel(), ys.ravel(), np.full(xs.size, float(iz))], axis=1).astype(np.float32)
+
+
+def ricker(nt, dt, fp):
+    t = np.arange(nt) * dt - 1.5 / fp
+    a = (np.pi * fp * t) ** 2
+    return ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
