@@ -25,6 +25,13 @@
 
 namespace klrtm {
 
+// The plan tables are read-only for the whole launch and indexed by the
+// wave-uniform sample index: in the constant address space they are fetched
+// with scalar loads, which keeps them out of the in-order vector-memory
+// counter that the streamed line data waits on.
+typedef const __attribute__((address_space(4))) float cfloat;
+__device__ __forceinline__ const cfloat *as_const(const float *p) { return (const cfloat *)p; }
+
 enum { AXZ = 0, AXX = 1, AXY = 2 };
 enum { P_GRAD = 0, P_GRAD_SCALAR = 1, P_DIV = 2, P_DIV_STEP = 3 };
 
@@ -37,6 +44,212 @@ struct CfdPassArgs {
   PmlDesc pml;
 };
 
+// ---------------------------------------------------------------------------
+// x and y lines: one lane owns one line, consecutive lanes own consecutive z,
+// so every access of a wavefront is one contiguous row.
+//
+// gfx9 retires vector-memory operations in issue order (one vmcnt), so a load
+// is only useful as a prefetch if nothing issued after it has to be waited on
+// first.  Both sweeps therefore issue the loads of batch k+1 before solving
+// batch k, every load inside a batch loop is unconditional (indices clamped to
+// the segment, results of clamped loads discarded), and PML state outside a
+// layer is reached through a buffer resource whose out-of-range accesses the
+// hardware drops.
+
+// PML slot of one damped derivative component along the sweep: the state of
+// sample ii sits at byte offset off + ii*step (both per lane) of the resource spanning both
+// layers of that axis (off >= 2^31: lane outside both layers).
+struct CrossSlot {
+  __amdgpu_buffer_rsrc_t rs;
+  uint32_t off, step;
+  float a, b, d;  // (1-g)/(1+g), 2g/(1-g^2), 1-g for this line's g
+  bool on;
+};
+
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, voff, soff, 0);
+}
+
+// Layer cells of face f (layout of state_index()).
+__device__ __forceinline__ uint32_t face_cells_dev(const PmlDesc &P, int f, int nx, int ny, int nz) {
+  const uint32_t w = (uint32_t)max(P.w[f], 0);
+  if(f == FZ1 || f == FZ2) return w * (uint32_t)nx * (uint32_t)ny;
+  if(f == FX1 || f == FX2) return w * (uint32_t)ny * (uint32_t)nz;
+  return w * (uint32_t)nx * (uint32_t)nz;
+}
+
+// Cross-axis slot of a line: the layer pair of axis `ax` (0 z, 1 x, 2 y) and
+// state component comp, for the line through (ix0, iy0, iz0) swept along
+// axis `sw` (1 x, 2 y; the swept coordinate is 0 in the arguments).
+__device__ __forceinline__ void cross_setup(const PmlDesc &P, int round, int ax, int comp, int sw, int ix0, int iy0,
+                                            int iz0, int nx, int ny, int nz, CrossSlot &cs) {
+  const int f1 = ax == 0 ? FZ1 : ax == 1 ? FX1 : FY1, f2 = f1 + 1;
+  const int pos = ax == 0 ? iz0 : ax == 1 ? ix0 : iy0, npos = ax == 0 ? nz : ax == 1 ? nx : ny;
+  const uint32_t c1 = face_cells_dev(P, f1, nx, ny, nz), c2 = face_cells_dev(P, f2, nx, ny, nz);
+  float *q1 = P.q[f1][round][comp], *q2 = P.q[f2][round][comp];
+  float *base = q1 ? q1 : q2;
+  cs.rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)((q1 ? c1 : 0u) * 4u + (q2 ? c2 : 0u) * 4u), 0x00020000);
+  int f = -1, j = 0;
+  if(q1 && pos < P.w[f1]) f = f1, j = pos;
+  else if(q2 && pos >= npos - P.w[f2]) f = f2, j = pos - (npos - P.w[f2]);
+  cs.on = f >= 0;
+  cs.a = 0.f, cs.b = 0.f, cs.d = 1.f;
+  cs.off = 0x80000000u;
+  cs.step = 0;
+  if(cs.on) {
+    const float g = P.g[f][j] * P.scaler;
+    cs.a = (1.0f - g) / (1.0f + g);
+    cs.b = g * 2.0f / (1.0f - g * g);
+    cs.d = 1.0f - g;
+    const uint32_t i0 = (uint32_t)state_index(f, P.w[f], j, ix0, iy0, iz0, nx, nz);
+    const uint32_t i1 = (uint32_t)state_index(f, P.w[f], j, ix0 + (sw == 1), iy0 + (sw == 2), iz0, nx, nz);
+    cs.off = ((f == f2 && q1 ? c1 : 0u) + i0) * 4u;
+    cs.step = (i1 - i0) * 4u;
+  }
+}
+
+template<int AXIS, int PASS, int MODEL>
+struct LineSweep {
+  // streams of the backward sweep
+  const float *t;  // b' (the forward sweep's output line)
+  float *tmp, *pr, *pz, *pb;
+  const float *vel, *w1, *eps, *del, *pjx, *pjy;
+  const float *prl, *pjyl;  // pr, pjy for loads (another valid stream in 2D, where the value is unused)
+  size_t base, s;
+  const cfloat *U;
+  int n, ny;
+  float Y0, Y1, Y2, Y3;
+  // own-axis layer of the current segment
+  float *qo;          // state of sample j = 0 of the layer for this lane
+  size_t qo_js;       // state stride per layer position
+  int jbase;          // layer position = ii - jbase
+  const cfloat *go;   // damping profile of the layer
+  CrossSlot cs1, cs2;
+
+  static constexpr int E = PASS == P_GRAD ? 0
+                           : PASS == P_GRAD_SCALAR ? (MODEL == KLRTM_ISO ? 0 : MODEL == KLRTM_VTI ? 4 : 6)
+                           : PASS == P_DIV ? 2 : 3;
+
+  template<bool OWN, bool CROSS, int NB>
+  struct Batch {
+    float tv[NB], o[E > 0 ? E : 1][NB], qo[OWN ? NB : 1], q1[CROSS ? NB : 1], q2[CROSS ? NB : 1];
+  };
+
+  template<bool OWN, bool CROSS, int NB>
+  __device__ __forceinline__ void load(Batch<OWN, CROSS, NB> &B, int hi, int lo) {
+#pragma unroll
+    for(int j = 0; j < NB; j++) {
+      const int ii = max(hi - j, lo);
+      const size_t c = base + (size_t)ii * s;
+      B.tv[j] = t[(size_t)ii * s];
+      if(PASS == P_GRAD_SCALAR && MODEL != KLRTM_ISO) {
+        B.o[0][j] = prl[c], B.o[1][j] = pz[c], B.o[2][j] = eps[c], B.o[3][j] = del[c];
+        if(MODEL == KLRTM_TTI) B.o[4][j] = pjx[c], B.o[5][j] = pjyl[c];
+      } else if(PASS == P_DIV || PASS == P_DIV_STEP) {
+        B.o[0][j] = pb[c], B.o[1][j] = vel[c];
+        if(PASS == P_DIV_STEP) B.o[2][j] = w1[c];
+      }
+      if(OWN) B.qo[j] = qo[(size_t)(ii - jbase) * qo_js];
+      if(CROSS) {
+        B.q1[j] = buf_ld(cs1.rs, cs1.off + (uint32_t)ii * cs1.step, 0);
+        B.q2[j] = buf_ld(cs2.rs, cs2.off + (uint32_t)ii * cs2.step, 0);
+      }
+    }
+  }
+
+  // back substitution of sample ii (rows ii >= n-4 use only the solved neighbours that exist)
+  __device__ __forceinline__ float solve(int ii, float v, bool edge) {
+    if(edge) {
+      const int l = n - 1 - ii;
+      if(l >= 1) v = v - U[ii * 5 + 1] * Y0;
+      if(l >= 2) v = v - U[ii * 5 + 2] * Y1;
+      if(l >= 3) v = v - U[ii * 5 + 3] * Y2;
+      v = v / U[ii * 5];
+    } else {
+      v = (v - U[ii * 5 + 1] * Y0 - U[ii * 5 + 2] * Y1 - U[ii * 5 + 3] * Y2 - U[ii * 5 + 4] * Y3) / U[ii * 5];
+    }
+    Y3 = Y2, Y2 = Y1, Y1 = Y0, Y0 = v;
+    return v;
+  }
+
+  // PML rounds (bgprtm/PML.cpp:62-70) of the own axis then the cross axes, and the pass epilogue
+  template<bool OWN, bool CROSS, int NB>
+  __device__ __forceinline__ void finish(const Batch<OWN, CROSS, NB> &B, int j, int ii, float v) {
+    float g = v;
+    if(OWN) {
+      const float gg = go[ii - jbase];
+      float qi = (1.0f - gg) / (1.0f + gg) * B.qo[j] - gg * 2.0f / (1.0f - gg * gg) * g;
+      qo[(size_t)(ii - jbase) * qo_js] = qi;
+      g = qi + g / (1.0f - gg);
+    }
+    if(CROSS) {
+      float qi = cs1.a * B.q1[j] - cs1.b * g;
+      float r = qi + g / cs1.d;
+      g = cs1.on ? r : g;
+      buf_st(qi, cs1.rs, cs1.off + (uint32_t)ii * cs1.step, 0);
+      qi = cs2.a * B.q2[j] - cs2.b * g;
+      r = qi + g / cs2.d;
+      g = cs2.on ? r : g;
+      buf_st(qi, cs2.rs, cs2.off + (uint32_t)ii * cs2.step, 0);
+    }
+    const size_t c = base + (size_t)ii * s;
+    if(PASS == P_GRAD) {
+      tmp[c] = g;
+    } else if(PASS == P_GRAD_SCALAR && MODEL == KLRTM_ISO) {
+      tmp[c] = g;
+    } else if(PASS == P_GRAD_SCALAR) {
+      float gx = g, gy = ny > 1 ? B.o[0][j] : 0.f, gz = B.o[1][j];
+      if(MODEL == KLRTM_VTI) scalar_vti(B.o[2][j], B.o[3][j], gx, gy, gz);
+      else scalar_tti(B.o[2][j], B.o[3][j], B.o[4][j], ny > 1 ? B.o[5][j] : 0.f, gx, gy, gz);
+      tmp[c] = gx;
+      if(ny > 1) pr[c] = gy;
+      if(MODEL == KLRTM_TTI) pz[c] = gz;
+    } else if(PASS == P_DIV) {
+      pb[c] = B.o[0][j] - g * B.o[1][j];
+    } else {
+      float w = B.o[0][j] - g * B.o[1][j];
+      pb[c] = (B.o[2][j] + B.o[2][j]) - w;
+    }
+  }
+
+  // samples ii, ii-1, ... of batch B (cnt of them); the edge rows take the short back substitution
+  template<bool OWN, bool CROSS, int NB>
+  __device__ __forceinline__ void compute(const Batch<OWN, CROSS, NB> &B, int ii, int cnt) {
+#pragma unroll
+    for(int j = 0; j < NB; j++)
+      if(j < cnt) finish<OWN, CROSS, NB>(B, j, ii - j, solve(ii - j, B.tv[j], ii - j >= n - 4));
+  }
+  // one batch: prefetch the next into nxt (unless cur is the last), then solve cur
+  template<bool OWN, bool CROSS, int NB>
+  __device__ __forceinline__ bool half(const Batch<OWN, CROSS, NB> &cur, Batch<OWN, CROSS, NB> &nxt, int &ii, int lo) {
+    const int left = ii - lo + 1;
+    if(left <= NB) {
+      compute<OWN, CROSS, NB>(cur, ii, left);
+      return false;
+    }
+    load<OWN, CROSS, NB>(nxt, ii - NB, lo);
+    compute<OWN, CROSS, NB>(cur, ii, NB);
+    ii -= NB;
+    return true;
+  }
+
+  // samples hi down to lo; batches alternate between two register sets so a
+  // prefetch is never copied (a copy would wait for it)
+  template<bool OWN, bool CROSS>
+  __device__ __forceinline__ void segment(int hi, int lo) {
+    constexpr int streams = 1 + E + (OWN ? 1 : 0) + (CROSS ? 2 : 0);
+    constexpr int NB = streams >= 8 ? 4 : streams >= 5 ? 6 : streams >= 3 ? 8 : 12;
+    Batch<OWN, CROSS, NB> b0, b1;
+    int ii = hi;
+    load<OWN, CROSS, NB>(b0, ii, lo);
+    while(half<OWN, CROSS, NB>(b0, b1, ii, lo) && half<OWN, CROSS, NB>(b1, b0, ii, lo)) {
+    }
+  }
+};
+
 template<int AXIS, int PASS, int MODEL>
 __global__ void __launch_bounds__(256) cfd_line_kernel(const float *in, float *tmp, float *__restrict__ pr,
                                                        float *__restrict__ pz, float *__restrict__ pb,
@@ -44,136 +257,340 @@ __global__ void __launch_bounds__(256) cfd_line_kernel(const float *in, float *t
                                                        const float *__restrict__ eps, const float *__restrict__ del,
                                                        const float *__restrict__ pjx, const float *__restrict__ pjy,
                                                        const CfdPassArgs a) {
+  static_assert(AXIS == AXX || AXIS == AXY, "z lines use cfd_zline_kernel");
   const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if(b >= a.nlines) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t nxz = (size_t)nx * nz;
-  int cx = 0, cy = 0, cz = 0, n;
+  int cx = 0, cy = 0, n;
+  const int cz = (int)(b % nz);
   size_t base, s;
-  if(AXIS == AXZ) {
-    cx = (int)(b % nx), cy = (int)(b / nx), base = (size_t)b * nz, s = 1, n = nz;
-  } else if(AXIS == AXX) {
-    cz = (int)(b % nz), cy = (int)(b / nz), base = (size_t)cy * nxz + cz, s = (size_t)nz, n = nx;
-  } else {
-    cz = (int)(b % nz), cx = (int)(b / nz), base = (size_t)cx * nz + cz, s = nxz, n = ny;
-  }
-  const float *__restrict__ A = a.am;
-  const float *__restrict__ U = a.bm;
-  const float *__restrict__ Lw = a.bl;
+  if(AXIS == AXX) cy = (int)(b / nz), base = (size_t)cy * nxz + cz, s = (size_t)nz, n = nx;
+  else cx = (int)(b / nz), base = (size_t)cx * nz + cz, s = nxz, n = ny;
+  const cfloat *A = as_const(a.am);
+  const cfloat *Lw = as_const(a.bl);
   const float *x = in + base;
   float *t = tmp + base;
 
   // ---- forward sweep: RHS band multiply + elimination (bgprtm/FdEngine_cfd.cpp:170-230)
-  float X[11];
+  // window sample k, zero beyond the line; the load itself is always issued (clamped)
+  auto xload = [&](int k) {
+    const float v = x[(size_t)min(k, n - 1) * s];
+    return k < n ? v : 0.f;
+  };
+  constexpr int NF = 12;  // interior rows per batch
+  float X[11], head[5], fa[NF], fb[NF];
+  // samples 0..10 for the edge rows and the first interior batch's window, all in flight together
 #pragma unroll
   for(int k = 0; k < 5; k++) X[k] = 0.f;
 #pragma unroll
   for(int k = 5; k < 11; k++) X[k] = x[(size_t)(k - 5) * s];
+#pragma unroll
+  for(int k = 0; k < 5; k++) head[k] = xload(k + 6);
+#pragma unroll
+  for(int j = 0; j < NF; j++) fa[j] = xload(11 + j);
   float B0 = 0.f, B1 = 0.f, B2 = 0.f, B3 = 0.f;
-#define KL_SHIFT_X(inext)                                  \
-  {                                                        \
-    _Pragma("unroll") for(int k = 0; k < 10; k++) X[k] = X[k + 1]; \
-    X[10] = ((inext) < n) ? x[(size_t)(inext) * s] : 0.f;  \
-  }
+  auto shift_in = [&](float v) {
+#pragma unroll
+    for(int k = 0; k < 10; k++) X[k] = X[k + 1];
+    X[10] = v;
+  };
+  auto eliminate = [&](int i, float acc) {
+    if(i >= 4) acc = acc - Lw[(i - 4) * 4 + 3] * B3;
+    if(i >= 3) acc = acc - Lw[(i - 3) * 4 + 2] * B2;
+    if(i >= 2) acc = acc - Lw[(i - 2) * 4 + 1] * B1;
+    if(i >= 1) acc = acc - Lw[(i - 1) * 4 + 0] * B0;
+    B3 = B2, B2 = B1, B1 = B0, B0 = acc;
+    return acc;
+  };
 #pragma unroll
   for(int i = 0; i < 5; i++) {
     float acc = 0.f;
 #pragma unroll
     for(int j = 5 - i; j < 11; j++) acc = acc + A[j * 5 + i] * X[j];
-    if(i >= 4) acc = acc - Lw[(i - 4) * 4 + 3] * B3;
-    if(i >= 3) acc = acc - Lw[(i - 3) * 4 + 2] * B2;
-    if(i >= 2) acc = acc - Lw[(i - 2) * 4 + 1] * B1;
-    if(i >= 1) acc = acc - Lw[(i - 1) * 4 + 0] * B0;
-    t[(size_t)i * s] = acc;
-    B3 = B2, B2 = B1, B1 = B0, B0 = acc;
-    KL_SHIFT_X(i + 6);
+    t[(size_t)i * s] = eliminate(i, acc);
+    shift_in(head[i]);
   }
   const float a55 = A[55], a56 = A[56], a57 = A[57], a58 = A[58], a59 = A[59];
-  for(int i = 5; i < n - 5; i++) {
-    float acc = a55 * (X[6] - X[4]) + a56 * (X[7] - X[3]) + a57 * (X[8] - X[2]) + a58 * (X[9] - X[1]) +
-                a59 * (X[10] - X[0]);
-    acc = acc - Lw[(i - 4) * 4 + 3] * B3;
-    acc = acc - Lw[(i - 3) * 4 + 2] * B2;
-    acc = acc - Lw[(i - 2) * 4 + 1] * B1;
-    acc = acc - Lw[(i - 1) * 4 + 0] * B0;
-    t[(size_t)i * s] = acc;
-    B3 = B2, B2 = B1, B1 = B0, B0 = acc;
-    KL_SHIFT_X(i + 6);
+  auto interior = [&]() {
+    return a55 * (X[6] - X[4]) + a56 * (X[7] - X[3]) + a57 * (X[8] - X[2]) + a58 * (X[9] - X[1]) + a59 * (X[10] - X[0]);
+  };
+  // interior rows [i, n-5): cur holds window samples i+6.., the next batch is prefetched into nxt
+  auto fhalf = [&](const float(&cur)[NF], float(&nxt)[NF], int &i) __attribute__((always_inline)) {
+    const int left = (n - 5) - i;
+    if(left <= NF) {
+#pragma unroll
+      for(int j = 0; j < NF; j++)
+        if(j < left) {
+          t[(size_t)(i + j) * s] = eliminate(i + j, interior());
+          shift_in(cur[j]);
+        }
+      i += left;
+      return false;
+    }
+#pragma unroll
+    for(int j = 0; j < NF; j++) nxt[j] = xload(i + NF + 6 + j);
+#pragma unroll
+    for(int j = 0; j < NF; j++) {
+      t[(size_t)(i + j) * s] = eliminate(i + j, interior());
+      shift_in(cur[j]);
+    }
+    i += NF;
+    return true;
+  };
+  {
+    int i = 5;
+    while(fhalf(fa, fb, i) && fhalf(fb, fa, i)) {
+    }
   }
 #pragma unroll
   for(int r = 0; r < 5; r++) {
-    const int i = n - 5 + r;
+    const int ii = n - 5 + r;
     float acc = 0.f;
 #pragma unroll
     for(int j = 0; j < 10 - r; j++) acc = acc - A[54 - r - 5 * j] * X[j];
-    acc = acc - Lw[(i - 4) * 4 + 3] * B3;
-    acc = acc - Lw[(i - 3) * 4 + 2] * B2;
-    acc = acc - Lw[(i - 2) * 4 + 1] * B1;
-    acc = acc - Lw[(i - 1) * 4 + 0] * B0;
-    t[(size_t)i * s] = acc;
-    B3 = B2, B2 = B1, B1 = B0, B0 = acc;
-    if(r < 4) KL_SHIFT_X(i + 6);
+    t[(size_t)ii * s] = eliminate(ii, acc);
+    if(r < 4) shift_in(0.f);  // samples n+1.. lie beyond the line
   }
-#undef KL_SHIFT_X
 
-  // ---- backward sweep + epilogue (bgprtm/FdEngine_cfd.cpp:231-254)
-  float Y0 = 0.f, Y1 = 0.f, Y2 = 0.f, Y3 = 0.f;
-  auto epilogue = [&](int i, float v) {
-    int ix = cx, iy = cy, iz = cz;
-    if(AXIS == AXZ) iz = i;
-    else if(AXIS == AXX) ix = i;
-    else iy = i;
-    float g = v;
-    if(a.has_pml) {
-      if(AXIS == AXZ) g = pml1_z(a.pml, a.round, g, ix, iy, iz, nx, ny, nz);
-      else if(AXIS == AXX) g = pml1_x(a.pml, a.round, g, ix, iy, iz, nx, ny, nz);
-      else g = pml1_y(a.pml, a.round, g, ix, iy, iz, nx, ny, nz);
-    }
-    const size_t c = base + (size_t)i * s;
-    if(PASS == P_GRAD) {
-      tmp[c] = g;
-    } else if(PASS == P_GRAD_SCALAR) {
-      float gx = g, gy = ny > 1 ? pr[c] : 0.f, gz = pz[c];
-      if(MODEL == KLRTM_VTI) scalar_vti(eps[c], del[c], gx, gy, gz);
-      else if(MODEL == KLRTM_TTI) scalar_tti(eps[c], del[c], pjx[c], ny > 1 ? pjy[c] : 0.f, gx, gy, gz);
-      tmp[c] = gx;
-      if(MODEL != KLRTM_ISO && ny > 1) pr[c] = gy;
-      if(MODEL == KLRTM_TTI) pz[c] = gz;
-    } else if(PASS == P_DIV) {
-      pb[c] = pb[c] - g * vel[c];
+  // ---- backward sweep + epilogue (bgprtm/FdEngine_cfd.cpp:231-254), by PML segment
+  LineSweep<AXIS, PASS, MODEL> L;
+  L.t = t, L.tmp = tmp, L.pr = pr, L.pz = pz, L.pb = pb;
+  L.vel = vel, L.w1 = w1, L.eps = eps, L.del = del, L.pjx = pjx, L.pjy = pjy;
+  L.prl = ny > 1 ? pr : pz, L.pjyl = ny > 1 ? pjy : pjx;
+  L.base = base, L.s = s, L.U = as_const(a.bm), L.n = n, L.ny = ny;
+  L.Y0 = L.Y1 = L.Y2 = L.Y3 = 0.f;
+  if(!a.has_pml) {
+    L.template segment<false, false>(n - 1, 0);
+    return;
+  }
+  const PmlDesc &P = a.pml;
+  const int r = a.round;
+  // cross axes: x sweep (z comp 1, y comp 2), y sweep (z comp 2, x comp 2); FdEngine::pml1_{x,y}bnd
+  const bool cross = P.pml3d && (AXIS == AXX || ny > 1);
+  if(cross) {
+    if(AXIS == AXX) {
+      cross_setup(P, r, 0, 1, 1, 0, cy, cz, nx, ny, nz, L.cs1);
+      cross_setup(P, r, 2, 2, 1, 0, cy, cz, nx, ny, nz, L.cs2);
     } else {
-      float w = pb[c] - g * vel[c];
-      pb[c] = (w1[c] + w1[c]) - w;
+      cross_setup(P, r, 0, 2, 2, cx, 0, cz, nx, ny, nz, L.cs1);
+      cross_setup(P, r, 1, 2, 2, cx, 0, cz, nx, ny, nz, L.cs2);
+    }
+  } else {
+    L.cs1.on = L.cs2.on = false;
+  }
+  const bool cross_any = cross && __any(L.cs1.on || L.cs2.on);
+  // own-axis layers: samples [0, chi) in layer 1, [alo, n) in layer 2 (layer 1 wins where they overlap)
+  const int f1 = AXIS == AXX ? FX1 : FY1, f2 = f1 + 1;
+  const int w1l = P.q[f1][r][0] ? P.w[f1] : 0, w2l = P.q[f2][r][0] ? P.w[f2] : 0;
+  const int chi = min(max(w1l, 0), n), alo = max(n - max(w2l, 0), chi);
+  auto own = [&](int f, int jbase) {
+    const int w = P.w[f];
+    L.jbase = jbase;
+    L.go = as_const(P.g[f]);
+    if(AXIS == AXX) L.qo = P.q[f][r][0] + (size_t)cy * w * nz + cz, L.qo_js = (size_t)nz;
+    else L.qo = P.q[f][r][0] + (size_t)cx * nz + cz, L.qo_js = nxz;
+  };
+  if(alo < n) {
+    own(f2, n - w2l);
+    if(cross_any) L.template segment<true, true>(n - 1, alo);
+    else L.template segment<true, false>(n - 1, alo);
+  }
+  if(chi < alo) {
+    if(cross_any) L.template segment<false, true>(alo - 1, chi);
+    else L.template segment<false, false>(alo - 1, chi);
+  }
+  if(chi > 0) {
+    own(f1, 0);
+    if(cross_any) L.template segment<true, true>(chi - 1, 0);
+    else L.template segment<true, false>(chi - 1, 0);
+  }
+}
+// ---------------------------------------------------------------------------
+// z lines (contiguous in memory): one wave owns 64 lines.  Chunks of 32 samples
+// of the 64 lines are staged through LDS with coalesced 128-byte row accesses
+// (the next chunk is prefetched into registers), each lane sweeps its own line
+// from LDS, and b' plus the epilogue go back through the same tiles, so every
+// global access of the pass is coalesced.  Per-sample arithmetic is identical
+// to cfd_line_kernel.
+constexpr int ZC = 32, ZSP = ZC + 1;  // chunk length, LDS row pitch (odd: conflict-free)
+
+template<int PASS>
+__global__ void __launch_bounds__(64) cfd_zline_kernel(const float *in, float *tmp, float *__restrict__ pb,
+                                                       const float *__restrict__ vel, const float *__restrict__ w1,
+                                                       const CfdPassArgs a) {
+  __shared__ float xin[2][64 * ZSP];
+  __shared__ float obuf[64 * ZSP];
+  const int lane = threadIdx.x;
+  const long line0 = (long)blockIdx.x * 64;
+  const int n = a.nz, nx = a.nx, ny = a.ny, nz = a.nz;
+  const long nlines = a.nlines;
+  const int nch = (n + ZC - 1) / ZC;
+  const cfloat *A = as_const(a.am);
+  const cfloat *U = as_const(a.bm);
+  const cfloat *Lw = as_const(a.bl);
+  const int jl = lane >> 5, kl = lane & 31;  // tile element of this lane in pass r: line 2r+jl, sample kl
+
+  float R[32];
+  auto load_regs = [&](const float *src, int c) {
+#pragma unroll
+    for(int r = 0; r < 32; r++) {
+      long L = line0 + 2 * r + jl;
+      int z = c * ZC + kl;
+      R[r] = (L < nlines && z < n) ? src[(size_t)L * n + z] : 0.f;
     }
   };
-  {
-    int i = n - 1;
-    float v = t[(size_t)i * s] / U[i * 5];
-    epilogue(i, v);
-    Y0 = v;
-    i = n - 2;
-    v = t[(size_t)i * s] - U[i * 5 + 1] * Y0;
-    v = v / U[i * 5];
-    epilogue(i, v);
-    Y1 = Y0, Y0 = v;
-    i = n - 3;
-    v = t[(size_t)i * s] - U[i * 5 + 1] * Y0;
-    v = v - U[i * 5 + 2] * Y1;
-    v = v / U[i * 5];
-    epilogue(i, v);
-    Y2 = Y1, Y1 = Y0, Y0 = v;
-    i = n - 4;
-    v = t[(size_t)i * s] - U[i * 5 + 1] * Y0;
-    v = v - U[i * 5 + 2] * Y1;
-    v = v - U[i * 5 + 3] * Y2;
-    v = v / U[i * 5];
-    epilogue(i, v);
-    Y3 = Y2, Y2 = Y1, Y1 = Y0, Y0 = v;
+  auto regs_to_lds = [&](float *buf) {
+#pragma unroll
+    for(int r = 0; r < 32; r++) buf[(2 * r + jl) * ZSP + kl] = R[r];
+  };
+
+  // ---- forward sweep
+  load_regs(in, 0);
+  regs_to_lds(xin[0]);
+  if(nch > 1) {
+    load_regs(in, 1);
+    regs_to_lds(xin[1]);
   }
-  for(int i = n - 5; i >= 0; i--) {
-    float v = (t[(size_t)i * s] - U[i * 5 + 1] * Y0 - U[i * 5 + 2] * Y1 - U[i * 5 + 3] * Y2 - U[i * 5 + 4] * Y3) / U[i * 5];
-    epilogue(i, v);
-    Y3 = Y2, Y2 = Y1, Y1 = Y0, Y0 = v;
+  if(nch > 2) load_regs(in, 2);
+  lds_barrier();
+  auto xat = [&](int i) -> float {
+    int c = i / ZC;
+    return xin[c & 1][lane * ZSP + (i - c * ZC)];
+  };
+  float X[11];
+#pragma unroll
+  for(int k = 0; k < 5; k++) X[k] = 0.f;
+#pragma unroll
+  for(int k = 5; k < 11; k++) X[k] = xat(k - 5);
+  float B0 = 0.f, B1 = 0.f, B2 = 0.f, B3 = 0.f;
+  const float a55 = A[55], a56 = A[56], a57 = A[57], a58 = A[58], a59 = A[59];
+  for(int c = 0; c < nch; c++) {
+    const int i0 = c * ZC, i1 = min(n, i0 + ZC);
+    for(int i = i0; i < i1; i++) {
+      float acc = 0.f;
+      if(i < 5) {
+        switch(i) {
+#define KL_ROW(II)                                                                    \
+  case II: {                                                                          \
+    _Pragma("unroll") for(int j = 5 - II; j < 11; j++) acc = acc + A[j * 5 + II] * X[j]; \
+  } break;
+          KL_ROW(0) KL_ROW(1) KL_ROW(2) KL_ROW(3) KL_ROW(4)
+#undef KL_ROW
+        }
+      } else if(i < n - 5) {
+        acc = a55 * (X[6] - X[4]) + a56 * (X[7] - X[3]) + a57 * (X[8] - X[2]) + a58 * (X[9] - X[1]) + a59 * (X[10] - X[0]);
+      } else {
+        switch(i - (n - 5)) {
+#define KL_END(RR)                                                                       \
+  case RR: {                                                                             \
+    _Pragma("unroll") for(int j = 0; j < 10 - RR; j++) acc = acc - A[54 - RR - 5 * j] * X[j]; \
+  } break;
+          KL_END(0) KL_END(1) KL_END(2) KL_END(3) KL_END(4)
+#undef KL_END
+        }
+      }
+      if(i >= 4) acc = acc - Lw[(i - 4) * 4 + 3] * B3;
+      if(i >= 3) acc = acc - Lw[(i - 3) * 4 + 2] * B2;
+      if(i >= 2) acc = acc - Lw[(i - 2) * 4 + 1] * B1;
+      if(i >= 1) acc = acc - Lw[(i - 1) * 4 + 0] * B0;
+      obuf[lane * ZSP + (i - i0)] = acc;
+      B3 = B2, B2 = B1, B1 = B0, B0 = acc;
+#pragma unroll
+      for(int k = 0; k < 10; k++) X[k] = X[k + 1];
+      X[10] = (i + 6 < n) ? xat(i + 6) : 0.f;
+    }
+    lds_barrier();
+#pragma unroll
+    for(int r = 0; r < 32; r++) {
+      long L = line0 + 2 * r + jl;
+      int z = i0 + kl;
+      if(L < nlines && z < i1) tmp[(size_t)L * n + z] = obuf[(2 * r + jl) * ZSP + kl];
+    }
+    if(c + 2 < nch) {
+      regs_to_lds(xin[c & 1]);
+      if(c + 3 < nch) load_regs(in, c + 3);
+    }
+    lds_barrier();
+  }
+
+  // ---- backward sweep over chunks in reverse, epilogue on each solved chunk
+  load_regs(tmp, nch - 1);
+  regs_to_lds(xin[0]);
+  if(nch > 1) {
+    load_regs(tmp, nch - 2);
+    regs_to_lds(xin[1]);
+  }
+  if(nch > 2) load_regs(tmp, nch - 3);
+  lds_barrier();
+  float Y0 = 0.f, Y1 = 0.f, Y2 = 0.f, Y3 = 0.f;
+  for(int cc = 0; cc < nch; cc++) {
+    const int c = nch - 1 - cc;
+    const float *buf = xin[cc & 1];
+    const int i0 = c * ZC, i1 = min(n, i0 + ZC);
+    for(int i = i1 - 1; i >= i0; i--) {
+      float v = buf[lane * ZSP + (i - i0)];
+      if(i >= n - 4) {
+        switch(n - 1 - i) {
+        case 0: break;
+        case 1: v = v - U[i * 5 + 1] * Y0; break;
+        case 2:
+          v = v - U[i * 5 + 1] * Y0;
+          v = v - U[i * 5 + 2] * Y1;
+          break;
+        default:
+          v = v - U[i * 5 + 1] * Y0;
+          v = v - U[i * 5 + 2] * Y1;
+          v = v - U[i * 5 + 3] * Y2;
+          break;
+        }
+        v = v / U[i * 5];
+      } else {
+        v = (v - U[i * 5 + 1] * Y0 - U[i * 5 + 2] * Y1 - U[i * 5 + 3] * Y2 - U[i * 5 + 4] * Y3) / U[i * 5];
+      }
+      obuf[lane * ZSP + (i - i0)] = v;
+      Y3 = Y2, Y2 = Y1, Y1 = Y0, Y0 = v;
+    }
+    lds_barrier();
+    for(int r0 = 0; r0 < 32; r0 += 8) {
+      float ob[8], pbv[8], vv[8], wv[8];
+      Pml3 pc[8];
+#pragma unroll
+      for(int r = 0; r < 8; r++) {
+        const long L = line0 + 2 * (r0 + r) + jl;
+        const int z = i0 + kl;
+        pc[r].n = 0;
+        if(L < nlines && z < i1) {
+          const int ix = (int)(L % nx), iy = (int)(L / nx);
+          const size_t cidx = (size_t)L * n + z;
+          ob[r] = obuf[(2 * (r0 + r) + jl) * ZSP + kl];
+          if(a.has_pml) pml_load<0>(a.pml, a.round, ix, iy, z, nx, ny, nz, pc[r]);
+          if(PASS != P_GRAD) pbv[r] = pb[cidx], vv[r] = vel[cidx];
+          if(PASS == P_DIV_STEP) wv[r] = w1[cidx];
+        }
+      }
+#pragma unroll
+      for(int r = 0; r < 8; r++) {
+        const long L = line0 + 2 * (r0 + r) + jl;
+        const int z = i0 + kl;
+        if(L < nlines && z < i1) {
+          const int ix = (int)(L % nx), iy = (int)(L / nx);
+          const size_t cidx = (size_t)L * n + z;
+          float g = pc[r].n ? pml_apply<0>(a.pml, a.round, ix, iy, z, nx, nz, pc[r], ob[r]) : ob[r];
+          if(PASS == P_GRAD) tmp[cidx] = g;
+          else if(PASS == P_DIV) pb[cidx] = pbv[r] - g * vv[r];
+          else {
+            float w = pbv[r] - g * vv[r];
+            pb[cidx] = (wv[r] + wv[r]) - w;
+          }
+        }
+      }
+    }
+    if(cc + 2 < nch) {
+      regs_to_lds(xin[cc & 1]);
+      if(cc + 3 < nch) load_regs(tmp, c - 3);
+    }
+    lds_barrier();
   }
 }
 
@@ -325,6 +742,23 @@ static int launch_line(Engine &e, int sg, int round, const float *in, float *tmp
   return check_launch(name);
 }
 
+template<int PASS>
+static int launch_zline(Engine &e, int sg, int round, const float *in, float *tmp, float *pb, const float *w1,
+                        const char *name) {
+  CfdPassArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nx = e.nx, a.ny = e.ny, a.nz = e.nz;
+  a.nlines = (long)e.nx * e.ny;
+  a.am = e.cfd->am[AXZ][sg], a.bm = e.cfd->bm[AXZ][sg], a.bl = e.cfd->bl[AXZ][sg];
+  a.round = round;
+  a.has_pml = e.has_pml;
+  a.pml = e.desc;
+  LaunchTimer tm(name);
+  hipLaunchKernelGGL((cfd_zline_kernel<PASS>), dim3((unsigned)div_up(a.nlines, 64)), dim3(64), 0, stream(), in, tmp, pb,
+                     e.vel, w1, a);
+  return check_launch(name);
+}
+
 template<int MODEL>
 static int cfd_run_m(Engine &e, int mode, const float *p0, float *pb, float *pr) {
   int rc;
@@ -332,18 +766,18 @@ static int cfd_run_m(Engine &e, int mode, const float *p0, float *pb, float *pr)
   float *px = e.d_px, *pz = e.d_pz;
   // gradient (isign=+1, plan 1, PML round 1): y, z, then x with the scalar operator
   if(y && (rc = launch_line<AXY, P_GRAD, MODEL>(e, 0, 0, p0, pr, pr, pz, pb, p0, "cfd_grad_y"))) return rc;
-  if((rc = launch_line<AXZ, P_GRAD, MODEL>(e, 0, 0, p0, pz, pr, pz, pb, p0, "cfd_grad_z"))) return rc;
+  if((rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z"))) return rc;
   if((rc = launch_line<AXX, P_GRAD_SCALAR, MODEL>(e, 0, 0, p0, px, pr, pz, pb, p0, "cfd_grad_x"))) return rc;
   // divergence (isign=-1, plan 2, PML round 2): x, z, y as bgprtm/KernelCPU.cpp:108-114
   const bool step = mode == MODE_STEP;
   if((rc = launch_line<AXX, P_DIV, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x"))) return rc;
   if(y) {
-    if((rc = launch_line<AXZ, P_DIV, MODEL>(e, 1, 1, pz, pz, pr, pz, pb, p0, "cfd_div_z"))) return rc;
+    if((rc = launch_zline<P_DIV>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z"))) return rc;
     if(step) return launch_line<AXY, P_DIV_STEP, MODEL>(e, 1, 1, pr, px, pr, pz, pb, p0, "cfd_div_y");
     return launch_line<AXY, P_DIV, MODEL>(e, 1, 1, pr, px, pr, pz, pb, p0, "cfd_div_y");
   }
-  if(step) return launch_line<AXZ, P_DIV_STEP, MODEL>(e, 1, 1, pz, pz, pr, pz, pb, p0, "cfd_div_z");
-  return launch_line<AXZ, P_DIV, MODEL>(e, 1, 1, pz, pz, pr, pz, pb, p0, "cfd_div_z");
+  if(step) return launch_zline<P_DIV_STEP>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z");
+  return launch_zline<P_DIV>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z");
 }
 
 int cfd_run(Engine &e, int model, int mode, const float *p0, float *pb, float *pr) {

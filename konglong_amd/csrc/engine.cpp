@@ -72,15 +72,23 @@ int Engine::alloc_state(int comps) {
       return KLRTM_ENOMEM;
     }
   }
+  // pool order [round][comp][face]: the two faces of one axis are adjacent for
+  // each (round, comp), so one 32-bit-offset buffer resource spans both
   float *p = d_state;
-  for(int f = 0; f < 6; f++)
-    for(int r = 0; r < 2; r++)
-      for(int c = 0; c < 3; c++) {
+  for(int r = 0; r < 2; r++)
+    for(int c = 0; c < 3; c++)
+      for(int f = 0; f < 6; f++) {
         if(c < comps && face_cells(f)) {
           desc.q[f][r][c] = p;
           p += face_cells(f);
         } else desc.q[f][r][c] = nullptr;
       }
+  for(int f = 0; f < 6; f += 2)
+    if((face_cells(f) + face_cells(f + 1)) * sizeof(float) >= (size_t(1) << 31)) {
+      set_error("PML layer pair %d holds %zu cells; the line kernels address at most 2 GiB per pair", f / 2,
+                face_cells(f) + face_cells(f + 1));
+      return KLRTM_EINVAL;
+    }
   return clean();
 }
 

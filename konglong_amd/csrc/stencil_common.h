@@ -83,4 +83,63 @@ static __device__ __forceinline__ float pml1_y(const PmlDesc &P, int r, float p,
   return p;
 }
 
+// ---------------------------------------------------------------------------
+// Batched form of the first-derivative PML for the line solvers: pml_load
+// fetches the state of every face acting on one derivative sample (own axis,
+// then the two cross axes, as FdEngine::pml1_*bnd orders them) so that loads of
+// several samples can be in flight together; pml_apply then runs the exact
+// rounds (reference divisions) and writes the state back.
+struct Pml3 {
+  float q[3], g[3];
+  int f[3], j[3];  // slot k: face and layer position (j < 0: inactive)
+  int n;           // number of active slots
+};
+template<int AXIS>  // 0 z, 1 x, 2 y
+__device__ __forceinline__ void pml_load(const PmlDesc &P, int round, int ix, int iy, int iz, int nx, int ny, int nz,
+                                         Pml3 &c) {
+  // slot order and state component per derivative axis (FdEngine::pml1_{z,x,y}bnd)
+  constexpr int ord0 = AXIS, ord1 = AXIS == 0 ? 1 : 0, ord2 = AXIS == 2 ? 1 : 2;
+  constexpr int cp1 = AXIS == 2 ? 2 : 1, cp2 = AXIS == 0 ? 1 : 2;
+  const bool cross = P.pml3d && (AXIS != 2 || ny > 1);
+  c.n = 0;
+#pragma unroll
+  for(int k = 0; k < 3; k++) {
+    const int ax = k == 0 ? ord0 : k == 1 ? ord1 : ord2;
+    const int comp = k == 0 ? 0 : k == 1 ? cp1 : cp2;
+    const int f1 = ax == 0 ? FZ1 : ax == 1 ? FX1 : FY1;
+    c.j[k] = -1;
+    c.f[k] = f1;
+    if(k == 0 || cross) {
+      int j = face_pos(P, f1, ix, iy, iz, nx, ny, nz);
+      int f = f1;
+      if(j < 0) {
+        f = f1 + 1;
+        j = face_pos(P, f, ix, iy, iz, nx, ny, nz);
+      }
+      if(j >= 0) {
+        c.f[k] = f, c.j[k] = j;
+        c.g[k] = P.g[f][j] * (k ? P.scaler : 1.f);
+        c.q[k] = P.q[f][round][comp][state_index(f, P.w[f], j, ix, iy, iz, nx, nz)];
+        c.n++;
+      }
+    }
+  }
+}
+template<int AXIS>
+__device__ __forceinline__ float pml_apply(const PmlDesc &P, int round, int ix, int iy, int iz, int nx, int nz, Pml3 &c,
+                                           float p) {
+  constexpr int cp1 = AXIS == 2 ? 2 : 1, cp2 = AXIS == 0 ? 1 : 2;
+#pragma unroll
+  for(int k = 0; k < 3; k++) {
+    if(c.j[k] >= 0) {
+      const int comp = k == 0 ? 0 : k == 1 ? cp1 : cp2;
+      const int f = c.f[k];
+      float q = c.q[k];
+      p = pml_round(c.g[k], p, q);
+      P.q[f][round][comp][state_index(f, P.w[f], c.j[k], ix, iy, iz, nx, nz)] = q;
+    }
+  }
+  return p;
+}
+
 }  // namespace klrtm
