@@ -90,6 +90,15 @@ int klrtm_kernel_tti(klrtm_kernel *k, const float *p0, float *pb, float *pr);
 /* fused Propagator::kernel() + update2nd(): w0 <- 2*w1 - (w0 - vel*Op(w1)) in one pass
  * where the engine allows it (model = KLRTM_ISO/VTI/TTI) */
 int klrtm_kernel_step(klrtm_kernel *k, int model, const float *w1, float *w0, float *wr);
+
+/* FdEngine_fd::laplacian (bgprtm/FdEngine.h:116-118, bgprtm/FdEngine_fd.cpp:204-325) over the whole
+ * volume: out = L(in) with the PML of the Laplacian form (pml2, layers z, x, y).  FD engines only. */
+int klrtm_kernel_laplacian(klrtm_kernel *k, const float *in, float *out);
+/* FdEngine::d{z,x,y}1 (bgprtm/FdEngine.h:61-84; FD: bgprtm/FdEngine_fd.cpp:330-470, CFD:
+ * bgprtm/FdEngine_cfd.cpp:24-297) over the whole volume, axis 0 z, 1 x, 2 y.  iacc 0: out = D(in);
+ * iacc 1: out -= vel*D(in) with the model's vel.  isign > 0 applies the gradient PML round (and the
+ * CFD gradient plan), isign <= 0 the divergence round and plan. */
+int klrtm_kernel_deriv1(klrtm_kernel *k, int axis, const float *in, float *out, int iacc, int isign);
 /* tile shape of the stencil kernels: tz (z lanes), tx (x rows), ychunk (y planes per
  * block); 0 keeps the current value.  Used by the autotuner. */
 int klrtm_kernel_set_tile(klrtm_kernel *k, int tz, int tx, int ychunk);
@@ -99,6 +108,17 @@ int klrtm_kernel_get_tile(klrtm_kernel *k, int *tz, int *tx, int *ychunk);
  * bands per row), bl[2][n*4] (elimination multipliers per row), am[2][60] (RHS
  * boundary table, interior weights at 55..59); [0] gradient plan, [1] divergence */
 int klrtm_cfd_plan(int n, float d, float *bm, float *bl, float *am);
+
+/* ---- PML class (bgprtm/PML.h:28-61, bgprtm/PML.cpp) ---------------------------- */
+/* PML::init (bgprtm/PML.cpp:23-60), host: coef[i] (layer 1, i = 0 outermost) and feoc (layer 2) */
+int klrtm_pml_profile(int npml, float dx, float dt, float vmax, float pml_eps, float *coef, float *feoc);
+/* PML::apply_single (q2 == NULL) / apply2_single (q2 != NULL) over nbatch lines, device arrays:
+ * line j uses p + j*pstride and q (q2) + j*npml; g = coef[i]*scaler; symmetry -1/0/1 sets
+ * p[npml] = p[npml-1]*symmetry afterwards (bgprtm/PML.cpp:62-119).  coef is a device array. */
+int klrtm_pml_apply_lines(const float *coef, int npml, int nbatch, long pstride, float scaler, float *p, float *q,
+                          float *q2, int symmetry);
+/* PML::apply_single_trans (q2 == NULL) / apply2_single_trans across nz entries (bgprtm/PML.cpp:121-141) */
+int klrtm_pml_apply_trans(float g, float *p, float *q, float *q2, int nz);
 
 /* ---- Util::CompactOrmsbySpreadCoeff (bgprtm/Util.cpp:317-397), host ---------- */
 /* coeff receives *nlimit taps (input: max taps, output: taps used); returns the

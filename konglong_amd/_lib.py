@@ -58,6 +58,11 @@ _SIGS = {
     "klrtm_timer_enable": (_i, [_i]),
     "klrtm_timer_read": (_i, [C.c_char_p, _pd, _pl]),
     "klrtm_timer_reset": (_i, []),
+    "klrtm_kernel_laplacian": (_i, [_vp, _vp, _vp]),
+    "klrtm_kernel_deriv1": (_i, [_vp, _i, _vp, _vp, _i, _i]),
+    "klrtm_pml_profile": (_i, [_i, _f, _f, _f, _f, _pf, _pf]),
+    "klrtm_pml_apply_lines": (_i, [_vp, _i, _i, _l, _f, _vp, _vp, _vp, _i]),
+    "klrtm_pml_apply_trans": (_i, [_f, _vp, _vp, _vp, _i]),
     "klrtm_vel2": (_i, [_vp, _sz, _f]),
     "klrtm_update2nd": (_i, [_vp, _vp, _sz]),
     "klrtm_kernel_create": (_i, [C.POINTER(_vp), _i, _i, _i, _f, _f, _f, _i, _i]),

@@ -25,13 +25,6 @@
 
 namespace klrtm {
 
-// The plan tables are read-only for the whole launch and indexed by the
-// wave-uniform sample index: in the constant address space they are fetched
-// with scalar loads, which keeps them out of the in-order vector-memory
-// counter that the streamed line data waits on.
-typedef const __attribute__((address_space(4))) float cfloat;
-__device__ __forceinline__ const cfloat *as_const(const float *p) { return (const cfloat *)p; }
-
 enum { AXZ = 0, AXX = 1, AXY = 2 };
 enum { P_GRAD = 0, P_GRAD_SCALAR = 1, P_DIV = 2, P_DIV_STEP = 3 };
 
@@ -66,20 +59,6 @@ struct CrossSlot {
   bool on;
 };
 
-__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
-}
-__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, voff, soff, 0);
-}
-
-// Layer cells of face f (layout of state_index()).
-__device__ __forceinline__ uint32_t face_cells_dev(const PmlDesc &P, int f, int nx, int ny, int nz) {
-  const uint32_t w = (uint32_t)max(P.w[f], 0);
-  if(f == FZ1 || f == FZ2) return w * (uint32_t)nx * (uint32_t)ny;
-  if(f == FX1 || f == FX2) return w * (uint32_t)ny * (uint32_t)nz;
-  return w * (uint32_t)nx * (uint32_t)nz;
-}
 
 // Cross-axis slot of a line: the layer pair of axis `ax` (0 z, 1 x, 2 y) and
 // state component comp, for the line through (ix0, iy0, iz0) swept along
@@ -407,193 +386,371 @@ __global__ void __launch_bounds__(256) cfd_line_kernel(const float *in, float *t
   }
 }
 // ---------------------------------------------------------------------------
-// z lines (contiguous in memory): one wave owns 64 lines.  Chunks of 32 samples
-// of the 64 lines are staged through LDS with coalesced 128-byte row accesses
-// (the next chunk is prefetched into registers), each lane sweeps its own line
-// from LDS, and b' plus the epilogue go back through the same tiles, so every
-// global access of the pass is coalesced.  Per-sample arithmetic is identical
-// to cfd_line_kernel.
+// z lines (contiguous in memory): one wave owns 64 lines.  Chunks of ZC
+// samples of the 64 lines are staged through LDS with coalesced row accesses
+// (chunk c+2 is in flight in registers while chunk c is swept), each lane
+// sweeps its own line from LDS, writing b' (forward) and the solution
+// (backward) in place, and both go back to global memory through the same
+// transposed tiles, so every global access of the pass is coalesced.
+// Interior chunks are fully unrolled: the 11-sample window lives in renamed
+// registers and the plan rows of a chunk arrive as a few scalar loads.
+// Global memory is reached through per-wave buffer resources; tile elements
+// beyond the line or the last line get an out-of-range offset, so their loads
+// return 0 and their stores are dropped, and no access needs a branch.
+// Per-sample arithmetic is identical to cfd_line_kernel.
 constexpr int ZC = 32, ZSP = ZC + 1;  // chunk length, LDS row pitch (odd: conflict-free)
+constexpr int ZLPR = 64 / ZC;         // lines per tile row
+constexpr int ZU = 8;                 // samples per unrolled step of an interior chunk sweep
+
+struct ZCross {  // cross-axis PML slot of one line: state offset of z = 0 (OOR: outside), coefficients
+  uint32_t off;
+  float a, b, d;
+};
 
 template<int PASS>
-__global__ void __launch_bounds__(64) cfd_zline_kernel(const float *in, float *tmp, float *__restrict__ pb,
-                                                       const float *__restrict__ vel, const float *__restrict__ w1,
-                                                       const CfdPassArgs a) {
+__global__ void __launch_bounds__(64) cfd_zline_kernel(const float *in, float *tmp, float *pb, const float *vel,
+                                                       const float *w1, const CfdPassArgs a) {
   __shared__ float xin[2][64 * ZSP];
-  __shared__ float obuf[64 * ZSP];
+  __shared__ ZCross lco[2][64];
   const int lane = threadIdx.x;
   const long line0 = (long)blockIdx.x * 64;
   const int n = a.nz, nx = a.nx, ny = a.ny, nz = a.nz;
-  const long nlines = a.nlines;
+  const int nl = (int)min(64L, a.nlines - line0);
   const int nch = (n + ZC - 1) / ZC;
   const cfloat *A = as_const(a.am);
   const cfloat *U = as_const(a.bm);
   const cfloat *Lw = as_const(a.bl);
-  const int jl = lane >> 5, kl = lane & 31;  // tile element of this lane in pass r: line 2r+jl, sample kl
+  const int jl = lane / ZC, kl = lane % ZC;  // tile element of this lane in row r: line r*ZLPR+jl, sample kl
+  const int wbytes = nl * n * 4;
+  auto rsrc = [&](const float *p) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(p + (size_t)line0 * n), 0, wbytes, 0x00020000);
+  };
+  const __amdgpu_buffer_rsrc_t rs_in = rsrc(in), rs_tmp = rsrc(tmp);
+  const __amdgpu_buffer_rsrc_t rs_pb = rsrc(PASS == P_GRAD ? in : pb), rs_vel = rsrc(PASS == P_GRAD ? in : vel);
+  const __amdgpu_buffer_rsrc_t rs_w1 = rsrc(PASS == P_DIV_STEP ? w1 : in);
+  // tile element (row r, chunk c) = lane part (VGPR, out of range beyond the line) + row part (SGPR);
+  // rows past the wave's last line fall outside the resource
+  auto voff = [&](int c) -> uint32_t {
+    const int z = c * ZC + kl;
+    return z < n ? (uint32_t)(jl * n + z) * 4u : OOR;
+  };
+  auto soff = [&](int r) -> uint32_t { return (uint32_t)(r * ZLPR * n) * 4u; };
 
-  float R[32];
-  auto load_regs = [&](const float *src, int c) {
+  float R[ZC];
+  auto load_regs = [&](__amdgpu_buffer_rsrc_t rs, int c) {
+    const uint32_t v = voff(c);
 #pragma unroll
-    for(int r = 0; r < 32; r++) {
-      long L = line0 + 2 * r + jl;
-      int z = c * ZC + kl;
-      R[r] = (L < nlines && z < n) ? src[(size_t)L * n + z] : 0.f;
-    }
+    for(int r = 0; r < ZC; r++) R[r] = buf_ld(rs, v, soff(r));
   };
   auto regs_to_lds = [&](float *buf) {
 #pragma unroll
-    for(int r = 0; r < 32; r++) buf[(2 * r + jl) * ZSP + kl] = R[r];
+    for(int r = 0; r < ZC; r++) buf[(r * ZLPR + jl) * ZSP + kl] = R[r];
+  };
+  auto lds_to_global = [&](const float *buf, __amdgpu_buffer_rsrc_t rs, int c) {
+    const uint32_t v = voff(c);
+#pragma unroll
+    for(int r = 0; r < ZC; r++) buf_st(buf[(r * ZLPR + jl) * ZSP + kl], rs, v, soff(r));
   };
 
+  // ---- PML of this wave: cross-axis slots per line (x layers, then y layers, state component 1)
+  const PmlDesc &P = a.pml;
+  const int rnd = a.round;
+  bool cross_any = false;
+  __amdgpu_buffer_rsrc_t rs_q[3];
+  uint32_t pairoff[3] = {0, 0, 0};  // cells of the first layer of each pair (second layer starts there)
+  if(a.has_pml) {
+    const int comp[3] = {0, 1, 1};
+    const int fa[3] = {FZ1, FX1, FY1};
+#pragma unroll
+    for(int k = 0; k < 3; k++) {
+      const int f1 = fa[k];
+      float *q1 = P.q[f1][rnd][comp[k]], *q2 = P.q[f1 + 1][rnd][comp[k]];
+      const uint32_t c1 = q1 ? face_cells_dev(P, f1, nx, ny, nz) : 0u, c2 = q2 ? face_cells_dev(P, f1 + 1, nx, ny, nz) : 0u;
+      pairoff[k] = c1;
+      rs_q[k] = __builtin_amdgcn_make_buffer_rsrc(q1 ? q1 : q2, 0, (int)((c1 + c2) * 4u), 0x00020000);
+    }
+    const long L = line0 + lane;
+    const int ix = (int)(L % nx), iy = (int)(L / nx);
+    bool on_any = false;
+#pragma unroll
+    for(int k = 0; k < 2; k++) {
+      const int f1 = k == 0 ? FX1 : FY1;
+      const int pos = k == 0 ? ix : iy, npos = k == 0 ? nx : ny;
+      ZCross zc;
+      zc.off = OOR, zc.a = 0.f, zc.b = 0.f, zc.d = 1.f;
+      if(P.pml3d && lane < nl) {
+        float *q1 = P.q[f1][rnd][1], *q2 = P.q[f1 + 1][rnd][1];
+        int f = -1, j = 0;
+        if(q1 && pos < P.w[f1]) f = f1, j = pos;
+        else if(q2 && pos >= npos - P.w[f1 + 1]) f = f1 + 1, j = pos - (npos - P.w[f1 + 1]);
+        if(f >= 0) {
+          const float g = P.g[f][j] * P.scaler;
+          zc.a = (1.0f - g) / (1.0f + g);
+          zc.b = g * 2.0f / (1.0f - g * g);
+          zc.d = 1.0f - g;
+          zc.off = ((f == f1 + 1 ? pairoff[k + 1] : 0u) + (uint32_t)state_index(f, P.w[f], j, ix, iy, 0, nx, nz)) * 4u;
+          on_any = true;
+        }
+      }
+      lco[k][lane] = zc;
+    }
+    cross_any = __any(on_any);
+  }
+  // own z layers: samples [0, zlo) in layer 1, [zhi, n) in layer 2
+  const int wz1 = (a.has_pml && P.q[FZ1][rnd][0]) ? P.w[FZ1] : 0;
+  const int wz2 = (a.has_pml && P.q[FZ2][rnd][0]) ? P.w[FZ2] : 0;
+  const int zlo = min(wz1, n), zhi = max(n - wz2, zlo);
+
   // ---- forward sweep
-  load_regs(in, 0);
+  load_regs(rs_in, 0);
   regs_to_lds(xin[0]);
   if(nch > 1) {
-    load_regs(in, 1);
+    load_regs(rs_in, 1);
     regs_to_lds(xin[1]);
   }
-  if(nch > 2) load_regs(in, 2);
+  if(nch > 2) load_regs(rs_in, 2);
   lds_barrier();
-  auto xat = [&](int i) -> float {
-    int c = i / ZC;
-    return xin[c & 1][lane * ZSP + (i - c * ZC)];
-  };
   float X[11];
 #pragma unroll
   for(int k = 0; k < 5; k++) X[k] = 0.f;
 #pragma unroll
-  for(int k = 5; k < 11; k++) X[k] = xat(k - 5);
+  for(int k = 5; k < 11; k++) X[k] = xin[0][lane * ZSP + (k - 5)];
   float B0 = 0.f, B1 = 0.f, B2 = 0.f, B3 = 0.f;
   const float a55 = A[55], a56 = A[56], a57 = A[57], a58 = A[58], a59 = A[59];
   for(int c = 0; c < nch; c++) {
     const int i0 = c * ZC, i1 = min(n, i0 + ZC);
-    for(int i = i0; i < i1; i++) {
-      float acc = 0.f;
-      if(i < 5) {
-        switch(i) {
+    float *cur = xin[c & 1] + lane * ZSP;
+    const float *nxt = xin[(c + 1) & 1] + lane * ZSP;
+    if(c > 0 && i0 + ZC <= n - 5) {
+      // interior chunk: rows i0..i0+ZC-1 all use the interior stencil; window samples up to i0+ZC+5
+      // sit in the two staged chunks (zero beyond the line)
+#pragma unroll 1
+      for(int jb = 0; jb < ZC; jb += ZU) {
+#pragma unroll
+        for(int jj = 0; jj < ZU; jj++) {
+          const int j = jb + jj, i = i0 + j;
+          float acc = a55 * (X[6] - X[4]) + a56 * (X[7] - X[3]) + a57 * (X[8] - X[2]) + a58 * (X[9] - X[1]) +
+                      a59 * (X[10] - X[0]);
+          acc = acc - Lw[(i - 4) * 4 + 3] * B3;
+          acc = acc - Lw[(i - 3) * 4 + 2] * B2;
+          acc = acc - Lw[(i - 2) * 4 + 1] * B1;
+          acc = acc - Lw[(i - 1) * 4 + 0] * B0;
+          B3 = B2, B2 = B1, B1 = B0, B0 = acc;
+          const float xn = j + 6 < ZC ? cur[j + 6] : nxt[j + 6 - ZC];
+          cur[j] = acc;
+#pragma unroll
+          for(int k = 0; k < 10; k++) X[k] = X[k + 1];
+          X[10] = xn;
+        }
+      }
+    } else {
+      for(int i = i0; i < i1; i++) {
+        float acc = 0.f;
+        if(i < 5) {
+          switch(i) {
 #define KL_ROW(II)                                                                    \
   case II: {                                                                          \
     _Pragma("unroll") for(int j = 5 - II; j < 11; j++) acc = acc + A[j * 5 + II] * X[j]; \
   } break;
-          KL_ROW(0) KL_ROW(1) KL_ROW(2) KL_ROW(3) KL_ROW(4)
+            KL_ROW(0) KL_ROW(1) KL_ROW(2) KL_ROW(3) KL_ROW(4)
 #undef KL_ROW
-        }
-      } else if(i < n - 5) {
-        acc = a55 * (X[6] - X[4]) + a56 * (X[7] - X[3]) + a57 * (X[8] - X[2]) + a58 * (X[9] - X[1]) + a59 * (X[10] - X[0]);
-      } else {
-        switch(i - (n - 5)) {
+          }
+        } else if(i < n - 5) {
+          acc = a55 * (X[6] - X[4]) + a56 * (X[7] - X[3]) + a57 * (X[8] - X[2]) + a58 * (X[9] - X[1]) +
+                a59 * (X[10] - X[0]);
+        } else {
+          switch(i - (n - 5)) {
 #define KL_END(RR)                                                                       \
   case RR: {                                                                             \
     _Pragma("unroll") for(int j = 0; j < 10 - RR; j++) acc = acc - A[54 - RR - 5 * j] * X[j]; \
   } break;
-          KL_END(0) KL_END(1) KL_END(2) KL_END(3) KL_END(4)
+            KL_END(0) KL_END(1) KL_END(2) KL_END(3) KL_END(4)
 #undef KL_END
+          }
         }
-      }
-      if(i >= 4) acc = acc - Lw[(i - 4) * 4 + 3] * B3;
-      if(i >= 3) acc = acc - Lw[(i - 3) * 4 + 2] * B2;
-      if(i >= 2) acc = acc - Lw[(i - 2) * 4 + 1] * B1;
-      if(i >= 1) acc = acc - Lw[(i - 1) * 4 + 0] * B0;
-      obuf[lane * ZSP + (i - i0)] = acc;
-      B3 = B2, B2 = B1, B1 = B0, B0 = acc;
+        if(i >= 4) acc = acc - Lw[(i - 4) * 4 + 3] * B3;
+        if(i >= 3) acc = acc - Lw[(i - 3) * 4 + 2] * B2;
+        if(i >= 2) acc = acc - Lw[(i - 2) * 4 + 1] * B1;
+        if(i >= 1) acc = acc - Lw[(i - 1) * 4 + 0] * B0;
+        B3 = B2, B2 = B1, B1 = B0, B0 = acc;
+        // the window sample (i+6) is read before row i+6 overwrites its slot
+        const int k6 = i + 6 - i0;
+        const float xn = (i + 6 < n) ? (k6 < ZC ? cur[k6] : nxt[k6 - ZC]) : 0.f;
+        cur[i - i0] = acc;
 #pragma unroll
-      for(int k = 0; k < 10; k++) X[k] = X[k + 1];
-      X[10] = (i + 6 < n) ? xat(i + 6) : 0.f;
+        for(int k = 0; k < 10; k++) X[k] = X[k + 1];
+        X[10] = xn;
+      }
     }
     lds_barrier();
-#pragma unroll
-    for(int r = 0; r < 32; r++) {
-      long L = line0 + 2 * r + jl;
-      int z = i0 + kl;
-      if(L < nlines && z < i1) tmp[(size_t)L * n + z] = obuf[(2 * r + jl) * ZSP + kl];
-    }
+    lds_to_global(xin[c & 1], rs_tmp, c);
+    lds_barrier();
     if(c + 2 < nch) {
       regs_to_lds(xin[c & 1]);
-      if(c + 3 < nch) load_regs(in, c + 3);
+      if(c + 3 < nch) load_regs(rs_in, c + 3);
     }
     lds_barrier();
   }
 
   // ---- backward sweep over chunks in reverse, epilogue on each solved chunk
-  load_regs(tmp, nch - 1);
+  load_regs(rs_tmp, nch - 1);
   regs_to_lds(xin[0]);
   if(nch > 1) {
-    load_regs(tmp, nch - 2);
+    load_regs(rs_tmp, nch - 2);
     regs_to_lds(xin[1]);
   }
-  if(nch > 2) load_regs(tmp, nch - 3);
+  if(nch > 2) load_regs(rs_tmp, nch - 3);
   lds_barrier();
   float Y0 = 0.f, Y1 = 0.f, Y2 = 0.f, Y3 = 0.f;
+  constexpr int NO = PASS == P_GRAD ? 1 : PASS == P_DIV ? 2 : 3;  // epilogue operand streams
+  constexpr int GR = 8;                                              // tile rows per epilogue group
+  struct Grp {
+    float o[NO][GR], q[3][GR];
+  };
+  constexpr int GRP = 4;  // tile rows per epilogue group with PML (three state streams more)
   for(int cc = 0; cc < nch; cc++) {
     const int c = nch - 1 - cc;
-    const float *buf = xin[cc & 1];
+    float *buf = xin[cc & 1];
+    float *row = buf + lane * ZSP;
     const int i0 = c * ZC, i1 = min(n, i0 + ZC);
-    for(int i = i1 - 1; i >= i0; i--) {
-      float v = buf[lane * ZSP + (i - i0)];
-      if(i >= n - 4) {
-        switch(n - 1 - i) {
-        case 0: break;
-        case 1: v = v - U[i * 5 + 1] * Y0; break;
-        case 2:
-          v = v - U[i * 5 + 1] * Y0;
-          v = v - U[i * 5 + 2] * Y1;
-          break;
-        default:
-          v = v - U[i * 5 + 1] * Y0;
-          v = v - U[i * 5 + 2] * Y1;
-          v = v - U[i * 5 + 3] * Y2;
-          break;
+    const int z = i0 + kl;
+    const bool pml = a.has_pml && (cross_any || i0 < zlo || i1 > zhi);
+    // own z-layer slot of this lane's sample
+    float oa = 0.f, ob = 0.f, od = 1.f;
+    uint32_t obase = OOR;
+    int ow = 0;
+    if(pml && z < n && (z < zlo || z >= zhi)) {
+      const int f = z < zlo ? FZ1 : FZ2, j = z < zlo ? z : z - (n - wz2);
+      const float g = P.g[f][j];
+      oa = (1.0f - g) / (1.0f + g);
+      ob = g * 2.0f / (1.0f - g * g);
+      od = 1.0f - g;
+      ow = P.w[f];
+      obase = ((f == FZ2 ? pairoff[0] : 0u) + (uint32_t)(line0 * ow + j)) * 4u;
+    }
+    const uint32_t vc = voff(c);
+    const uint32_t ov = obase == OOR ? OOR : obase + (uint32_t)(jl * ow) * 4u;  // own state, lane part
+    // (the row part r*ZLPR*ow differs between the two layers, so it goes into the per-lane offset too)
+    auto load_grp = [&](Grp &G, int r0, bool with_pml) __attribute__((always_inline)) {
+      const int gr = with_pml ? GRP : GR;
+#pragma unroll
+      for(int rr = 0; rr < GR; rr++) {
+        if(rr >= gr) break;
+        const int r = r0 + rr;
+        if(PASS != P_GRAD) G.o[0][rr] = buf_ld(rs_pb, vc, soff(r)), G.o[1][rr] = buf_ld(rs_vel, vc, soff(r));
+        if(PASS == P_DIV_STEP) G.o[2][rr] = buf_ld(rs_w1, vc, soff(r));
+        if(with_pml) {
+          const int l = r * ZLPR + jl;
+          G.q[0][rr] = buf_ld(rs_q[0], l < nl ? ov + (uint32_t)(r * ZLPR * ow) * 4u : OOR, 0);
+          const uint32_t ox = lco[0][l].off, oy = lco[1][l].off;
+          G.q[1][rr] = buf_ld(rs_q[1], ox == OOR || z >= n ? OOR : ox + (uint32_t)z * 4u, 0);
+          G.q[2][rr] = buf_ld(rs_q[2], oy == OOR || z >= n ? OOR : oy + (uint32_t)z * 4u, 0);
         }
-        v = v / U[i * 5];
-      } else {
-        v = (v - U[i * 5 + 1] * Y0 - U[i * 5 + 2] * Y1 - U[i * 5 + 3] * Y2 - U[i * 5 + 4] * Y3) / U[i * 5];
       }
-      obuf[lane * ZSP + (i - i0)] = v;
-      Y3 = Y2, Y2 = Y1, Y1 = Y0, Y0 = v;
+    };
+    auto finish_grp = [&](const Grp &G, int r0, bool with_pml) __attribute__((always_inline)) {
+      const int gr = with_pml ? GRP : GR;
+#pragma unroll
+      for(int rr = 0; rr < GR; rr++) {
+        if(rr >= gr) break;
+        const int r = r0 + rr;
+        const int l = r * ZLPR + jl;
+        float g = buf[l * ZSP + kl];
+        if(with_pml) {
+          const uint32_t oo = l < nl ? ov + (uint32_t)(r * ZLPR * ow) * 4u : OOR;  // ow = 0 outside: stays OOR
+          float qi = oa * G.q[0][rr] - ob * g;
+          float rv = qi + g / od;
+          g = obase != OOR && l < nl ? rv : g;
+          buf_st(qi, rs_q[0], oo, 0);
+          const ZCross cx = lco[0][l], cy = lco[1][l];
+          qi = cx.a * G.q[1][rr] - cx.b * g;
+          rv = qi + g / cx.d;
+          g = cx.off != OOR ? rv : g;
+          // a sample past the end of the line would land on the next layer row's state
+          buf_st(qi, rs_q[1], cx.off == OOR || z >= n ? OOR : cx.off + (uint32_t)z * 4u, 0);
+          qi = cy.a * G.q[2][rr] - cy.b * g;
+          rv = qi + g / cy.d;
+          g = cy.off != OOR ? rv : g;
+          buf_st(qi, rs_q[2], cy.off == OOR || z >= n ? OOR : cy.off + (uint32_t)z * 4u, 0);
+        }
+        if(PASS == P_GRAD) {
+          buf_st(g, rs_tmp, vc, soff(r));
+        } else if(PASS == P_DIV) {
+          buf_st(G.o[0][rr] - g * G.o[1][rr], rs_pb, vc, soff(r));
+        } else {
+          const float w = G.o[0][rr] - g * G.o[1][rr];
+          buf_st((G.o[2][rr] + G.o[2][rr]) - w, rs_pb, vc, soff(r));
+        }
+      }
+    };
+    // epilogue operands of the first group are in flight during the solve
+    Grp g0, g1;
+    if(pml) load_grp(g0, 0, true);
+    else load_grp(g0, 0, false);
+    if(i1 - 1 < n - 4) {
+      // full chunk (only the last chunk can be short, and it holds the edge rows)
+#pragma unroll 1
+      for(int jb = ZC - ZU; jb >= 0; jb -= ZU) {
+#pragma unroll
+        for(int jj = ZU - 1; jj >= 0; jj--) {
+          const int j = jb + jj, i = i0 + j;
+          float v = (row[j] - U[i * 5 + 1] * Y0 - U[i * 5 + 2] * Y1 - U[i * 5 + 3] * Y2 - U[i * 5 + 4] * Y3) / U[i * 5];
+          row[j] = v;
+          Y3 = Y2, Y2 = Y1, Y1 = Y0, Y0 = v;
+        }
+      }
+    } else {
+      for(int i = i1 - 1; i >= i0; i--) {
+        float v = row[i - i0];
+        if(i >= n - 4) {
+          switch(n - 1 - i) {
+          case 0: break;
+          case 1: v = v - U[i * 5 + 1] * Y0; break;
+          case 2:
+            v = v - U[i * 5 + 1] * Y0;
+            v = v - U[i * 5 + 2] * Y1;
+            break;
+          default:
+            v = v - U[i * 5 + 1] * Y0;
+            v = v - U[i * 5 + 2] * Y1;
+            v = v - U[i * 5 + 3] * Y2;
+            break;
+          }
+          v = v / U[i * 5];
+        } else {
+          v = (v - U[i * 5 + 1] * Y0 - U[i * 5 + 2] * Y1 - U[i * 5 + 3] * Y2 - U[i * 5 + 4] * Y3) / U[i * 5];
+        }
+        row[i - i0] = v;
+        Y3 = Y2, Y2 = Y1, Y1 = Y0, Y0 = v;
+      }
     }
     lds_barrier();
-    for(int r0 = 0; r0 < 32; r0 += 8) {
-      float ob[8], pbv[8], vv[8], wv[8];
-      Pml3 pc[8];
+    static_assert(ZC % (2 * GR) == 0, "epilogue groups alternate between two register sets");
+    if(pml) {
 #pragma unroll
-      for(int r = 0; r < 8; r++) {
-        const long L = line0 + 2 * (r0 + r) + jl;
-        const int z = i0 + kl;
-        pc[r].n = 0;
-        if(L < nlines && z < i1) {
-          const int ix = (int)(L % nx), iy = (int)(L / nx);
-          const size_t cidx = (size_t)L * n + z;
-          ob[r] = obuf[(2 * (r0 + r) + jl) * ZSP + kl];
-          if(a.has_pml) pml_load<0>(a.pml, a.round, ix, iy, z, nx, ny, nz, pc[r]);
-          if(PASS != P_GRAD) pbv[r] = pb[cidx], vv[r] = vel[cidx];
-          if(PASS == P_DIV_STEP) wv[r] = w1[cidx];
-        }
+      for(int r0 = 0; r0 < ZC; r0 += 2 * GRP) {
+        load_grp(g1, r0 + GRP, true);
+        finish_grp(g0, r0, true);
+        if(r0 + 2 * GRP < ZC) load_grp(g0, r0 + 2 * GRP, true);
+        finish_grp(g1, r0 + GRP, true);
       }
+    } else {
 #pragma unroll
-      for(int r = 0; r < 8; r++) {
-        const long L = line0 + 2 * (r0 + r) + jl;
-        const int z = i0 + kl;
-        if(L < nlines && z < i1) {
-          const int ix = (int)(L % nx), iy = (int)(L / nx);
-          const size_t cidx = (size_t)L * n + z;
-          float g = pc[r].n ? pml_apply<0>(a.pml, a.round, ix, iy, z, nx, nz, pc[r], ob[r]) : ob[r];
-          if(PASS == P_GRAD) tmp[cidx] = g;
-          else if(PASS == P_DIV) pb[cidx] = pbv[r] - g * vv[r];
-          else {
-            float w = pbv[r] - g * vv[r];
-            pb[cidx] = (wv[r] + wv[r]) - w;
-          }
-        }
+      for(int r0 = 0; r0 < ZC; r0 += 2 * GR) {
+        load_grp(g1, r0 + GR, false);
+        finish_grp(g0, r0, false);
+        if(r0 + 2 * GR < ZC) load_grp(g0, r0 + 2 * GR, false);
+        finish_grp(g1, r0 + GR, false);
       }
     }
+    lds_barrier();
     if(cc + 2 < nch) {
-      regs_to_lds(xin[cc & 1]);
-      if(cc + 3 < nch) load_regs(tmp, c - 3);
+      regs_to_lds(buf);
+      if(cc + 3 < nch) load_regs(rs_tmp, c - 3);
     }
     lds_barrier();
   }
 }
-
 // ---------------------------------------------------------------------------
 // host side
 
@@ -778,6 +935,24 @@ static int cfd_run_m(Engine &e, int mode, const float *p0, float *pb, float *pr)
   }
   if(step) return launch_zline<P_DIV_STEP>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z");
   return launch_zline<P_DIV>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z");
+}
+
+// FdEngine_cfd::d{x,y,z}1 over the whole volume (bgprtm/FdEngine_cfd.cpp:24-297): out = D(in) with the
+// PML of that component (isign > 0: gradient plan, PML round 1; else divergence plan, round 2), or with
+// iacc out -= vel*D(in).  tmp holds b' when accumulating (in is left untouched).
+int cfd_deriv1(Engine &e, int axis, int iacc, int isign, const float *in, float *out, float *tmp) {
+  const int sg = isign > 0 ? 0 : 1, rnd = sg;
+  float *t = iacc ? tmp : out;
+  if(axis == 0) {
+    if(iacc) return launch_zline<P_DIV>(e, sg, rnd, in, t, out, nullptr, "cfd_deriv1_z");
+    return launch_zline<P_GRAD>(e, sg, rnd, in, t, nullptr, nullptr, "cfd_deriv1_z");
+  }
+  if(axis == 1) {
+    if(iacc) return launch_line<AXX, P_DIV, KLRTM_ISO>(e, sg, rnd, in, t, nullptr, nullptr, out, nullptr, "cfd_deriv1_x");
+    return launch_line<AXX, P_GRAD, KLRTM_ISO>(e, sg, rnd, in, t, nullptr, nullptr, nullptr, nullptr, "cfd_deriv1_x");
+  }
+  if(iacc) return launch_line<AXY, P_DIV, KLRTM_ISO>(e, sg, rnd, in, t, nullptr, nullptr, out, nullptr, "cfd_deriv1_y");
+  return launch_line<AXY, P_GRAD, KLRTM_ISO>(e, sg, rnd, in, t, nullptr, nullptr, nullptr, nullptr, "cfd_deriv1_y");
 }
 
 int cfd_run(Engine &e, int model, int mode, const float *p0, float *pb, float *pr) {

@@ -243,6 +243,8 @@ int klrtm_kernel_create(klrtm_kernel **out, int nx, int ny, int nz, float dx, fl
   *out = nullptr;
   KL_REQUIRE(nx > 2 && nz > 2 && ny >= 1, KLRTM_EINVAL, "The current code requires nx(%d), nz(%d) > 2", nx, nz);
   KL_REQUIRE(dx > 0 && dy > 0 && dz > 0, KLRTM_EINVAL, "grid spacing must be positive");
+  KL_REQUIRE((size_t)nx * nz * sizeof(float) < (size_t(1) << 31), KLRTM_EINVAL,
+             "an x-z plane of %d x %d cells exceeds the 2 GiB the stencil kernels address per plane", nx, nz);
   KL_REQUIRE(engine == KLRTM_ENGINE_FD || engine == KLRTM_ENGINE_CFD, KLRTM_EINVAL, "unknown engine %d", engine);
   if(engine == KLRTM_ENGINE_FD) {
     if(order < 4) order = 8;  // bgprtm/FdEngine_fd.cpp:146
@@ -314,6 +316,33 @@ int klrtm_kernel_step(klrtm_kernel *k, int model, const float *w1, float *w0, fl
   KL_REQUIRE(k, KLRTM_EINVAL, "null engine");
   KL_REQUIRE(model >= KLRTM_ISO && model <= KLRTM_TTI, KLRTM_EINVAL, "unknown model type %d", model);
   return reinterpret_cast<Engine *>(k)->run(model, MODE_STEP, w1, w0, wr);
+}
+int klrtm_kernel_laplacian(klrtm_kernel *kk, const float *in, float *out) {
+  Engine *k = reinterpret_cast<Engine *>(kk);
+  KL_REQUIRE(k && in && out, KLRTM_EINVAL, "null argument");
+  KL_REQUIRE(k->engine == KLRTM_ENGINE_FD, KLRTM_EINVAL, "laplacian() is an FD-engine operation");
+  KL_REQUIRE(in != out, KLRTM_EINVAL, "laplacian() cannot run in place");
+  IsoArgs a;
+  k->fill_args(a, 2);
+  return launch_iso(k->ne, MODE_LAP, in, nullptr, nullptr, out, a);
+}
+int klrtm_kernel_deriv1(klrtm_kernel *kk, int axis, const float *in, float *out, int iacc, int isign) {
+  Engine *k = reinterpret_cast<Engine *>(kk);
+  KL_REQUIRE(k && in && out, KLRTM_EINVAL, "null argument");
+  KL_REQUIRE(axis >= 0 && axis <= 2, KLRTM_EINVAL, "axis must be 0 (z), 1 (x) or 2 (y)");
+  KL_REQUIRE(iacc == 0 || k->vel, KLRTM_ESTATE, "iacc needs the model (setModel)");
+  KL_REQUIRE(axis != 2 || k->ny > 1, KLRTM_EINVAL, "no y derivative on a 2D grid");
+  int rc;
+  if((rc = k->ensure_comps(k->pml3d ? 3 : 1))) return rc;
+  if(k->engine == KLRTM_ENGINE_CFD) {
+    KL_REQUIRE(in != out || !iacc, KLRTM_EINVAL, "accumulating derivative cannot run in place");
+    if(iacc && (rc = k->scratch())) return rc;
+    return cfd_deriv1(*k, axis, iacc, isign, in, out, k->d_px);
+  }
+  KL_REQUIRE(in != out, KLRTM_EINVAL, "FD derivative cannot run in place");
+  IsoArgs a;
+  k->fill_args(a, 1);
+  return launch_deriv1(k->ne, axis, iacc, isign > 0 ? 0 : 1, in, out, k->vel, a);
 }
 int klrtm_kernel_set_tile(klrtm_kernel *kk, int tz, int tx, int ychunk) {
   Engine *k = reinterpret_cast<Engine *>(kk);

@@ -4,7 +4,7 @@
 
 namespace klrtm {
 
-enum { MODE_KERNEL = 0, MODE_STEP = 1 };
+enum { MODE_KERNEL = 0, MODE_STEP = 1, MODE_LAP = 2 };  // pb -= vel*L (+pr = L) | + leapfrog | pr = L only
 
 // Arguments of the y-streaming stencil kernels (passed by value).
 struct IsoArgs {
@@ -22,6 +22,7 @@ int launch_grad(int R, int model, const float *u, float *px, float *py, float *p
                 const float *pjx, const float *pjy, const IsoArgs &a);
 int launch_div(int R, int mode, const float *px, const float *py, const float *pz, const float *w1, float *w0,
                const float *vel, const IsoArgs &a);
+int launch_deriv1(int R, int axis, int iacc, int round, const float *in, float *out, const float *vel, const IsoArgs &a);
 int launch_vel2(float *v, size_t n, float dt);
 int launch_update2nd(float *w0, const float *w1, size_t n);
 

@@ -17,6 +17,7 @@
 #include "klrtm_internal.h"
 #include "fd_kernels.h"
 #include "stencil_common.h"
+#include <type_traits>
 
 namespace klrtm {
 
@@ -61,7 +62,25 @@ __device__ __forceinline__ Halo load_halo(const float *__restrict__ u, size_t pl
 }
 
 // ---------------------------------------------------------------------------
-// ISO Laplacian step
+// ISO Laplacian step, streaming version.
+//
+// gfx9 retires vector-memory operations in issue order, so the next plane's
+// loads only overlap the current plane if nothing issued after them must be
+// waited on first.  Two register sets hold alternate planes (a copy out of a
+// register that a load is still filling would wait for it), every global
+// access goes through a per-plane buffer resource with out-of-range offsets
+// for lanes that have nothing to load or store (no exec-mask branches), and
+// the PML state of the next plane is loaded with its data.  PML layers are
+// handled by y segment (inside a y layer or not) and by a block-uniform flag
+// for z/x layers, so interior blocks carry no PML work at all.  The layers of
+// one axis never overlap (setBoundary rejects that) and a plane is addressed
+// with 31-bit byte offsets (checked when the engine is created).
+struct IsoPlane {
+  float qa, w, v;          // u at y+R (own column), w0 and vel at y
+  float hz0, hz1, hx0, hx1;  // z and x halos of u at y
+  float sz[2], sx[2], sy[2];  // PML state (round 1, round 2) of the z, x and y layers
+};
+
 template<int R, int MODE>
 __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict__ u, float *__restrict__ w0,
                                                         const float *__restrict__ vel, float *__restrict__ pr,
@@ -70,8 +89,59 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
   const int S = a.tz + 2 * R;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t nxz = (size_t)nx * nz;
+  const uint32_t pbytes = (uint32_t)(nxz * 4);
   const TileCoord c = tile_coord(a);
   const int ix = c.ix, iz = c.iz;
+  auto prs = [&](const float *base, int y) { return buf_rsrc(base + (size_t)y * nxz, pbytes); };
+  const uint32_t oc = c.act ? (uint32_t)c.col * 4u : OOR;
+  const uint32_t oz0 = (c.tz < R && ix < nx && iz - R >= 0) ? (uint32_t)(c.col - R) * 4u : OOR;
+  const uint32_t oz1 = (c.tz >= a.tz - R && ix < nx && iz + R < nz) ? (uint32_t)(c.col + R) * 4u : OOR;
+  const uint32_t ox0 = (c.tx < R && iz < nz && ix - R >= 0) ? (uint32_t)(c.col - (size_t)R * nz) * 4u : OOR;
+  const uint32_t ox1 = (c.tx >= a.tx - R && iz < nz && ix + R < nx) ? (uint32_t)(c.col + (size_t)R * nz) * 4u : OOR;
+
+  // ---- PML of this column: z and x layers are fixed per thread (state offset of plane 0 and stride per plane)
+  const PmlDesc &P = a.pml;
+  __amdgpu_buffer_rsrc_t rq[3][2];  // [z, x, y][round]: the two layers of the axis
+  uint32_t pair1[3] = {0, 0, 0};    // cells of layer 1 (layer 2 starts there)
+  uint32_t zo = OOR, zs = 0, xo = OOR, xs = 0;
+  float zc[3] = {0.f, 0.f, 1.f}, xc[3] = {0.f, 0.f, 1.f};
+  bool xz_any = false;
+  if(a.has_pml) {
+    const int fa[3] = {FZ1, FX1, FY1};
+#pragma unroll
+    for(int k = 0; k < 3; k++) {
+      const int f1 = fa[k];
+#pragma unroll
+      for(int r = 0; r < 2; r++) {
+        float *q1 = P.q[f1][r][0], *q2 = P.q[f1 + 1][r][0];
+        const uint32_t c1 = q1 ? face_cells_dev(P, f1, nx, ny, nz) : 0u, c2 = q2 ? face_cells_dev(P, f1 + 1, nx, ny, nz) : 0u;
+        pair1[k] = c1;
+        rq[k][r] = buf_rsrc(q1 ? q1 : q2, (c1 + c2) * 4u);
+      }
+    }
+    if(c.act) {
+      int f = -1, j = 0;
+      if(P.q[FZ1][0][0] && iz < P.w[FZ1]) f = FZ1, j = iz;
+      else if(P.q[FZ2][0][0] && iz >= nz - P.w[FZ2]) f = FZ2, j = iz - (nz - P.w[FZ2]);
+      if(f >= 0) {
+        const int w = P.w[f];
+        zo = ((f == FZ2 ? pair1[0] : 0u) + (uint32_t)ix * w + j) * 4u;  // ((iy*nx + ix)*w + j)
+        zs = (uint32_t)nx * w * 4u;
+        zc[0] = P.abc[f][0][3 * j], zc[1] = P.abc[f][0][3 * j + 1], zc[2] = P.abc[f][0][3 * j + 2];
+      }
+      f = -1;
+      if(P.q[FX1][0][0] && ix < P.w[FX1]) f = FX1, j = ix;
+      else if(P.q[FX2][0][0] && ix >= nx - P.w[FX2]) f = FX2, j = ix - (nx - P.w[FX2]);
+      if(f >= 0) {
+        const int w = P.w[f];
+        xo = ((f == FX2 ? pair1[1] : 0u) + (uint32_t)j * nz + iz) * 4u;  // ((iy*w + j)*nz + iz)
+        xs = (uint32_t)w * nz * 4u;
+        xc[0] = P.abc[f][0][3 * j], xc[1] = P.abc[f][0][3 * j + 1], xc[2] = P.abc[f][0][3 * j + 2];
+      }
+    }
+    xz_any = __syncthreads_or(zo != OOR || xo != OOR) != 0;
+  }
+
   float q[2 * R + 1];
 #pragma unroll
   for(int k = 0; k < 2 * R; k++) {
@@ -79,34 +149,52 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
     q[k] = (c.act && yy >= 0 && yy < ny) ? u[(size_t)yy * nxz + c.col] : 0.f;
   }
   float *srow = s + (c.tx + R) * S + c.tz + R;
-  // prefetch of plane y0
-  float qn = 0.f, wn = 0.f, vn = 0.f;
-  Halo hn = load_halo<R>(u, (size_t)c.y0 * nxz, c, a, true, true);
-  if(c.act) {
-    if(c.y0 + R < ny) qn = u[(size_t)(c.y0 + R) * nxz + c.col];
-    wn = w0[(size_t)c.y0 * nxz + c.col], vn = vel[(size_t)c.y0 * nxz + c.col];
-  }
-  for(int iy = c.y0; iy < c.y1; iy++) {
-    const size_t pl = (size_t)iy * nxz;
-    q[2 * R] = qn;
-    const Halo h = hn;
-    const float wv = wn, vv = vn;
-    if(iy + 1 < c.y1) {  // issue next plane's loads; they stay in flight across the barriers
-      const size_t pn = pl + nxz;
-      hn = load_halo<R>(u, pn, c, a, true, true);
-      if(c.act) {
-        qn = (iy + 1 + R < ny) ? u[pn + (size_t)R * nxz + c.col] : 0.f;
-        wn = w0[pn + c.col], vn = vel[pn + c.col];
+
+  auto round2 = [](float t0, float t1, float t2, float p, float &q1, float &q2) {
+    float qi = t0 * q1 - t1 * p;
+    q1 = qi;
+    p = qi + p * t2;
+    qi = t0 * q2 - t1 * p;
+    q2 = qi;
+    return qi + p * t2;
+  };
+
+  // planes [ya, yb) of one segment; XZ: z/x layer work, YL: inside y layer fy (position iy - jy0)
+  auto segment = [&](auto xz_tag, auto yl_tag, int ya, int yb, int fy, int jy0) __attribute__((always_inline)) {
+    constexpr bool XZ = decltype(xz_tag)::value, YL = decltype(yl_tag)::value;
+    const uint32_t yoff = (YL && fy == FY2) ? pair1[2] : 0u;
+    auto fetch = [&](IsoPlane &N, int y) __attribute__((always_inline)) {
+      const __amdgpu_buffer_rsrc_t ru = prs(u, y);
+      N.qa = buf_ld(prs(u, min(y + R, ny - 1)), y + R < ny ? oc : OOR, 0);
+      if(MODE != MODE_LAP) {
+        N.w = buf_ld(prs(w0, y), oc, 0);
+        N.v = buf_ld(prs(vel, y), oc, 0);
       }
-    }
-    lds_barrier();
-    srow[0] = q[R];
-    if(c.tz < R) srow[-R] = h.z0;
-    if(c.tz >= a.tz - R) srow[R] = h.z1;
-    if(c.tx < R) srow[-R * S] = h.x0;
-    if(c.tx >= a.tx - R) srow[R * S] = h.x1;
-    lds_barrier();
-    if(c.act) {
+      N.hz0 = buf_ld(ru, oz0, 0), N.hz1 = buf_ld(ru, oz1, 0);
+      N.hx0 = buf_ld(ru, ox0, 0), N.hx1 = buf_ld(ru, ox1, 0);
+      if(XZ) {
+#pragma unroll
+        for(int r = 0; r < 2; r++) {
+          N.sz[r] = buf_ld(rq[0][r], zo == OOR ? OOR : zo + (uint32_t)y * zs, 0);
+          N.sx[r] = buf_ld(rq[1][r], xo == OOR ? OOR : xo + (uint32_t)y * xs, 0);
+        }
+      }
+      if(YL) {
+        const uint32_t so = (yoff + (uint32_t)(y - jy0) * (uint32_t)nxz) * 4u;
+#pragma unroll
+        for(int r = 0; r < 2; r++) N.sy[r] = buf_ld(rq[2][r], oc, so);
+      }
+    };
+    auto plane = [&](IsoPlane &C, IsoPlane &N, int y) __attribute__((always_inline)) {
+      fetch(N, min(y + 1, yb - 1));  // the last plane of the segment fetches itself again (unused)
+      q[2 * R] = C.qa;
+      lds_barrier();
+      srow[0] = q[R];
+      if(c.tz < R) srow[-R] = C.hz0;
+      if(c.tz >= a.tz - R) srow[R] = C.hz1;
+      if(c.tx < R) srow[-R * S] = C.hx0;
+      if(c.tx >= a.tx - R) srow[R * S] = C.hx1;
+      lds_barrier();
       const float ctr = q[R];
       // z part, reference order m = -R..-1, 1..R (bgprtm/FdEngine_fd.cpp:223-236)
       float lz = a.cz[0] * ctr;
@@ -131,33 +219,63 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
         }
         L = L + ly;
       }
-      // PML on the summed Laplacian, order z, x, y (bgprtm/FdEngine.cpp:391-395)
-      if(a.has_pml) {
-        const PmlDesc &P = a.pml;
-        const int faces[6] = {FZ1, FZ2, FX1, FX2, FY1, FY2};
-#pragma unroll
-        for(int k = 0; k < 6; k++) {
-          int f = faces[k];
-          int j = face_pos(P, f, ix, iy, iz, nx, ny, nz);
-          if(j >= 0) {
-            const float *t = P.abc[f][0] + 3 * j;
-            size_t st = state_index(f, P.w[f], j, ix, iy, iz, nx, nz);
-            float q1 = P.q[f][0][0][st], q2 = P.q[f][1][0][st];
-            L = pml_round_abc(t, L, q1);
-            L = pml_round_abc(t, L, q2);
-            P.q[f][0][0][st] = q1, P.q[f][1][0][st] = q2;
-          }
-        }
+      // PML on the summed Laplacian, layers in the order z, x, y (bgprtm/FdEngine.cpp:391-395)
+      if(XZ) {
+        float s1 = C.sz[0], s2 = C.sz[1];
+        float Lz = round2(zc[0], zc[1], zc[2], L, s1, s2);
+        L = zo != OOR ? Lz : L;
+        const uint32_t o = zo == OOR ? OOR : zo + (uint32_t)y * zs;
+        buf_st(s1, rq[0][0], o, 0), buf_st(s2, rq[0][1], o, 0);
+        s1 = C.sx[0], s2 = C.sx[1];
+        float Lx = round2(xc[0], xc[1], xc[2], L, s1, s2);
+        L = xo != OOR ? Lx : L;
+        const uint32_t ox = xo == OOR ? OOR : xo + (uint32_t)y * xs;
+        buf_st(s1, rq[1][0], ox, 0), buf_st(s2, rq[1][1], ox, 0);
+      }
+      if(YL) {
+        const cfloat *t = (const cfloat *)P.abc[fy][0] + 3 * (y - jy0);
+        float s1 = C.sy[0], s2 = C.sy[1];
+        L = round2(t[0], t[1], t[2], L, s1, s2);
+        const uint32_t so = (yoff + (uint32_t)(y - jy0) * (uint32_t)nxz) * 4u;
+        buf_st(s1, rq[2][0], oc, so), buf_st(s2, rq[2][1], oc, so);
       }
       if(MODE == MODE_STEP) {
-        w0[pl + c.col] = (ctr + ctr) - fmaf(-vv, L, wv);
+        buf_st((ctr + ctr) - fmaf(-C.v, L, C.w), prs(w0, y), oc, 0);
+      } else if(MODE == MODE_KERNEL) {
+        buf_st(L, prs(pr, y), oc, 0);
+        buf_st(fmaf(-L, C.v, C.w), prs(w0, y), oc, 0);
       } else {
-        pr[pl + c.col] = L;
-        w0[pl + c.col] = fmaf(-L, vv, wv);
+        buf_st(L, prs(pr, y), oc, 0);
       }
-    }
 #pragma unroll
-    for(int k = 0; k < 2 * R; k++) q[k] = q[k + 1];
+      for(int k = 0; k < 2 * R; k++) q[k] = q[k + 1];
+    };
+    IsoPlane A, B;
+    fetch(A, ya);
+    for(int y = ya;;) {
+      plane(A, B, y);
+      if(++y >= yb) break;
+      plane(B, A, y);
+      if(++y >= yb) break;
+    }
+  };
+  using T_ = std::true_type;
+  using F_ = std::false_type;
+  // y segments: layer 1 [0, wy1), interior, layer 2 [ny - wy2, ny)
+  const int wy1 = (a.has_pml && P.q[FY1][0][0]) ? P.w[FY1] : 0;
+  const int wy2 = (a.has_pml && P.q[FY2][0][0]) ? P.w[FY2] : 0;
+  const int s1e = min(c.y1, max(c.y0, wy1)), s3b = max(s1e, min(c.y1, ny - wy2));
+  if(c.y0 < s1e) {
+    if(xz_any) segment(T_(), T_(), c.y0, s1e, FY1, 0);
+    else segment(F_(), T_(), c.y0, s1e, FY1, 0);
+  }
+  if(s1e < s3b) {
+    if(xz_any) segment(T_(), F_(), s1e, s3b, 0, 0);
+    else segment(F_(), F_(), s1e, s3b, 0, 0);
+  }
+  if(s3b < c.y1) {
+    if(xz_any) segment(T_(), T_(), s3b, c.y1, FY2, ny - wy2);
+    else segment(F_(), T_(), s3b, c.y1, FY2, ny - wy2);
   }
 }
 
@@ -335,6 +453,66 @@ __global__ void __launch_bounds__(1024) div_kernel(const float *__restrict__ px,
 }
 
 // ---------------------------------------------------------------------------
+// Single-axis first derivative with the PML of that component and optional
+// accumulate, FdEngine::d{x,y,z}1 over a whole volume (bgprtm/FdEngine_fd.cpp:
+// 330-470, bgprtm/FdEngine.cpp:301-350): out = D(in), or out -= vel*D(in).
+// One thread per cell, neighbours through the caches: this is the API form of
+// what grad_kernel/div_kernel fuse for the step.
+template<int R, int AXIS, int IACC>
+__global__ void __launch_bounds__(256) fd_deriv1_kernel(const float *__restrict__ in, float *__restrict__ out,
+                                                        const float *__restrict__ vel, const IsoArgs a, int round) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const size_t nxz = (size_t)nx * nz;
+  if(i >= (long)nxz * ny) return;
+  const int iz = (int)(i % nz), ix = (int)((i / nz) % nx), iy = (int)(i / (long)nxz);
+  const int pos = AXIS == 0 ? iz : AXIS == 1 ? ix : iy, np = AXIS == 0 ? nz : AXIS == 1 ? nx : ny;
+  const size_t st = AXIS == 0 ? 1 : AXIS == 1 ? (size_t)nz : nxz;
+  const float *c = AXIS == 0 ? a.cz : AXIS == 1 ? a.cx : a.cy;
+  float d = 0.f;
+  if(np > 1) {
+#pragma unroll
+    for(int m = R; m >= 1; m--) d = fmaf(-c[m], pos - m >= 0 ? in[i - m * st] : 0.f, d);
+#pragma unroll
+    for(int m = 1; m <= R; m++) d = fmaf(c[m], pos + m < np ? in[i + m * st] : 0.f, d);
+    if(a.has_pml) {
+      if(AXIS == 0) d = pml1_z<true>(a.pml, round, d, ix, iy, iz, nx, ny, nz);
+      else if(AXIS == 1) d = pml1_x<true>(a.pml, round, d, ix, iy, iz, nx, ny, nz);
+      else d = pml1_y<true>(a.pml, round, d, ix, iy, iz, nx, ny, nz);
+    }
+  }
+  if(IACC) out[i] = fmaf(-d, vel[i], out[i]);
+  else out[i] = d;
+}
+
+template<int R, int AXIS>
+static int launch_deriv1_ra(int iacc, int round, const float *in, float *out, const float *vel, const IsoArgs &a) {
+  const long n = (long)a.nx * a.ny * a.nz;
+  LaunchTimer tm("fd_deriv1");
+  if(iacc)
+    hipLaunchKernelGGL((fd_deriv1_kernel<R, AXIS, 1>), dim3(div_up(n, 256)), dim3(256), 0, stream(), in, out, vel, a, round);
+  else
+    hipLaunchKernelGGL((fd_deriv1_kernel<R, AXIS, 0>), dim3(div_up(n, 256)), dim3(256), 0, stream(), in, out, vel, a, round);
+  return check_launch("fd_deriv1_kernel");
+}
+template<int R>
+static int launch_deriv1_r(int axis, int iacc, int round, const float *in, float *out, const float *vel, const IsoArgs &a) {
+  if(axis == 0) return launch_deriv1_ra<R, 0>(iacc, round, in, out, vel, a);
+  if(axis == 1) return launch_deriv1_ra<R, 1>(iacc, round, in, out, vel, a);
+  return launch_deriv1_ra<R, 2>(iacc, round, in, out, vel, a);
+}
+int launch_deriv1(int R, int axis, int iacc, int round, const float *in, float *out, const float *vel, const IsoArgs &a) {
+  switch(R) {
+  case 4: return launch_deriv1_r<4>(axis, iacc, round, in, out, vel, a);
+  case 5: return launch_deriv1_r<5>(axis, iacc, round, in, out, vel, a);
+  case 6: return launch_deriv1_r<6>(axis, iacc, round, in, out, vel, a);
+  case 7: return launch_deriv1_r<7>(axis, iacc, round, in, out, vel, a);
+  case 8: return launch_deriv1_r<8>(axis, iacc, round, in, out, vel, a);
+  default: set_error("FD half order %d not supported (4..8)", R); return KLRTM_EINVAL;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 static void grid_of(IsoArgs &a) {
   a.nbz = div_up(a.nz, a.tz);
@@ -342,17 +520,20 @@ static void grid_of(IsoArgs &a) {
   a.nby = div_up(a.ny, a.ychunk);
 }
 
+template<int R, int MODE>
+static int launch_iso_rm(const float *u, float *w0, const float *vel, float *pr, const IsoArgs &a, int nb, size_t lds) {
+  hipLaunchKernelGGL((iso_step_kernel<R, MODE>), dim3(nb), dim3(a.tz * a.tx), lds, stream(), u, w0, vel, pr, a);
+  return check_launch("iso_step_kernel");
+}
 template<int R>
 static int launch_iso_r(int mode, const float *u, float *w0, const float *vel, float *pr, IsoArgs a) {
   grid_of(a);
   int nb = a.nbz * a.nbx * a.nby;
   size_t lds = sizeof(float) * (size_t)(a.tx + 2 * R) * (a.tz + 2 * R);
-  LaunchTimer tm(mode == MODE_STEP ? "iso_step" : "iso_kernel");
-  if(mode == MODE_STEP)
-    hipLaunchKernelGGL((iso_step_kernel<R, MODE_STEP>), dim3(nb), dim3(a.tz * a.tx), lds, stream(), u, w0, vel, pr, a);
-  else
-    hipLaunchKernelGGL((iso_step_kernel<R, MODE_KERNEL>), dim3(nb), dim3(a.tz * a.tx), lds, stream(), u, w0, vel, pr, a);
-  return check_launch("iso_step_kernel");
+  LaunchTimer tm(mode == MODE_STEP ? "iso_step" : mode == MODE_KERNEL ? "iso_kernel" : "laplacian");
+  if(mode == MODE_STEP) return launch_iso_rm<R, MODE_STEP>(u, w0, vel, pr, a, nb, lds);
+  if(mode == MODE_KERNEL) return launch_iso_rm<R, MODE_KERNEL>(u, w0, vel, pr, a, nb, lds);
+  return launch_iso_rm<R, MODE_LAP>(u, w0, vel, pr, a, nb, lds);
 }
 
 int launch_iso(int R, int mode, const float *u, float *w0, const float *vel, float *pr, const IsoArgs &a) {

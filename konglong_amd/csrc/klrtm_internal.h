@@ -90,6 +90,35 @@ __device__ __forceinline__ float pml_round_abc(const float *t, float p, float &q
   return qi + p * t[2];
 }
 
+// Tables that are read-only for a whole launch and indexed by wave-uniform
+// values (plan rows, PML profiles): in the constant address space they are
+// fetched with scalar loads, which keeps them out of the in-order
+// vector-memory counter that streamed data waits on.
+typedef const __attribute__((address_space(4))) float cfloat;
+__device__ __forceinline__ const cfloat *as_const(const float *p) { return (const cfloat *)p; }
+
+// Layer cells of face f (layout of state_index()).
+__device__ __forceinline__ uint32_t face_cells_dev(const PmlDesc &P, int f, int nx, int ny, int nz) {
+  const uint32_t w = (uint32_t)max(P.w[f], 0);
+  if(f == FZ1 || f == FZ2) return w * (uint32_t)nx * (uint32_t)ny;
+  if(f == FX1 || f == FX2) return w * (uint32_t)ny * (uint32_t)nz;
+  return w * (uint32_t)nx * (uint32_t)nz;
+}
+
+// Raw buffer access (gfx9 buffer resource, stride 0): an offset at or past
+// the resource's byte size reads 0 and drops the store, so lanes outside the
+// data get OOR instead of a branch.  The SGPR offset must be wave-uniform.
+constexpr uint32_t OOR = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, voff, soff, 0);
+}
+
 // Workgroup barrier for LDS hand-off only: waits for this wave's LDS accesses
 // (lgkmcnt) but not for outstanding global loads, so prefetches issued before
 // the barrier stay in flight (__syncthreads would drain them).

@@ -122,29 +122,59 @@ struct GroupArgs {
   int offz, lz, bx0, by0, bw, bh, lxmax, lymax, L;
 };
 
-// T[iy][ix] = sum_r (val_r*ky_r)*kx_r over receivers covering (ix,iy)
-__global__ void inject_plane_kernel(const GroupArgs a, const int *__restrict__ bin_start, const int *__restrict__ bin_rcv,
-                                    const float *__restrict__ kx, const float *__restrict__ ky,
-                                    const int *__restrict__ lx, const int *__restrict__ ly,
-                                    const int *__restrict__ offx, const int *__restrict__ offy,
-                                    const float *__restrict__ val, float scaler, float *__restrict__ plane) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if(c >= a.bw * a.bh) return;
-  int px = c % a.bw, py = c / a.bw;
-  int ix = a.bx0 + px, iy = a.by0 + py;
+// T[iy][ix] = sum_r (val_r*ky_r)*kx_r over receivers covering (ix,iy).
+// One block owns a 16x16 tile of the plane.  The receivers whose kernels can
+// reach the tile are those with origin bins in [tile - lmax + 1, tile end]:
+// for each bin row they are one contiguous CSR range, staged through LDS in
+// chunks (scaled value, origin, extents, x and y taps), and every cell of
+// the tile walks the chunk.  Bin rows ascend and each range is in CSR order,
+// so a cell adds its terms in the same order as a per-cell walk of the bins.
+constexpr int ITX = 16, ITY = 16;
+__global__ void __launch_bounds__(ITX *ITY)
+    inject_plane_kernel(const GroupArgs a, const int *__restrict__ bin_start, const int *__restrict__ bin_rcv,
+                        const float *__restrict__ kx, const float *__restrict__ ky, const int *__restrict__ lx,
+                        const int *__restrict__ ly, const int *__restrict__ offx, const int *__restrict__ offy,
+                        const float *__restrict__ val, float scaler, float *__restrict__ plane, int ICH) {
+  extern __shared__ float smem[];
+  float *s_val = smem;
+  int *s_ox = (int *)(s_val + ICH), *s_oy = s_ox + ICH, *s_lx = s_oy + ICH, *s_ly = s_lx + ICH;
+  float *s_kx = (float *)(s_ly + ICH), *s_ky = s_kx + ICH * a.L;
+  const int tid = threadIdx.x;
+  const int ntx = (a.bw + ITX - 1) / ITX;
+  const int tx0 = (blockIdx.x % ntx) * ITX, ty0 = (blockIdx.x / ntx) * ITY;
+  const int px = tx0 + tid % ITX, py = ty0 + tid / ITX;
+  const int jx0 = a.bx0 + px, jy0 = a.by0 + py;  // absolute cell
+  const int qx_lo = max(0, tx0 - a.lxmax + 1), qx_hi = min(a.bw - 1, tx0 + ITX - 1);
+  const int qy_lo = max(0, ty0 - a.lymax + 1), qy_hi = min(a.bh - 1, ty0 + ITY - 1);
   float T = 0.f;
-  int qy0 = max(0, py - a.lymax + 1), qx0 = max(0, px - a.lxmax + 1);
-  for(int qy = qy0; qy <= py; qy++)
-    for(int qx = qx0; qx <= px; qx++) {
-      int b = qy * a.bw + qx;
-      for(int k = bin_start[b]; k < bin_start[b + 1]; k++) {
-        int r = bin_rcv[k];
-        int jx = ix - offx[r], jy = iy - offy[r];
-        if(jx >= 0 && jy >= 0 && jx < lx[r] && jy < ly[r]) T += ((val[r] * scaler) * ky[(size_t)r * a.L + jy]) * kx[(size_t)r * a.L + jx];
+  for(int qy = qy_lo; qy <= qy_hi; qy++) {
+    const int k0 = bin_start[qy * a.bw + qx_lo], k1 = bin_start[qy * a.bw + qx_hi + 1];
+    for(int kb = k0; kb < k1; kb += ICH) {
+      const int cnt = min(ICH, k1 - kb);
+      __syncthreads();
+      for(int j = tid; j < cnt; j += ITX * ITY) {
+        const int r = bin_rcv[kb + j];
+        s_val[j] = val[r] * scaler;
+        s_ox[j] = offx[r], s_oy[j] = offy[r], s_lx[j] = lx[r], s_ly[j] = ly[r];
+      }
+      for(int e = tid; e < cnt * a.L; e += ITX * ITY) {
+        const int j = e / a.L, t = e - j * a.L;
+        const int r = bin_rcv[kb + j];
+        s_kx[e] = kx[(size_t)r * a.L + t];
+        s_ky[e] = ky[(size_t)r * a.L + t];
+      }
+      __syncthreads();
+      for(int j = 0; j < cnt; j++) {
+        const int ex = jx0 - s_ox[j], ey = jy0 - s_oy[j];
+        if(ex >= 0 && ey >= 0 && ex < s_lx[j] && ey < s_ly[j]) T += (s_val[j] * s_ky[j * a.L + ey]) * s_kx[j * a.L + ex];
       }
     }
-  plane[c] = T;
+  }
+  if(px < a.bw && py < a.bh) plane[py * a.bw + px] = T;
 }
+// receivers per LDS chunk: 48 KiB of staging at most
+static int inject_chunk(int L) { return std::max(8, std::min(128, 49152 / ((5 + 2 * L) * (int)sizeof(float)))); }
+static size_t inject_smem(int L) { return (size_t)inject_chunk(L) * (5 + 2 * L) * sizeof(float); }
 
 // w0[col, offz+k] += T*kz[k]*vel
 __global__ void inject_apply_kernel(const GroupArgs a, const float *__restrict__ plane, const float *__restrict__ kz,
@@ -207,8 +237,10 @@ int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scale
   for(auto &g : R.groups) {
     GroupArgs a = gargs(R, g);
     int cells = g.bw * g.bh;
-    hipLaunchKernelGGL(inject_plane_kernel, dim3(div_up(cells, 256)), dim3(256), 0, stream(), a, g.bin_start, g.bin_rcv,
-                       R.d_kx, R.d_ky, R.d_lx, R.d_ly, R.d_offx, R.d_offy, val, scaler, R.d_plane);
+    const int tiles = div_up(g.bw, ITX) * div_up(g.bh, ITY);
+    hipLaunchKernelGGL(inject_plane_kernel, dim3(tiles), dim3(ITX * ITY), inject_smem(R.L), stream(), a, g.bin_start,
+                       g.bin_rcv, R.d_kx, R.d_ky, R.d_lx, R.d_ly, R.d_offx, R.d_offy, val, scaler, R.d_plane,
+                       inject_chunk(R.L));
     long tot = (long)cells * g.lz;
     hipLaunchKernelGGL(inject_apply_kernel, dim3(div_up(tot, 256)), dim3(256), 0, stream(), a, R.d_plane, g.kz, w0, vel);
   }

@@ -64,3 +64,59 @@ class KernelCPU:
             self.close()
         except Exception:
             pass
+
+
+class FdEngine:
+    """FdEngine volume operations (bgprtm/FdEngine.h:45-119) on device arrays.
+
+    ``laplacian(in, out)`` (FD engines) and ``deriv1(axis, in, out, iacc, isign)``
+    (axis 0 z, 1 x, 2 y; iacc 1 accumulates ``out -= vel*D(in)`` with the model set
+    through ``setModel``).  The PML state is the engine's own, as in the reference.
+    """
+
+    ENGINE = ENGINE_FD
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, order=8):
+        self._k = KernelCPU(nx, ny, nz, dx, dy, dz, self.ENGINE, order)
+
+    def setModel(self, vel, rho=None, del_=None, eps=None, pjx=None, pjy=None):
+        self._k.setModel(vel, rho, del_, eps, pjx, pjy)
+
+    def setPmlOptions(self, **kw):
+        self._k.setPmlOptions(**kw)
+
+    def setBoundary(self, *a):
+        self._k.setBoundary(*a)
+
+    def cleanPMLMemory(self):
+        self._k.cleanPMLMemory()
+
+    def laplacian(self, u, out):
+        check(lib().klrtm_kernel_laplacian(self._k.h, u.ptr, out.ptr))
+
+    def deriv1(self, axis, u, out, iacc=0, isign=1):
+        check(lib().klrtm_kernel_deriv1(self._k.h, axis, u.ptr, out.ptr, iacc, isign))
+
+    def dz1(self, u, out, iacc=0, isign=1):
+        self.deriv1(0, u, out, iacc, isign)
+
+    def dx1(self, u, out, iacc=0, isign=1):
+        self.deriv1(1, u, out, iacc, isign)
+
+    def dy1(self, u, out, iacc=0, isign=1):
+        self.deriv1(2, u, out, iacc, isign)
+
+    def close(self):
+        self._k.close()
+
+
+class FdEngine_fd(FdEngine):
+    ENGINE = ENGINE_FD
+
+
+class FdEngine_cfd(FdEngine):
+    ENGINE = ENGINE_CFD
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, order=10):
+        super().__init__(nx, ny, nz, dx, dy, dz, 10)
+

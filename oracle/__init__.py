@@ -43,6 +43,8 @@ def lib():
             getattr(L, f"orc_engine_kernel_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
             getattr(L, f"orc_engine_step_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
             getattr(L, f"orc_engine_laplacian_{s}").argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+            getattr(L, f"orc_engine_deriv1_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                                            C.c_int]
             getattr(L, f"orc_update2nd_{s}").argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.orc_taylor_coef2.argtypes = [C.c_int, _pd]
         L.orc_taylor_coef1.argtypes = [C.c_int, _pd]
@@ -111,6 +113,14 @@ class Engine:
 
     def step(self, model, w1, w0, wr):
         getattr(lib(), f"orc_engine_step_{self.s}")(self.h, model, w1.ctypes.data, w0.ctypes.data, wr.ctypes.data)
+
+    def laplacian(self, u, out):
+        """FdEngine_fd::laplacian with the Laplacian-form PML (FD engines)."""
+        getattr(lib(), f"orc_engine_laplacian_{self.s}")(self.h, u.ctypes.data, out.ctypes.data)
+
+    def deriv1(self, axis, u, out, iacc, isign):
+        """One first derivative over the volume (axis 0 z, 1 x, 2 y) with its PML; iacc: out -= vel*D(u)."""
+        getattr(lib(), f"orc_engine_deriv1_{self.s}")(self.h, axis, u.ctypes.data, out.ctypes.data, iacc, isign)
 
     def close(self):
         if getattr(self, "h", None):

@@ -520,6 +520,18 @@ template<class T> struct Engine {
     if(iacc)
       for(size_t i = 0; i < nxz; i++) out[i] = out[i] - dst[i] * vslice[i];
   }
+  // one axis over the volume (0 z, 1 x, 2 y), as the slice drivers below apply it
+  void deriv1(int axis, const T *in, T *out, int iacc, int isign) {
+    if(axis == 2) {
+      dy_vol(in, out, iacc, isign, vel);
+      return;
+    }
+    for(int iy = 0; iy < ny; iy++) {
+      const size_t o = (size_t)iy * nxz;
+      if(axis == 0) dz_slice(in + o, out + o, iy, iacc, isign, vel ? vel + o : nullptr);
+      else dx_slice(in + o, out + o, iy, iacc, isign, vel ? vel + o : nullptr);
+    }
+  }
   // y-derivative of the volume: FdEngine_fd::dy_3D / FdEngine::dy_3D + FdEngine_cfd::dy1_2D
   void dy_vol(const T *in, T *out, int iacc, int isign, const T *v) {
     if(ny <= 1) return;
@@ -849,6 +861,10 @@ void orc_cfd_derive_d(int n, double d, int sgn, const double *x, double *b, int 
     ((Engine<T> *)h)->kernel(model, p0, pb, pr);                                                                      \
   }                                                                                                                   \
   void orc_engine_laplacian_##S(void *h, const T *in, T *out) { ((Engine<T> *)h)->laplacian(in, out); }             \
+  void orc_engine_deriv1_##S(void *h, int axis, const T *in, T *out, int iacc, int isign) {                          \
+    Engine<T> *e = (Engine<T> *)h;                                                                                    \
+    e->deriv1(axis, in, out, iacc, isign);                                                                            \
+  }                                                                                                                   \
   void orc_update2nd_##S(T *w0, const T *w1, size_t n) { update2nd(w0, w1, n, 1); }                                   \
   void orc_engine_step_##S(void *h, int model, const T *w1, T *w0, T *wr) {                                           \
     Engine<T> *e = (Engine<T> *)h;                                                                                    \

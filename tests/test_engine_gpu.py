@@ -84,6 +84,40 @@ def test_step_parity(klrtm, case):
         assert err <= 1e-4
 
 
+@pytest.mark.parametrize("engine", [ENGINE_FD, ENGINE_CFD], ids=["FD8", "CFD10"])
+def test_step_parity_layers_fill_axis(klrtm, engine):
+    """x and z are covered entirely by their two PML layers (no interior along them)."""
+    ny, nx, nz = 18, 15, 14
+    nb = (8, 7, 5, 6, 6, 8)
+    shape = (ny, nx, nz)
+    order = 8 if engine == ENGINE_FD else 10
+    v = smooth_field(shape, 1500.0, 2500.0, 3)
+    m = (v * v * np.float32(DT * DT)).astype(np.float32)
+    w1 = gaussian_blob(ny, nx, nz, ny / 2, nx / 2, nz / 2, 2.0)
+    w0 = (0.97 * w1).astype(np.float32)
+    o = oracle.Engine(nx, ny, nz, D, D, D, engine, order, np.float32)
+    o.set_model(m)
+    o.set_boundary(nb, DT, 2500.0, dim_pml=3)
+    ow1, ow0, owr = w1.copy(), w0.copy(), np.zeros_like(w1)
+    k = KernelCPU(nx, ny, nz, D, D, D, engine, order)
+    k.setModel(DeviceArray.from_host(m))
+    k.setBoundary(*nb, DT, 2500.0)
+    g1, g0, gr = DeviceArray.from_host(w1), DeviceArray.from_host(w0), DeviceArray(shape)
+    for _ in range(10):
+        o.step(ISO, ow1, ow0, owr)
+        ow0, ow1 = ow1, ow0
+        k.step(ISO, g1, g0, gr)
+        g0, g1 = g1, g0
+    got = g1.download()
+    err = rel_l2(got, ow1)
+    print(f"[engine-parity] {'FD8' if engine == ENGINE_FD else 'CFD10'}-ISO layers filling x and z, "
+          f"{ny}x{nx}x{nz}: rel_l2 = {err:.3e}", flush=True)
+    if engine == ENGINE_CFD:
+        assert np.array_equal(got.view(np.uint32), ow1.view(np.uint32)), f"CFD not bit-exact (rel_l2={err:.3e})"
+    else:
+        assert err <= 1e-4
+
+
 @pytest.mark.parametrize("case", [(ENGINE_FD, 8, ISO), (ENGINE_FD, 16, VTI), (ENGINE_CFD, 10, VTI)],
                          ids=["FD8-ISO", "FD16-VTI", "CFD10-VTI"])
 def test_kernel_api_parity(klrtm, case):
