@@ -169,7 +169,9 @@ void Engine::default_tile(int R) {
   while(tz * tx > 1024) tx = std::max(R, tx / 2);
   if(ychunk <= 0) {
     long tiles = (long)div_up(nz, tz) * div_up(nx, tx);
-    long want = 2048;  // >= 8 blocks per CU on 256 CUs
+    // about four blocks per CU slot in flight over the launch (2 resident per CU on 256 CUs), so the
+    // tail round stays short; longer chunks re-read fewer y halo planes
+    long want = 1536;
     int chunks = (int)std::max(1L, std::min((long)ny, (long)div_up(want, std::max(1L, tiles))));
     ychunk = std::max(std::min(ny, 2 * R), div_up(ny, chunks));
   }
@@ -355,6 +357,9 @@ int klrtm_kernel_set_tile(klrtm_kernel *kk, int tz, int tx, int ychunk) {
   if(tz > 0) k->tz = tz;
   if(tx > 0) k->tx = tx;
   if(ychunk > 0) k->ychunk = ychunk;
+  if(ychunk < 0 || tz > 0 || tx > 0) {
+    if(ychunk <= 0) k->ychunk = 0;  // recomputed for the new tile count
+  }
   return KLRTM_OK;
 }
 int klrtm_kernel_get_tile(klrtm_kernel *kk, int *tz, int *tx, int *ychunk) {

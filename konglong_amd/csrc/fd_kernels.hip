@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
   uint32_t pair1[3] = {0, 0, 0};    // cells of layer 1 (layer 2 starts there)
   uint32_t zo = OOR, zs = 0, xo = OOR, xs = 0;
   float zc[3] = {0.f, 0.f, 1.f}, xc[3] = {0.f, 0.f, 1.f};
-  bool xz_any = false;
+  bool z_any = false, x_any = false;
   if(a.has_pml) {
     const int fa[3] = {FZ1, FX1, FY1};
 #pragma unroll
@@ -139,7 +139,8 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
         xc[0] = P.abc[f][0][3 * j], xc[1] = P.abc[f][0][3 * j + 1], xc[2] = P.abc[f][0][3 * j + 2];
       }
     }
-    xz_any = __syncthreads_or(zo != OOR || xo != OOR) != 0;
+    z_any = __syncthreads_or(zo != OOR) != 0;
+    x_any = __syncthreads_or(xo != OOR) != 0;
   }
 
   float q[2 * R + 1];
@@ -148,7 +149,8 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
     int yy = c.y0 - R + k;
     q[k] = (c.act && yy >= 0 && yy < ny) ? u[(size_t)yy * nxz + c.col] : 0.f;
   }
-  float *srow = s + (c.tx + R) * S + c.tz + R;
+  float *srow0 = s + (c.tx + R) * S + c.tz + R;                // LDS tile of even planes
+  float *srow1 = srow0 + (size_t)(a.tx + 2 * R) * S;            // and of odd planes
 
   auto round2 = [](float t0, float t1, float t2, float p, float &q1, float &q2) {
     float qi = t0 * q1 - t1 * p;
@@ -159,36 +161,38 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
     return qi + p * t2;
   };
 
-  // planes [ya, yb) of one segment; XZ: z/x layer work, YL: inside y layer fy (position iy - jy0)
-  auto segment = [&](auto xz_tag, auto yl_tag, int ya, int yb, int fy, int jy0) __attribute__((always_inline)) {
-    constexpr bool XZ = decltype(xz_tag)::value, YL = decltype(yl_tag)::value;
+  // planes [ya, yb) of one segment; ZL/XL: the block has z/x layer cells, YL: inside y layer fy
+  // (position iy - jy0)
+  auto segment = [&](auto z_tag, auto x_tag, auto yl_tag, int ya, int yb, int fy, int jy0)
+                     __attribute__((always_inline)) {
+    constexpr bool ZL = decltype(z_tag)::value, XL = decltype(x_tag)::value, YL = decltype(yl_tag)::value;
     const uint32_t yoff = (YL && fy == FY2) ? pair1[2] : 0u;
     auto fetch = [&](IsoPlane &N, int y) __attribute__((always_inline)) {
       const __amdgpu_buffer_rsrc_t ru = prs(u, y);
       N.qa = buf_ld(prs(u, min(y + R, ny - 1)), y + R < ny ? oc : OOR, 0);
       if(MODE != MODE_LAP) {
-        N.w = buf_ld(prs(w0, y), oc, 0);
-        N.v = buf_ld(prs(vel, y), oc, 0);
+        N.w = buf_ld_nt(prs(w0, y), oc, 0);
+        N.v = buf_ld_nt(prs(vel, y), oc, 0);
       }
       N.hz0 = buf_ld(ru, oz0, 0), N.hz1 = buf_ld(ru, oz1, 0);
       N.hx0 = buf_ld(ru, ox0, 0), N.hx1 = buf_ld(ru, ox1, 0);
-      if(XZ) {
 #pragma unroll
-        for(int r = 0; r < 2; r++) {
-          N.sz[r] = buf_ld(rq[0][r], zo == OOR ? OOR : zo + (uint32_t)y * zs, 0);
-          N.sx[r] = buf_ld(rq[1][r], xo == OOR ? OOR : xo + (uint32_t)y * xs, 0);
-        }
+      for(int r = 0; r < 2; r++) {
+        if(ZL) N.sz[r] = buf_ld_nt(rq[0][r], zo == OOR ? OOR : zo + (uint32_t)y * zs, 0);
+        if(XL) N.sx[r] = buf_ld_nt(rq[1][r], xo == OOR ? OOR : xo + (uint32_t)y * xs, 0);
       }
       if(YL) {
         const uint32_t so = (yoff + (uint32_t)(y - jy0) * (uint32_t)nxz) * 4u;
 #pragma unroll
-        for(int r = 0; r < 2; r++) N.sy[r] = buf_ld(rq[2][r], oc, so);
+        for(int r = 0; r < 2; r++) N.sy[r] = buf_ld_nt(rq[2][r], oc, so);
       }
     };
-    auto plane = [&](IsoPlane &C, IsoPlane &N, int y) __attribute__((always_inline)) {
-      fetch(N, min(y + 1, yb - 1));  // the last plane of the segment fetches itself again (unused)
+    // plane y: the loads of y+2 go out first; y's tile goes to LDS buffer y&1 (the barrier of y-1
+    // guarantees nobody still reads that buffer, so one barrier per plane suffices)
+    auto plane = [&](IsoPlane &C, IsoPlane &N2, int y) __attribute__((always_inline)) {
+      fetch(N2, min(y + 2, yb - 1));  // near the segment end the last plane is fetched again (unused)
       q[2 * R] = C.qa;
-      lds_barrier();
+      float *srow = (y & 1) ? srow1 : srow0;
       srow[0] = q[R];
       if(c.tz < R) srow[-R] = C.hz0;
       if(c.tz >= a.tz - R) srow[R] = C.hz1;
@@ -220,42 +224,47 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
         L = L + ly;
       }
       // PML on the summed Laplacian, layers in the order z, x, y (bgprtm/FdEngine.cpp:391-395)
-      if(XZ) {
+      if(ZL) {
         float s1 = C.sz[0], s2 = C.sz[1];
         float Lz = round2(zc[0], zc[1], zc[2], L, s1, s2);
         L = zo != OOR ? Lz : L;
         const uint32_t o = zo == OOR ? OOR : zo + (uint32_t)y * zs;
-        buf_st(s1, rq[0][0], o, 0), buf_st(s2, rq[0][1], o, 0);
-        s1 = C.sx[0], s2 = C.sx[1];
+        buf_st_nt(s1, rq[0][0], o, 0), buf_st_nt(s2, rq[0][1], o, 0);
+      }
+      if(XL) {
+        float s1 = C.sx[0], s2 = C.sx[1];
         float Lx = round2(xc[0], xc[1], xc[2], L, s1, s2);
         L = xo != OOR ? Lx : L;
         const uint32_t ox = xo == OOR ? OOR : xo + (uint32_t)y * xs;
-        buf_st(s1, rq[1][0], ox, 0), buf_st(s2, rq[1][1], ox, 0);
+        buf_st_nt(s1, rq[1][0], ox, 0), buf_st_nt(s2, rq[1][1], ox, 0);
       }
       if(YL) {
         const cfloat *t = (const cfloat *)P.abc[fy][0] + 3 * (y - jy0);
         float s1 = C.sy[0], s2 = C.sy[1];
         L = round2(t[0], t[1], t[2], L, s1, s2);
         const uint32_t so = (yoff + (uint32_t)(y - jy0) * (uint32_t)nxz) * 4u;
-        buf_st(s1, rq[2][0], oc, so), buf_st(s2, rq[2][1], oc, so);
+        buf_st_nt(s1, rq[2][0], oc, so), buf_st_nt(s2, rq[2][1], oc, so);
       }
       if(MODE == MODE_STEP) {
-        buf_st((ctr + ctr) - fmaf(-C.v, L, C.w), prs(w0, y), oc, 0);
+        buf_st_nt((ctr + ctr) - fmaf(-C.v, L, C.w), prs(w0, y), oc, 0);
       } else if(MODE == MODE_KERNEL) {
-        buf_st(L, prs(pr, y), oc, 0);
-        buf_st(fmaf(-L, C.v, C.w), prs(w0, y), oc, 0);
+        buf_st_nt(L, prs(pr, y), oc, 0);
+        buf_st_nt(fmaf(-L, C.v, C.w), prs(w0, y), oc, 0);
       } else {
-        buf_st(L, prs(pr, y), oc, 0);
+        buf_st_nt(L, prs(pr, y), oc, 0);
       }
 #pragma unroll
       for(int k = 0; k < 2 * R; k++) q[k] = q[k + 1];
     };
-    IsoPlane A, B;
+    IsoPlane A, B, C3;  // three planes in rotation: current, next (in flight), next-but-one (issued now)
     fetch(A, ya);
+    fetch(B, min(ya + 1, yb - 1));
     for(int y = ya;;) {
-      plane(A, B, y);
+      plane(A, C3, y);
       if(++y >= yb) break;
       plane(B, A, y);
+      if(++y >= yb) break;
+      plane(C3, B, y);
       if(++y >= yb) break;
     }
   };
@@ -265,18 +274,15 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
   const int wy1 = (a.has_pml && P.q[FY1][0][0]) ? P.w[FY1] : 0;
   const int wy2 = (a.has_pml && P.q[FY2][0][0]) ? P.w[FY2] : 0;
   const int s1e = min(c.y1, max(c.y0, wy1)), s3b = max(s1e, min(c.y1, ny - wy2));
-  if(c.y0 < s1e) {
-    if(xz_any) segment(T_(), T_(), c.y0, s1e, FY1, 0);
-    else segment(F_(), T_(), c.y0, s1e, FY1, 0);
-  }
-  if(s1e < s3b) {
-    if(xz_any) segment(T_(), F_(), s1e, s3b, 0, 0);
-    else segment(F_(), F_(), s1e, s3b, 0, 0);
-  }
-  if(s3b < c.y1) {
-    if(xz_any) segment(T_(), T_(), s3b, c.y1, FY2, ny - wy2);
-    else segment(F_(), T_(), s3b, c.y1, FY2, ny - wy2);
-  }
+  auto run = [&](auto yl, int ya, int yb, int fy, int jy0) __attribute__((always_inline)) {
+    if(z_any && x_any) segment(T_(), T_(), yl, ya, yb, fy, jy0);
+    else if(z_any) segment(T_(), F_(), yl, ya, yb, fy, jy0);
+    else if(x_any) segment(F_(), T_(), yl, ya, yb, fy, jy0);
+    else segment(F_(), F_(), yl, ya, yb, fy, jy0);
+  };
+  if(c.y0 < s1e) run(T_(), c.y0, s1e, FY1, 0);
+  if(s1e < s3b) run(F_(), s1e, s3b, 0, 0);
+  if(s3b < c.y1) run(T_(), s3b, c.y1, FY2, ny - wy2);
 }
 
 // ---------------------------------------------------------------------------
@@ -529,7 +535,7 @@ template<int R>
 static int launch_iso_r(int mode, const float *u, float *w0, const float *vel, float *pr, IsoArgs a) {
   grid_of(a);
   int nb = a.nbz * a.nbx * a.nby;
-  size_t lds = sizeof(float) * (size_t)(a.tx + 2 * R) * (a.tz + 2 * R);
+  size_t lds = 2 * sizeof(float) * (size_t)(a.tx + 2 * R) * (a.tz + 2 * R);  // two plane tiles
   LaunchTimer tm(mode == MODE_STEP ? "iso_step" : mode == MODE_KERNEL ? "iso_kernel" : "laplacian");
   if(mode == MODE_STEP) return launch_iso_rm<R, MODE_STEP>(u, w0, vel, pr, a, nb, lds);
   if(mode == MODE_KERNEL) return launch_iso_rm<R, MODE_KERNEL>(u, w0, vel, pr, a, nb, lds);

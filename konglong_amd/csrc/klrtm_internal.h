@@ -118,6 +118,15 @@ __device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t rs, uint32_t voff
 __device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, voff, soff, 0);
 }
+// Streaming forms (cache policy NT): for data read or written once, so it does
+// not evict lines that neighbouring tiles still re-read from L2.
+constexpr int CPOL_NT = 2;
+__device__ __forceinline__ float buf_ld_nt(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, CPOL_NT));
+}
+__device__ __forceinline__ void buf_st_nt(float v, __amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, voff, soff, CPOL_NT);
+}
 
 // Workgroup barrier for LDS hand-off only: waits for this wave's LDS accesses
 // (lgkmcnt) but not for outstanding global loads, so prefetches issued before
