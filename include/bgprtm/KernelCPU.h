@@ -1,0 +1,66 @@
+// KernelCPU.h -- bgprtm's per-step operator (bgprtm/KernelCPU.h:19-35) on the GPU.
+//
+// Same constructor and methods; every volume argument is a device pointer of
+// nx*ny*nz floats.  The FD order comes from konglong::params().fd_order (the
+// reference parses it from global_pars "Engine"); nThreads is accepted and
+// unused.  Contract as in the reference: p0 = u^n (read), pb = u^(n-1) updated
+// in place to u^(n-1) - vel*Op(u^n), pr = scratch (holds py for VTI/TTI).
+#ifndef KONGLONG_KERNELCPU_H_
+#define KONGLONG_KERNELCPU_H_
+
+#include "konglong_gpu.h"
+
+class Grid;
+
+class KernelCPU {
+public:
+  KernelCPU(int nx, int ny, int nz, float dx, float dy, float dz, int engine, int nThreads)
+      : nx_(nx), ny_(ny), nz_(nz) {
+    (void)nThreads;
+    konglong::check(klrtm_kernel_create(&k_, nx, ny, nz, dx, dy, dz, engine, konglong::params().fd_order));
+    const konglong::Params &p = konglong::params();
+    konglong::check(klrtm_kernel_set_pml_options(k_, p.pml_eps, p.pml3d, p.pml3d_scaler, p.top_pml_flag));
+  }
+  virtual ~KernelCPU() { klrtm_kernel_destroy(k_); }
+  KernelCPU(const KernelCPU &) = delete;
+  KernelCPU &operator=(const KernelCPU &) = delete;
+
+  // borrowed device pointers; vel holds v^2 dt^2 (Propagator::Vel2); rho must be null
+  void setModel(float *vel, float *rho, float *del, float *eps, float *pjx, float *pjy) {
+    konglong::check(klrtm_kernel_set_model(k_, vel, rho, del, eps, pjx, pjy));
+  }
+  // only the identity (regular grid) is supported
+  void setJacob(Grid *grid, float *jacobx, float *jacoby, float *jacobz) {
+    if(jacobx || jacoby || jacobz)
+      throw konglong::Error(KLRTM_EINVAL, "konglong: irregular-grid jacobians are not supported");
+    (void)grid;
+  }
+  void setBoundary(int nxbnd1, int nxbnd2, int nybnd1, int nybnd2, int nzbnd1, int nzbnd2, float dt, float vmax) {
+    konglong::check(klrtm_kernel_set_boundary(k_, nxbnd1, nxbnd2, nybnd1, nybnd2, nzbnd1, nzbnd2, dt, vmax));
+  }
+  void TTI(float *p0, float *pb, float *pr, ModelType modelType = ::TTI) {
+    if(modelType == ::VTI) return VTI(p0, pb, pr);
+    konglong::check(klrtm_kernel_tti(k_, p0, pb, pr));
+  }
+  void VTI(float *p0, float *pb, float *pr) { konglong::check(klrtm_kernel_vti(k_, p0, pb, pr)); }
+  // bndType/iz0/spread select the free-surface mirror in the reference; ABSORB (0) only here
+  void ISO(float *p0, float *pb, float *pr, int bndType = 0, int iz0 = 0, int spread = 0) {
+    if(bndType != 0)
+      throw konglong::Error(KLRTM_EINVAL, "konglong: only the absorbing surface (bndType 0) is supported");
+    (void)iz0, (void)spread;
+    konglong::check(klrtm_kernel_iso(k_, p0, pb, pr));
+  }
+  void cleanPMLMemory() { konglong::check(klrtm_kernel_clean_pml(k_)); }
+
+  // Propagator::kernel() + update2nd() fused: w0 <- 2 w1 - (w0 - vel*Op(w1))
+  void step(ModelType model, const float *w1, float *w0, float *wr) {
+    konglong::check(klrtm_kernel_step(k_, model, w1, w0, wr));
+  }
+  klrtm_kernel *handle() const { return k_; }
+
+private:
+  klrtm_kernel *k_ = nullptr;
+  int nx_, ny_, nz_;
+};
+
+#endif  // KONGLONG_KERNELCPU_H_

@@ -153,6 +153,18 @@ int klrtm_compress16(const float *in, uint16_t *out, int nx, int ny, int nz);
 /* decode with the reference's edge taper (20 cells x/y, 10 cells z) */
 int klrtm_uncompress16(const uint16_t *in, float *out, int nx, int ny, int nz);
 
+/* ---- WaveFieldCompress COMPRESS_16P / COMPRESS_8P / COMPRESS_NONE (bgprtm/WaveFieldCompress.cpp:
+ * 30-44, 209-530): per z trace (nz floats) of the nx*ny traces, ushort stride nshort_pack(nz, mode).
+ * mode 2 (16P) / 3 (8P): sum/difference pairs with a per-trace bit split; decode applies the edge
+ * taper.  The reference forms 2^b as (1 << b) in int, undefined for b = 31, 32; exact powers here.
+ * COMPRESS_NONE keeps whole floats (2*nz ushorts per trace: the reference's nz+2 stride cannot hold
+ * them) and its decode applies the edge taper. */
+size_t klrtm_nshort_pack(size_t nfloats, int mode);
+int klrtm_compress_pair(const float *in, uint16_t *out, int nx, int ny, int nz, int mode);
+int klrtm_uncompress_pair(const uint16_t *in, float *out, int nx, int ny, int nz, int mode);
+int klrtm_compress_none(const float *in, uint16_t *out, int nx, int ny, int nz);
+int klrtm_uncompress_none(const uint16_t *in, float *out, int nx, int ny, int nz);
+
 /* ---- block-scaled half snapshots (one FP32 scale per 8x8x8 block) ------------ */
 /* fmt KLRTM_SNAP_FP16 or KLRTM_SNAP_BF16; payload nx*ny*nz uint16, scales
  * ceil(nz/8)*ceil(nx/8)*ceil(ny/8) floats (device) */
@@ -161,9 +173,11 @@ int klrtm_blockq_encode(const float *in, uint16_t *payload, float *scales, int n
 int klrtm_blockq_decode(const uint16_t *payload, const float *scales, float *out, int nx, int ny, int nz, int fmt);
 
 /* ---- ImagingCorrelator::crossone + ImagingCondition::calIllum (zero lag) ----- */
-/* image += S*R, illum += S*S with S = src*taper (taper = wtaperT or z taper as in
- * ImagingCondition::applyTape); any output pointer may be NULL.  FP64 outputs
- * accumulate the same products in double. */
+/* image += S*R, illum += S*S with S = src*taper as in ImagingCondition::applyTape:
+ * taper_t >= 0 multiplies every cell by taper_t (time taper wtaperT[it]); taper_t < 0
+ * multiplies the first ntaper_z cells of each column by taper_z[iz] (water-depth taper,
+ * device array).  Any output pointer may be NULL.  FP64 outputs accumulate the same
+ * products in double. */
 int klrtm_ic_correlate(const float *src, const float *rec, int nx, int ny, int nz, float taper_t, const float *taper_z,
                        int ntaper_z, float *image, float *illum, double *image64, double *illum64);
 

@@ -9,6 +9,7 @@
 // blocks along z; lanes run along z so every row access is a 256-byte
 // coalesced segment, each lane keeps its 8 y-samples in registers between the
 // max reduction and the store.
+#pragma clang fp contract(off)
 #include "klrtm_internal.h"
 #include "codec_common.h"
 #include "codec.h"
@@ -110,6 +111,98 @@ int c16_decode(const uint16_t *in, float *out, int nx, int ny, int nz) {
   hipLaunchKernelGGL(c16_decode_kernel, dim3((unsigned)div_up((long)n, 256)), dim3(256), 0, stream(), in, out, nx, ny, nz,
                      make_edge_taper());
   return check_launch("c16_decode_kernel");
+}
+
+// ---------------------------------------------------------------------------
+// COMPRESS_16P / COMPRESS_8P: one wavefront per z trace.  Pass 1 reduces the
+// pair ranges across the wave, pass 2 (re-reading the trace through the
+// cache) writes two units per pair; lane 0 writes the odd last sample and the
+// range trailer.  Decode is one pass with the edge taper.
+template<int MODE>
+__global__ void __launch_bounds__(256) pair_encode_kernel(const float *__restrict__ in, uint16_t *__restrict__ out,
+                                                          int nxy, int n) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if(t >= nxy) return;
+  const float *tr = in + (size_t)t * n;
+  const size_t ns = pair_nshort((size_t)n, MODE);
+  uint16_t *o = out + (size_t)t * ns;
+  const int np = n / 2;
+  float m1 = 0.f, m2 = 0.f;
+  for(int p = lane; p < np; p += 64) {
+    const float a = tr[2 * p], b = tr[2 * p + 1];
+    m1 = fmaxf(m1, fabsf(0.5f * (a + b)));
+    m2 = fmaxf(m2, fabsf(0.5f * (a - b)));
+  }
+  if(lane == 0 && (n & 1)) m1 = fmaxf(m1, fabsf(tr[n - 1]));
+  for(int k = 32; k > 0; k >>= 1) {
+    m1 = fmaxf(m1, __shfl_xor(m1, k, 64));
+    m2 = fmaxf(m2, __shfl_xor(m2, k, 64));
+  }
+  const PairCodec c = pair_encoder(2 * m1, 2 * m2, MODE);
+  for(int p = lane; p < np; p += 64) {
+    unsigned u0, u1;
+    pair_encode(c, tr[2 * p], tr[2 * p + 1], u0, u1);
+    if(MODE == 2) o[2 * p] = (uint16_t)u0, o[2 * p + 1] = (uint16_t)u1;
+    else o[p] = (uint16_t)((u0 & 0xffu) | ((u1 & 0xffu) << 8));
+  }
+  if(lane == 0) {
+    if(n & 1) {
+      unsigned u0, u1;
+      pair_encode_last(c, tr[n - 1], u0, u1);
+      if(MODE == 2) o[2 * np] = (uint16_t)u0, o[2 * np + 1] = (uint16_t)u1;
+      else o[np] = (uint16_t)((u0 & 0xffu) | ((u1 & 0xffu) << 8));
+    }
+    // ranges at unit offset (n+1)/2*2: ushorts for 16P, bytes (an even count) for 8P
+    const size_t off = MODE == 2 ? (size_t)(n + 1) / 2 * 2 : (size_t)(n + 1) / 2;
+    const unsigned r1 = __float_as_uint(c.range1), r2 = __float_as_uint(c.range2);
+    o[off] = (uint16_t)(r1 & 0xffffu), o[off + 1] = (uint16_t)(r1 >> 16);
+    o[off + 2] = (uint16_t)(r2 & 0xffffu), o[off + 3] = (uint16_t)(r2 >> 16);
+  }
+}
+
+template<int MODE>
+__global__ void __launch_bounds__(256) pair_decode_kernel(const uint16_t *__restrict__ in, float *__restrict__ out, int nx,
+                                                          int ny, int n, const EdgeTaper T) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if(t >= nx * ny) return;
+  const int ix = t % nx, iy = t / nx;
+  const size_t ns = pair_nshort((size_t)n, MODE);
+  const uint16_t *c = in + (size_t)t * ns;
+  float *tr = out + (size_t)t * n;
+  const size_t off = MODE == 2 ? (size_t)(n + 1) / 2 * 2 : (size_t)(n + 1) / 2;
+  const float r1 = __uint_as_float((unsigned)c[off] | ((unsigned)c[off + 1] << 16));
+  const float r2 = __uint_as_float((unsigned)c[off + 2] | ((unsigned)c[off + 3] << 16));
+  const PairCodec d = pair_decoder(r1, r2, MODE);
+  const int np = n / 2;
+  for(int p = lane; p < np; p += 64) {
+    unsigned u0, u1;
+    if(MODE == 2) u0 = c[2 * p], u1 = c[2 * p + 1];
+    else u0 = c[p] & 0xffu, u1 = c[p] >> 8;
+    float a, b;
+    pair_decode(d, u0, u1, a, b);
+    tr[2 * p] = edge_taper(T, a, ix, iy, 2 * p, nx, ny, n);
+    tr[2 * p + 1] = edge_taper(T, b, ix, iy, 2 * p + 1, nx, ny, n);
+  }
+  if(lane == 0 && (n & 1)) {
+    unsigned u0, u1;
+    if(MODE == 2) u0 = c[2 * np], u1 = c[2 * np + 1];
+    else u0 = c[np] & 0xffu, u1 = c[np] >> 8;
+    tr[n - 1] = edge_taper(T, pair_decode_last(d, u0, u1), ix, iy, n - 1, nx, ny, n);
+  }
+}
+
+// COMPRESS_NONE: the floats of each trace, whole (two ushorts each); decode applies the edge taper
+__global__ void none_decode_kernel(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, int nz,
+                                   const EdgeTaper T) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nxz = (size_t)nx * nz;
+  if(i >= nxz * ny) return;
+  const int iy = (int)(i / nxz);
+  const size_t j = i - (size_t)iy * nxz;
+  const int ix = (int)(j / nz), iz = (int)(j - (size_t)ix * nz);
+  out[i] = edge_taper(T, in[i], ix, iy, iz, nx, ny, nz);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,6 +329,44 @@ int klrtm_uncompress16(const uint16_t *in, float *out, int nx, int ny, int nz) {
   KL_REQUIRE(in && out, KLRTM_EINVAL, "null pointer");
   KL_REQUIRE(nx > 0 && ny > 0 && nz > 0, KLRTM_EINVAL, "bad dimensions");
   return c16_decode(in, out, nx, ny, nz);
+}
+size_t klrtm_nshort_pack(size_t nfloats, int mode) { return pair_nshort(nfloats, mode); }
+int klrtm_compress_pair(const float *in, uint16_t *out, int nx, int ny, int nz, int mode) {
+  KL_REQUIRE(in && out, KLRTM_EINVAL, "null argument");
+  KL_REQUIRE(mode == 2 || mode == 3, KLRTM_EINVAL, "mode must be COMPRESS_16P (2) or COMPRESS_8P (3)");
+  KL_REQUIRE(nx > 0 && ny > 0 && nz > 0, KLRTM_EINVAL, "bad sizes");
+  const int nxy = nx * ny;
+  LaunchTimer tm("pair_encode");
+  if(mode == 2)
+    hipLaunchKernelGGL(pair_encode_kernel<2>, dim3(div_up(nxy, 4)), dim3(256), 0, stream(), in, out, nxy, nz);
+  else
+    hipLaunchKernelGGL(pair_encode_kernel<3>, dim3(div_up(nxy, 4)), dim3(256), 0, stream(), in, out, nxy, nz);
+  return check_launch("pair_encode_kernel");
+}
+int klrtm_uncompress_pair(const uint16_t *in, float *out, int nx, int ny, int nz, int mode) {
+  KL_REQUIRE(in && out, KLRTM_EINVAL, "null argument");
+  KL_REQUIRE(mode == 2 || mode == 3, KLRTM_EINVAL, "mode must be COMPRESS_16P (2) or COMPRESS_8P (3)");
+  KL_REQUIRE(nx > 0 && ny > 0 && nz > 0, KLRTM_EINVAL, "bad sizes");
+  LaunchTimer tm("pair_decode");
+  if(mode == 2)
+    hipLaunchKernelGGL(pair_decode_kernel<2>, dim3(div_up(nx * ny, 4)), dim3(256), 0, stream(), in, out, nx, ny, nz,
+                       make_edge_taper());
+  else
+    hipLaunchKernelGGL(pair_decode_kernel<3>, dim3(div_up(nx * ny, 4)), dim3(256), 0, stream(), in, out, nx, ny, nz,
+                       make_edge_taper());
+  return check_launch("pair_decode_kernel");
+}
+int klrtm_compress_none(const float *in, uint16_t *out, int nx, int ny, int nz) {
+  KL_REQUIRE(in && out, KLRTM_EINVAL, "null argument");
+  KL_HIP(hipMemcpyAsync(out, in, (size_t)nx * ny * nz * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+  return KLRTM_OK;
+}
+int klrtm_uncompress_none(const uint16_t *in, float *out, int nx, int ny, int nz) {
+  KL_REQUIRE(in && out, KLRTM_EINVAL, "null argument");
+  const size_t n = (size_t)nx * ny * nz;
+  hipLaunchKernelGGL(none_decode_kernel, dim3((unsigned)div_up((long)n, 256)), dim3(256), 0, stream(),
+                     (const float *)in, out, nx, ny, nz, make_edge_taper());
+  return check_launch("none_decode_kernel");
 }
 size_t klrtm_blockq_nscales(int nx, int ny, int nz) { return blockq_nscales(nx, ny, nz); }
 int klrtm_blockq_encode(const float *in, uint16_t *payload, float *scales, int nx, int ny, int nz, int fmt) {

@@ -64,6 +64,10 @@ def lib():
         L.orc_compress16.argtypes = [_pf, C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_uncompress16.argtypes = [C.c_void_p, _pf, C.c_int, C.c_int, C.c_int]
         L.orc_codec16_table.argtypes = [_pf]
+        L.orc_compress_pair.argtypes = [_pf, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_uncompress_pair.argtypes = [C.c_void_p, _pf, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_pair_autobit.restype = C.c_int
+        L.orc_pair_autobit.argtypes = [C.c_float, C.c_float, C.c_int]
         L.orc_crossone.argtypes = [_pf, _pf, _pf, C.c_size_t]
         L.orc_illum.argtypes = [_pf, _pf, C.c_size_t]
         L.orc_apply_tape.argtypes = [_pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, C.c_int, _pf, C.c_int]
@@ -217,3 +221,33 @@ def codec16_table():
     t = np.zeros(65536, np.float32)
     lib().orc_codec16_table(t.ctypes.data_as(_pf))
     return t
+
+
+def nshort_pack(n, mode):
+    """ushorts per trace of n floats for COMPRESS_16P (2) / COMPRESS_8P (3) / COMPRESS_NONE (0)."""
+    if mode == 2:
+        return (n + 1) // 2 * 2 + 4
+    if mode == 3:
+        return (n + 1) // 2 + 4
+    return 2 * n
+
+
+def compress_pair(vol, mode):
+    """COMPRESS_16P / COMPRESS_8P of a (ny, nx, nz) volume: uint16 [ny*nx, nshort_pack(nz)]."""
+    vol = np.ascontiguousarray(vol, np.float32)
+    ny, nx, nz = vol.shape
+    out = np.zeros((ny * nx, nshort_pack(nz, mode)), np.uint16)
+    lib().orc_compress_pair(vol.ctypes.data_as(_pf), out.ctypes.data, nx, ny, nz, mode)
+    return out
+
+
+def uncompress_pair(code, ny, nx, nz, mode):
+    code = np.ascontiguousarray(code, np.uint16)
+    out = np.zeros((ny, nx, nz), np.float32)
+    lib().orc_uncompress_pair(code.ctypes.data, out.ctypes.data_as(_pf), nx, ny, nz, mode)
+    return out
+
+
+def pair_autobit(range1, range2, nbits):
+    return lib().orc_pair_autobit(range1, range2, nbits)
+

@@ -12,6 +12,7 @@
 //
 // Index convention (bgprtm/FdEngine.h:42): idx = (iy*nx + ix)*nz + iz.
 #include <cmath>
+#include <cfloat>
 #include <cstdint>
 #include <cstring>
 #include <cstdio>
@@ -805,6 +806,106 @@ struct Codec16 {
 // FP64 instantiations.
 using namespace orc;
 
+// COMPRESS_16P / COMPRESS_8P per z trace: bgprtm/WaveFieldCompress.cpp:281-530
+// (compress16p/uncompress16p/compress8p/uncompress8p/autobit16/autobit8).  The
+// reference's bit ranges are ints built as (1 << b), undefined at b = 31, 32;
+// this restatement uses the exact powers 2^b and saturates a code that rounds
+// past 2^32 - 1.
+static double pow2_exact(int b) { return (double)(1ull << b); }
+static unsigned bits_full(int b) { return b >= 32 ? 0xffffffffu : (1u << b) - 1u; }
+static int orc_autobit(float range1, float range2, int nbits) {
+  if(range1 > 0 && range2 == 0) return nbits;
+  if(range1 > 0 && range2 > 0) {
+    float minerr = range1 + range2;
+    int bitmin = nbits / 2;
+    for(int bit1 = 0; bit1 <= nbits; bit1++) {
+      int bit2 = nbits - bit1;
+      float totalerr = range1 / (float)pow2_exact(bit1) + range2 / (float)pow2_exact(bit2);
+      if(totalerr < minerr) {
+        minerr = totalerr;
+        bitmin = bit1;
+      }
+    }
+    return bitmin;
+  }
+  return nbits / 2;
+}
+static unsigned orc_round_code(float x) {
+  float r = nearbyintf(x);
+  return r >= 4294967295.0f ? 0xffffffffu : (unsigned)r;
+}
+// one trace of n floats -> units (16-bit for nbits 32, 8-bit for nbits 16) + two range floats
+template<typename U>
+static void orc_pack_trace(const float *in, U *c, size_t n, int nbits) {
+  const int half = nbits / 2;
+  float v1, v2, v1max, v1min, v2max, v2min;
+  size_t i;
+  if(n == 1) v1min = v1max = in[0], v2min = v2max = 0;
+  else v1min = v1max = 0.5f * (in[0] + in[1]), v2min = v2max = 0.5f * (in[0] - in[1]);
+  for(i = 0; i + 1 < n; i += 2) {
+    v1 = 0.5f * (in[i] + in[i + 1]);
+    v2 = 0.5f * (in[i] - in[i + 1]);
+    v1max = std::max(v1max, v1), v1min = std::min(v1min, v1);
+    v2max = std::max(v2max, v2), v2min = std::min(v2min, v2);
+  }
+  if(i < n) v1max = std::max(v1max, in[i]), v1min = std::min(v1min, in[i]);
+  float range1 = 2 * std::max(fabsf(v1max), fabsf(v1min));
+  float range2 = 2 * std::max(fabsf(v2max), fabsf(v2min));
+  const int bit1 = orc_autobit(range1, range2, nbits);
+  const int bit2 = nbits - bit1, bit3 = abs(bit1 - half);
+  const unsigned br1 = bits_full(bit1), br2 = bits_full(bit2), br3 = bits_full(bit3);
+  range1 = std::max(range1, FLT_MIN * (float)br1);
+  range2 = std::max(range2, FLT_MIN * (float)br2);
+  const float bias1 = -0.5f * range1, bias2 = -0.5f * range2;
+  const float scale1 = range1 > 0 ? (float)br1 / range1 : 0.0f, scale2 = range2 > 0 ? (float)br2 / range2 : 0.0f;
+  const float *p = in;
+  for(i = 0; i + 1 < n; i += 2, p += 2) {
+    float x1 = 0.5f * (p[0] + p[1]) - bias1, x2 = 0.5f * (p[0] - p[1]) - bias2;
+    x1 = scale1 * x1;
+    x2 = scale2 * x2;
+    unsigned ui1 = orc_round_code(bit1 < half ? x2 : x1), ui2 = orc_round_code(bit1 < half ? x1 : x2);
+    *c++ = (U)(ui1 >> bit3);
+    *c++ = (U)(((ui1 & br3) << (bit1 < half ? bit1 : bit2)) + ui2);
+  }
+  if(i < n) {
+    unsigned ui1 = orc_round_code(scale1 * (p[0] - bias1));
+    *c++ = (U)(ui1 >> bit3);
+    *c++ = (U)(ui1 & br3);
+  }
+  memcpy(c, &range1, 4);
+  memcpy((char *)c + 4, &range2, 4);
+}
+template<typename U>
+static void orc_unpack_trace(const U *c, float *out, size_t n, int nbits) {
+  const int half = nbits / 2;
+  float range1, range2;
+  memcpy(&range1, c + (n + 1) / 2 * 2, 4);
+  memcpy(&range2, (const char *)(c + (n + 1) / 2 * 2) + 4, 4);
+  const int bit1 = orc_autobit(range1, range2, nbits);
+  const int bit2 = nbits - bit1, bit3 = abs(bit1 - half);
+  const unsigned br1 = bits_full(bit1), br2 = bits_full(bit2);
+  const float unscale1 = br1 ? range1 / (float)br1 : 0, unscale2 = br2 ? range2 / (float)br2 : 0;
+  const float bias1 = -0.5f * unscale1 * (float)br1, bias2 = -0.5f * unscale2 * (float)br2;
+  size_t i;
+  for(i = 0; i + 1 < n; i += 2, c += 2) {
+    unsigned ui1 = (((unsigned)c[0]) << bit3) + (c[1] >> (bit1 < half ? bit1 : bit2));
+    unsigned ui2 = (unsigned)(c[1] & (bit1 < half ? br1 : br2));
+    float x1 = (float)(bit1 < half ? ui2 : ui1), x2 = (float)(bit1 < half ? ui1 : ui2);
+    x1 = x1 * unscale1 + bias1;
+    x2 = x2 * unscale2 + bias2;
+    *out++ = x1 + x2;
+    *out++ = x1 - x2;
+  }
+  if(i < n) {
+    unsigned ui1 = (((unsigned)c[0]) << bit3) + c[1];
+    *out++ = (float)ui1 * unscale1 + bias1;
+  }
+}
+static size_t orc_nshort_pack(size_t n, int mode) {
+  if(mode == 2) return (n + 1) / 2 * 2 + 4;
+  if(mode == 3) return (n + 1) / 2 + 4;
+  return 2 * n;
+}
 extern "C" {
 
 void orc_taylor_coef2(int M, double *c) { taylor_coef2(M, c); }
@@ -938,6 +1039,42 @@ void orc_codec16_table(float *t) {
   static Codec16 c;
   memcpy(t, c.table.data(), 65536 * 4);
 }
+
+// volume forms (bgprtm/WaveFieldCompress.cpp:209-279): trace ixy at ushort offset ixy*nshort_pack(nz);
+// decode applies the x/y (20) and z (10) edge tapers
+void orc_compress_pair(const float *in, uint16_t *out, int nx, int ny, int nz, int mode) {
+  const size_t ns = orc_nshort_pack(nz, mode), nxy = (size_t)nx * ny;
+  for(size_t t = 0; t < nxy; t++) {
+    if(mode == 2) orc_pack_trace<uint16_t>(in + t * nz, out + t * ns, nz, 32);
+    else orc_pack_trace<uint8_t>(in + t * nz, (uint8_t *)(out + t * ns), nz, 16);
+  }
+}
+void orc_uncompress_pair(const uint16_t *in, float *out, int nx, int ny, int nz, int mode) {
+  static Codec16 c;  // for its taper weights
+  const size_t ns = orc_nshort_pack(nz, mode);
+  for(int iy = 0; iy < ny; iy++)
+    for(int ix = 0; ix < nx; ix++) {
+      const size_t t = (size_t)iy * nx + ix;
+      float *po = out + t * nz;
+      if(mode == 2) orc_unpack_trace<uint16_t>(in + t * ns, po, nz, 32);
+      else orc_unpack_trace<uint8_t>((const uint8_t *)(in + t * ns), po, nz, 16);
+      for(int iz = 0; iz < nz; iz++) {
+        if(ny > c.ntaper * 2) {
+          if(iy < c.ntaper) po[iz] *= c.wtaper[iy];
+          if(iy > ny - c.ntaper - 1) po[iz] *= c.wtaper[ny - iy - 1];
+        }
+        if(nx > c.ntaper * 2) {
+          if(ix < c.ntaper) po[iz] *= c.wtaper[ix];
+          if(ix > nx - c.ntaper - 1) po[iz] *= c.wtaper[nx - ix - 1];
+        }
+        if(nz > c.nztaper * 2) {
+          if(iz < c.nztaper) po[iz] *= c.wztaper[iz];
+          if(iz > nz - c.nztaper - 1) po[iz] *= c.wztaper[nz - iz - 1];
+        }
+      }
+    }
+}
+int orc_pair_autobit(float range1, float range2, int nbits) { return orc_autobit(range1, range2, nbits); }
 
 // zero-lag correlation: bgprtm/ImagingCorrelator.cpp:180-194 and illumination
 // bgprtm/ImagingCondition.cpp:143-149

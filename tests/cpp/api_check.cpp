@@ -1,0 +1,132 @@
+// api_check.cpp -- drives the bgprtm drop-in headers (include/bgprtm) the way
+// a bgprtm caller would and compares with the CPU oracle (liboracle.so, test
+// infrastructure).  Prints one line per check and "API_CHECK PASS" at the end;
+// exits non-zero on a mismatch.  Needs a GPU.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "../../include/bgprtm/FdEngine.h"
+#include "../../include/bgprtm/ImagingCorrelator.h"
+#include "../../include/bgprtm/KernelCPU.h"
+#include "../../include/bgprtm/PML.h"
+#include "../../include/bgprtm/WaveFieldCompress.h"
+
+extern "C" {  // oracle (oracle/rtm_oracle.cpp)
+void *orc_engine_create_f(int nx, int ny, int nz, float dx, float dy, float dz, int engine, int order);
+void orc_engine_destroy_f(void *h);
+void orc_engine_set_model_f(void *h, const float *vel, const float *eps, const float *del, const float *pjx,
+                            const float *pjy);
+void orc_engine_set_boundary_f(void *h, const int *nb, float dt, float vmax, float pml_eps, int pml3d, float scaler,
+                               int top, int dimPML);
+void orc_engine_kernel_f(void *h, int model, const float *p0, float *pb, float *pr);
+void orc_engine_laplacian_f(void *h, const float *in, float *out);
+void orc_compress_pair(const float *in, uint16_t *out, int nx, int ny, int nz, int mode);
+void orc_pml_round(float g, float *p, float *q, int n);
+}
+
+static int failures = 0;
+static void report(const char *what, bool ok, double err) {
+  std::printf("[api_check] %-44s %s (err %.3e)\n", what, ok ? "ok" : "FAIL", err);
+  if(!ok) failures++;
+}
+static double rel_l2(const std::vector<float> &a, const std::vector<float> &b) {
+  double num = 0, den = 0;
+  for(size_t i = 0; i < a.size(); i++) num += (double)(a[i] - b[i]) * (a[i] - b[i]), den += (double)b[i] * b[i];
+  return std::sqrt(num / (den > 0 ? den : 1));
+}
+
+int main() {
+  const int nx = 29, ny = 23, nz = 31, n = nx * ny * nz;
+  const float d = 10.f, dt = 1e-3f, vmax = 2500.f;
+  const int nb[6] = {5, 6, 4, 7, 5, 6};
+  std::mt19937 gen(11);
+  std::uniform_real_distribution<float> U(-1.f, 1.f), V(1500.f, 2500.f), E(0.f, 0.2f);
+  std::vector<float> vel(n), eps(n), del(n), p0(n), pb(n), pr(n, 0.f);
+  for(int i = 0; i < n; i++) {
+    float v = V(gen);
+    vel[i] = v * v * dt * dt, eps[i] = E(gen), del[i] = 0.5f * E(gen), p0[i] = U(gen), pb[i] = 0.9f * U(gen);
+  }
+  konglong::DeviceBuffer<float> dv(n), de(n), dd(n), d0(n), dbb(n), dr(n);
+  dv.upload(vel.data()), de.upload(eps.data()), dd.upload(del.data()), d0.upload(p0.data()), dbb.upload(pb.data());
+
+  // KernelCPU::VTI with the compact engine: bit for bit
+  {
+    KernelCPU k(nx, ny, nz, d, d, d, FD::CFD, 1);
+    k.setModel(dv.get(), nullptr, dd.get(), de.get(), nullptr, nullptr);
+    k.setBoundary(nb[0], nb[1], nb[2], nb[3], nb[4], nb[5], dt, vmax);
+    void *o = orc_engine_create_f(nx, ny, nz, d, d, d, FD::CFD, 10);
+    orc_engine_set_model_f(o, vel.data(), eps.data(), del.data(), nullptr, nullptr);
+    orc_engine_set_boundary_f(o, nb, dt, vmax, 1e-3f, 1, 0.3f, 1, 3);
+    std::vector<float> opb = pb, opr(n, 0.f);
+    for(int it = 0; it < 3; it++) {
+      k.VTI(d0.get(), dbb.get(), dr.get());
+      orc_engine_kernel_f(o, VTI, p0.data(), opb.data(), opr.data());
+    }
+    std::vector<float> got = dbb.host();
+    report("KernelCPU::VTI (CFD) bit-exact", std::memcmp(got.data(), opb.data(), n * 4) == 0, rel_l2(got, opb));
+    orc_engine_destroy_f(o);
+  }
+  // FdEngine_fd::laplacian (FD8, contracted multiply-adds): 1e-5
+  {
+    konglong::params().fd_order = 8;
+    FdEngine_fd f(nx, ny, nz, d, d, d, 1);
+    f.setModel(dv.get());
+    f.setBoundary(nb[0], nb[1], nb[2], nb[3], nb[4], nb[5], dt, vmax);
+    void *o = orc_engine_create_f(nx, ny, nz, d, d, d, FD::FD, 8);
+    orc_engine_set_model_f(o, vel.data(), nullptr, nullptr, nullptr, nullptr);
+    orc_engine_set_boundary_f(o, nb, dt, vmax, 1e-3f, 1, 0.3f, 1, 3);
+    std::vector<float> want(n);
+    orc_engine_laplacian_f(o, p0.data(), want.data());
+    f.laplacian(d0.get(), dr.get());
+    std::vector<float> got = dr.host();
+    double e = rel_l2(got, want);
+    report("FdEngine_fd::laplacian", e < 1e-5, e);
+    orc_engine_destroy_f(o);
+  }
+  // WaveFieldCompress COMPRESS_16P codes
+  {
+    WaveFieldCompress::compression_ref() = WaveFieldCompress::COMPRESS_16P;
+    WaveFieldCompress w(1);
+    const size_t ns = WaveFieldCompress::nshort_volume(nz, nx, ny);
+    konglong::DeviceBuffer<ushort> code(ns);
+    w.compress(d0.get(), code.get(), nx, ny, nz);
+    std::vector<ushort> got = code.host();
+    std::vector<uint16_t> want(ns);
+    orc_compress_pair(p0.data(), want.data(), nx, ny, nz, 2);
+    report("WaveFieldCompress COMPRESS_16P codes", std::memcmp(got.data(), want.data(), ns * 2) == 0, 0.0);
+    WaveFieldCompress::compression_ref() = WaveFieldCompress::COMPRESS_16;
+  }
+  // PML::apply2_single_trans: two rounds with one g
+  {
+    PML pml(40, 12, 1, 12.5f, 1e-3f, 4000.f, false);
+    const int m = 300;
+    std::vector<float> p(m), q(m), q2(m);
+    for(int i = 0; i < m; i++) p[i] = U(gen), q[i] = 0.1f * U(gen), q2[i] = 0.1f * U(gen);
+    konglong::DeviceBuffer<float> dp(m), dq(m), dq2(m);
+    dp.upload(p.data()), dq.upload(q.data()), dq2.upload(q2.data());
+    const float g = pml.coef[3];
+    pml.apply2_single_trans(g, dp.get(), dq.get(), dq2.get(), m);
+    orc_pml_round(g, p.data(), q.data(), m);
+    orc_pml_round(g, p.data(), q2.data(), m);
+    std::vector<float> gp = dp.host();
+    report("PML::apply2_single_trans bit-exact", std::memcmp(gp.data(), p.data(), m * 4) == 0, rel_l2(gp, p));
+  }
+  // ImagingCorrelator::crossone
+  {
+    ImagingCorrelator ic(nx, ny, nz);
+    konglong::DeviceBuffer<float> img(n);
+    ic.crossone(img.get(), d0.get(), dbb.get());
+    std::vector<float> got = img.host(), b = dbb.host(), want(n);
+    for(int i = 0; i < n; i++) want[i] = 0.f + p0[i] * b[i];
+    report("ImagingCorrelator::crossone", std::memcmp(got.data(), want.data(), n * 4) == 0, rel_l2(got, want));
+  }
+  if(failures) {
+    std::printf("API_CHECK FAIL (%d)\n", failures);
+    return 1;
+  }
+  std::printf("API_CHECK PASS\n");
+  return 0;
+}

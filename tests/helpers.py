@@ -32,3 +32,18 @@ def tilt_axis(shape, seed):
     ph = smooth_field(shape, 0.0, 360.0, seed + 1) * np.float32(np.pi / 180)
     st = np.sin(th)
     return (-st * np.cos(ph)).astype(np.float32), (-st * np.sin(ph)).astype(np.float32)
+
+
+def build_api_check(out_path):
+    """Compile tests/cpp/api_check.cpp against the built libraries (the bgprtm drop-in headers)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(root, "include"),
+           os.path.join(root, "tests", "cpp", "api_check.cpp"),
+           "-L", os.path.join(root, "konglong_amd"), "-lkonglong_rtm", "-L", os.path.join(root, "oracle"), "-loracle",
+           "-Wl,-rpath," + os.path.join(root, "konglong_amd"), "-Wl,-rpath," + os.path.join(root, "oracle"),
+           "-o", str(out_path)]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return str(out_path)
+

@@ -79,3 +79,11 @@ def test_nshort_volume_matches_reference_formula():
     assert _lib.lib().klrtm_nshort_volume(128, 128, 128) == (128 * 128 + 2) * 128
     assert _lib.lib().klrtm_blockq_nscales(400, 400, 400) == 50 ** 3
     assert _lib.lib().klrtm_blockq_nscales(9, 17, 8) == 2 * 3 * 1
+
+
+def test_cpp_dropin_headers_build(tmp_path):
+    """The bgprtm drop-in headers compile warning-free and link against the library (no GPU call)."""
+    from tests.helpers import build_api_check
+    exe = build_api_check(tmp_path / "api_check")
+    assert os.path.exists(exe)
+

@@ -320,3 +320,53 @@ def test_codec16_shift_known_answers():
     codes = c[2:].reshape(4, 4)
     assert codes[0, 0] == 0x1 and codes[1, 1] == 0x2000 and codes[2, 2] == 0x8000 | 0x1000
     assert c[0:2].view(np.float32)[0] == 2.0
+
+
+# ---- COMPRESS_16P / COMPRESS_8P (per-trace sum/difference codes) ------------
+@pytest.mark.parametrize("mode,nbits", [(2, 32), (3, 16)], ids=["16P", "8P"])
+@pytest.mark.parametrize("n", [1, 2, 7, 13, 20])
+def test_pair_codec_error_bound(mode, nbits, n):
+    """Round trip within half a quantisation step of each channel (no taper for nz <= 20, nx, ny <= 40)."""
+    rng = np.random.default_rng(n + mode)
+    v = (rng.standard_normal((3, 4, n)) * np.float32(1e-3)).astype(np.float32)
+    code = oracle.compress_pair(v, mode)
+    assert code.shape == (12, oracle.nshort_pack(n, mode))
+    back = oracle.uncompress_pair(code, 3, 4, n, mode)
+    for t in range(12):
+        tr, rt = v.reshape(12, n)[t], back.reshape(12, n)[t]
+        s = 0.5 * (tr[0:n - 1:2] + tr[1:n:2])
+        d = 0.5 * (tr[0:n - 1:2] - tr[1:n:2])
+        r1 = 2 * max(np.abs(s).max() if s.size else 0.0, abs(tr[-1]) if n % 2 else 0.0)
+        r2 = 2 * (np.abs(d).max() if d.size else 0.0)
+        b1 = oracle.pair_autobit(r1, r2, nbits)
+        step1 = r1 / (2.0 ** b1 - 1) if b1 else r1
+        step2 = r2 / (2.0 ** (nbits - b1) - 1) if nbits - b1 else r2
+        bound = 0.5 * step1 + 0.5 * step2 + 4e-7 * (r1 + r2)
+        assert np.abs(rt - tr).max() <= bound, (t, np.abs(rt - tr).max(), bound)
+
+
+@pytest.mark.parametrize("nbits", [16, 32])
+def test_pair_autobit_is_the_error_minimiser(nbits):
+    """autobit picks the first bit split with the least r1/2^b1 + r2/2^b2 (brute force)."""
+    rng = np.random.default_rng(nbits)
+    for _ in range(200):
+        r1, r2 = (np.float32(10.0 ** rng.uniform(-8, 3)) for _ in range(2))
+        best, bmin = np.float32(r1 + r2), nbits // 2
+        for b in range(nbits + 1):
+            e = np.float32(r1 / np.float32(2.0 ** b)) + np.float32(r2 / np.float32(2.0 ** (nbits - b)))
+            if e < best:
+                best, bmin = e, b
+        assert oracle.pair_autobit(r1, r2, nbits) == bmin
+    assert oracle.pair_autobit(1.0, 0.0, nbits) == nbits      # pairs all equal: every bit to the sums
+    assert oracle.pair_autobit(0.0, 0.0, nbits) == nbits // 2  # zero trace
+
+
+@pytest.mark.parametrize("mode", [2, 3], ids=["16P", "8P"])
+def test_pair_codec_special_traces(mode):
+    n = 12
+    const = np.full((1, 1, n), 0.37, np.float32)  # d = 0: the sums get every bit
+    back = oracle.uncompress_pair(oracle.compress_pair(const, mode), 1, 1, n, mode)
+    tol = 1e-6 if mode == 2 else 0.37 / 65535 + 1e-7
+    assert np.abs(back - const).max() <= tol
+    zero = np.zeros((2, 2, n), np.float32)  # ranges floor at FLT_MIN per level: decodes to ~1e-38
+    assert np.abs(oracle.uncompress_pair(oracle.compress_pair(zero, mode), 2, 2, n, mode)).max() <= 1e-30

@@ -227,3 +227,37 @@ def test_rtm_image_matches_oracle(klrtm, case):
     print(f"  image rel_l2={e1:.2e} illum rel_l2={e2:.2e} image64-vs-oracle32={e3:.2e}", flush=True)
     assert np.abs(img).max() > 0
     assert e1 < 1e-4 and e2 < 1e-4 and e3 < 1e-4
+
+
+@pytest.mark.parametrize("mode", [0, 2, 3], ids=["NONE", "16P", "8P"])
+@pytest.mark.parametrize("shape", [(23, 31, 37), (44, 42, 45)], ids=["small", "tapered"])
+def test_wavefield_codecs_match_oracle(klrtm, mode, shape):
+    """COMPRESS_16P / COMPRESS_8P codes and decodes bit for bit; COMPRESS_NONE decode (edge taper)."""
+    from konglong_amd.codec import WaveFieldCompress
+    ny, nx, nz = shape
+    rng = np.random.default_rng(mode + nz)
+    v = (rng.standard_normal(shape) * np.float32(2e-3)).astype(np.float32)
+    v[3, 4, :] = 0.25   # a constant trace (differences vanish)
+    v[5, 6, :] = 0.0    # a zero trace
+    wfc = WaveFieldCompress(compression=mode)
+    ns = wfc.nshort_volume(nz, nx, ny)
+    d_in, d_code, d_out = DeviceArray.from_host(v), DeviceArray((ns,), np.uint16), DeviceArray(shape)
+    wfc.compress(d_in, d_code, nx, ny, nz)
+    wfc.uncompress(d_code, d_out, nx, ny, nz)
+    got = d_out.download()
+    print(f"[codec] mode={mode} shape={shape}: max |decode - input| = {np.abs(got - v).max():.3e}", flush=True)
+    if mode == 0:
+        assert np.array_equal(d_code.download().view(np.float32).reshape(shape), v)
+        # decode = input times the edge taper: compare with the oracle's COMPRESS_16 taper on a flat field
+        ones = np.ones(shape, np.float32)
+        d1 = DeviceArray.from_host(ones)
+        wfc.compress(d1, d_code, nx, ny, nz)
+        wfc.uncompress(d_code, d_out, nx, ny, nz)
+        taper = d_out.download()
+        # taper factors multiply one at a time (y, x, z); a unit field gives their product, so allow 1 ulp
+        assert np.allclose(got, v * taper, rtol=2.5e-7, atol=0)
+        return
+    want_code = oracle.compress_pair(v, mode)
+    assert np.array_equal(d_code.download().reshape(want_code.shape), want_code)
+    want = oracle.uncompress_pair(want_code, ny, nx, nz, mode)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
