@@ -243,7 +243,10 @@ __global__ void __launch_bounds__(512) blockq_encode_kernel(const float *__restr
   const int zb = lz >> 3;
 #pragma unroll
   for(int k = 0; k < 8; k++) bm = fmaxf(bm, red[0][zb * 8 + k]);
-  const float s = (FMT == KLRTM_SNAP_FP16) ? bm * 0x1p-15f : bm;
+  // a block whose scale would be subnormal (max |x| below ~4e-34 for FP16, 1.2e-38 for BF16) is
+  // stored as zero: 1/scale would overflow and turn its zeros into NaN
+  const float s0 = (FMT == KLRTM_SNAP_FP16) ? bm * 0x1p-15f : bm;
+  const float s = s0 >= 1.17549435e-38f ? s0 : 0.f;
   const float inv = s > 0.f ? 1.0f / s : 0.f;
   // lane 0 of each z block in x row 0 (always inside the grid) owns the scale slot
   if(act && (lz & 7) == 0 && lx == 0) scales[block_index(ix, y0, iz, nx, nz)] = s;

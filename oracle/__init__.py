@@ -62,6 +62,10 @@ def lib():
         L.orc_extract.argtypes = [_pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, _pf, _pf, _pf, C.c_int, C.c_int,
                                   C.c_int, _pi, _pi, _pi, _pi, _pi, _pi]
         L.orc_compress16.argtypes = [_pf, C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.orc_inject_d.argtypes = [_pd, _pd, C.c_int, C.c_int, C.c_int, C.c_int, _pd, _pf, _pf, _pf, C.c_int, C.c_int,
+                                   C.c_int, _pi, _pi, _pi, _pi, _pi, _pi]
+        L.orc_extract_d.argtypes = [_pd, C.c_int, C.c_int, C.c_int, C.c_int, _pd, _pf, _pf, _pf, C.c_int, C.c_int,
+                                    C.c_int, _pi, _pi, _pi, _pi, _pi, _pi]
         L.orc_uncompress16.argtypes = [C.c_void_p, _pf, C.c_int, C.c_int, C.c_int]
         L.orc_codec16_table.argtypes = [_pf]
         L.orc_compress_pair.argtypes = [_pf, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]

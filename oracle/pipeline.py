@@ -15,7 +15,7 @@ import math
 
 import numpy as np
 
-from . import lib, Engine, spread_coeff, taper_kernel, xy_taper, compress16, uncompress16
+from . import lib, Engine, spread_coeff, taper_kernel, xy_taper, compress16, uncompress16, _pd
 
 _pf, _pi = C.POINTER(C.c_float), C.POINTER(C.c_int)
 
@@ -165,12 +165,102 @@ class Shot:
         return img, ill
 
 
-def _edge_taper(v):
-    """Edge taper of WaveFieldCompress::uncompress (20 x/y cells, 10 z cells)."""
+class Shot64:
+    """The same shot in FP64 throughout (engine, injection, extraction, snapshots, imaging): the
+    reference algorithm without rounding, against which reduced-precision images are measured.
+    The spread taps keep their FP32 values (they are part of the problem definition)."""
+
+    def __init__(self, nx, ny, nz, d, dt, nt, engine, order, model, nbnd, vmax, src_xyz, f_peak, spread, k1, k2,
+                 rcv_xyz, snap_every=1, tmute_frac=0.1, threads=4):
+        self.shape = (ny, nx, nz)
+        self.nx, self.ny, self.nz, self.dt, self.nt = nx, ny, nz, dt, nt
+        self.model_type, self.snap_every, self.tmute_frac = model, snap_every, tmute_frac
+        self.eng = Engine(nx, ny, nz, d, d, d, engine, order, np.float64, threads=threads)
+        self.nbnd, self.vmax = nbnd, vmax
+        self.dt_dxyz = 1.0 / (float(d) ** 3)
+        self.src = spread_tables(nx, ny, nz, np.asarray(src_xyz, np.float32).reshape(1, 3), spread, k1, k2)
+        self.rcv = spread_tables(nx, ny, nz, rcv_xyz, spread, k1, k2)
+        self.wavelet = ricker(nt, dt, f_peak, dtype=np.float64)
+
+    def set_model(self, vel, eps=None, del_=None, pjx=None, pjy=None):
+        v = np.asarray(vel, np.float64)
+        self.m = v * v * (self.dt * self.dt)
+        f64 = [None if a is None else np.asarray(a, np.float64) for a in (eps, del_, pjx, pjy)]
+        self.eng.set_model(self.m, *f64)
+        self.eng.set_boundary(self.nbnd, self.dt, self.vmax, dim_pml=3)
+
+    def _inject(self, w0, t, vals):
+        ny, nx, nz = self.shape
+        vals = np.ascontiguousarray(vals, np.float64)
+        lib().orc_inject_d(w0.ctypes.data_as(_pd), self.m.ctypes.data_as(_pd), nx, ny, nz, vals.size,
+                           vals.ctypes.data_as(_pd), *_tab_args(t))
+
+    def _extract(self, w, t):
+        ny, nx, nz = self.shape
+        out = np.zeros(t["lx"].size, np.float64)
+        lib().orc_extract_d(w.ctypes.data_as(_pd), nx, ny, nz, out.size, out.ctypes.data_as(_pd), *_tab_args(t))
+        return out
+
+    def _fresh(self):
+        self.eng.clean_pml()
+        z = np.zeros(self.shape, np.float64)
+        return z.copy(), z.copy(), z.copy()
+
+    def model(self):
+        w0, w1, wr = self._fresh()
+        rec = np.zeros((self.rcv["lx"].size, self.nt), np.float64)
+        for it in range(self.nt):
+            self.eng.step(self.model_type, w1, w0, wr)
+            if 0 <= it - 1 < self.nt:
+                self._inject(w0, self.src, self.wavelet[it - 1:it] * self.dt_dxyz)
+            rec[:, it] = self._extract(w0, self.rcv)
+            w0, w1 = w1, w0
+        return rec
+
+    def rtm(self, record):
+        k = self.snap_every
+        ntsav = (self.nt + k - 1) // k
+        snaps = [None] * ntsav
+        w0, w1, wr = self._fresh()
+        for it in range(self.nt):
+            self.eng.step(self.model_type, w1, w0, wr)
+            if 0 <= it - 1 < self.nt:
+                self._inject(w0, self.src, self.wavelet[it - 1:it] * self.dt_dxyz)
+            if it % k == 0 and it // k < ntsav:
+                snaps[it // k] = w0.copy()
+            w0, w1 = w1, w0
+        img = np.zeros(self.shape, np.float64)
+        ill = np.zeros(self.shape, np.float64)
+        ntaperT = int(np.rint(self.tmute_frac * self.nt * self.dt / (k * self.dt)))
+        w0, w1, wr = self._fresh()
+        rec = np.asarray(record, np.float64)
+        for it in range(self.nt - 1, -1, -1):
+            self.eng.step(self.model_type, w1, w0, wr)
+            ia = it + 1
+            if 0 <= ia < self.nt:
+                self._inject(w0, self.rcv, rec[:, ia] * self.dt_dxyz)
+            if it % k == 0 and it // k < ntsav:
+                s = it // k
+                S = _edge_taper(snaps[s], np.float64)
+                R = _edge_taper(w0, np.float64)
+                if s < ntaperT:
+                    S = S * (0.5 * (1 - math.cos(math.pi * s / ntaperT)))
+                img += S * R
+                ill += S * S
+            w0, w1 = w1, w0
+        return img, ill
+
+
+def _edge_taper(v, dtype=np.float32):
+    """Edge taper of WaveFieldCompress::uncompress (20 x/y cells, 10 z cells), in dtype."""
     ny, nx, nz = v.shape
-    out = np.array(v, np.float32, copy=True)
-    wt = (0.5 * (1 - np.cos(np.float32(np.pi) * np.arange(20, dtype=np.float32) / np.float32(20)))).astype(np.float32)
-    wz = (0.5 * (1 - np.cos(np.float32(np.pi) * np.arange(10, dtype=np.float32) / np.float32(10)))).astype(np.float32)
+    out = np.array(v, dtype, copy=True)
+    if dtype == np.float32:
+        wt = (0.5 * (1 - np.cos(np.float32(np.pi) * np.arange(20, dtype=np.float32) / np.float32(20)))).astype(np.float32)
+        wz = (0.5 * (1 - np.cos(np.float32(np.pi) * np.arange(10, dtype=np.float32) / np.float32(10)))).astype(np.float32)
+    else:
+        wt = 0.5 * (1 - np.cos(np.pi * np.arange(20) / 20.0))
+        wz = 0.5 * (1 - np.cos(np.pi * np.arange(10) / 10.0))
     if ny > 40:
         out[:20] *= wt[:, None, None]
         out[ny - 20:] *= wt[::-1, None, None]

@@ -1027,6 +1027,35 @@ void orc_extract(const float *w, int nx, int ny, int nz, int nr, float *out, con
   }
 }
 
+// FP64 forms of the two above (the spread taps stay the FP32 tables: they define the problem)
+void orc_inject_d(double *w0, const double *vel, int nx, int ny, int nz, int nr, const double *val, const float *kx,
+                  const float *ky, const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz,
+                  const int *offx, const int *offy, const int *offz) {
+  for(int ir = 0; ir < nr; ir++)
+    for(int iy = 0; iy < ly[ir]; iy++) {
+      const double cy = val[ir] * (double)ky[(size_t)ir * Ly + iy];
+      for(int ix = 0; ix < lx[ir]; ix++) {
+        const double cx = cy * (double)kx[(size_t)ir * Lx + ix];
+        const size_t off = ((size_t)(iy + offy[ir]) * nx + ix + offx[ir]) * nz + offz[ir];
+        for(int iz = 0; iz < lz[ir]; iz++) w0[off + iz] += cx * (double)kz[(size_t)ir * Lz + iz] * vel[off + iz];
+      }
+    }
+}
+void orc_extract_d(const double *w, int nx, int ny, int nz, int nr, double *out, const float *kx, const float *ky,
+                   const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz, const int *offx,
+                   const int *offy, const int *offz) {
+  for(int ir = 0; ir < nr; ir++) {
+    double val = 0;
+    for(int iy = 0; iy < ly[ir]; iy++)
+      for(int ix = 0; ix < lx[ir]; ix++) {
+        const double cx = (double)ky[(size_t)ir * Ly + iy] * (double)kx[(size_t)ir * Lx + ix];
+        const size_t off = ((size_t)(iy + offy[ir]) * nx + ix + offx[ir]) * nz + offz[ir];
+        for(int iz = 0; iz < lz[ir]; iz++) val += cx * (double)kz[(size_t)ir * Lz + iz] * w[off + iz];
+      }
+    out[ir] = val;
+  }
+}
+
 void orc_compress16(const float *in, uint16_t *out, int nx, int ny, int nz) {
   static Codec16 c;
   c.compress(in, out, nx, ny, nz);

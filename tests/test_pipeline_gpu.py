@@ -120,6 +120,10 @@ def test_blockq_roundtrip(klrtm, fmt, bound):
     ny, nx, nz = shape
     v = (rng.standard_normal(shape) * np.exp(rng.uniform(-6, 3, shape))).astype(np.float32)
     v[:8, :8, :8] = 0.0
+    # a block holding only subnormal-scale residue next to exact zeros: stored as zero, never NaN
+    v[8:16, :8, :8] = 0.0
+    v[9, 2, 3] = np.float32(3e-39)
+    v[10, 4, 5] = np.float32(-1e-36)
     dv = DeviceArray.from_host(v)
     dq = DeviceArray(shape, np.uint16)
     ns = blockq_nscales(nx, ny, nz)
@@ -129,15 +133,22 @@ def test_blockq_roundtrip(klrtm, fmt, bound):
     blockq_decode(dq, ds, do, nx, ny, nz, fmt)
     out = do.download()
     sc = ds.download()
+    assert np.isfinite(out).all()
     assert not out[:8, :8, :8].any()
-    # per-block max |x| is what the scale records (FP16 keeps a 2^-15 factor)
+    if fmt == SNAP_FP16:  # scale 1e-36 * 2^-15 is subnormal: the block is stored as zero
+        assert not out[8:16, :8, :8].any()
+    else:                 # BF16 keeps scale 1e-36
+        assert abs(out[10, 4, 5] - v[10, 4, 5]) <= bound * abs(v[10, 4, 5])
+    # per-block max |x| is what the scale records (FP16 keeps a 2^-15 factor); below FLT_MIN it is 0
     bmax = np.zeros(((ny + 7) // 8, (nx + 7) // 8, (nz + 7) // 8), np.float32)
     for by in range(bmax.shape[0]):
         for bx in range(bmax.shape[1]):
             for bz in range(bmax.shape[2]):
                 bmax[by, bx, bz] = np.abs(v[by*8:by*8+8, bx*8:bx*8+8, bz*8:bz*8+8]).max()
     fac = 2.0 ** -15 if fmt == SNAP_FP16 else 1.0
-    assert np.array_equal(sc.reshape(bmax.shape), (bmax * np.float32(fac)).astype(np.float32))
+    want = (bmax * np.float32(fac)).astype(np.float32)
+    want[want < np.float32(1.17549435e-38)] = 0.0
+    assert np.array_equal(sc.reshape(bmax.shape), want)
     err = np.abs(out - v)
     rel = err[np.abs(v) > 1e-3 * np.abs(v).max()] / np.abs(v[np.abs(v) > 1e-3 * np.abs(v).max()])
     assert rel.max() <= bound * 1.01
