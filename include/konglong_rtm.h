@@ -100,8 +100,9 @@ int klrtm_kernel_laplacian(klrtm_kernel *k, const float *in, float *out);
  * CFD gradient plan), isign <= 0 the divergence round and plan. */
 int klrtm_kernel_deriv1(klrtm_kernel *k, int axis, const float *in, float *out, int iacc, int isign);
 /* tile shape of the stencil kernels: tz (z lanes), tx (x rows), ychunk (y planes per
- * block); 0 keeps the current value (ychunk <= 0 with a new tz or tx, or ychunk < 0,
- * picks the default chunking again).  Used by the autotuner. */
+ * block), tz*tx <= 512; 0 keeps the current value, ychunk < 0 returns to the default
+ * chunking.  get_tile reports the shape the Laplacian step runs with.  Used by the
+ * autotuner. */
 int klrtm_kernel_set_tile(klrtm_kernel *k, int tz, int tx, int ychunk);
 int klrtm_kernel_get_tile(klrtm_kernel *k, int *tz, int *tx, int *ychunk);
 

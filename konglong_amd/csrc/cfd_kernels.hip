@@ -398,7 +398,7 @@ __global__ void __launch_bounds__(256) cfd_line_kernel(const float *in, float *t
 // beyond the line or the last line get an out-of-range offset, so their loads
 // return 0 and their stores are dropped, and no access needs a branch.
 // Per-sample arithmetic is identical to cfd_line_kernel.
-constexpr int ZC = 32, ZSP = ZC + 1;  // chunk length, LDS row pitch (odd: conflict-free)
+constexpr int ZC = 16, ZSP = ZC + 1;  // chunk length, LDS row pitch (odd: conflict-free)
 constexpr int ZLPR = 64 / ZC;         // lines per tile row
 constexpr int ZU = 8;                 // samples per unrolled step of an interior chunk sweep
 

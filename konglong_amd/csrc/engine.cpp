@@ -163,25 +163,26 @@ int Engine::set_boundary(const int *nbnd, float dt_, float vmax_) {
   return KLRTM_OK;
 }
 
-void Engine::default_tile(int R) {
-  if(tz <= 0) tz = std::max(R, std::min(64, nz));
-  if(tx <= 0) tx = std::max(R, 8);
-  while(tz * tx > 1024) tx = std::max(R, tx / 2);
-  if(ychunk <= 0) {
-    long tiles = (long)div_up(nz, tz) * div_up(nx, tx);
-    // about four blocks per CU slot in flight over the launch (2 resident per CU on 256 CUs), so the
-    // tail round stays short; longer chunks re-read fewer y halo planes
-    long want = 1536;
+// effective tile: the values set through klrtm_kernel_set_tile, the defaults for the rest
+void Engine::default_tile(int R, long want, int &etz, int &etx, int &eyc) const {
+  etz = tz > 0 ? tz : std::max(R, std::min(64, nz));
+  etx = tx > 0 ? tx : std::max(R, 8);
+  while(etz * etx > 512) etx = std::max(R, etx / 2);  // the stencil kernels are built for <= 512 threads
+  eyc = ychunk;
+  if(eyc <= 0) {
+    long tiles = (long)div_up(nz, etz) * div_up(nx, etx);
     int chunks = (int)std::max(1L, std::min((long)ny, (long)div_up(want, std::max(1L, tiles))));
-    ychunk = std::max(std::min(ny, 2 * R), div_up(ny, chunks));
+    eyc = std::max(std::min(ny, 2 * R), div_up(ny, chunks));
   }
 }
 
 void Engine::fill_args(IsoArgs &a, int deriv_order) {
   memset(&a, 0, sizeof(a));
   a.nx = nx, a.ny = ny, a.nz = nz;
-  default_tile(ne);
-  a.tz = tz, a.tx = tx, a.ychunk = ychunk;
+  // blocks per launch: the Laplacian step keeps about three rounds of resident blocks (2 per CU on
+  // 256 CUs), so the tail round is short and chunks stay long (fewer y halo planes re-read); the
+  // first-derivative kernels run best with smaller chunks
+  default_tile(ne, deriv_order == 2 ? 1536 : 2048, a.tz, a.tx, a.ychunk);
   a.do_y = ny > 2;
   a.has_pml = has_pml;
   a.pml = desc;
@@ -353,22 +354,21 @@ int klrtm_kernel_set_tile(klrtm_kernel *kk, int tz, int tx, int ychunk) {
   if(tz > 0) KL_REQUIRE(tz >= R, KLRTM_EINVAL, "tile tz must be >= stencil radius %d", R);
   if(tx > 0) KL_REQUIRE(tx >= R, KLRTM_EINVAL, "tile tx must be >= stencil radius %d", R);
   int ntz = tz > 0 ? tz : k->tz, ntx = tx > 0 ? tx : k->tx;
-  if(ntz > 0 && ntx > 0) KL_REQUIRE(ntz * ntx <= 1024, KLRTM_EINVAL, "tile tz*tx must be <= 1024");
+  if(ntz > 0 && ntx > 0) KL_REQUIRE(ntz * ntx <= 512, KLRTM_EINVAL, "tile tz*tx must be <= 512");
   if(tz > 0) k->tz = tz;
   if(tx > 0) k->tx = tx;
   if(ychunk > 0) k->ychunk = ychunk;
-  if(ychunk < 0 || tz > 0 || tx > 0) {
-    if(ychunk <= 0) k->ychunk = 0;  // recomputed for the new tile count
-  }
+  if(ychunk < 0) k->ychunk = 0;  // back to the default chunking
   return KLRTM_OK;
 }
 int klrtm_kernel_get_tile(klrtm_kernel *kk, int *tz, int *tx, int *ychunk) {
   Engine *k = reinterpret_cast<Engine *>(kk);
   KL_REQUIRE(k, KLRTM_EINVAL, "null engine");
-  k->default_tile(k->ne);
-  if(tz) *tz = k->tz;
-  if(tx) *tx = k->tx;
-  if(ychunk) *ychunk = k->ychunk;
+  int a, b, c;
+  k->default_tile(k->ne, 1536, a, b, c);  // as the Laplacian step would run
+  if(tz) *tz = a;
+  if(tx) *tx = b;
+  if(ychunk) *ychunk = c;
   return KLRTM_OK;
 }
 int klrtm_vel2(float *vel, size_t n, float dt) {

@@ -28,7 +28,7 @@ struct Engine {
   PmlDesc desc{};
   // scratch volumes of the first-derivative form (x and z gradients)
   float *d_px = nullptr, *d_pz = nullptr;
-  // stencil tile
+  // stencil tile set through klrtm_kernel_set_tile (0: default)
   int tz = 0, tx = 0, ychunk = 0;
   // compact-FD plans (device)
   CfdDevPlan *cfd = nullptr;
@@ -39,7 +39,7 @@ struct Engine {
   int ensure_comps(int comps);
   int clean();
   int set_boundary(const int *nbnd, float dt, float vmax);
-  void default_tile(int R);
+  void default_tile(int R, long want_blocks, int &tz_out, int &tx_out, int &ychunk_out) const;
   void fill_args(IsoArgs &a, int deriv_order);
   int check_model(int model);
   int scratch();

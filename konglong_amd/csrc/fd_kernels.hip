@@ -82,7 +82,7 @@ struct IsoPlane {
 };
 
 template<int R, int MODE>
-__global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict__ u, float *__restrict__ w0,
+__global__ void __launch_bounds__(512) iso_step_kernel(const float *__restrict__ u, float *__restrict__ w0,
                                                         const float *__restrict__ vel, float *__restrict__ pr,
                                                         const IsoArgs a) {
   extern __shared__ float s[];
@@ -119,24 +119,24 @@ __global__ void __launch_bounds__(1024) iso_step_kernel(const float *__restrict_
         rq[k][r] = buf_rsrc(q1 ? q1 : q2, (c1 + c2) * 4u);
       }
     }
+    // (faces are picked with selects: indexing the argument struct with a lane value would copy it
+    // to scratch memory)
     if(c.act) {
-      int f = -1, j = 0;
-      if(P.q[FZ1][0][0] && iz < P.w[FZ1]) f = FZ1, j = iz;
-      else if(P.q[FZ2][0][0] && iz >= nz - P.w[FZ2]) f = FZ2, j = iz - (nz - P.w[FZ2]);
-      if(f >= 0) {
-        const int w = P.w[f];
-        zo = ((f == FZ2 ? pair1[0] : 0u) + (uint32_t)ix * w + j) * 4u;  // ((iy*nx + ix)*w + j)
+      const bool z1 = P.q[FZ1][0][0] && iz < P.w[FZ1], z2 = !z1 && P.q[FZ2][0][0] && iz >= nz - P.w[FZ2];
+      if(z1 || z2) {
+        const int w = z1 ? P.w[FZ1] : P.w[FZ2], j = z1 ? iz : iz - (nz - w);
+        const float *t = (z1 ? P.abc[FZ1][0] : P.abc[FZ2][0]) + 3 * j;
+        zo = ((z2 ? pair1[0] : 0u) + (uint32_t)ix * w + j) * 4u;  // ((iy*nx + ix)*w + j)
         zs = (uint32_t)nx * w * 4u;
-        zc[0] = P.abc[f][0][3 * j], zc[1] = P.abc[f][0][3 * j + 1], zc[2] = P.abc[f][0][3 * j + 2];
+        zc[0] = t[0], zc[1] = t[1], zc[2] = t[2];
       }
-      f = -1;
-      if(P.q[FX1][0][0] && ix < P.w[FX1]) f = FX1, j = ix;
-      else if(P.q[FX2][0][0] && ix >= nx - P.w[FX2]) f = FX2, j = ix - (nx - P.w[FX2]);
-      if(f >= 0) {
-        const int w = P.w[f];
-        xo = ((f == FX2 ? pair1[1] : 0u) + (uint32_t)j * nz + iz) * 4u;  // ((iy*w + j)*nz + iz)
+      const bool x1 = P.q[FX1][0][0] && ix < P.w[FX1], x2 = !x1 && P.q[FX2][0][0] && ix >= nx - P.w[FX2];
+      if(x1 || x2) {
+        const int w = x1 ? P.w[FX1] : P.w[FX2], j = x1 ? ix : ix - (nx - w);
+        const float *t = (x1 ? P.abc[FX1][0] : P.abc[FX2][0]) + 3 * j;
+        xo = ((x2 ? pair1[1] : 0u) + (uint32_t)j * nz + iz) * 4u;  // ((iy*w + j)*nz + iz)
         xs = (uint32_t)w * nz * 4u;
-        xc[0] = P.abc[f][0][3 * j], xc[1] = P.abc[f][0][3 * j + 1], xc[2] = P.abc[f][0][3 * j + 2];
+        xc[0] = t[0], xc[1] = t[1], xc[2] = t[2];
       }
     }
     z_any = __syncthreads_or(zo != OOR) != 0;
