@@ -88,7 +88,8 @@ int klrtm_kernel_iso(klrtm_kernel *k, const float *p0, float *pb, float *pr);
 int klrtm_kernel_vti(klrtm_kernel *k, const float *p0, float *pb, float *pr);
 int klrtm_kernel_tti(klrtm_kernel *k, const float *p0, float *pb, float *pr);
 /* fused Propagator::kernel() + update2nd(): w0 <- 2*w1 - (w0 - vel*Op(w1)) in one pass
- * where the engine allows it (model = KLRTM_ISO/VTI/TTI) */
+ * where the engine allows it (model = KLRTM_ISO/VTI/TTI).  wr is work space: its contents
+ * afterwards are unspecified (Propagator::kernel reads nothing from it either). */
 int klrtm_kernel_step(klrtm_kernel *k, int model, const float *w1, float *w0, float *wr);
 
 /* FdEngine_fd::laplacian (bgprtm/FdEngine.h:116-118, bgprtm/FdEngine_fd.cpp:204-325) over the whole

@@ -26,7 +26,10 @@
 namespace klrtm {
 
 enum { AXZ = 0, AXX = 1, AXY = 2 };
-enum { P_GRAD = 0, P_GRAD_SCALAR = 1, P_DIV = 2, P_DIV_STEP = 3 };
+// line-pass epilogues: store D (P_GRAD), D + scalar operator (P_GRAD_SCALAR), pb -= vel*D (P_DIV, +
+// leapfrog P_DIV_STEP), or the whole divergence accumulate pb - vel*Dx - vel*Dz - vel*D from the stored
+// Dx, Dz + leapfrog (P_DIV3_STEP)
+enum { P_GRAD = 0, P_GRAD_SCALAR = 1, P_DIV = 2, P_DIV_STEP = 3, P_DIV3_STEP = 4 };
 
 struct CfdPassArgs {
   int nx, ny, nz;
@@ -110,7 +113,7 @@ struct LineSweep {
 
   static constexpr int E = PASS == P_GRAD ? 0
                            : PASS == P_GRAD_SCALAR ? (MODEL == KLRTM_ISO ? 0 : MODEL == KLRTM_VTI ? 4 : 6)
-                           : PASS == P_DIV ? 2 : 3;
+                           : PASS == P_DIV ? 2 : PASS == P_DIV_STEP ? 3 : 5;
 
   template<bool OWN, bool CROSS, int NB>
   struct Batch {
@@ -130,6 +133,8 @@ struct LineSweep {
       } else if(PASS == P_DIV || PASS == P_DIV_STEP) {
         B.o[0][j] = pb[c], B.o[1][j] = vel[c];
         if(PASS == P_DIV_STEP) B.o[2][j] = w1[c];
+      } else if(PASS == P_DIV3_STEP) {  // pr carries Dx here, pz Dz
+        B.o[0][j] = pb[c], B.o[1][j] = vel[c], B.o[2][j] = pr[c], B.o[3][j] = pz[c], B.o[4][j] = w1[c];
       }
       if(OWN) B.qo[j] = qo[(size_t)(ii - jbase) * qo_js];
       if(CROSS) {
@@ -188,9 +193,15 @@ struct LineSweep {
       if(MODEL == KLRTM_TTI) pz[c] = gz;
     } else if(PASS == P_DIV) {
       pb[c] = B.o[0][j] - g * B.o[1][j];
-    } else {
+    } else if(PASS == P_DIV_STEP) {
       float w = B.o[0][j] - g * B.o[1][j];
       pb[c] = (B.o[2][j] + B.o[2][j]) - w;
+    } else {
+      // the x, z, y accumulates of bgprtm/KernelCPU.cpp:108-114 in their order
+      float w = B.o[0][j] - B.o[2][j] * B.o[1][j];
+      w = w - B.o[3][j] * B.o[1][j];
+      w = w - g * B.o[1][j];
+      pb[c] = (B.o[4][j] + B.o[4][j]) - w;
     }
   }
 
@@ -927,10 +938,17 @@ static int cfd_run_m(Engine &e, int mode, const float *p0, float *pb, float *pr)
   if((rc = launch_line<AXX, P_GRAD_SCALAR, MODEL>(e, 0, 0, p0, px, pr, pz, pb, p0, "cfd_grad_x"))) return rc;
   // divergence (isign=-1, plan 2, PML round 2): x, z, y as bgprtm/KernelCPU.cpp:108-114
   const bool step = mode == MODE_STEP;
+  if(y && step) {
+    // Dx and Dz stay in px and pz; the y pass applies all three accumulates and the leapfrog, so pb
+    // is read and written once.  The y pass keeps its b' in pr, which the step leaves as work space
+    // (the un-fused kernel call leaves the scalar operator's y input there, as KernelCPU does).
+    if((rc = launch_line<AXX, P_GRAD, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x"))) return rc;
+    if((rc = launch_zline<P_GRAD>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z"))) return rc;
+    return launch_line<AXY, P_DIV3_STEP, MODEL>(e, 1, 1, pr, pr, px, pz, pb, p0, "cfd_div_y");
+  }
   if((rc = launch_line<AXX, P_DIV, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x"))) return rc;
   if(y) {
     if((rc = launch_zline<P_DIV>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z"))) return rc;
-    if(step) return launch_line<AXY, P_DIV_STEP, MODEL>(e, 1, 1, pr, px, pr, pz, pb, p0, "cfd_div_y");
     return launch_line<AXY, P_DIV, MODEL>(e, 1, 1, pr, px, pr, pz, pb, p0, "cfd_div_y");
   }
   if(step) return launch_zline<P_DIV_STEP>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z");
