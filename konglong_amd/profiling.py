@@ -8,7 +8,7 @@ FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by the ratio of the
 known bytes of a plain device-to-device copy in the same run (the largest
 __amd_rocclr_copyBuffer dispatch, `copy_bytes` read and written) to what the
 counter reports for it.  The per-step totals of the kernels whose names start
-with one of the group prefixes are written as `fetch_bytes_per_step` and
+with one of the group prefixes (each weighted by its launches per step) are written as `fetch_bytes_per_step` and
 `write_bytes_per_step` lines, which bench.py reads as the roofline traffic.
 """
 import argparse
@@ -75,15 +75,20 @@ def main():
     print(f"# counter scale: FETCH_SIZE x{kf:.3f}, WRITE_SIZE x{kw:.3f} "
           f"(largest device copy of {a.copy_bytes / 1e6:.1f} MB read and written)")
     print(f"{'kernel':60s} {'calls':>6s} {'avg_us':>10s} {'pct':>6s} {'fetch_MB':>10s} {'write_MB':>10s}")
+    # a kernel launched twice per step (the same template for two passes) has twice the calls of the
+    # others in the group; the step count is the smallest call count in the group
+    in_group = [k for k in stats if any(k.startswith(p) for p in prefixes)]
+    steps = min(stats[k][0] for k in in_group) if in_group else 1
     tf = tw = tus = 0.0
     for k, (calls, us, pct) in sorted(stats.items(), key=lambda kv: -kv[1][2]):
         f = sum(fv[k]) / len(fv[k]) * 1024 * kf if fv.get(k) else float("nan")
         w = sum(wv[k]) / len(wv[k]) * 1024 * kw if wv.get(k) else float("nan")
         print(f"{k[:60]:60s} {calls:6d} {us:10.1f} {pct:6.2f} {f / 1e6:10.1f} {w / 1e6:10.1f}")
-        if any(k.startswith(p) for p in prefixes):
-            tf += f
-            tw += w
-            tus += us
+        if k in in_group:
+            per = calls / steps
+            tf += f * per
+            tw += w * per
+            tus += us * per
     print(f"step_kernel_us: {tus:.1f}")
     print(f"fetch_bytes_per_step: {tf:.0f}")
     print(f"write_bytes_per_step: {tw:.0f}")
