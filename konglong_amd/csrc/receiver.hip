@@ -89,6 +89,8 @@ struct RcvGroup {
   float *kz;             // device, lz taps
   int *bin_start;        // device, bw*bh+1 (CSR over kernel origins)
   int *bin_rcv;          // device, receiver ids
+  int2 *corg;            // device, kernel origins (x, y) in CSR order
+  float *ckx, *cky;      // device, [L][nr] taps in CSR order (zero past each kernel's length)
   int nr;
 };
 
@@ -105,14 +107,19 @@ struct Receiver {
   float *d_traces = nullptr;  // [nt][nr]
   float *d_record = nullptr;  // [nt][nr]
   float *d_plane = nullptr;   // scratch plane (largest group bbox)
+  float *d_wy = nullptr;      // scratch y factors of the injection terms (largest group, [L][nr])
   size_t plane_cells = 0;
   ~Receiver();
 };
 
 Receiver::~Receiver() {
-  for(auto &g : groups) (void)hipFree(g.kz), (void)hipFree(g.bin_start), (void)hipFree(g.bin_rcv);
+  for(auto &g : groups) {
+    (void)hipFree(g.kz), (void)hipFree(g.bin_start), (void)hipFree(g.bin_rcv);
+    (void)hipFree(g.corg), (void)hipFree(g.ckx), (void)hipFree(g.cky);
+  }
   (void)hipFree(d_kx), (void)hipFree(d_ky), (void)hipFree(d_lx), (void)hipFree(d_ly);
   (void)hipFree(d_offx), (void)hipFree(d_offy), (void)hipFree(d_traces), (void)hipFree(d_record), (void)hipFree(d_plane);
+  (void)hipFree(d_wy);
 }
 
 // ---------------------------------------------------------------------------
@@ -122,59 +129,58 @@ struct GroupArgs {
   int offz, lz, bx0, by0, bw, bh, lxmax, lymax, L;
 };
 
+// wy[j][k] = (val_r*scaler)*ky_r[j] for the receiver r at CSR position k of
+// the group's nr: the y factor of every term, formed once per step.
+__global__ void inject_wy_kernel(int n, int nr, const int *__restrict__ bin_rcv, const float *__restrict__ cky,
+                                 const float *__restrict__ val, float scaler, float *__restrict__ wy) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= n) return;
+  wy[e] = (val[bin_rcv[e % nr]] * scaler) * cky[e];
+}
+
 // T[iy][ix] = sum_r (val_r*ky_r)*kx_r over receivers covering (ix,iy).
-// One block owns a 16x16 tile of the plane.  The receivers whose kernels can
-// reach the tile are those with origin bins in [tile - lmax + 1, tile end]:
-// for each bin row they are one contiguous CSR range, staged through LDS in
-// chunks (scaled value, origin, extents, x and y taps), and every cell of
-// the tile walks the chunk.  Bin rows ascend and each range is in CSR order,
-// so a cell adds its terms in the same order as a per-cell walk of the bins.
-constexpr int ITX = 16, ITY = 16;
-__global__ void __launch_bounds__(ITX *ITY)
-    inject_plane_kernel(const GroupArgs a, const int *__restrict__ bin_start, const int *__restrict__ bin_rcv,
-                        const float *__restrict__ kx, const float *__restrict__ ky, const int *__restrict__ lx,
-                        const int *__restrict__ ly, const int *__restrict__ offx, const int *__restrict__ offy,
-                        const float *__restrict__ val, float scaler, float *__restrict__ plane, int ICH) {
-  extern __shared__ float smem[];
-  float *s_val = smem;
-  int *s_ox = (int *)(s_val + ICH), *s_oy = s_ox + ICH, *s_lx = s_oy + ICH, *s_ly = s_lx + ICH;
-  float *s_kx = (float *)(s_ly + ICH), *s_ky = s_kx + ICH * a.L;
-  const int tid = threadIdx.x;
-  const int ntx = (a.bw + ITX - 1) / ITX;
-  const int tx0 = (blockIdx.x % ntx) * ITX, ty0 = (blockIdx.x / ntx) * ITY;
-  const int px = tx0 + tid % ITX, py = ty0 + tid / ITX;
-  const int jx0 = a.bx0 + px, jy0 = a.by0 + py;  // absolute cell
-  const int qx_lo = max(0, tx0 - a.lxmax + 1), qx_hi = min(a.bw - 1, tx0 + ITX - 1);
-  const int qy_lo = max(0, ty0 - a.lymax + 1), qy_hi = min(a.bh - 1, ty0 + ITY - 1);
+// A receiver can reach a cell only if its kernel origin lies in [cell - lmax
+// + 1, cell] on both axes; for each origin row those receivers are one
+// contiguous CSR range of the bins.  IG adjacent lanes share a cell, lane j
+// taking the origin rows j, j + IG, ... of its window, and the IG partial sums
+// are combined in a fixed tree, so the result does not depend on scheduling.
+// The receivers' origins, x taps and y factors are stored in CSR order, taps
+// tap-major ([tap][k]; zero past a kernel's length): neighbouring cells walk
+// neighbouring receivers at the same tap offsets, so the loads of a wave are
+// contiguous, and there is no bounds branch: an index outside [0, L) is
+// clamped and its term dropped by a select (adding +0 leaves a sum that
+// started at +0 unchanged).
+template<int IG>
+__global__ void __launch_bounds__(256)
+    inject_plane_kernel(const GroupArgs a, const int *__restrict__ bin_start, const int2 *__restrict__ corg,
+                        const float *__restrict__ ckx, const float *__restrict__ wy, float *__restrict__ plane) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = (int)(t / IG), sub = (int)(t % IG);
+  const bool act = c < a.bw * a.bh;
+  const int L = a.L;
   float T = 0.f;
-  for(int qy = qy_lo; qy <= qy_hi; qy++) {
-    const int k0 = bin_start[qy * a.bw + qx_lo], k1 = bin_start[qy * a.bw + qx_hi + 1];
-    for(int kb = k0; kb < k1; kb += ICH) {
-      const int cnt = min(ICH, k1 - kb);
-      __syncthreads();
-      for(int j = tid; j < cnt; j += ITX * ITY) {
-        const int r = bin_rcv[kb + j];
-        s_val[j] = val[r] * scaler;
-        s_ox[j] = offx[r], s_oy[j] = offy[r], s_lx[j] = lx[r], s_ly[j] = ly[r];
-      }
-      for(int e = tid; e < cnt * a.L; e += ITX * ITY) {
-        const int j = e / a.L, t = e - j * a.L;
-        const int r = bin_rcv[kb + j];
-        s_kx[e] = kx[(size_t)r * a.L + t];
-        s_ky[e] = ky[(size_t)r * a.L + t];
-      }
-      __syncthreads();
-      for(int j = 0; j < cnt; j++) {
-        const int ex = jx0 - s_ox[j], ey = jy0 - s_oy[j];
-        if(ex >= 0 && ey >= 0 && ex < s_lx[j] && ey < s_ly[j]) T += (s_val[j] * s_ky[j * a.L + ey]) * s_kx[j * a.L + ex];
+  if(act) {
+    const int px = c % a.bw, py = c / a.bw;
+    const int jx = a.bx0 + px, jy = a.by0 + py;  // absolute cell
+    const int qx_lo = max(0, px - a.lxmax + 1), qy_lo = max(0, py - a.lymax + 1);
+    for(int qy = qy_lo + sub; qy <= py; qy += IG) {
+      const int k0 = bin_start[qy * a.bw + qx_lo], k1 = bin_start[qy * a.bw + px + 1];
+#pragma unroll 4
+      for(int k = k0; k < k1; k++) {
+        const int2 o = corg[k];
+        const int ex = jx - o.x, ey = jy - o.y;
+        const bool ok = ex >= 0 && ey >= 0 && ex < L && ey < L;
+        const float term = wy[(size_t)min(max(ey, 0), L - 1) * a.nr + k] * ckx[(size_t)min(max(ex, 0), L - 1) * a.nr + k];
+        T += ok ? term : 0.f;
       }
     }
   }
-  if(px < a.bw && py < a.bh) plane[py * a.bw + px] = T;
+  // lanes c*IG .. c*IG+IG-1 are adjacent in one wave: (T0 + T1) + (T2 + T3) in every lane
+#pragma unroll
+  for(int m = 1; m < IG; m *= 2) T = T + __shfl_xor(T, m);
+  if(act && sub == 0) plane[c] = T;
 }
-// receivers per LDS chunk: 48 KiB of staging at most
-static int inject_chunk(int L) { return std::max(8, std::min(128, 49152 / ((5 + 2 * L) * (int)sizeof(float)))); }
-static size_t inject_smem(int L) { return (size_t)inject_chunk(L) * (5 + 2 * L) * sizeof(float); }
+constexpr int IG = 4;
 
 // w0[col, offz+k] += T*kz[k]*vel
 __global__ void inject_apply_kernel(const GroupArgs a, const float *__restrict__ plane, const float *__restrict__ kz,
@@ -237,10 +243,11 @@ int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scale
   for(auto &g : R.groups) {
     GroupArgs a = gargs(R, g);
     int cells = g.bw * g.bh;
-    const int tiles = div_up(g.bw, ITX) * div_up(g.bh, ITY);
-    hipLaunchKernelGGL(inject_plane_kernel, dim3(tiles), dim3(ITX * ITY), inject_smem(R.L), stream(), a, g.bin_start,
-                       g.bin_rcv, R.d_kx, R.d_ky, R.d_lx, R.d_ly, R.d_offx, R.d_offy, val, scaler, R.d_plane,
-                       inject_chunk(R.L));
+    const int nw = g.nr * R.L;
+    hipLaunchKernelGGL(inject_wy_kernel, dim3(div_up(nw, 256)), dim3(256), 0, stream(), nw, g.nr, g.bin_rcv, g.cky, val,
+                       scaler, R.d_wy);
+    hipLaunchKernelGGL(inject_plane_kernel<IG>, dim3(div_up((long)cells * IG, 256)), dim3(256), 0, stream(), a,
+                       g.bin_start, g.corg, g.ckx, R.d_wy, R.d_plane);
     long tot = (long)cells * g.lz;
     hipLaunchKernelGGL(inject_apply_kernel, dim3(div_up(tot, 256)), dim3(256), 0, stream(), a, R.d_plane, g.kz, w0, vel);
   }
@@ -358,9 +365,22 @@ int receiver_create(Receiver **out, int nx, int ny, int nz, int nr, const float 
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
     for(size_t i = 0; i < ids.size(); i++) rid[fill[keyof[i]]++] = ids[i];
     std::vector<float> kzv(R->kz.begin() + (size_t)r0 * limz, R->kz.begin() + (size_t)r0 * limz + std::max(g.lz, 1));
-    if((rc = upload(&g.kz, kzv)) || (rc = upload(&g.bin_start, cnt)) || (rc = upload(&g.bin_rcv, rid))) break;
+    std::vector<int2> corg(ids.size());
+    std::vector<float> ckx(ids.size() * (size_t)R->L), cky(ids.size() * (size_t)R->L);
+    for(size_t k = 0; k < ids.size(); k++) {
+      const int r = rid[k];
+      corg[k] = make_int2(R->offx[r], R->offy[r]);
+      for(int j = 0; j < R->L; j++) {
+        ckx[(size_t)j * ids.size() + k] = R->kx[(size_t)r * R->L + j];
+        cky[(size_t)j * ids.size() + k] = R->ky[(size_t)r * R->L + j];
+      }
+    }
+    R->groups.push_back(g);  // owned by R from here (freed with it on any error below)
+    RcvGroup &G = R->groups.back();
+    if((rc = upload(&G.kz, kzv)) || (rc = upload(&G.bin_start, cnt)) || (rc = upload(&G.bin_rcv, rid)) ||
+       (rc = upload(&G.corg, corg)) || (rc = upload(&G.ckx, ckx)) || (rc = upload(&G.cky, cky)))
+      break;
     R->plane_cells = std::max(R->plane_cells, (size_t)g.bw * g.bh);
-    R->groups.push_back(g);
   }
   if(!rc) {
     if((rc = upload(&R->d_kx, R->kx)) || (rc = upload(&R->d_ky, R->ky)) || (rc = upload(&R->d_lx, R->lx)) ||
@@ -368,7 +388,12 @@ int receiver_create(Receiver **out, int nx, int ny, int nz, int nr, const float 
     }
   }
   if(!rc) {
+  }
+  if(!rc) {
     hipError_t e = hipMalloc(&R->d_plane, std::max<size_t>(1, R->plane_cells) * sizeof(float));
+    size_t nwy = 1;
+    for(auto &g : R->groups) nwy = std::max(nwy, (size_t)g.nr * R->L);
+    if(e == hipSuccess) e = hipMalloc(&R->d_wy, nwy * sizeof(float));
     if(e == hipSuccess) e = hipMalloc(&R->d_record, (size_t)nt * nr * sizeof(float));
     if(e == hipSuccess) e = hipMemset(R->d_record, 0, (size_t)nt * nr * sizeof(float));
     if(e != hipSuccess) set_error("receiver allocation failed: %s", hipGetErrorString(e)), rc = KLRTM_ENOMEM;
