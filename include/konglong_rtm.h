@@ -34,8 +34,9 @@ enum { KLRTM_OK = 0, KLRTM_EINVAL = 1, KLRTM_EDEVICE = 2, KLRTM_ENOMEM = 3, KLRT
 enum { KLRTM_ENGINE_CFD = 4, KLRTM_ENGINE_FD = 7 };
 /* model kinds use the reference's ModelType values (bgprtm/Model.h:13) */
 enum { KLRTM_ISO = 1, KLRTM_VTI = 2, KLRTM_TTI = 3 };
-/* snapshot storage formats (wf_compress, bgprtm/WaveFieldCompress.h:6, plus block-scaled half types) */
-enum { KLRTM_SNAP_FP32 = 0, KLRTM_SNAP_C16 = 1, KLRTM_SNAP_FP16 = 16, KLRTM_SNAP_BF16 = 17 };
+/* snapshot storage formats (wf_compress, bgprtm/WaveFieldCompress.h:6, plus block-scaled half types
+ * and the learned codec) */
+enum { KLRTM_SNAP_FP32 = 0, KLRTM_SNAP_C16 = 1, KLRTM_SNAP_FP16 = 16, KLRTM_SNAP_BF16 = 17, KLRTM_SNAP_LEARNED = 32 };
 
 /* ---- errors, device, memory ------------------------------------------------ */
 const char *klrtm_last_error(void);
@@ -174,6 +175,28 @@ size_t klrtm_blockq_nscales(int nx, int ny, int nz);
 int klrtm_blockq_encode(const float *in, uint16_t *payload, float *scales, int nx, int ny, int nz, int fmt);
 int klrtm_blockq_decode(const uint16_t *payload, const float *scales, float *out, int nx, int ny, int nz, int fmt);
 
+/* ---- learned snapshot codec (replaces WaveFieldCompress in the snapshot cache,
+ * bgprtm/WaveFieldCompress.cpp:25-279 as called from bgprtm/Propagator.cpp:1233-1243) ----
+ * Learned transform coding on 8x8x8 blocks: the 512 samples x of a block (order (ly*8+lx)*8+lz)
+ * are stored as nl BF16 values z = x W (a linear autoencoder's code) plus the residual
+ * r = x - z W^T quantised to int8 with one FP32 scale per block (max|r|/127); decode is
+ * z W^T + q*scale.  W (host, [512][nl] row-major, nl 32, 64, 128 or 256) comes from training:
+ * klrtm_lcodec_gram accumulates G += sum over blocks of (s x)^T (s x) into a device [512][512]
+ * FP64 matrix; W = its nl leading eigenvectors.  Grid sizes must be multiples of 8.  A code
+ * (klrtm_lcodec_code_bytes) is z [nblk][nl] uint16 | scale [nblk] float | q [nblk][512] int8,
+ * each part 256-byte aligned, block index (by*nx/8 + bx)*nz/8 + bz. */
+typedef struct klrtm_lcodec klrtm_lcodec;
+int klrtm_lcodec_create(klrtm_lcodec **c, int nl, const float *W);
+int klrtm_lcodec_destroy(klrtm_lcodec *c);
+size_t klrtm_lcodec_code_bytes(const klrtm_lcodec *c, int nx, int ny, int nz);
+int klrtm_lcodec_encode(const klrtm_lcodec *c, const float *vol, void *code, int nx, int ny, int nz);
+int klrtm_lcodec_decode(const klrtm_lcodec *c, const void *code, float *vol, int nx, int ny, int nz);
+int klrtm_lcodec_gram(const float *vol, int nx, int ny, int nz, float scale, double *G);
+/* klrtm_ic_correlate with the source wavefield given as a learned code (decoded in the same pass) */
+int klrtm_lcodec_correlate(const klrtm_lcodec *c, const void *code, const float *rec, int nx, int ny, int nz,
+                           float taper_t, const float *taper_z, int ntaper_z, float *image, float *illum,
+                           double *image64, double *illum64);
+
 /* ---- ImagingCorrelator::crossone + ImagingCondition::calIllum (zero lag) ----- */
 /* image += S*R, illum += S*S with S = src*taper as in ImagingCondition::applyTape:
  * taper_t >= 0 multiplies every cell by taper_t (time taper wtaperT[it]); taper_t < 0
@@ -199,7 +222,8 @@ typedef struct {
   float khpass, khcut;
   int nrcv;               /* receivers; coordinates supplied separately */
   int snap_every;         /* store every k-th step (nprop_per_ic) */
-  int snap_format;        /* KLRTM_SNAP_* */
+  int snap_format;        /* KLRTM_SNAP_* (KLRTM_SNAP_LEARNED: set the codec with
+                             klrtm_propagator_set_lcodec before the first rtm call) */
   int rec_through_codec;  /* 1: receiver field passes COMPRESS_16 like the reference cache */
   float tmute_frac;       /* IC time taper length as a fraction of the record (tmutePerc) */
   int ntaper_z;           /* IC water-depth taper cells */
@@ -235,6 +259,8 @@ int klrtm_propagator_backward_steps(klrtm_propagator *p, int it0, int nsteps, in
 int klrtm_propagator_set_record(klrtm_propagator *p, const float *record_host);
 /* reset wavefields and PML state */
 int klrtm_propagator_reset(klrtm_propagator *p);
+/* learned codec of KLRTM_SNAP_LEARNED snapshots (borrowed: it must outlive the propagator's use) */
+int klrtm_propagator_set_lcodec(klrtm_propagator *p, const klrtm_lcodec *c);
 
 #ifdef __cplusplus
 }

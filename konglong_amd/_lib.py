@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libkonglong_rtm.so")
 KLRTM_OK = 0
 ENGINE_CFD, ENGINE_FD = 4, 7
 ISO, VTI, TTI = 1, 2, 3
-SNAP_FP32, SNAP_C16, SNAP_FP16, SNAP_BF16 = 0, 1, 16, 17
+SNAP_FP32, SNAP_C16, SNAP_FP16, SNAP_BF16, SNAP_LEARNED = 0, 1, 16, 17, 32
 
 _vp, _i, _f, _d, _sz, _l = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t, C.c_long
 _pi, _pf, _pd, _pl = C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_long)
@@ -111,6 +111,14 @@ _SIGS = {
     "klrtm_propagator_reset": (_i, [_vp]),
     "klrtm_propagator_backward_steps": (_i, [_vp, _i, _i, _i]),
     "klrtm_propagator_set_record": (_i, [_vp, _pf]),
+    "klrtm_propagator_set_lcodec": (_i, [_vp, _vp]),
+    "klrtm_lcodec_create": (_i, [C.POINTER(_vp), _i, _pf]),
+    "klrtm_lcodec_destroy": (_i, [_vp]),
+    "klrtm_lcodec_code_bytes": (_sz, [_vp, _i, _i, _i]),
+    "klrtm_lcodec_encode": (_i, [_vp, _vp, _vp, _i, _i, _i]),
+    "klrtm_lcodec_decode": (_i, [_vp, _vp, _vp, _i, _i, _i]),
+    "klrtm_lcodec_gram": (_i, [_vp, _i, _i, _i, _f, _vp]),
+    "klrtm_lcodec_correlate": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

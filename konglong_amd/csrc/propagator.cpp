@@ -13,6 +13,7 @@
 #include "receiver.h"
 #include "codec.h"
 #include "ic.h"
+#include "lcodec.h"
 #include <cmath>
 #include <cstring>
 #include <algorithm>
@@ -35,6 +36,7 @@ struct Propagator {
   float *d_img = nullptr, *d_ill = nullptr;
   double *d_img64 = nullptr, *d_ill64 = nullptr;
   float *d_taper_z = nullptr;
+  const LCodec *lc = nullptr;  // KLRTM_SNAP_LEARNED codec (borrowed)
   std::vector<float> wtaperT;
   double t_fwd = 0, t_bwd = 0, t_ic = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -80,6 +82,12 @@ int Propagator::alloc_snapshots() {
     snap_stride_bytes = n * sizeof(uint16_t);
     nscales = blockq_nscales(c.nx, c.ny, c.nz);
     break;
+  case KLRTM_SNAP_LEARNED:
+    KL_REQUIRE(lc, KLRTM_ESTATE, "learned snapshots need a codec (klrtm_propagator_set_lcodec)");
+    KL_REQUIRE(c.nx % 8 == 0 && c.ny % 8 == 0 && c.nz % 8 == 0, KLRTM_EINVAL,
+               "learned snapshots need grid sizes that are multiples of 8");
+    snap_stride_bytes = lcodec_code_bytes(*lc, c.nx, c.ny, c.nz);
+    break;
   default: set_error("unknown snapshot format %d", c.snap_format); return KLRTM_EINVAL;
   }
   snap_stride_bytes = (snap_stride_bytes + 255) & ~(size_t)255;
@@ -114,6 +122,7 @@ int Propagator::store(int itsav) {
     KL_HIP(hipMemcpyAsync(dst, w0, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
     return KLRTM_OK;
   case KLRTM_SNAP_C16: return c16_encode(w0, (uint16_t *)dst, c.nx, c.ny, c.nz, d_smax);
+  case KLRTM_SNAP_LEARNED: return lcodec_encode(*lc, w0, dst, c.nx, c.ny, c.nz);
   default: return blockq_encode(w0, (uint16_t *)dst, d_scales + nscales * (size_t)itsav, c.nx, c.ny, c.nz, c.snap_format);
   }
 }
@@ -132,6 +141,8 @@ int Propagator::image(int itsav) {
     if(rc) return rc;
   }
   const char *snap = d_snap + snap_stride_bytes * (size_t)itsav;
+  if(c.snap_format == KLRTM_SNAP_LEARNED)
+    return lcodec_image(*lc, a, snap, w0, d_img, d_ill, d_img64, d_ill64);
   const float *sc = nscales ? d_scales + nscales * (size_t)itsav : nullptr;
   return launch_ic(c.snap_format, rq, a, snap, sc, w0, d_smax, d_img, d_ill, d_img64, d_ill64);
 }
@@ -408,6 +419,16 @@ int klrtm_propagator_set_record(klrtm_propagator *p, const float *record) {
 int klrtm_propagator_reset(klrtm_propagator *p) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
   return reinterpret_cast<Propagator *>(p)->reset();
+}
+
+int klrtm_propagator_set_lcodec(klrtm_propagator *p, const klrtm_lcodec *c) {
+  KL_REQUIRE(p && c, KLRTM_EINVAL, "null argument");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  KL_REQUIRE(!P.d_snap || P.lc, KLRTM_ESTATE, "snapshots are already allocated for another format");
+  const LCodec *lc = reinterpret_cast<const LCodec *>(c);
+  KL_REQUIRE(!P.d_snap || P.lc->nl == lc->nl, KLRTM_ESTATE, "snapshots are allocated for a codec of another width");
+  P.lc = lc;
+  return KLRTM_OK;
 }
 
 }  // extern "C"

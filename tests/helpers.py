@@ -47,3 +47,14 @@ def build_api_check(out_path):
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     return str(out_path)
 
+
+
+def bf16_round(a):
+    """float32 -> nearest BF16 (ties to even), returned as float32 (finite inputs)."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def bf16_bits_to_float(h):
+    return (np.asarray(h, np.uint16).astype(np.uint32) << 16).view(np.float32)

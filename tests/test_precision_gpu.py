@@ -101,3 +101,54 @@ def test_fp64_multishot_stack(klrtm):
     e = nrmse(stack_gpu, stack_64)
     print(f"[precision] TTI-FD16 3-shot FP64 stack, FP16 snapshots: NRMSE vs FP64 oracle = {e:.3e}", flush=True)
     assert e <= 1e-2
+
+
+@pytest.mark.parametrize("case", [(ENGINE_FD, 16, TTI), (ENGINE_CFD, 10, VTI)], ids=["TTI-FD16", "VTI-CFD10"])
+def test_learned_snapshot_nrmse(klrtm, case):
+    """Learned-codec snapshots (32 BF16 code values + int8 residual per 8^3 block, trained on ISO
+    forward snapshots of other random models on the same grid) against the FP64 oracle, beside
+    FP32 snapshots."""
+    from konglong_amd import learned
+    from konglong_amd._lib import SNAP_LEARNED
+    engine, order, model = case
+    n = 48
+    shape = (n, n, n)
+    vel = smooth_field(shape, 1600.0, 2800.0, 31)
+    eps = smooth_field(shape, 0.0, 0.25, 32)
+    dl = smooth_field(shape, 0.0, 0.12, 33)
+    pjx = pjy = None
+    if model == TTI:
+        pjx, pjy = tilt_axis(shape, 34)
+    dt = 0.9 * cfl_dt(vel, D, engine, order, VTI, eps, dl)
+    nt = int(0.36 / dt)
+    rcv = np.array([[x, y, NB + 2.0] for y in range(NB, n - NB, 3) for x in range(NB, n - NB, 3)], np.float32)
+    src = (n / 2, n / 2, NB + 2.0)
+    codec, info = learned.train(shape, seeds=[501, 502, 503, 504], d=D, dt=dt, nt=nt, snap_every=3, nl=32, nbnd=NB,
+                                f_peak=18.0, spread=5, vmin=1600.0, vmax=2800.0, sigma=4.0)
+    print(f"[precision] learned codec trained on {info['snapshots']} snapshots, energy kept {info['energy_kept']:.4f}",
+          flush=True)
+    o = pipeline.Shot64(n, n, n, D, dt, nt, engine, order, model, [NB] * 6, float(vel.max()), src, 18.0, 5, 0.3, 0.3,
+                        rcv, snap_every=3, tmute_frac=0.1, threads=8)
+    o.set_model(vel, eps, dl, pjx, pjy)
+    img64, _ = o.rtm(o.model())
+    res = {}
+    for fmt in (SNAP_FP32, SNAP_LEARNED):
+        cfg = shot_config(n, n, n, D, dt, nt, engine, order, model, nbnd=NB, vmax=float(vel.max()), src_xyz=src,
+                          f_peak=18.0, spread=5, khpass=0.3, khcut=0.3, nrcv=rcv.shape[0], snap_every=3, snap_format=fmt,
+                          tmute_frac=0.1)
+        p = Propagator(cfg)
+        if fmt == SNAP_LEARNED:
+            from konglong_amd._lib import lib as _l, check as _c
+            _c(_l().klrtm_propagator_set_lcodec(p.h, codec.h))
+        p.set_model(vel, eps, dl, pjx, pjy)
+        p.set_receivers(rcv)
+        p.set_source_wavelet(o.wavelet.astype(np.float32))
+        out = p.rtm(p.model())
+        p.close()
+        res[fmt] = nrmse(out["image64"], img64)
+    name = "TTI-FD16" if model == TTI else "VTI-CFD10"
+    ratio = n ** 3 * 4 / codec.code_bytes(shape)
+    print(f"[precision] {name} {n}^3 nt={nt}: image NRMSE vs FP64 oracle, fp32 snapshots {res[SNAP_FP32]:.3e}, "
+          f"learned ({ratio:.2f}x smaller than fp32) {res[SNAP_LEARNED]:.3e}", flush=True)
+    assert res[SNAP_LEARNED] <= 1e-2
+    codec.close()
