@@ -103,6 +103,64 @@ def cpu_baseline(vel, eps, dl, w, eng, mdl, dt, budget_s):
                       f"{ny}x{nx}x{nz} model, FP32, {threads} OpenMP threads"}
 
 
+def batched_vs_plain(nshots, nt, batch, seed=505):
+    """Batched multi-shot ISO RTM against the plain per-shot FP32 path on the same shots.
+
+    Workload: the first `nshots` of 64 seeded apertures (200-450 cells square, multiples of 8, x 400
+    deep) on a 1000 x 1000 x 400 layered model with a 4500 m/s salt body, dx 12.5 m, FD8, PML 32,
+    15 Hz Ricker, receivers every 4th surface cell, snapshots every 4 steps, nt steps per shot
+    (shots/hour is scaled to the full 2500-step record).  Plain: one shot at a time, FP32
+    snapshots, default tiles.  Batched: `batch` shots side by side on their own streams, learned
+    snapshots (trained beforehand, untimed, on two seeded 256^3 random models), autotuned tiles.
+    Both stack FP32 shot images into an FP64 image; the batched stack's NRMSE is taken against the
+    plain one."""
+    from konglong_amd import models, learned
+    from konglong_amd.batch import BatchRTM, Shot, Autotuner
+    from konglong_amd._lib import ENGINE_FD, ISO, SNAP_FP32, SNAP_LEARNED
+    nxF = nyF = 1000
+    nz, d, nb, fp, k = 400, 12.5, 32, 15.0, 4
+    model = models.SaltLayeredModel(nxF, nyF, nz, d, seed)
+    dt = 2.0 / models.fd_k1max(8) / (4500.0 * np.sqrt(3.0) / d) / 1.01   # bgprtm/ModelPrepare.cpp:701-718 rule
+    rng = np.random.default_rng(seed)
+    shots = []
+    for i in range(64):
+        a = (int(rng.uniform(200, 451)) // 8 * 8, int(rng.uniform(200, 451)) // 8 * 8)
+        x0, y0 = int(rng.uniform(0, nxF - a[0])), int(rng.uniform(0, nyF - a[1]))
+        rx = models.surface_receivers(a[0], a[1], nb + 2, step=4)
+        keep = (rx[:, 0] >= nb) & (rx[:, 0] < a[0] - nb) & (rx[:, 1] >= nb) & (rx[:, 1] < a[1] - nb)
+        shots.append(Shot(x0, y0, a[0], a[1], ((a[0] - 1) / 2, (a[1] - 1) / 2, nb + 2.0), rx[keep]))
+    shots = shots[:nshots]
+    cache = {}
+
+    def vel(x0, y0, nx, ny):
+        key = (x0, y0, nx, ny)
+        if key not in cache:
+            cache[key] = model(x0, y0, nx, ny)
+        return cache[key]
+    common = dict(d=d, dt=float(dt), nt=nt, engine=ENGINE_FD, order=8, model=ISO, nbnd=nb, f_peak=fp, spread=10,
+                  snap_every=k, tmute_frac=0.1, vmax=4500.0)
+    t0 = time.perf_counter()
+    codec, info = learned.train((256, 256, 256), [4001, 4002], d, float(dt), 300, k, nl=32, nbnd=nb, f_peak=fp,
+                                vmin=1500.0, vmax=4500.0, sigma=20.0)
+    t_train = time.perf_counter() - t0
+    plain = BatchRTM((nyF, nxF, nz), vel, common, batch=1, snap_format=SNAP_FP32)
+    records = plain.run(shots)
+    fast = BatchRTM((nyF, nxF, nz), vel, common, batch=batch, snap_format=SNAP_LEARNED, lcodec=codec,
+                    tuner=Autotuner())
+    fast.run(shots, records)
+    a64, b64 = plain.stack.download(), fast.stack.download()
+    err = float(np.linalg.norm(b64 - a64) / max(np.linalg.norm(a64), 1e-300))
+    sph_plain, sph_fast = plain.shots_per_hour(2500), fast.shots_per_hour(2500)
+    ratio = codec.code_bytes((nz, nz, nz)) / (nz ** 3 * 4.0)
+    plain.close(), fast.close(), codec.close()
+    return {"workload": "batched ISO FD8 RTM, 1000x1000x400 salt model, apertures 200-450 x 400",
+            "shots": nshots, "steps_per_shot_measured": nt, "steps_per_shot_scaled_to": 2500, "batch": batch,
+            "plain_shots_per_hour": round(sph_plain, 2), "batched_shots_per_hour": round(sph_fast, 2),
+            "speedup": round(sph_fast / sph_plain, 3), "stack_nrmse_vs_plain": err,
+            "learned_bytes_vs_fp32": round(ratio, 4), "codec_training_s": round(t_train, 2),
+            "codec_training": info}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +169,9 @@ def main():
     ap.add_argument("--workload", default="vti-cfd10-400", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--batched-shots", type=int, default=8, help="shots of the batched-vs-plain leg (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=4)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -214,6 +275,12 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kern,
     }
+    if args.batched_shots > 0:
+        p.close()
+        try:
+            out["batched"] = batched_vs_plain(args.batched_shots, args.batched_steps, args.batch)
+        except Exception as e:  # the headline line is still printed
+            out["batched"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out))
 
 

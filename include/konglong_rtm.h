@@ -52,6 +52,11 @@ int klrtm_memcpy_d2h(void *dst_host, const void *src_device, size_t bytes);
 int klrtm_memcpy_d2d(void *dst_device, const void *src_device, size_t bytes);
 int klrtm_memset(void *dst_device, int value, size_t bytes);
 int klrtm_synchronize(void);
+/* extra streams (default priority, non-blocking) for work that runs side by side: make one current
+ * with klrtm_set_stream; klrtm_stream_synchronize(NULL) waits for the current stream */
+int klrtm_stream_create(void **s);
+int klrtm_stream_destroy(void *s);
+int klrtm_stream_synchronize(void *s);
 
 /* ---- kernel timing (HIP events recorded around each launch of a kernel) ---- */
 int klrtm_timer_enable(int on);
@@ -229,6 +234,8 @@ typedef struct {
   int ntaper_z;           /* IC water-depth taper cells */
   int fuse_step;          /* 1: fused kernel+update2nd */
   int snap_slots;         /* > 0: snapshot slots kept (ring for benchmarks); 0: all */
+  int ic_fp32_only;       /* 1: the imaging step accumulates only the FP32 image and illumination
+                             (FP64 is left to the multi-shot stack, klrtm_stack_image) */
 } klrtm_shot_config;
 
 typedef struct klrtm_propagator klrtm_propagator;
@@ -261,6 +268,18 @@ int klrtm_propagator_set_record(klrtm_propagator *p, const float *record_host);
 int klrtm_propagator_reset(klrtm_propagator *p);
 /* learned codec of KLRTM_SNAP_LEARNED snapshots (borrowed: it must outlive the propagator's use) */
 int klrtm_propagator_set_lcodec(klrtm_propagator *p, const klrtm_lcodec *c);
+/* the propagator's engine (for klrtm_kernel_set_tile) */
+int klrtm_propagator_kernel(klrtm_propagator *p, klrtm_kernel **k);
+/* device pointers of the shot's image accumulators (FP32 image, illumination; FP64 image, illumination) */
+int klrtm_propagator_images(klrtm_propagator *p, float **img, float **ill, double **img64, double **ill64);
+/* zero the image accumulators (the step API accumulates into them; klrtm_propagator_rtm clears them itself) */
+int klrtm_propagator_clear_images(klrtm_propagator *p);
+
+/* ---- multi-shot stack (ImagingCondition::saveImage, bgprtm/ImagingCondition.cpp:874-945, in FP64) ----
+ * full[(y0+iy)*nxF + x0+ix][z0+iz] += img[iy][ix][iz] for a shot image of nx*ny*nz on its aperture at
+ * (x0, y0, z0) of the nxF*nyF*nzF output grid; img is FP32 (is_double 0) or FP64 (is_double 1). */
+int klrtm_stack_image(const void *img, int is_double, int nx, int ny, int nz, double *full, int nxF, int nyF, int nzF,
+                      int x0, int y0, int z0);
 
 #ifdef __cplusplus
 }

@@ -37,7 +37,7 @@ class ShotConfig(C.Structure):
         ("nrcv", _i),
         ("snap_every", _i), ("snap_format", _i), ("rec_through_codec", _i),
         ("tmute_frac", _f), ("ntaper_z", _i), ("fuse_step", _i),
-        ("snap_slots", _i),
+        ("snap_slots", _i), ("ic_fp32_only", _i),
     ]
 
 
@@ -112,6 +112,13 @@ _SIGS = {
     "klrtm_propagator_backward_steps": (_i, [_vp, _i, _i, _i]),
     "klrtm_propagator_set_record": (_i, [_vp, _pf]),
     "klrtm_propagator_set_lcodec": (_i, [_vp, _vp]),
+    "klrtm_propagator_kernel": (_i, [_vp, C.POINTER(_vp)]),
+    "klrtm_propagator_images": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "klrtm_propagator_clear_images": (_i, [_vp]),
+    "klrtm_stack_image": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _i, _i, _i, _i, _i]),
+    "klrtm_stream_create": (_i, [C.POINTER(_vp)]),
+    "klrtm_stream_destroy": (_i, [_vp]),
+    "klrtm_stream_synchronize": (_i, [_vp]),
     "klrtm_lcodec_create": (_i, [C.POINTER(_vp), _i, _pf]),
     "klrtm_lcodec_destroy": (_i, [_vp]),
     "klrtm_lcodec_code_bytes": (_sz, [_vp, _i, _i, _i]),

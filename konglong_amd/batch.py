@@ -1,0 +1,204 @@
+"""Many shots on one GPU: side-by-side shots, tuned stencil tiles, FP64 multi-shot stack.
+
+The reference migrates one shot per process and stacks images on the output
+grid (bgprtm/ImagingCondition.cpp:874-945).  Here a batch of shots with their
+own aperture sub-grids runs side by side on one GPU: every shot is a
+Propagator on its own HIP stream, the host issues the shots' time steps in
+chunks of the snapshot cadence, round-robin, so the kernels of different
+shots overlap (small apertures and the short injection / imaging kernels no
+longer leave the GPU idle), and each finished shot image is added in FP64
+into the stack on the full output grid.  Stencil tile shapes are chosen per
+aperture shape by ``Autotuner`` (timed candidates, cached).
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from ._lib import check, lib, ENGINE_FD, ISO, SNAP_FP32, SNAP_LEARNED
+from .propagator import Propagator, shot_config
+
+
+class Autotuner:
+    """Picks (tz, tx, ychunk) for a propagator's stencil step by timing candidates.
+
+    Candidates are z x x tiles of at most 512 threads and y-chunk lengths; each is timed over a few
+    forward steps with the library's kernel timers, and the fastest is kept per (grid, engine, order,
+    model).  The propagator's wavefields are reset afterwards."""
+
+    TILES = ((64, 8), (64, 4), (32, 8), (128, 4))
+    YCHUNKS = (0, 16, 32, 64)   # 0: the engine's default chunking
+
+    def __init__(self, steps=4):
+        self.steps = steps
+        self.cache = {}
+
+    @staticmethod
+    def _kernel(p):
+        k = C.c_void_p()
+        check(lib().klrtm_propagator_kernel(p.h, C.byref(k)))
+        return k.value
+
+    def tune(self, p):
+        cfg = p.cfg
+        key = (cfg.nx, cfg.ny, cfg.nz, cfg.engine, cfg.order, cfg.model)
+        k = self._kernel(p)
+        if key not in self.cache:
+            if cfg.engine != ENGINE_FD:
+                self.cache[key] = (None, {})
+            else:
+                self.cache[key] = self._search(p, k)
+        best = self.cache[key][0]
+        if best is not None:
+            check(lib().klrtm_kernel_set_tile(k, best[0], best[1], best[2] if best[2] > 0 else -1))
+        return best
+
+    def _search(self, p, k):
+        L = lib()
+        name = b"iso_step" if p.cfg.model == ISO else b"fd_grad"
+        R = p.cfg.order // 2
+        table = {}
+        p.reset()
+        for tz, tx in self.TILES:
+            if tz < R or tx < R or tz * tx > 512:
+                continue
+            for yc in self.YCHUNKS:
+                check(L.klrtm_kernel_set_tile(k, tz, tx, yc if yc > 0 else -1))
+                p.forward_steps(0, 1, False)   # warm
+                L.klrtm_synchronize()
+                L.klrtm_timer_reset()
+                L.klrtm_timer_enable(1)
+                p.forward_steps(1, self.steps, False)
+                L.klrtm_synchronize()
+                L.klrtm_timer_enable(0)
+                tot, cnt = C.c_double(), C.c_long()
+                L.klrtm_timer_read(name, C.byref(tot), C.byref(cnt))
+                if cnt.value:
+                    table[(tz, tx, yc)] = tot.value / cnt.value
+        L.klrtm_timer_reset()
+        p.reset()
+        best = min(table, key=table.get) if table else None
+        return best, table
+
+
+class Shot:
+    """One shot of a batch: aperture origin (x0, y0) and size (nx, ny) on the output grid, local
+    source and receiver coordinates."""
+
+    def __init__(self, x0, y0, nx, ny, src_xyz, rcv_xyz):
+        self.x0, self.y0, self.nx, self.ny = int(x0), int(y0), int(nx), int(ny)
+        self.src_xyz = tuple(float(v) for v in src_xyz)
+        self.rcv_xyz = np.ascontiguousarray(rcv_xyz, np.float32).reshape(-1, 3)
+
+
+class BatchRTM:
+    """RTM of many shots into one FP64 stack on an output grid of (nyF, nxF, nz).
+
+    velocity(x0, y0, nx, ny) returns the shot's (ny, nx, nz) velocity sub-volume; common holds the
+    shot_config keywords shared by all shots (d, dt, nt, engine, order, model, nbnd, ...)."""
+
+    def __init__(self, out_shape, velocity, common, batch=4, snap_format=SNAP_FP32, lcodec=None, tuner=None):
+        from ._lib import DeviceArray
+        self.nyF, self.nxF, self.nz = out_shape
+        self.velocity = velocity
+        self.common = dict(common)
+        self.batch = max(1, int(batch))
+        self.snap_format = snap_format
+        self.lcodec = lcodec
+        self.tuner = tuner
+        self.stack = DeviceArray(out_shape, np.float64)
+        check(lib().klrtm_synchronize())
+        self.streams = []
+        for _ in range(self.batch):
+            s = C.c_void_p()
+            check(lib().klrtm_stream_create(C.byref(s)))
+            self.streams.append(s.value)
+        self.times = dict(migrate_s=0.0, shots=0, steps=0)
+
+    def _make(self, shot):
+        c = dict(self.common)
+        nt = c.pop("nt")
+        vel = self.velocity(shot.x0, shot.y0, shot.nx, shot.ny)
+        c.setdefault("vmax", float(vel.max()))
+        cfg = shot_config(shot.nx, shot.ny, self.nz, c.pop("d"), c.pop("dt"), nt, nrcv=shot.rcv_xyz.shape[0],
+                          src_xyz=shot.src_xyz, snap_format=self.snap_format, ic_fp32_only=1, **c)
+        p = Propagator(cfg)
+        if self.snap_format == SNAP_LEARNED:
+            check(lib().klrtm_propagator_set_lcodec(p.h, self.lcodec.h))
+        p.set_model(vel)
+        p.set_receivers(shot.rcv_xyz)
+        return p
+
+    def run(self, shots, records=None):
+        """Migrate the shots; records[i] is shot i's [nrcv][nt] record (None: modelled here, untimed).
+        Returns the per-shot records used."""
+        L = lib()
+        used = []
+        for g0 in range(0, len(shots), self.batch):
+            group = shots[g0:g0 + self.batch]
+            props = []
+            for j, sh in enumerate(group):
+                L.klrtm_set_stream(self.streams[j])
+                p = self._make(sh)
+                if self.tuner is not None:
+                    self.tuner.tune(p)
+                rec = records[g0 + j] if records is not None else p.model()
+                p.set_record(rec)
+                used.append(rec)
+                p.forward_steps(0, 1, True)   # allocates the snapshot store outside the timed region
+                props.append(p)
+            for j in range(len(group)):
+                L.klrtm_stream_synchronize(self.streams[j])
+            t0 = time.perf_counter()
+            nt = self.common["nt"]
+            k = max(1, self.common.get("snap_every", 1))
+            for j, p in enumerate(props):
+                L.klrtm_set_stream(self.streams[j])
+                p.reset()
+            for it0 in range(0, nt, k):
+                for j, p in enumerate(props):
+                    L.klrtm_set_stream(self.streams[j])
+                    p.forward_steps(it0, min(k, nt - it0), True)
+            for j, p in enumerate(props):
+                L.klrtm_set_stream(self.streams[j])
+                p.reset()
+                check(L.klrtm_propagator_clear_images(p.h))
+            it = nt - 1
+            while it >= 0:
+                n = it % k + 1  # down to the next imaged step
+                for j, p in enumerate(props):
+                    L.klrtm_set_stream(self.streams[j])
+                    p.backward_steps(it, n, True)
+                it -= n
+            # apertures overlap: the stack is updated shot after shot on one stream, in shot order
+            for j in range(len(group)):
+                check(L.klrtm_stream_synchronize(self.streams[j]))
+            L.klrtm_set_stream(self.streams[0])
+            for p, sh in zip(props, group):
+                img = C.c_void_p()
+                check(L.klrtm_propagator_images(p.h, C.byref(img), None, None, None))
+                check(L.klrtm_stack_image(img.value, 0, sh.nx, sh.ny, self.nz, self.stack.ptr, self.nxF, self.nyF,
+                                          self.nz, sh.x0, sh.y0, 0))
+            check(L.klrtm_stream_synchronize(self.streams[0]))
+            self.times["migrate_s"] += time.perf_counter() - t0
+            self.times["shots"] += len(group)
+            self.times["steps"] += len(group) * nt
+            for p in props:
+                p.close()
+        L.klrtm_set_stream(None)
+        return used
+
+    def shots_per_hour(self, nt_full=None):
+        """Shots per hour of migration (forward + backward + imaging), scaled to nt_full steps per shot
+        when the measured shots ran fewer."""
+        if not self.times["shots"]:
+            return 0.0
+        per_shot = self.times["migrate_s"] / self.times["shots"]
+        if nt_full:
+            per_shot *= nt_full / self.common["nt"]
+        return 3600.0 / per_shot
+
+    def close(self):
+        for s in self.streams:
+            lib().klrtm_stream_destroy(s)
+        self.streams = []

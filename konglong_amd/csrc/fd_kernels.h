@@ -25,5 +25,7 @@ int launch_div(int R, int mode, const float *px, const float *py, const float *p
 int launch_deriv1(int R, int axis, int iacc, int round, const float *in, float *out, const float *vel, const IsoArgs &a);
 int launch_vel2(float *v, size_t n, float dt);
 int launch_update2nd(float *w0, const float *w1, size_t n);
+// p[0..n) = v (a kernel, so that it can sit in a captured graph)
+int launch_fill(float *p, float v, size_t n);
 
 }  // namespace klrtm

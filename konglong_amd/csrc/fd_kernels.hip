@@ -625,6 +625,15 @@ int launch_vel2(float *v, size_t n, float dt) {
   hipLaunchKernelGGL(vel2_kernel, dim3((unsigned)div_up((long)n, 256)), dim3(256), 0, stream(), v, n, dt * dt);
   return check_launch("vel2_kernel");
 }
+__global__ void fill_kernel(float *__restrict__ p, float v, size_t n) {
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+int launch_fill(float *p, float v, size_t n) {
+  if(n == 0) return KLRTM_OK;
+  const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(fill_kernel, dim3(nb), dim3(256), 0, stream(), p, v, n);
+  return check_launch("fill_kernel");
+}
 int launch_update2nd(float *w0, const float *w1, size_t n) {
   if(n == 0) return KLRTM_OK;
   LaunchTimer tm("update2nd");

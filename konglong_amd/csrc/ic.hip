@@ -23,12 +23,12 @@ template<int FMT, bool RQ>
 __global__ void ic_fused_kernel(const IcArgs a, const void *__restrict__ snap, const float *__restrict__ scales,
                                 const float *__restrict__ w, const unsigned *__restrict__ rmax, float *__restrict__ img,
                                 float *__restrict__ ill, double *__restrict__ img64, double *__restrict__ ill64) {
-  const size_t nxz = (size_t)a.nx * a.nz;
-  const size_t n = nxz * a.ny;
-  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int iy = (int)(i / nxz);
-    const size_t j = i - (size_t)iy * nxz;
-    const int ix = (int)(j / a.nz), iz = (int)(j - (size_t)ix * a.nz);
+  // one y plane per blockIdx.y; a plane (nx*nz < 2^31 cells) is indexed with 32-bit arithmetic
+  const unsigned nxz = (unsigned)a.nx * (unsigned)a.nz;
+  const int iy = blockIdx.y;
+  for(unsigned j = blockIdx.x * blockDim.x + threadIdx.x; j < nxz; j += gridDim.x * blockDim.x) {
+    const size_t i = (size_t)iy * nxz + j;
+    const int ix = (int)(j / (unsigned)a.nz), iz = (int)(j - (unsigned)ix * (unsigned)a.nz);
     float S;
     if(FMT == KLRTM_SNAP_FP32) {
       S = ((const float *)snap)[i];
@@ -58,11 +58,12 @@ __global__ void ic_fused_kernel(const IcArgs a, const void *__restrict__ snap, c
 
 int launch_ic(int fmt, bool rq, const IcArgs &a, const void *snap, const float *scales, const float *w, const unsigned *rmax,
               float *img, float *ill, double *img64, double *ill64) {
-  size_t n = (size_t)a.nx * a.ny * a.nz;
-  unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+  KL_REQUIRE((size_t)a.nx * a.nz < (1ull << 31) && a.ny <= 65535, KLRTM_EINVAL, "grid too large for the imaging kernel");
+  const size_t nxz = (size_t)a.nx * a.nz;
+  const dim3 grid((unsigned)std::min<size_t>((nxz + 255) / 256, 1024), (unsigned)a.ny);
   LaunchTimer tm("ic_correlate");
 #define KL_IC(F, Q) \
-  hipLaunchKernelGGL((ic_fused_kernel<F, Q>), dim3(nb), dim3(256), 0, stream(), a, snap, scales, w, rmax, img, ill, img64, ill64)
+  hipLaunchKernelGGL((ic_fused_kernel<F, Q>), grid, dim3(256), 0, stream(), a, snap, scales, w, rmax, img, ill, img64, ill64)
   if(fmt == KLRTM_SNAP_FP32) {
     if(rq) KL_IC(KLRTM_SNAP_FP32, true);
     else KL_IC(KLRTM_SNAP_FP32, false);
@@ -99,9 +100,43 @@ __global__ void ic_plain_kernel(const float *__restrict__ src, const float *__re
   }
 }
 
+// full[(y0+iy)*nxF + x0+ix][z0+iz] += img[iy][ix][iz] in FP64 (ImagingCondition::saveImage stacking,
+// bgprtm/ImagingCondition.cpp:932-933, with FP64 accumulation across shots)
+template<class T>
+__global__ void stack_kernel(const T *__restrict__ img, int nx, int ny, int nz, double *__restrict__ full, int nxF,
+                             int nzF, int x0, int y0, int z0) {
+  const size_t n = (size_t)nx * ny * nz, nxz = (size_t)nx * nz;
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int iy = (int)(i / nxz);
+    const size_t j = i - (size_t)iy * nxz;
+    const int ix = (int)(j / nz), iz = (int)(j - (size_t)ix * nz);
+    const size_t o = ((size_t)(y0 + iy) * nxF + (x0 + ix)) * nzF + (z0 + iz);
+    full[o] = full[o] + (double)img[i];
+  }
+}
+
 }  // namespace klrtm
 
 using namespace klrtm;
+
+extern "C" int klrtm_stack_image(const void *img, int is_double, int nx, int ny, int nz, double *full, int nxF, int nyF,
+                                 int nzF, int x0, int y0, int z0) {
+  KL_REQUIRE(img && full, KLRTM_EINVAL, "null image");
+  KL_REQUIRE(nx > 0 && ny > 0 && nz > 0 && x0 >= 0 && y0 >= 0 && z0 >= 0 && x0 + nx <= nxF && y0 + ny <= nyF &&
+                 z0 + nz <= nzF,
+             KLRTM_EINVAL, "shot image %dx%dx%d at (%d,%d,%d) does not fit the %dx%dx%d output grid", nx, ny, nz, x0, y0,
+             z0, nxF, nyF, nzF);
+  const size_t n = (size_t)nx * ny * nz;
+  const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+  LaunchTimer tm("stack");
+  if(is_double)
+    hipLaunchKernelGGL(stack_kernel<double>, dim3(nb), dim3(256), 0, stream(), (const double *)img, nx, ny, nz, full, nxF,
+                       nzF, x0, y0, z0);
+  else
+    hipLaunchKernelGGL(stack_kernel<float>, dim3(nb), dim3(256), 0, stream(), (const float *)img, nx, ny, nz, full, nxF,
+                       nzF, x0, y0, z0);
+  return check_launch("stack_kernel");
+}
 
 extern "C" int klrtm_ic_correlate(const float *src, const float *rec, int nx, int ny, int nz, float taper_t,
                                   const float *taper_z, int ntaper_z, float *image, float *illum, double *image64,

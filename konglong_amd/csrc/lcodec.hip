@@ -105,12 +105,24 @@ __global__ void __launch_bounds__(256) lc_encode_kernel(const float *__restrict_
 //   DEC: x' = x^ + q*scale stored to the volume
 //   IC:  x' correlated with the receiver wavefield (ImagingCorrelator::crossone with the snapshot
 //        tapers, as the fused imaging kernel)
+// A workgroup owns one tile of 32 blocks and its four waves split the 16 feature tiles (wave w takes
+// tiles 4w..4w+3; the residual scale combines the four waves' maxima through LDS).  The volume side
+// goes through a per-wave LDS tile [32 blocks][32 features] (row stride LCS): in the product layout a
+// lane holds one feature of 16 blocks, in the volume layout lane l takes z offset l % 8 of blocks
+// l / 8 + 8s (s < 4) on each of the four block rows, so a wave's loads and stores are runs of 64
+// consecutive z samples.
 enum { LC_RES = 0, LC_DEC = 1, LC_IC = 2 };
+constexpr int LCS = 40;  // LDS row stride in floats: both access patterns are free of bank conflicts
 struct LcCode {
   const uint16_t *z;  // [nblk][nl] BF16
   float *scale;       // [nblk]
   int8_t *q;          // [nblk][512]
 };
+__device__ __forceinline__ void wave_lds_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's LDS writes are done
+  asm volatile("" ::: "memory");
+}
 template<int NL, int MODE>
 __global__ void __launch_bounds__(256)
     lc_decode_kernel(const LcCode code, const bf16x8 *__restrict__ wfrag, const float *__restrict__ xin,
@@ -118,9 +130,12 @@ __global__ void __launch_bounds__(256)
                      float *__restrict__ img, float *__restrict__ ill, double *__restrict__ img64,
                      double *__restrict__ ill64) {
   constexpr int KS = NL / 16;
-  const int lane = threadIdx.x & 63;
-  const long b0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
-  if(b0 >= g.nblk) return;  // whole wave
+  __shared__ float tiles[4][32 * LCS];
+  __shared__ float wmax[4][32];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float *T = tiles[wv];
+  const long b0 = (long)blockIdx.x * 32;  // the grid holds exactly the tiles: every wave has work
+  const int nt0 = wv * 4, nt1 = nt0 + 4;
   const long b = b0 + (lane & 31);
   bf16x8 za[KS];
 #pragma unroll
@@ -128,89 +143,120 @@ __global__ void __launch_bounds__(256)
     za[s] = bf16x8{};
     if(b < g.nblk) za[s] = *(const bf16x8 *)(code.z + b * NL + s * 16 + (lane >> 5) * 8);
   }
-  // the 16 output blocks of this lane (register r) and their origins
-  size_t org[16];
+  // product layout: register r of this lane is block b0 + ti(r), feature f0 + (lane & 31)
+  auto ti = [&](int r) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); };
   bool okr[16];
   float sc[16];
 #pragma unroll
   for(int r = 0; r < 16; r++) {
-    const long bb = b0 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-    okr[r] = bb < g.nblk;
-    org[r] = okr[r] ? block_origin(g, bb) : 0;
-    sc[r] = (MODE != LC_RES && okr[r]) ? code.scale[bb] : 0.f;
+    okr[r] = b0 + ti(r) < g.nblk;
+    sc[r] = (MODE != LC_RES && okr[r]) ? code.scale[b0 + ti(r)] : 0.f;
   }
+  // volume layout: blocks b0 + (lane >> 3) + 8s, z offset lane & 7; coordinates packed
+  // (by << 20) | (bx << 10) | bz (block counts < 1024 per axis, checked on the host)
   const int lz = lane & 7;
+  bool okv[4];
+  unsigned pk[4];
+#pragma unroll
+  for(int s = 0; s < 4; s++) {
+    const long bb = b0 + (lane >> 3) + 8 * s;
+    okv[s] = bb < g.nblk;
+    const unsigned ub = okv[s] ? (unsigned)bb : 0u, bz = ub % (unsigned)g.nbz, t = ub / (unsigned)g.nbz;
+    pk[s] = ((t / (unsigned)g.nbx) << 20) | ((t % (unsigned)g.nbx) << 10) | bz;
+  }
+  auto cell = [&](int s, int row, int &ix, int &iy, int &iz) {
+    iy = (int)(pk[s] >> 20) * 8 + (row >> 3), ix = (int)((pk[s] >> 10) & 1023u) * 8 + (row & 7);
+    iz = (int)(pk[s] & 1023u) * 8 + lz;
+    return ((size_t)iy * g.nx + ix) * g.nz + iz;
+  };
   auto xhat = [&](int nt) {
     f32x16 acc = {};
 #pragma unroll
     for(int s = 0; s < KS; s++) acc = mfma32(za[s], wfrag[(nt * KS + s) * 64 + lane], acc);
     return acc;
   };
-  auto cell_off = [&](int nt) {
-    const int row = nt * 4 + ((lane & 31) >> 3), ly = row >> 3, lx = row & 7;
-    return ((size_t)ly * g.nx + lx) * g.nz + lz;
+  // x of feature tile nt into the LDS tile (volume layout in, product layout out)
+  auto stage_x = [&](int nt) {
+#pragma unroll
+    for(int q = 0; q < 4; q++)
+#pragma unroll
+      for(int s = 0; s < 4; s++) {
+        int ix, iy, iz;
+        const size_t i = cell(s, nt * 4 + q, ix, iy, iz);
+        T[((lane >> 3) + 8 * s) * LCS + q * 8 + lz] = okv[s] ? xin[i] : 0.f;
+      }
+    wave_lds_sync();
   };
   if(MODE == LC_RES) {
     float m[16];
 #pragma unroll
     for(int r = 0; r < 16; r++) m[r] = 0.f;
-    for(int nt = 0; nt < 16; nt++) {
+    for(int nt = nt0; nt < nt1; nt++) {
       const f32x16 acc = xhat(nt);
-      const size_t off = cell_off(nt);
+      stage_x(nt);
 #pragma unroll
-      for(int r = 0; r < 16; r++)
-        if(okr[r]) m[r] = fmaxf(m[r], fabsf(xin[org[r] + off] - acc[r]));
+      for(int r = 0; r < 16; r++) m[r] = fmaxf(m[r], fabsf(T[ti(r) * LCS + (lane & 31)] - acc[r]));
+      wave_lds_sync();
     }
-    // block max over the 32 lanes of this half-wave (features), all 16 nt already folded in
+    // block max over the 32 lanes of this half-wave (the features of this wave's tiles), then over the
+    // four waves through LDS
 #pragma unroll
     for(int r = 0; r < 16; r++) {
 #pragma unroll
       for(int k = 1; k < 32; k *= 2) m[r] = fmaxf(m[r], __shfl_xor(m[r], k));
-      sc[r] = m[r] * (1.f / 127.f);
-      m[r] = m[r] > 0.f ? 127.f / m[r] : 0.f;  // inverse scale
-      const long bb = b0 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-      if(okr[r] && (lane & 31) == 0) code.scale[bb] = sc[r];
+      if((lane & 31) == 0) wmax[wv][ti(r)] = m[r];
     }
-    for(int nt = 0; nt < 16; nt++) {
+    __syncthreads();
+#pragma unroll
+    for(int r = 0; r < 16; r++) {
+      const int t = ti(r);
+      m[r] = fmaxf(fmaxf(wmax[0][t], wmax[1][t]), fmaxf(wmax[2][t], wmax[3][t]));
+      if(wv == 0 && okr[r] && (lane & 31) == 0) code.scale[b0 + t] = m[r] * (1.f / 127.f);
+      m[r] = m[r] > 0.f ? 127.f / m[r] : 0.f;  // inverse scale
+    }
+    for(int nt = nt0; nt < nt1; nt++) {
       const f32x16 acc = xhat(nt);
-      const size_t off = cell_off(nt);
+      stage_x(nt);
 #pragma unroll
       for(int r = 0; r < 16; r++) {
-        if(!okr[r]) continue;
-        const long bb = b0 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        const float qv = fminf(fmaxf(rintf((xin[org[r] + off] - acc[r]) * m[r]), -127.f), 127.f);
-        code.q[bb * 512 + nt * 32 + (lane & 31)] = (int8_t)qv;
+        const float qv = fminf(fmaxf(rintf((T[ti(r) * LCS + (lane & 31)] - acc[r]) * m[r]), -127.f), 127.f);
+        if(okr[r]) code.q[(b0 + ti(r)) * 512 + nt * 32 + (lane & 31)] = (int8_t)qv;
       }
+      wave_lds_sync();
     }
     return;
   }
-  for(int nt = 0; nt < 16; nt++) {
+  for(int nt = nt0; nt < nt1; nt++) {
     const f32x16 acc = xhat(nt);
-    const size_t off = cell_off(nt);
 #pragma unroll
     for(int r = 0; r < 16; r++) {
-      if(!okr[r]) continue;
-      const long bb = b0 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-      const float x = acc[r] + (float)code.q[bb * 512 + nt * 32 + (lane & 31)] * sc[r];
-      const size_t i = org[r] + off;
-      if(MODE == LC_DEC) {
-        vol[i] = x;
-      } else {
-        const size_t nxz = (size_t)g.nx * g.nz;
-        const int iy = (int)(i / nxz);
-        const size_t j = i - (size_t)iy * nxz;
-        const int ix = (int)(j / g.nz), iz = (int)(j - (size_t)ix * g.nz);
-        float S = edge_taper(a.T, x, ix, iy, iz, g.nx, g.ny, g.nz);
-        if(a.taper_t >= 0.f) S = S * a.taper_t;
-        else if(iz < a.ntaper_z) S = S * a.taper_z[iz];
-        const float R = edge_taper(a.T, w[i], ix, iy, iz, g.nx, g.ny, g.nz);
-        const float d = S * R;
-        if(img) img[i] = img[i] + d;
-        if(ill) ill[i] = ill[i] + S * S;
-        if(img64) img64[i] = img64[i] + (double)S * (double)R;
-        if(ill64) ill64[i] = ill64[i] + (double)S * (double)S;
-      }
+      const float qv = okr[r] ? (float)code.q[(b0 + ti(r)) * 512 + nt * 32 + (lane & 31)] : 0.f;
+      T[ti(r) * LCS + (lane & 31)] = acc[r] + qv * sc[r];
     }
+    wave_lds_sync();
+#pragma unroll
+    for(int q = 0; q < 4; q++)
+#pragma unroll
+      for(int s = 0; s < 4; s++) {
+        if(!okv[s]) continue;
+        int ix, iy, iz;
+        const size_t i = cell(s, nt * 4 + q, ix, iy, iz);
+        const float x = T[((lane >> 3) + 8 * s) * LCS + q * 8 + lz];
+        if(MODE == LC_DEC) {
+          vol[i] = x;
+        } else {
+          float S = edge_taper(a.T, x, ix, iy, iz, g.nx, g.ny, g.nz);
+          if(a.taper_t >= 0.f) S = S * a.taper_t;
+          else if(iz < a.ntaper_z) S = S * a.taper_z[iz];
+          const float R = edge_taper(a.T, w[i], ix, iy, iz, g.nx, g.ny, g.nz);
+          const float d = S * R;
+          if(img) img[i] = img[i] + d;
+          if(ill) ill[i] = ill[i] + S * S;
+          if(img64) img64[i] = img64[i] + (double)S * (double)R;
+          if(ill64) ill64[i] = ill64[i] + (double)S * (double)S;
+        }
+      }
+    wave_lds_sync();
   }
 }
 
@@ -269,6 +315,7 @@ __global__ void lc_gram_reduce_kernel(const float *__restrict__ part, int nchunk
 static int block_grid(BlockGrid &g, int nx, int ny, int nz) {
   KL_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nx % 8 == 0 && ny % 8 == 0 && nz % 8 == 0, KLRTM_EINVAL,
              "the learned codec needs grid sizes that are multiples of 8 (got %d x %d x %d)", nx, ny, nz);
+  KL_REQUIRE(nx <= 8184 && ny <= 8184 && nz <= 8184, KLRTM_EINVAL, "the learned codec handles up to 8184 cells per axis");
   g.nx = nx, g.ny = ny, g.nz = nz, g.nbx = nx / 8, g.nbz = nz / 8;
   g.nblk = (long)(nx / 8) * (ny / 8) * (nz / 8);
   return KLRTM_OK;
@@ -293,7 +340,7 @@ size_t lcodec_code_bytes(const LCodec &c, int nx, int ny, int nz) {
 template<int MODE>
 static void launch_dec(const LCodec &c, const LcCode &k, const float *xin, float *vol, const BlockGrid &g, const IcArgs &a,
                        const float *w, float *img, float *ill, double *img64, double *ill64) {
-  const dim3 grid(tiles_of(g.nblk)), blk(256);
+  const dim3 grid((unsigned)((g.nblk + 31) / 32)), blk(256);  // one 32-block tile per workgroup
   const bf16x8 *wd = (const bf16x8 *)c.d_wdec;
   switch(c.nl) {
   case 32: hipLaunchKernelGGL((lc_decode_kernel<32, MODE>), grid, blk, 0, stream(), k, wd, xin, vol, g, a, w, img, ill, img64, ill64); break;

@@ -14,6 +14,7 @@
 #include "codec.h"
 #include "ic.h"
 #include "lcodec.h"
+#include "fd_kernels.h"
 #include <cmath>
 #include <cstring>
 #include <algorithm>
@@ -43,6 +44,7 @@ struct Propagator {
   ~Propagator();
   int alloc_snapshots();
   int reset();
+  int clear_images();
   int step(int it);
   int store(int itsav);
   int image(int itsav);
@@ -101,6 +103,14 @@ int Propagator::alloc_snapshots() {
   return KLRTM_OK;
 }
 
+int Propagator::clear_images() {
+  // fill kernels rather than memset nodes, so that a captured step sequence stays replayable
+  int rc;
+  if((rc = launch_fill(d_img, 0.f, n)) || (rc = launch_fill(d_ill, 0.f, n))) return rc;
+  if((rc = launch_fill((float *)d_img64, 0.f, 2 * n)) || (rc = launch_fill((float *)d_ill64, 0.f, 2 * n))) return rc;
+  return KLRTM_OK;
+}
+
 int Propagator::reset() {
   KL_HIP(hipMemsetAsync(w0, 0, n * sizeof(float), stream()));
   KL_HIP(hipMemsetAsync(w1, 0, n * sizeof(float), stream()));
@@ -141,10 +151,10 @@ int Propagator::image(int itsav) {
     if(rc) return rc;
   }
   const char *snap = d_snap + snap_stride_bytes * (size_t)itsav;
-  if(c.snap_format == KLRTM_SNAP_LEARNED)
-    return lcodec_image(*lc, a, snap, w0, d_img, d_ill, d_img64, d_ill64);
+  double *i64 = c.ic_fp32_only ? nullptr : d_img64, *l64 = c.ic_fp32_only ? nullptr : d_ill64;
+  if(c.snap_format == KLRTM_SNAP_LEARNED) return lcodec_image(*lc, a, snap, w0, d_img, d_ill, i64, l64);
   const float *sc = nscales ? d_scales + nscales * (size_t)itsav : nullptr;
-  return launch_ic(c.snap_format, rq, a, snap, sc, w0, d_smax, d_img, d_ill, d_img64, d_ill64);
+  return launch_ic(c.snap_format, rq, a, snap, sc, w0, d_smax, d_img, d_ill, i64, l64);
 }
 
 static float ricker(float t, float fp) {
@@ -290,10 +300,7 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
   // backward: receiver wavefield from the record, imaging at the stored steps
   if(record && (rc = receiver_set_traces(*P.rcv, record))) return rc;
   if(!record && (rc = receiver_set_traces_device(*P.rcv, receiver_record_device(*P.rcv)))) return rc;
-  KL_HIP(hipMemsetAsync(P.d_img, 0, P.n * sizeof(float), stream()));
-  KL_HIP(hipMemsetAsync(P.d_ill, 0, P.n * sizeof(float), stream()));
-  KL_HIP(hipMemsetAsync(P.d_img64, 0, P.n * sizeof(double), stream()));
-  KL_HIP(hipMemsetAsync(P.d_ill64, 0, P.n * sizeof(double), stream()));
+  if((rc = P.clear_images())) return rc;
   if((rc = P.reset())) return rc;
   KL_HIP(hipEventRecord(P.ev[2], stream()));
   for(int it = nt - 1; it >= 0; it--) {
@@ -419,6 +426,27 @@ int klrtm_propagator_set_record(klrtm_propagator *p, const float *record) {
 int klrtm_propagator_reset(klrtm_propagator *p) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
   return reinterpret_cast<Propagator *>(p)->reset();
+}
+
+int klrtm_propagator_kernel(klrtm_propagator *p, klrtm_kernel **k) {
+  KL_REQUIRE(p && k, KLRTM_EINVAL, "null argument");
+  *k = reinterpret_cast<klrtm_kernel *>(reinterpret_cast<Propagator *>(p)->eng);
+  return KLRTM_OK;
+}
+
+int klrtm_propagator_images(klrtm_propagator *p, float **img, float **ill, double **img64, double **ill64) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  if(img) *img = P.d_img;
+  if(ill) *ill = P.d_ill;
+  if(img64) *img64 = P.d_img64;
+  if(ill64) *ill64 = P.d_ill64;
+  return KLRTM_OK;
+}
+
+int klrtm_propagator_clear_images(klrtm_propagator *p) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  return reinterpret_cast<Propagator *>(p)->clear_images();
 }
 
 int klrtm_propagator_set_lcodec(klrtm_propagator *p, const klrtm_lcodec *c) {

@@ -130,6 +130,21 @@ int klrtm_set_stream(void *s) {
   t_user_stream_set = s != nullptr;
   return KLRTM_OK;
 }
+int klrtm_stream_create(void **s) {
+  KL_REQUIRE(s, KLRTM_EINVAL, "null stream output");
+  hipStream_t h = nullptr;
+  KL_HIP(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));  // default priority
+  *s = (void *)h;
+  return KLRTM_OK;
+}
+int klrtm_stream_destroy(void *s) {
+  if(s) KL_HIP(hipStreamDestroy((hipStream_t)s));
+  return KLRTM_OK;
+}
+int klrtm_stream_synchronize(void *s) {
+  KL_HIP(hipStreamSynchronize(s ? (hipStream_t)s : stream()));
+  return KLRTM_OK;
+}
 int klrtm_malloc(void **ptr, size_t bytes) {
   KL_REQUIRE(ptr, KLRTM_EINVAL, "null pointer output");
   *ptr = nullptr;

@@ -113,3 +113,38 @@ def ricker(nt, dt, fp):
     t = np.arange(nt) * dt - 1.5 / fp
     a = (np.pi * fp * t) ** 2
     return ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
+
+
+class SaltLayeredModel:
+    """Layered model with a salt-like body, evaluated on sub-volumes (apertures) on demand.
+
+    Six flat layers with seeded interface depths and velocities (1500-4000 m/s, increasing with depth
+    on average), plus an ellipsoidal body at `vsalt` whose semi-axes and centre are seeded and whose
+    boundary is made irregular by a low-order random radial perturbation.  The full model
+    (nyF, nxF, nz) is never formed: __call__(x0, y0, nx, ny) returns the (ny, nx, nz) block at
+    (x0, y0)."""
+
+    def __init__(self, nxF=1000, nyF=1000, nz=400, d=12.5, seed=505, vsalt=4500.0):
+        rng = np.random.default_rng(seed)
+        self.nxF, self.nyF, self.nz, self.d = nxF, nyF, nz, d
+        depths = np.sort(rng.uniform(0.05, 0.95, 5)) * nz
+        vels = np.sort(rng.uniform(1500.0, 4000.0, 6))
+        vels[1:] = np.maximum(vels[1:], vels[:-1])
+        self.profile = vels[np.searchsorted(depths, np.arange(nz) + 0.5)].astype(np.float32)
+        self.c = np.array([rng.uniform(0.3, 0.7) * nyF, rng.uniform(0.3, 0.7) * nxF, rng.uniform(0.45, 0.7) * nz])
+        self.ax = np.array([rng.uniform(0.12, 0.25) * nyF, rng.uniform(0.12, 0.25) * nxF, rng.uniform(0.1, 0.2) * nz])
+        self.pert = rng.uniform(-0.15, 0.15, (3, 2))   # radius modulation: cos/sin of azimuth orders 1..3
+        self.vsalt = np.float32(vsalt)
+
+    def __call__(self, x0, y0, nx, ny):
+        y = (np.arange(y0, y0 + ny, dtype=np.float64) - self.c[0]) / self.ax[0]
+        x = (np.arange(x0, x0 + nx, dtype=np.float64) - self.c[1]) / self.ax[1]
+        z = (np.arange(self.nz, dtype=np.float64) - self.c[2]) / self.ax[2]
+        v = np.broadcast_to(self.profile[None, None, :], (ny, nx, self.nz)).copy()
+        phi = np.arctan2(y[:, None], x[None, :])
+        rr = 1.0 + sum(self.pert[m, 0] * np.cos((m + 1) * phi) + self.pert[m, 1] * np.sin((m + 1) * phi)
+                       for m in range(3))
+        h2 = (y[:, None] ** 2 + x[None, :] ** 2) / (rr * rr)          # (ny, nx)
+        inside = (h2[:, :, None] + z[None, None, :] ** 2) <= 1.0
+        v[inside] = self.vsalt
+        return v
