@@ -145,20 +145,28 @@ def batched_vs_plain(nshots, nt, batch, seed=505):
     t_train = time.perf_counter() - t0
     plain = BatchRTM((nyF, nxF, nz), vel, common, batch=1, snap_format=SNAP_FP32)
     records = plain.run(shots)
-    fast = BatchRTM((nyF, nxF, nz), vel, common, batch=batch, snap_format=SNAP_LEARNED, lcodec=codec,
-                    tuner=Autotuner())
+    tuner = Autotuner()
+    fast32 = BatchRTM((nyF, nxF, nz), vel, common, batch=batch, snap_format=SNAP_FP32, tuner=tuner)
+    fast32.run(shots, records)
+    fast = BatchRTM((nyF, nxF, nz), vel, common, batch=batch, snap_format=SNAP_LEARNED, lcodec=codec, tuner=tuner)
     fast.run(shots, records)
-    a64, b64 = plain.stack.download(), fast.stack.download()
-    err = float(np.linalg.norm(b64 - a64) / max(np.linalg.norm(a64), 1e-300))
-    sph_plain, sph_fast = plain.shots_per_hour(2500), fast.shots_per_hour(2500)
+    a64 = plain.stack.download()
+    na = max(np.linalg.norm(a64), 1e-300)
+    err = float(np.linalg.norm(fast.stack.download() - a64) / na)
+    err32 = float(np.linalg.norm(fast32.stack.download() - a64) / na)
+    sph = [b.shots_per_hour(2500) for b in (plain, fast32, fast)]
     ratio = codec.code_bytes((nz, nz, nz)) / (nz ** 3 * 4.0)
-    plain.close(), fast.close(), codec.close()
+    for b in (plain, fast32, fast, codec):
+        b.close()
     return {"workload": "batched ISO FD8 RTM, 1000x1000x400 salt model, apertures 200-450 x 400",
             "shots": nshots, "steps_per_shot_measured": nt, "steps_per_shot_scaled_to": 2500, "batch": batch,
-            "plain_shots_per_hour": round(sph_plain, 2), "batched_shots_per_hour": round(sph_fast, 2),
-            "speedup": round(sph_fast / sph_plain, 3), "stack_nrmse_vs_plain": err,
-            "learned_bytes_vs_fp32": round(ratio, 4), "codec_training_s": round(t_train, 2),
-            "codec_training": info}
+            "plain_shots_per_hour": round(sph[0], 2),
+            "batched_fp32_shots_per_hour": round(sph[1], 2), "batched_fp32_speedup": round(sph[1] / sph[0], 3),
+            "batched_fp32_stack_nrmse_vs_plain": err32,
+            "batched_learned_shots_per_hour": round(sph[2], 2), "batched_learned_speedup": round(sph[2] / sph[0], 3),
+            "batched_learned_stack_nrmse_vs_plain": err, "learned_bytes_vs_fp32": round(ratio, 4),
+            "codec_training_s": round(t_train, 2), "codec_training": info,
+            "tiles": {f"{k[0]}x{k[1]}": list(v[0]) if v[0] else None for k, v in tuner.cache.items()}}
 
 
 def main():

@@ -111,8 +111,9 @@ __global__ void __launch_bounds__(256) lc_encode_kernel(const float *__restrict_
 // lane holds one feature of 16 blocks, in the volume layout lane l takes z offset l % 8 of blocks
 // l / 8 + 8s (s < 4) on each of the four block rows, so a wave's loads and stores are runs of 64
 // consecutive z samples.
-enum { LC_RES = 0, LC_DEC = 1, LC_IC = 2 };
+enum { LC_RES = 0, LC_DEC = 1, LC_IC = 2, LC_IC64 = 3 };  // LC_IC64: imaging with FP64 outputs too
 constexpr int LCS = 40;  // LDS row stride in floats: both access patterns are free of bank conflicts
+constexpr int LCB = 4;   // cells per lane whose loads are issued together in the imaging phase
 struct LcCode {
   const uint16_t *z;  // [nblk][nl] BF16
   float *scale;       // [nblk]
@@ -176,15 +177,17 @@ __global__ void __launch_bounds__(256)
     return acc;
   };
   // x of feature tile nt into the LDS tile (volume layout in, product layout out)
+  // (all global loads of a phase are issued before its LDS stores: a flat load may alias LDS, so the
+  // compiler keeps their order, and interleaving would wait for every load in turn)
   auto stage_x = [&](int nt) {
+    float v[16];
 #pragma unroll
-    for(int q = 0; q < 4; q++)
+    for(int e = 0; e < 16; e++) {
+      int ix, iy, iz;
+      v[e] = xin[cell(e & 3, nt * 4 + (e >> 2), ix, iy, iz)];  // a block past the end reads block 0's cells
+    }
 #pragma unroll
-      for(int s = 0; s < 4; s++) {
-        int ix, iy, iz;
-        const size_t i = cell(s, nt * 4 + q, ix, iy, iz);
-        T[((lane >> 3) + 8 * s) * LCS + q * 8 + lz] = okv[s] ? xin[i] : 0.f;
-      }
+    for(int e = 0; e < 16; e++) T[((lane >> 3) + 8 * (e & 3)) * LCS + (e >> 2) * 8 + lz] = okv[e & 3] ? v[e] : 0.f;
     wave_lds_sync();
   };
   if(MODE == LC_RES) {
@@ -228,34 +231,61 @@ __global__ void __launch_bounds__(256)
   }
   for(int nt = nt0; nt < nt1; nt++) {
     const f32x16 acc = xhat(nt);
+    int8_t qv[16];
 #pragma unroll
     for(int r = 0; r < 16; r++) {
-      const float qv = okr[r] ? (float)code.q[(b0 + ti(r)) * 512 + nt * 32 + (lane & 31)] : 0.f;
-      T[ti(r) * LCS + (lane & 31)] = acc[r] + qv * sc[r];
+      // unconditional load: a block past the end reads block b0's codes (sc is 0 there)
+      const long bq = okr[r] ? b0 + ti(r) : b0;
+      qv[r] = code.q[bq * 512 + nt * 32 + (lane & 31)];
     }
+#pragma unroll
+    for(int r = 0; r < 16; r++) T[ti(r) * LCS + (lane & 31)] = acc[r] + (float)qv[r] * sc[r];
     wave_lds_sync();
+    // every load below is unconditional (a block past the end stands on block 0's cells, which exist),
+    // issued for LCB cells at a time before their stores; only the stores are masked
 #pragma unroll
-    for(int q = 0; q < 4; q++)
+    for(int h = 0; h < 16 / LCB; h++) {
+      float xv[LCB], wv[LCB], iv[LCB], lv[LCB];
+      double iv64[LCB], lv64[LCB];
 #pragma unroll
-      for(int s = 0; s < 4; s++) {
-        if(!okv[s]) continue;
+      for(int e = 0; e < LCB; e++) {
+        const int q = (LCB * h + e) >> 2, s = e & 3;
         int ix, iy, iz;
         const size_t i = cell(s, nt * 4 + q, ix, iy, iz);
-        const float x = T[((lane >> 3) + 8 * s) * LCS + q * 8 + lz];
-        if(MODE == LC_DEC) {
-          vol[i] = x;
-        } else {
-          float S = edge_taper(a.T, x, ix, iy, iz, g.nx, g.ny, g.nz);
-          if(a.taper_t >= 0.f) S = S * a.taper_t;
-          else if(iz < a.ntaper_z) S = S * a.taper_z[iz];
-          const float R = edge_taper(a.T, w[i], ix, iy, iz, g.nx, g.ny, g.nz);
-          const float d = S * R;
-          if(img) img[i] = img[i] + d;
-          if(ill) ill[i] = ill[i] + S * S;
-          if(img64) img64[i] = img64[i] + (double)S * (double)R;
-          if(ill64) ill64[i] = ill64[i] + (double)S * (double)S;
+        xv[e] = T[((lane >> 3) + 8 * s) * LCS + q * 8 + lz];
+        if(MODE != LC_DEC) {
+          wv[e] = w[i];
+          if(img) iv[e] = img[i];
+          if(ill) lv[e] = ill[i];
+          if(MODE == LC_IC64) {
+            if(img64) iv64[e] = img64[i];
+            if(ill64) lv64[e] = ill64[i];
+          }
         }
       }
+#pragma unroll
+      for(int e = 0; e < LCB; e++) {
+        const int q = (LCB * h + e) >> 2, s = e & 3;
+        const bool ok = okv[s];
+        int ix, iy, iz;
+        const size_t i = cell(s, nt * 4 + q, ix, iy, iz);
+        if(MODE == LC_DEC) {
+          if(ok) vol[i] = xv[e];
+        } else {
+          float S = edge_taper(a.T, xv[e], ix, iy, iz, g.nx, g.ny, g.nz);
+          if(a.taper_t >= 0.f) S = S * a.taper_t;
+          else if(iz < a.ntaper_z) S = S * a.taper_z[iz];
+          const float R = edge_taper(a.T, wv[e], ix, iy, iz, g.nx, g.ny, g.nz);
+          const float d = S * R;
+          if(ok && img) img[i] = iv[e] + d;
+          if(ok && ill) ill[i] = lv[e] + S * S;
+          if(MODE == LC_IC64) {
+            if(ok && img64) img64[i] = iv64[e] + (double)S * (double)R;
+            if(ok && ill64) ill64[i] = lv64[e] + (double)S * (double)S;
+          }
+        }
+      }
+    }
     wave_lds_sync();
   }
 }
@@ -385,7 +415,10 @@ int lcodec_image(const LCodec &c, const IcArgs &a, const void *code, const float
   int rc = block_grid(g, a.nx, a.ny, a.nz);
   if(rc) return rc;
   LaunchTimer tm("ic_correlate");
-  launch_dec<LC_IC>(c, code_parts(c, code, g.nblk), nullptr, nullptr, g, a, w, img, ill, img64, ill64);
+  if(img64 || ill64)
+    launch_dec<LC_IC64>(c, code_parts(c, code, g.nblk), nullptr, nullptr, g, a, w, img, ill, img64, ill64);
+  else
+    launch_dec<LC_IC>(c, code_parts(c, code, g.nblk), nullptr, nullptr, g, a, w, img, ill, nullptr, nullptr);
   return check_launch("lc_decode_kernel");
 }
 
