@@ -236,6 +236,10 @@ typedef struct {
   int snap_slots;         /* > 0: snapshot slots kept (ring for benchmarks); 0: all */
   int ic_fp32_only;       /* 1: the imaging step accumulates only the FP32 image and illumination
                              (FP64 is left to the multi-shot stack, klrtm_stack_image) */
+  int free_surface_iz;    /* >= 0: FREESURFACE at this z index on both sides: the top PML is off and
+                             klrtm_apply_symmetry(w0, sym -1, taper nzbnd1) follows every injection;
+                             < 0: absorbing surface */
+  int surface_taper;      /* taper_apply_symmetry (the reference default is 1) */
 } klrtm_shot_config;
 
 typedef struct klrtm_propagator klrtm_propagator;
@@ -274,6 +278,12 @@ int klrtm_propagator_kernel(klrtm_propagator *p, klrtm_kernel **k);
 int klrtm_propagator_images(klrtm_propagator *p, float **img, float **ill, double **img64, double **ill64);
 /* zero the image accumulators (the step API accumulates into them; klrtm_propagator_rtm clears them itself) */
 int klrtm_propagator_clear_images(klrtm_propagator *p);
+
+/* ---- Propagator::apply_symmetry (bgprtm/Propagator.cpp:1857-1918) --------------
+ * Free-surface mirror in place on a device volume: above z index iz0 each column takes
+ * sym * its mirror image about iz0 (sym -1 also zeroes iz0); with taper != 0 the top tprlen
+ * samples take the tapered, [1 4 6 4 1]/16-smoothed mirror (tprlen <= iz0, 2*iz0 + 2 < nz). */
+int klrtm_apply_symmetry(float *wf, int nx, int ny, int nz, int iz0, int sym, int tprlen, int taper);
 
 /* ---- multi-shot stack (ImagingCondition::saveImage, bgprtm/ImagingCondition.cpp:874-945, in FP64) ----
  * full[(y0+iy)*nxF + x0+ix][z0+iz] += img[iy][ix][iz] for a shot image of nx*ny*nz on its aperture at

@@ -37,7 +37,7 @@ class ShotConfig(C.Structure):
         ("nrcv", _i),
         ("snap_every", _i), ("snap_format", _i), ("rec_through_codec", _i),
         ("tmute_frac", _f), ("ntaper_z", _i), ("fuse_step", _i),
-        ("snap_slots", _i), ("ic_fp32_only", _i),
+        ("snap_slots", _i), ("ic_fp32_only", _i), ("free_surface_iz", _i), ("surface_taper", _i),
     ]
 
 
@@ -116,6 +116,7 @@ _SIGS = {
     "klrtm_propagator_images": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "klrtm_propagator_clear_images": (_i, [_vp]),
     "klrtm_stack_image": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _i, _i, _i, _i, _i]),
+    "klrtm_apply_symmetry": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i]),
     "klrtm_stream_create": (_i, [C.POINTER(_vp)]),
     "klrtm_stream_destroy": (_i, [_vp]),
     "klrtm_stream_synchronize": (_i, [_vp]),

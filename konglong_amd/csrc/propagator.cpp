@@ -15,6 +15,7 @@
 #include "ic.h"
 #include "lcodec.h"
 #include "fd_kernels.h"
+#include "surface.h"
 #include <cmath>
 #include <cstring>
 #include <algorithm>
@@ -45,6 +46,7 @@ struct Propagator {
   int alloc_snapshots();
   int reset();
   int clear_images();
+  int surface();
   int step(int it);
   int store(int itsav);
   int image(int itsav);
@@ -101,6 +103,13 @@ int Propagator::alloc_snapshots() {
   }
   if(nscales) KL_HIP(hipMalloc(&d_scales, sizeof(float) * nscales * ntsav));
   return KLRTM_OK;
+}
+
+// Propagator::apply_symmetry(w0, -1) after the injection (bgprtm/Propagator.cpp:1119-1125, 1228-1230);
+// the taper length is the z boundary width
+int Propagator::surface() {
+  if(c.free_surface_iz < 0) return KLRTM_OK;
+  return apply_symmetry(w0, c.nx, c.ny, c.nz, c.free_surface_iz, -1, c.nbnd[4], c.surface_taper);
 }
 
 int Propagator::clear_images() {
@@ -179,7 +188,14 @@ int propagator_create(Propagator **out, const klrtm_shot_config *cfg) {
     return rc;
   }
   P->eng = reinterpret_cast<Engine *>(kh);
-  if((rc = klrtm_kernel_set_pml_options(kh, c.pml_eps > 0 ? c.pml_eps : 1e-3f, c.pml3d, c.pml3d_scaler, c.top_pml)) ||
+  // FREESURFACE switches the top PML off (bgprtm/FdEngine.cpp:45)
+  const int top_pml = c.free_surface_iz >= 0 ? 0 : c.top_pml;
+  if(c.free_surface_iz >= 0 &&
+     (rc = apply_symmetry_check(c.nx, c.ny, c.nz, c.free_surface_iz, c.surface_taper ? c.nbnd[4] : 0))) {
+    delete P;
+    return rc;
+  }
+  if((rc = klrtm_kernel_set_pml_options(kh, c.pml_eps > 0 ? c.pml_eps : 1e-3f, c.pml3d, c.pml3d_scaler, top_pml)) ||
      (rc = klrtm_kernel_set_boundary(kh, c.nbnd[0], c.nbnd[1], c.nbnd[2], c.nbnd[3], c.nbnd[4], c.nbnd[5], c.dt, c.vmax))) {
     delete P;
     return rc;
@@ -266,7 +282,7 @@ int propagator_model(Propagator &P, float *record) {
   KL_HIP(hipEventRecord(P.ev[0], stream()));
   for(int it = 0; it < P.c.nt; it++) {
     if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f))) return rc;  // it_shift_force = -1
+    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f)) || (rc = P.surface())) return rc;  // it_shift_force = -1
     if((rc = receiver_extract(*P.rcv, P.w0, it))) return rc;
     std::swap(P.w0, P.w1);
   }
@@ -292,7 +308,7 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
   KL_HIP(hipEventRecord(P.ev[0], stream()));
   for(int it = 0; it < nt; it++) {
     if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f))) return rc;
+    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f)) || (rc = P.surface())) return rc;
     if(it % k == 0 && it / k < P.ntsav && (rc = P.store(it / k))) return rc;
     std::swap(P.w0, P.w1);
   }
@@ -305,7 +321,7 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
   KL_HIP(hipEventRecord(P.ev[2], stream()));
   for(int it = nt - 1; it >= 0; it--) {
     if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f))) return rc;
+    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f)) || (rc = P.surface())) return rc;
     if(it % k == 0 && it / k < P.ntsav && (rc = P.image(it / k))) return rc;
     std::swap(P.w0, P.w1);
   }
@@ -332,7 +348,7 @@ int propagator_forward_steps(Propagator &P, int it0, int nsteps, int store_snaps
   const int k = std::max(1, P.c.snap_every);
   for(int it = it0; it < it0 + nsteps; it++) {
     if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f))) return rc;
+    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f)) || (rc = P.surface())) return rc;
     if(store_snapshots && it % k == 0) {
       int s = (it / k) % std::max(1, P.ntsav);
       if((rc = P.store(s))) return rc;
@@ -349,7 +365,7 @@ int propagator_backward_steps(Propagator &P, int it0, int nsteps, int image) {
   const int k = std::max(1, P.c.snap_every);
   for(int it = it0; it > it0 - nsteps; it--) {
     if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f))) return rc;
+    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f)) || (rc = P.surface())) return rc;
     if(image && it >= 0 && it % k == 0 && (rc = P.image((it / k) % std::max(1, P.ntsav)))) return rc;
     std::swap(P.w0, P.w1);
   }

@@ -9,7 +9,7 @@ from ._lib import lib, check, fptr, dptr, ShotConfig, ENGINE_FD, ISO, SNAP_FP32
 def shot_config(nx, ny, nz, d, dt, nt, engine=ENGINE_FD, order=8, model=ISO, nbnd=20, vmax=None, src_xyz=None,
                 f_peak=15.0, spread=10, khpass=0.3, khcut=0.3, nrcv=0, snap_every=1, snap_format=SNAP_FP32,
                 rec_through_codec=0, tmute_frac=0.1, ntaper_z=0, fuse_step=1, pml_eps=1e-3, pml3d=1,
-                pml3d_scaler=0.3, top_pml=1, snap_slots=0, ic_fp32_only=0):
+                pml3d_scaler=0.3, top_pml=1, snap_slots=0, ic_fp32_only=0, free_surface_iz=-1, surface_taper=1):
     c = ShotConfig()
     c.nx, c.ny, c.nz = nx, ny, nz
     dx, dy, dz = (d, d, d) if np.isscalar(d) else d
@@ -31,6 +31,7 @@ def shot_config(nx, ny, nz, d, dt, nt, engine=ENGINE_FD, order=8, model=ISO, nbn
     c.tmute_frac, c.ntaper_z, c.fuse_step = tmute_frac, ntaper_z, fuse_step
     c.snap_slots = snap_slots
     c.ic_fp32_only = ic_fp32_only
+    c.free_surface_iz, c.surface_taper = free_surface_iz, surface_taper
     return c
 
 

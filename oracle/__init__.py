@@ -73,6 +73,8 @@ def lib():
         L.orc_pair_autobit.restype = C.c_int
         L.orc_pair_autobit.argtypes = [C.c_float, C.c_float, C.c_int]
         L.orc_crossone.argtypes = [_pf, _pf, _pf, C.c_size_t]
+        L.orc_apply_symmetry.argtypes = [_pf] + [C.c_int] * 7
+        L.orc_apply_symmetry_d.argtypes = [_pd] + [C.c_int] * 7
         L.orc_illum.argtypes = [_pf, _pf, C.c_size_t]
         L.orc_apply_tape.argtypes = [_pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, C.c_int, _pf, C.c_int]
         _lib = L
@@ -255,3 +257,14 @@ def uncompress_pair(code, ny, nx, nz, mode):
 def pair_autobit(range1, range2, nbits):
     return lib().orc_pair_autobit(range1, range2, nbits)
 
+
+def apply_symmetry(wf, iz0, sym, tprlen, taper=1):
+    """Propagator::apply_symmetry (bgprtm/Propagator.cpp:1857-1918) in place on a (ny, nx, nz)
+    float32 or float64 volume."""
+    ny, nx, nz = wf.shape
+    if wf.dtype == np.float64:
+        lib().orc_apply_symmetry_d(wf.ctypes.data_as(_pd), nx, ny, nz, iz0, sym, tprlen, int(taper))
+    else:
+        assert wf.dtype == np.float32 and wf.flags.c_contiguous
+        lib().orc_apply_symmetry(wf.ctypes.data_as(_pf), nx, ny, nz, iz0, sym, tprlen, int(taper))
+    return wf

@@ -15,7 +15,7 @@ import math
 
 import numpy as np
 
-from . import lib, Engine, spread_coeff, taper_kernel, xy_taper, compress16, uncompress16, _pd
+from . import lib, Engine, spread_coeff, taper_kernel, xy_taper, compress16, uncompress16, apply_symmetry, _pd
 
 _pf, _pi = C.POINTER(C.c_float), C.POINTER(C.c_int)
 
@@ -84,8 +84,9 @@ class Shot:
     """One shot: model (record) and rtm (image) in FP32, reference step order."""
 
     def __init__(self, nx, ny, nz, d, dt, nt, engine, order, model, nbnd, vmax, src_xyz, f_peak, spread, k1, k2,
-                 rcv_xyz, snap_every=1, snap_format=0, tmute_frac=0.1, threads=4):
+                 rcv_xyz, snap_every=1, snap_format=0, tmute_frac=0.1, threads=4, free_surface_iz=-1, surface_taper=1):
         self.shape = (ny, nx, nz)
+        self.fs, self.fs_taper = free_surface_iz, surface_taper
         self.nx, self.ny, self.nz, self.dt, self.nt = nx, ny, nz, dt, nt
         self.model_type, self.snap_every, self.snap_format = model, snap_every, snap_format
         self.eng = Engine(nx, ny, nz, d, d, d, engine, order, np.float32, threads=threads)
@@ -102,7 +103,15 @@ class Shot:
         dt = np.float32(self.dt)
         self.m = ((v * v) * (dt * dt)).astype(np.float32)  # Propagator::Vel2: dt*dt in FP32
         self.eng.set_model(self.m, eps, del_, pjx, pjy)
-        self.eng.set_boundary(self.nbnd, self.dt, self.vmax, dim_pml=3)
+        # FREESURFACE switches the top PML off (bgprtm/FdEngine.cpp:45)
+        self.eng.set_boundary(self.nbnd, self.dt, self.vmax, dim_pml=3, top=self.fs < 0)
+
+    def _surface(self, w0):
+        """Propagator::apply_symmetry(w0, -1) after the injection (bgprtm/Propagator.cpp:1119-1125,
+        1228-1230); the taper length is the z boundary width nzbnd."""
+        if self.fs >= 0:
+            nzb = self.nbnd[4] if not np.isscalar(self.nbnd) else self.nbnd
+            apply_symmetry(w0, self.fs, -1, nzb, self.fs_taper)
 
     def _fresh(self):
         self.eng.clean_pml()
@@ -117,6 +126,7 @@ class Shot:
             self.eng.step(self.model_type, w1, w0, wr)
             if 0 <= it - 1 < self.nt:
                 inject(w0, self.m, self.shape, self.src, self.wavelet[it - 1:it] * self.dt_dxyz)
+            self._surface(w0)
             rec[:, it] = extract(w0, self.shape, self.rcv)
             w0, w1 = w1, w0
         self.last = w1
@@ -137,6 +147,7 @@ class Shot:
             self.eng.step(self.model_type, w1, w0, wr)
             if 0 <= it - 1 < self.nt:
                 inject(w0, self.m, self.shape, self.src, self.wavelet[it - 1:it] * self.dt_dxyz)
+            self._surface(w0)
             if it % k == 0 and it // k < ntsav:
                 snaps[it // k] = compress16(w0) if self.snap_format == 1 else w0.copy()
             w0, w1 = w1, w0
@@ -153,6 +164,7 @@ class Shot:
             ia = it + 1
             if 0 <= ia < self.nt:
                 inject(w0, self.m, self.shape, self.rcv, rec[:, ia] * self.dt_dxyz)
+            self._surface(w0)
             if it % k == 0 and it // k < ntsav:
                 s = it // k
                 S = self._decode(snaps[s])

@@ -1027,6 +1027,56 @@ void orc_extract(const float *w, int nx, int ny, int nz, int nr, float *out, con
   }
 }
 
+// Propagator::apply_symmetry (bgprtm/Propagator.cpp:1857-1918): free-surface mirror above iz0 with
+// sign sym; with taper (taper_apply_symmetry, default 1) the top tprlen samples take the tapered,
+// [1 4 6 4 1]/16-smoothed mirror.  FP32 form with the source's promotions (the smoothing and its
+// (1 - f) factor in float, "/ 16." and the rest in double); FP64 form in double throughout.
+void orc_apply_symmetry(float *wf0, int nx, int ny, int nz, int iz0, int sym, int tprlen, int taper) {
+  const int iz2 = iz0 * 2;
+  std::vector<float> tp(std::max(tprlen, 0));
+  for(int iz = 0; iz < tprlen; iz++) tp[iz] = 0.5f * (1 - cosf((float)M_PI * (iz + 0.5) / tprlen));
+  for(long c = 0; c < (long)nx * ny; c++) {
+    float *wf = wf0 + (size_t)c * nz;
+    if(taper) {
+      for(int iz = tprlen; iz < iz0; iz++) wf[iz] = sym * wf[iz2 - iz];
+      if(sym == -1) wf[iz0] = 0;
+      for(int iz = 0; iz < tprlen; iz++) {
+        float f = tp[iz];
+        wf[iz] = sym *
+                 (wf[iz2 - iz] * f +
+                  (1 - f) * (wf[iz2 - iz - 2] + wf[iz2 - iz + 2] + 4 * (wf[iz2 - iz - 1] + wf[iz2 - iz + 1]) + 6 * wf[iz2 - iz]) /
+                      16.) *
+                 f;
+      }
+    } else {
+      for(int iz = 0; iz < iz0; iz++) wf[iz] = sym * wf[iz2 - iz];
+      if(sym == -1) wf[iz0] = 0;
+      wf[0] = 0.5 * (wf[0] + wf[1]);
+    }
+  }
+}
+void orc_apply_symmetry_d(double *wf0, int nx, int ny, int nz, int iz0, int sym, int tprlen, int taper) {
+  const int iz2 = iz0 * 2;
+  std::vector<double> tp(std::max(tprlen, 0));
+  for(int iz = 0; iz < tprlen; iz++) tp[iz] = 0.5 * (1 - cos(M_PI * (iz + 0.5) / tprlen));
+  for(long c = 0; c < (long)nx * ny; c++) {
+    double *wf = wf0 + (size_t)c * nz;
+    if(taper) {
+      for(int iz = tprlen; iz < iz0; iz++) wf[iz] = sym * wf[iz2 - iz];
+      if(sym == -1) wf[iz0] = 0;
+      for(int iz = 0; iz < tprlen; iz++) {
+        const double f = tp[iz];
+        const int m = iz2 - iz;
+        wf[iz] = sym * (wf[m] * f + (1 - f) * (wf[m - 2] + wf[m + 2] + 4 * (wf[m - 1] + wf[m + 1]) + 6 * wf[m]) / 16.) * f;
+      }
+    } else {
+      for(int iz = 0; iz < iz0; iz++) wf[iz] = sym * wf[iz2 - iz];
+      if(sym == -1) wf[iz0] = 0;
+      wf[0] = 0.5 * (wf[0] + wf[1]);
+    }
+  }
+}
+
 // FP64 forms of the two above (the spread taps stay the FP32 tables: they define the problem)
 void orc_inject_d(double *w0, const double *vel, int nx, int ny, int nz, int nr, const double *val, const float *kx,
                   const float *ky, const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz,

@@ -370,3 +370,45 @@ def test_pair_codec_special_traces(mode):
     assert np.abs(back - const).max() <= tol
     zero = np.zeros((2, 2, n), np.float32)  # ranges floor at FLT_MIN per level: decodes to ~1e-38
     assert np.abs(oracle.uncompress_pair(oracle.compress_pair(zero, mode), 2, 2, n, mode)).max() <= 1e-30
+
+
+# ---- free surface (Propagator::apply_symmetry) --------------------------------------------------
+def test_symmetry_untapered_mirrors():
+    rng = np.random.default_rng(8)
+    ny, nx, nz, iz0 = 3, 4, 30, 9
+    for sym in (1, -1):
+        v = rng.standard_normal((ny, nx, nz)).astype(np.float32)
+        w = oracle.apply_symmetry(v.copy(), iz0, sym, 0, taper=0)
+        assert np.array_equal(w[:, :, iz0 + 1:], v[:, :, iz0 + 1:])           # below the surface untouched
+        for k in range(1, iz0):                                                # mirror image above it
+            assert np.array_equal(w[:, :, iz0 - k], sym * v[:, :, iz0 + k])
+        if sym == -1:
+            assert not w[:, :, iz0].any()                                      # pressure release
+        else:
+            assert np.array_equal(w[:, :, iz0], v[:, :, iz0])
+        assert np.array_equal(w[:, :, 0], (0.5 * (sym * v[:, :, 2 * iz0].astype(np.float64) +
+                                                   sym * v[:, :, 2 * iz0 - 1])).astype(np.float32))
+
+
+def test_symmetry_taper_known_answers():
+    """A constant field c (sym +1) stays c from tprlen down and becomes c*f(iz) in the taper zone (the
+    [1 4 6 4 1]/16 smoothing of a constant is the constant); a field linear in z mirrors to the
+    reflected line (sym -1), times f in the taper zone; FP32 and FP64 forms agree to rounding."""
+    ny, nx, nz, iz0, tpr = 2, 3, 40, 12, 8
+    c = np.float32(1.75)
+    w = oracle.apply_symmetry(np.full((ny, nx, nz), c, np.float32), iz0, 1, tpr)
+    f = 0.5 * (1 - np.cos(np.pi * (np.arange(tpr) + 0.5) / tpr))
+    assert np.allclose(w[0, 0, :tpr], c * f, rtol=5e-6, atol=0)   # f in FP32 (cosf near 1 for small iz)
+    assert np.array_equal(w[:, :, tpr:], np.full((ny, nx, nz - tpr), c, np.float32))
+    z = np.arange(nz, dtype=np.float32)
+    lin = np.broadcast_to(z, (ny, nx, nz)).astype(np.float32).copy()
+    w = oracle.apply_symmetry(lin.copy(), iz0, -1, tpr)
+    assert np.array_equal(w[0, 0, tpr:iz0], -(2 * iz0 - z[tpr:iz0]))
+    assert w[0, 0, iz0] == 0
+    # the [1 4 6 4 1]/16 smoothing of a linear function is the function: -(mirror)*f in the taper zone
+    assert np.allclose(w[0, 0, :tpr], -(2 * iz0 - z[:tpr]) * f, rtol=5e-6)
+    rng = np.random.default_rng(9)
+    v = rng.standard_normal((ny, nx, nz)).astype(np.float32)
+    a = oracle.apply_symmetry(v.copy(), iz0, -1, tpr)
+    b = oracle.apply_symmetry(v.astype(np.float64), iz0, -1, tpr)
+    assert np.allclose(a, b, rtol=1e-6, atol=1e-6)
