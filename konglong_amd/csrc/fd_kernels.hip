@@ -31,11 +31,14 @@ struct TileCoord {
 };
 __device__ __forceinline__ TileCoord tile_coord(const IsoArgs &a) {
   TileCoord c;
+  // Consecutive logical tiles run together on one XCD (xcd_remap); z tiles vary fastest, so the
+  // co-resident set is a patch of whole z columns over neighbouring x: the z and x halos a block
+  // reads are mostly its neighbours' rows, already in that XCD's L2.
   int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int bx = t % a.nbx;
-  t /= a.nbx;
   const int bz = t % a.nbz;
-  const int by = t / a.nbz;
+  t /= a.nbz;
+  const int bx = t % a.nbx;
+  const int by = t / a.nbx;
   c.tz = threadIdx.x % a.tz, c.tx = threadIdx.x / a.tz;
   c.iz = bz * a.tz + c.tz, c.ix = bx * a.tx + c.tx;
   c.y0 = by * a.ychunk, c.y1 = min(a.ny, c.y0 + a.ychunk);
