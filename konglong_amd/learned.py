@@ -181,3 +181,40 @@ def train(shape, seeds, d, dt, nt, snap_every, nl=32, engine=ENGINE_FD, order=8,
     W, ev = leading_eigvecs(G, nl)
     kept = float(ev[:nl].sum() / max(ev.sum(), 1e-300))
     return LearnedCompressor(W), dict(snapshots=nsnap, blocks=acc.count, energy_kept=kept)
+
+
+def _bench(n=400, nl=32, reps=5):
+    """Times encode, decode and decode+imaging of one n^3 snapshot (random W, random field) against the
+    FP32 imaging kernel; used for the codec's kernel profile."""
+    import time
+    from ._lib import synchronize
+    rng = np.random.default_rng(0)
+    W, _ = np.linalg.qr(rng.standard_normal((FEATURES, nl)))
+    c = LearnedCompressor(W.astype(np.float32))
+    shape = (n, n, n)
+    x = DeviceArray.from_host(rng.standard_normal(shape).astype(np.float32))
+    w = DeviceArray.from_host(rng.standard_normal(shape).astype(np.float32))
+    img, ill, out = DeviceArray(shape), DeviceArray(shape), DeviceArray(shape)
+    code = c.new_code(shape)
+    L = lib()
+    res = {}
+    for name, f in (("encode", lambda: c.encode(x, code)), ("decode", lambda: c.decode(code, out)),
+                    ("decode+imaging", lambda: check(L.klrtm_lcodec_correlate(c.h, code.ptr, w.ptr, n, n, n, 0.5, None,
+                                                                              0, img.ptr, ill.ptr, None, None))),
+                    ("fp32 imaging", lambda: check(L.klrtm_ic_correlate(x.ptr, w.ptr, n, n, n, 0.5, None, 0, img.ptr,
+                                                                        ill.ptr, None, None)))):
+        f()
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        synchronize()
+        res[name] = (time.perf_counter() - t0) / reps * 1e3
+    c.close()
+    return res
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+    print(json.dumps({k: round(v, 4) for k, v in _bench(int(sys.argv[1]) if len(sys.argv) > 1 else 400).items()}))
