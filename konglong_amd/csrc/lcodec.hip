@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256)
       double iv64[LCB], lv64[LCB];
 #pragma unroll
       for(int e = 0; e < LCB; e++) {
-        const int q = (LCB * h + e) >> 2, s = e & 3;
+        const int q = (LCB * h + e) >> 2, s = (LCB * h + e) & 3;
         int ix, iy, iz;
         const size_t i = cell(s, nt * 4 + q, ix, iy, iz);
         xv[e] = T[((lane >> 3) + 8 * s) * LCS + q * 8 + lz];
@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(256)
       }
 #pragma unroll
       for(int e = 0; e < LCB; e++) {
-        const int q = (LCB * h + e) >> 2, s = e & 3;
+        const int q = (LCB * h + e) >> 2, s = (LCB * h + e) & 3;
         const bool ok = okv[s];
         int ix, iy, iz;
         const size_t i = cell(s, nt * 4 + q, ix, iy, iz);
