@@ -285,6 +285,21 @@ int klrtm_propagator_clear_images(klrtm_propagator *p);
  * samples take the tapered, [1 4 6 4 1]/16-smoothed mirror (tprlen <= iz0, 2*iz0 + 2 < nz). */
 int klrtm_apply_symmetry(float *wf, int nx, int ny, int nz, int iz0, int sym, int tprlen, int taper);
 
+/* ---- illumination compensation (ImagingCondition::computeDamping / applyDamping,
+ * bgprtm/ImagingCondition.cpp:600-672) -------------------------------------------------------
+ * In place on the device illumination volume: when smooth != 0 a box mean of half widths
+ * smooth_xy (x, y) and smooth_z (avgVolume3D: y, x, then z with edge replication; 2D: x, z;
+ * smoothSrcAmpXY/Z default 10), then with average = mean(illum)^amp_damping (ampDamping,
+ * default 1) illum <- 1/(average*(a + damp_weight*exp(-a))), a = illum/average (dampWeight,
+ * default 0.07), tapered over 10 cells at the x and y edges.  average_out (host, may be NULL)
+ * receives average.  klrtm_apply_damping multiplies an image by those weights. */
+int klrtm_illum_damping(float *illum, int nx, int ny, int nz, int smooth, int smooth_xy, int smooth_z,
+                        float amp_damping, float damp_weight, double *average_out);
+int klrtm_apply_damping(const float *illum, float *img, size_t n);
+/* avgVolume3D (bgprtm/volumefilter.cpp:59-98) in place: box means of half widths sx, sy, sz along
+ * y, then x, then z, edge samples replicated (ny > 1) */
+int klrtm_avg_volume3d(float *v, int nx, int ny, int nz, int sx, int sy, int sz);
+
 /* ---- multi-shot stack (ImagingCondition::saveImage, bgprtm/ImagingCondition.cpp:874-945, in FP64) ----
  * full[(y0+iy)*nxF + x0+ix][z0+iz] += img[iy][ix][iz] for a shot image of nx*ny*nz on its aperture at
  * (x0, y0, z0) of the nxF*nyF*nzF output grid; img is FP32 (is_double 0) or FP64 (is_double 1). */

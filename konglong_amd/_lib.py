@@ -117,6 +117,9 @@ _SIGS = {
     "klrtm_propagator_clear_images": (_i, [_vp]),
     "klrtm_stack_image": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _i, _i, _i, _i, _i]),
     "klrtm_apply_symmetry": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i]),
+    "klrtm_illum_damping": (_i, [_vp, _i, _i, _i, _i, _i, _i, _f, _f, _pd]),
+    "klrtm_apply_damping": (_i, [_vp, _vp, _sz]),
+    "klrtm_avg_volume3d": (_i, [_vp, _i, _i, _i, _i, _i, _i]),
     "klrtm_stream_create": (_i, [C.POINTER(_vp)]),
     "klrtm_stream_destroy": (_i, [_vp]),
     "klrtm_stream_synchronize": (_i, [_vp]),

@@ -1,0 +1,193 @@
+// postproc.hip -- image post-processing after the imaging condition: the
+// source-illumination compensation of ImagingCondition::computeDamping /
+// applyDamping (bgprtm/ImagingCondition.cpp:600-672), with the box smoothing
+// of avgVolume3D / avgVolume1D (bgprtm/volumefilter.cpp:59-135).
+//
+// The box filter is a running sum along one axis with edge replication: one
+// lane per line keeps the sum and walks the line in the reference's order
+// (y lines and x lines: lanes over z, so every step is a coalesced row; z
+// lines: one lane per column), so the smoothed volume equals the reference's
+// bit for bit.  The mean of the smoothed illumination is a fixed-order FP64
+// tree reduction; the damping weights then follow the reference's mixed
+// FP32/FP64 arithmetic.  FP contraction is off.
+#pragma clang fp contract(off)
+#include "klrtm_internal.h"
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace klrtm {
+
+// box mean of half width h along an axis of n samples with stride s, for every line of the volume
+// (lines enumerated by the lane index); out is another buffer
+template<int AXIS>
+__global__ void box_kernel(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, int nz, int h,
+                           float scaler, float sizea1) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t base, s;
+  int n;
+  if(AXIS == 2) {  // y lines: lanes over (ix, iz)
+    if(c >= (long)nx * nz) return;
+    base = (size_t)c, s = (size_t)nx * nz, n = ny;
+  } else if(AXIS == 1) {  // x lines: lanes over (iy, iz)
+    if(c >= (long)ny * nz) return;
+    const int iy = (int)(c / nz), iz = (int)(c % nz);
+    base = (size_t)iy * nx * nz + iz, s = (size_t)nz, n = nx;
+  } else {  // z lines: one lane per column
+    if(c >= (long)nx * ny) return;
+    base = (size_t)c * nz, s = 1, n = nz;
+  }
+  // avgVolume1D: bufm = (h+1)*a[0] + a[1..h]; b[0] = bufm*scaler; then bufm += a[k2] - a[k1]
+  float bufm = sizea1 * in[base];
+  for(int j = 1; j <= h; j++) bufm += in[base + (size_t)j * s];
+  out[base] = bufm * scaler;
+  for(int k = 1; k < n; k++) {
+    const int k1 = max(0, k - h - 1), k2 = min(n - 1, k + h);
+    bufm += (in[base + (size_t)k2 * s] - in[base + (size_t)k1 * s]);
+    out[base + (size_t)k * s] = bufm * scaler;
+  }
+}
+
+__global__ void sum_kernel(const float *__restrict__ v, size_t n, double *__restrict__ part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    acc += (double)v[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for(int w = 128; w > 0; w >>= 1) {
+    if((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+struct DampArgs {
+  int nx, ny, nz, ntaper;
+  double average, oaverage;
+  float damp_weight;
+  float wtaper[10];
+};
+
+// illum <- 1 / (average * (aa + w*exp(-aa))), aa = illum/average, then the x/y edge taper
+__global__ void damping_kernel(float *__restrict__ illum, const DampArgs a) {
+  const size_t n = (size_t)a.nx * a.ny * a.nz;
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t col = i / a.nz;
+    const int ix = (int)(col % a.nx), iy = (int)(col / a.nx);
+    const float aa = (float)((double)illum[i] * a.oaverage);
+    const float bb = expf(-aa);
+    float v = (float)(1.0 / (a.average * (double)(aa + a.damp_weight * bb)));
+    if(iy < a.ntaper && a.ny > 1) v *= a.wtaper[iy];
+    if(iy > a.ny - a.ntaper - 1 && a.ny > 1) v *= a.wtaper[a.ny - iy - 1];
+    if(ix < a.ntaper) v *= a.wtaper[ix];
+    if(ix > a.nx - a.ntaper - 1) v *= a.wtaper[a.nx - ix - 1];
+    illum[i] = v;
+  }
+}
+
+__global__ void multiply_kernel(float *__restrict__ img, const float *__restrict__ w, size_t n) {
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    img[i] = img[i] * w[i];
+}
+
+static unsigned grid_of(long n) { return (unsigned)std::max(1L, std::min((n + 255) / 256, 256L * 64)); }
+
+// avgVolume3D: y, then x, then z (bgprtm/volumefilter.cpp:59-98)
+static int box3d(float *v, float *tmp, int nx, int ny, int nz, int sx, int sy, int sz) {
+  auto pass = [&](int axis, const float *in, float *out, int h) {
+    const float scaler = (float)(1.0f / ((double)(2.0f * h) + 1.0)), sizea1 = h + 1.0f;
+    if(axis == 2)
+      hipLaunchKernelGGL(box_kernel<2>, dim3(grid_of((long)nx * nz)), dim3(256), 0, stream(), in, out, nx, ny, nz, h,
+                         scaler, sizea1);
+    else if(axis == 1)
+      hipLaunchKernelGGL(box_kernel<1>, dim3(grid_of((long)ny * nz)), dim3(256), 0, stream(), in, out, nx, ny, nz, h,
+                         scaler, sizea1);
+    else
+      hipLaunchKernelGGL(box_kernel<0>, dim3(grid_of((long)nx * ny)), dim3(256), 0, stream(), in, out, nx, ny, nz, h,
+                         scaler, sizea1);
+  };
+  pass(2, v, tmp, sy);
+  pass(1, tmp, v, sx);
+  pass(0, v, tmp, sz);
+  KL_HIP(hipMemcpyAsync(v, tmp, sizeof(float) * (size_t)nx * ny * nz, hipMemcpyDeviceToDevice, stream()));
+  return check_launch("box_kernel");
+}
+
+}  // namespace klrtm
+
+using namespace klrtm;
+
+extern "C" {
+
+int klrtm_illum_damping(float *illum, int nx, int ny, int nz, int smooth, int smooth_xy, int smooth_z,
+                        float amp_damping, float damp_weight, double *average_out) {
+  KL_REQUIRE(illum && nx > 0 && ny > 0 && nz > 0, KLRTM_EINVAL, "bad illumination volume");
+  KL_REQUIRE(!smooth || (smooth_xy >= 0 && smooth_z >= 0 && smooth_xy < nx && smooth_z < nz &&
+                         (ny == 1 || smooth_xy < ny)),
+             KLRTM_EINVAL, "smoothing half widths %d, %d must be shorter than the axes", smooth_xy, smooth_z);
+  const size_t n = (size_t)nx * ny * nz;
+  LaunchTimer tm("illum_damping");
+  if(smooth) {
+    float *tmp = nullptr;
+    KL_HIP(hipMallocAsync(&tmp, n * sizeof(float), stream()));
+    int rc;
+    if(ny > 1) {
+      rc = box3d(illum, tmp, nx, ny, nz, smooth_xy, smooth_xy, smooth_z);
+    } else {  // avgVolume2D: x then z
+      const float sx = (float)(1.0f / ((double)(2.0f * smooth_xy) + 1.0)), ax = smooth_xy + 1.0f;
+      const float sz = (float)(1.0f / ((double)(2.0f * smooth_z) + 1.0)), az = smooth_z + 1.0f;
+      hipLaunchKernelGGL(box_kernel<1>, dim3(grid_of((long)nz)), dim3(256), 0, stream(), illum, tmp, nx, 1, nz, smooth_xy,
+                         sx, ax);
+      hipLaunchKernelGGL(box_kernel<0>, dim3(grid_of((long)nx)), dim3(256), 0, stream(), tmp, illum, nx, 1, nz, smooth_z,
+                         sz, az);
+      rc = check_launch("box_kernel");
+    }
+    KL_HIP(hipFreeAsync(tmp, stream()));
+    if(rc) return rc;
+  }
+  // mean: per-block FP64 partials in a fixed order, summed on the host in block order
+  const unsigned nb = 1024;
+  double *part = nullptr;
+  KL_HIP(hipMallocAsync(&part, nb * sizeof(double), stream()));
+  hipLaunchKernelGGL(sum_kernel, dim3(nb), dim3(256), 0, stream(), illum, n, part);
+  std::vector<double> hp(nb);
+  KL_HIP(hipMemcpyAsync(hp.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  KL_HIP(hipStreamSynchronize(stream()));
+  KL_HIP(hipFreeAsync(part, stream()));
+  double total = 0.0;
+  for(double v : hp) total += v;
+  double average = total / (double)(unsigned long)n;
+  average = pow(average, (double)amp_damping);
+  KL_REQUIRE(average > 0.0 && std::isfinite(average), KLRTM_EINVAL, "illumination mean %g is not positive", average);
+  DampArgs a{};
+  a.nx = nx, a.ny = ny, a.nz = nz, a.ntaper = 10;
+  a.average = average, a.oaverage = 1.0 / average;
+  a.damp_weight = damp_weight;
+  for(int i = 0; i < 10; i++) {
+    float temp = (float)(M_PI * (double)float(i) / (double)float(10));
+    a.wtaper[i] = 0.5f * (1.0f - cosf(temp));
+  }
+  hipLaunchKernelGGL(damping_kernel, dim3(grid_of((long)n)), dim3(256), 0, stream(), illum, a);
+  if(average_out) *average_out = average;
+  return check_launch("damping_kernel");
+}
+
+int klrtm_avg_volume3d(float *v, int nx, int ny, int nz, int sx, int sy, int sz) {
+  KL_REQUIRE(v && nx > 0 && ny > 1 && nz > 0, KLRTM_EINVAL, "bad volume (3D, ny > 1)");
+  KL_REQUIRE(sx >= 0 && sy >= 0 && sz >= 0 && sx < nx && sy < ny && sz < nz, KLRTM_EINVAL,
+             "half widths %d, %d, %d must be shorter than the axes", sx, sy, sz);
+  float *tmp = nullptr;
+  KL_HIP(hipMallocAsync(&tmp, (size_t)nx * ny * nz * sizeof(float), stream()));
+  const int rc = box3d(v, tmp, nx, ny, nz, sx, sy, sz);
+  KL_HIP(hipFreeAsync(tmp, stream()));
+  return rc;
+}
+
+int klrtm_apply_damping(const float *illum, float *img, size_t n) {
+  KL_REQUIRE(illum && img, KLRTM_EINVAL, "null volume");
+  hipLaunchKernelGGL(multiply_kernel, dim3(grid_of((long)n)), dim3(256), 0, stream(), img, illum, n);
+  return check_launch("multiply_kernel");
+}
+
+}  // extern "C"

@@ -75,6 +75,8 @@ def lib():
         L.orc_crossone.argtypes = [_pf, _pf, _pf, C.c_size_t]
         L.orc_apply_symmetry.argtypes = [_pf] + [C.c_int] * 7
         L.orc_apply_symmetry_d.argtypes = [_pd] + [C.c_int] * 7
+        L.orc_illum_damping.argtypes = [_pf] + [C.c_int] * 6 + [C.c_float, C.c_float, _pd]
+        L.orc_avg_volume3d.argtypes = [_pf] + [C.c_int] * 6
         L.orc_illum.argtypes = [_pf, _pf, C.c_size_t]
         L.orc_apply_tape.argtypes = [_pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, C.c_int, _pf, C.c_int]
         _lib = L
@@ -268,3 +270,22 @@ def apply_symmetry(wf, iz0, sym, tprlen, taper=1):
         assert wf.dtype == np.float32 and wf.flags.c_contiguous
         lib().orc_apply_symmetry(wf.ctypes.data_as(_pf), nx, ny, nz, iz0, sym, tprlen, int(taper))
     return wf
+
+
+def illum_damping(illum, smooth=True, smooth_xy=10, smooth_z=10, amp_damping=1.0, damp_weight=0.07):
+    """ImagingCondition::computeDamping (bgprtm/ImagingCondition.cpp:600-660) in place on a (ny, nx, nz)
+    float32 volume; returns the damping average."""
+    ny, nx, nz = illum.shape
+    assert illum.dtype == np.float32 and illum.flags.c_contiguous
+    avg = C.c_double()
+    lib().orc_illum_damping(illum.ctypes.data_as(_pf), nx, ny, nz, int(smooth), smooth_xy, smooth_z, amp_damping,
+                            damp_weight, C.byref(avg))
+    return avg.value
+
+
+def avg_volume3d(v, sx, sy, sz):
+    """avgVolume3D (bgprtm/volumefilter.cpp:59-98) in place on a (ny, nx, nz) float32 volume."""
+    ny, nx, nz = v.shape
+    assert v.dtype == np.float32 and v.flags.c_contiguous
+    lib().orc_avg_volume3d(v.ctypes.data_as(_pf), nx, ny, nz, sx, sy, sz)
+    return v

@@ -412,3 +412,33 @@ def test_symmetry_taper_known_answers():
     a = oracle.apply_symmetry(v.copy(), iz0, -1, tpr)
     b = oracle.apply_symmetry(v.astype(np.float64), iz0, -1, tpr)
     assert np.allclose(a, b, rtol=1e-6, atol=1e-6)
+
+
+# ---- illumination compensation (computeDamping, avgVolume3D) -------------------------------------
+def test_box_mean_known_answers():
+    """avgVolume3D keeps a constant, keeps a linear function of x away from the x edges, and at the
+    first sample of a line gives ((h+1)*a0 + a1 + ... + ah)/(2h+1) (edge replication)."""
+    ny, nx, nz, h = 9, 30, 11, 3
+    c = oracle.avg_volume3d(np.full((ny, nx, nz), 2.5, np.float32), h, 2, 1)
+    assert np.allclose(c, 2.5, rtol=1e-6)
+    x = np.broadcast_to(np.arange(nx, dtype=np.float32)[None, :, None], (ny, nx, nz)).copy()
+    s = oracle.avg_volume3d(x.copy(), h, 2, 1)
+    assert np.allclose(s[:, h:nx - h, :], x[:, h:nx - h, :], rtol=1e-6, atol=1e-5)
+    edge = ((h + 1) * 0.0 + sum(range(1, h + 1))) / (2 * h + 1)
+    assert np.allclose(s[:, 0, :], edge, rtol=1e-6)
+
+
+def test_damping_known_answer_and_order():
+    """A constant illumination c: the interior weight is 1/(c*(1 + w/e)) (a = 1); a brighter cell gets
+    a smaller weight; the x/y edges are tapered to 0 at the border."""
+    ny, nx, nz = 24, 26, 18
+    v = np.full((ny, nx, nz), 3.0, np.float32)
+    avg = oracle.illum_damping(v, smooth_xy=3, smooth_z=2, damp_weight=0.07)
+    assert abs(avg - 3.0) < 1e-5
+    want = 1.0 / (3.0 * (1.0 + 0.07 * np.exp(-1.0)))
+    assert np.allclose(v[10:14, 10:16, :], want, rtol=1e-6)
+    assert not v[0].any() and not v[:, 0].any()
+    r = np.full((ny, nx, nz), 1.0, np.float32)
+    r[12, 13, 9] = 50.0
+    oracle.illum_damping(r, smooth=False)
+    assert r[12, 13, 9] < r[12, 13, 3]
