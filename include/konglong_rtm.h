@@ -296,6 +296,14 @@ int klrtm_apply_symmetry(float *wf, int nx, int ny, int nz, int iz0, int sym, in
 int klrtm_illum_damping(float *illum, int nx, int ny, int nz, int smooth, int smooth_xy, int smooth_z,
                         float amp_damping, float damp_weight, double *average_out);
 int klrtm_apply_damping(const float *illum, float *img, size_t n);
+/* ImagingCondition::imageConeMute (bgprtm/ImagingCondition.cpp:690-756) on a rectangular grid (origin
+ * x0, y0, z0, spacing dx, dy, dz; source at sx, sy, sz, all in metres): image (device, may be NULL)
+ * *= w and weight (device, may be NULL) = w, with w = 1 within the lateral radius
+ * r1 = (z - sz)*tan(angle_deg) + offset_min (ic_conemute_angle, ic_conemute_offset_min) or, when map
+ * is given, r1 linear in the nmap host (depth, offset) pairs (ic_mute_depth_offset_map); a cosine
+ * taper out to 2|r1|, 0 beyond (and wherever r1 <= 0). */
+int klrtm_cone_mute(float *image, float *weight, int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy,
+                    float dz, float sx, float sy, float sz, float angle_deg, float offset_min, const float *map, int nmap);
 /* avgVolume3D (bgprtm/volumefilter.cpp:59-98) in place: box means of half widths sx, sy, sz along
  * y, then x, then z, edge samples replicated (ny > 1) */
 int klrtm_avg_volume3d(float *v, int nx, int ny, int nz, int sx, int sy, int sz);

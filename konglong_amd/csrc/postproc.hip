@@ -1,7 +1,8 @@
 // postproc.hip -- image post-processing after the imaging condition: the
 // source-illumination compensation of ImagingCondition::computeDamping /
 // applyDamping (bgprtm/ImagingCondition.cpp:600-672), with the box smoothing
-// of avgVolume3D / avgVolume1D (bgprtm/volumefilter.cpp:59-135).
+// of avgVolume3D / avgVolume1D (bgprtm/volumefilter.cpp:59-135), and the cone
+// mute of ImagingCondition::imageConeMute (bgprtm/ImagingCondition.cpp:690-756).
 //
 // The box filter is a running sum along one axis with edge replication: one
 // lane per line keeps the sum and walks the line in the reference's order
@@ -89,6 +90,47 @@ __global__ void damping_kernel(float *__restrict__ illum, const DampArgs a) {
 __global__ void multiply_kernel(float *__restrict__ img, const float *__restrict__ w, size_t n) {
   for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     img[i] = img[i] * w[i];
+}
+
+// ImagingCondition::imageConeMute / radius_mute (bgprtm/ImagingCondition.cpp:690-756) on a rectangular
+// grid: cells farther than r1(z) laterally from the source are muted with a cosine taper out to 2|r1|;
+// r1 = (z - zs)*tan(angle) + offset_min, or linear in a (depth, offset) map
+constexpr int MUTE_MAP_MAX = 64;
+struct MuteArgs {
+  int nx, ny, nz, is_angle, nmap;
+  float x0, y0, z0, dx, dy, dz, sx, sy, sz, tangent, offset_min;
+  float mz[MUTE_MAP_MAX], mo[MUTE_MAP_MAX];
+};
+__device__ float radius_mute_dev(const MuteArgs &a, float z) {
+  if(a.is_angle) return z * a.tangent + a.offset_min;
+  const int n = a.nmap;
+  int i = 0;
+  if(z > a.mz[0])
+    for(i = 0; i < n - 1; i++)
+      if(a.mz[i + 1] >= z) break;
+  if(i > n - 2) i = n - 2;
+  const float z0 = a.mz[i], z1 = a.mz[i + 1];
+  return ((z - z0) * a.mo[i + 1] + (z1 - z) * a.mo[i]) / (z1 - z0);
+}
+__global__ void cone_mute_kernel(float *__restrict__ image, float *__restrict__ weight, const MuteArgs a) {
+  const size_t n = (size_t)a.nx * a.ny * a.nz;
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t col = i / a.nz;
+    const int iz = (int)(i - col * a.nz), ix = (int)(col % a.nx), iy = (int)(col / a.nx);
+    const float mz = (a.z0 + iz * a.dz) - a.sz;
+    const float r1 = radius_mute_dev(a, mz);
+    const float r2 = 2.0f * fabsf(r1);
+    const float my = (a.y0 + iy * a.dy) - a.sy, mx = (a.x0 + ix * a.dx) - a.sx;
+    const float rr = sqrtf(mx * mx + my * my);
+    float w = 1.0f;
+    if(rr > r1) {
+      const float d1 = rr - r1, d2 = r2 - r1;
+      if(r1 <= 0 || d1 > d2) w = 0.0f;
+      else w = 0.5f * (1.0f + cosf((float)(M_PI * (double)d1 / (double)d2)));
+    }
+    if(image) image[i] *= w;
+    if(weight) weight[i] = w;
+  }
 }
 
 static unsigned grid_of(long n) { return (unsigned)std::max(1L, std::min((n + 255) / 256, 256L * 64)); }
@@ -182,6 +224,22 @@ int klrtm_avg_volume3d(float *v, int nx, int ny, int nz, int sx, int sy, int sz)
   const int rc = box3d(v, tmp, nx, ny, nz, sx, sy, sz);
   KL_HIP(hipFreeAsync(tmp, stream()));
   return rc;
+}
+
+int klrtm_cone_mute(float *image, float *weight, int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy,
+                    float dz, float sx, float sy, float sz, float angle_deg, float offset_min, const float *map, int nmap) {
+  KL_REQUIRE(nx > 0 && ny > 0 && nz > 0, KLRTM_EINVAL, "bad image grid");
+  KL_REQUIRE(!map || (nmap >= 2 && nmap <= MUTE_MAP_MAX), KLRTM_EINVAL, "the depth-offset map needs 2..%d pairs",
+             MUTE_MAP_MAX);
+  MuteArgs a{};
+  a.nx = nx, a.ny = ny, a.nz = nz, a.is_angle = map ? 0 : 1, a.nmap = map ? nmap : 0;
+  a.x0 = x0, a.y0 = y0, a.z0 = z0, a.dx = dx, a.dy = dy, a.dz = dz, a.sx = sx, a.sy = sy, a.sz = sz;
+  a.tangent = tanf((float)(angle_deg * M_PI / 180.0f));
+  a.offset_min = offset_min;
+  for(int i = 0; map && i < nmap; i++) a.mz[i] = map[2 * i], a.mo[i] = map[2 * i + 1];
+  const size_t n = (size_t)nx * ny * nz;
+  hipLaunchKernelGGL(cone_mute_kernel, dim3(grid_of((long)n)), dim3(256), 0, stream(), image, weight, a);
+  return check_launch("cone_mute_kernel");
 }
 
 int klrtm_apply_damping(const float *illum, float *img, size_t n) {

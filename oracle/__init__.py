@@ -77,6 +77,7 @@ def lib():
         L.orc_apply_symmetry_d.argtypes = [_pd] + [C.c_int] * 7
         L.orc_illum_damping.argtypes = [_pf] + [C.c_int] * 6 + [C.c_float, C.c_float, _pd]
         L.orc_avg_volume3d.argtypes = [_pf] + [C.c_int] * 6
+        L.orc_cone_mute.argtypes = [_pf, _pf] + [C.c_int] * 3 + [C.c_float] * 11 + [_pf, C.c_int]
         L.orc_illum.argtypes = [_pf, _pf, C.c_size_t]
         L.orc_apply_tape.argtypes = [_pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, C.c_int, _pf, C.c_int]
         _lib = L
@@ -289,3 +290,15 @@ def avg_volume3d(v, sx, sy, sz):
     assert v.dtype == np.float32 and v.flags.c_contiguous
     lib().orc_avg_volume3d(v.ctypes.data_as(_pf), nx, ny, nz, sx, sy, sz)
     return v
+
+
+def cone_mute(image, origin, spacing, source, angle_deg=30.0, offset_min=0.0, mute_map=None):
+    """ImagingCondition::imageConeMute (bgprtm/ImagingCondition.cpp:690-756) on a rectangular grid: mutes
+    the (ny, nx, nz) float32 image in place and returns the weights."""
+    ny, nx, nz = image.shape
+    w = np.zeros_like(image)
+    m = None if mute_map is None else np.ascontiguousarray(mute_map, np.float32).reshape(-1, 2)
+    lib().orc_cone_mute(image.ctypes.data_as(_pf), w.ctypes.data_as(_pf), nx, ny, nz, *origin, *spacing, *source,
+                        angle_deg, offset_min, None if m is None else m.ctypes.data_as(_pf),
+                        0 if m is None else m.shape[0])
+    return w

@@ -1097,6 +1097,48 @@ static void transpose(const float *in, int n1, int n2, float *out) {  // in[i2][
   for(int i2 = 0; i2 < n2; i2++)
     for(int i1 = 0; i1 < n1; i1++) out[(size_t)i1 * n2 + i2] = in[(size_t)i2 * n1 + i1];
 }
+// ImagingCondition::radius_mute + imageConeMute (bgprtm/ImagingCondition.cpp:690-756), rectangular grid
+static float radius_mute(float z, bool isAngleMute, float tangent, float offset_min, const float *map, int n) {
+  if(isAngleMute) return z * tangent + offset_min;
+  int i = 0;
+  if(z > map[0])
+    for(i = 0; i < n - 1; i++)
+      if(map[2 * (i + 1)] >= z) break;
+  if(i > n - 2) i = n - 2;
+  float z0 = map[2 * i], z1 = map[2 * (i + 1)];
+  return ((z - z0) * map[2 * (i + 1) + 1] + (z1 - z) * map[2 * i + 1]) / (z1 - z0);
+}
+void orc_cone_mute(float *image, float *weight, int nx, int ny, int nz, float x0, float y0, float zg0, float dx, float dy,
+                   float dz, float sourceX, float sourceY, float sourceZ, float angle, float offset_min, const float *map,
+                   int nmap) {
+  bool is_angle_mute = map == nullptr;
+  float tangent = tanf(angle * M_PI / 180.0f);
+  for(int iz = 0; iz < nz; iz++) {
+    float mz = (zg0 + iz * dz) - sourceZ;
+    float r1 = radius_mute(mz, is_angle_mute, tangent, offset_min, map, nmap);
+    float r2 = 2.0f * fabsf(r1);
+    for(int iy = 0; iy < ny; iy++) {
+      float my = (y0 + iy * dy) - sourceY;
+      float yy = my * my;
+      for(int ix = 0; ix < nx; ix++) {
+        float mx = (x0 + ix * dx) - sourceX;
+        float xx = mx * mx;
+        float rr = sqrtf(xx + yy);
+        float w = 1.0f;
+        if(rr > r1) {
+          float d1 = rr - r1;
+          float d2 = r2 - r1;
+          if(r1 <= 0 || d1 > d2) w = 0.0f;
+          else w = 0.5f * (1.0f + cosf(M_PI * d1 / d2));
+        }
+        size_t idd = ((size_t)iy * nx + ix) * nz + iz;
+        if(image) image[idd] *= w;
+        if(weight) weight[idd] = w;
+      }
+    }
+  }
+}
+
 // avgVolume3D (bgprtm/volumefilter.cpp:59-98): y, then x, then z
 void orc_avg_volume3d(float *v, int nx, int ny, int nz, int sx, int sy, int sz) {
   const size_t nxz = (size_t)nx * nz;

@@ -442,3 +442,30 @@ def test_damping_known_answer_and_order():
     r[12, 13, 9] = 50.0
     oracle.illum_damping(r, smooth=False)
     assert r[12, 13, 9] < r[12, 13, 3]
+
+
+def test_cone_mute_known_answers():
+    """Inside the cone the weight is 1, beyond twice the radius 0, half way through the taper 0.5; above
+    the source (r1 <= 0) everything is muted; a depth-offset map interpolates linearly."""
+    ny, nx, nz = 1, 81, 41
+    img = np.ones((ny, nx, nz), np.float32)
+    d = 10.0
+    # source at x = 400 m, z = 0; 45 degrees: r1 = z
+    w = oracle.cone_mute(img, (0.0, 0.0, -100.0), (d, d, d), (400.0, 0.0, 0.0), angle_deg=45.0)
+    z = -100.0 + d * np.arange(nz)
+    x = d * np.arange(nx) - 400.0
+    for iz in (15, 25, 40):
+        r1 = z[iz]
+        inside = np.abs(x) <= r1 * (1 - 1e-6)
+        assert np.all(w[0, inside, iz] == 1.0)
+        beyond = np.abs(x) > 2 * r1 * (1 + 1e-6)
+        assert not w[0, beyond, iz].any()
+        mid = np.argmin(np.abs(np.abs(x) - 1.5 * r1) + (x < 0) * 1e9)
+        if abs(abs(x[mid]) - 1.5 * r1) < 1e-3:
+            assert abs(w[0, mid, iz] - 0.5) < 1e-6
+    assert not w[0, :, :10].any()                       # above the source: r1 <= 0
+    assert np.array_equal(img, w)                       # the image was multiplied by w
+    # map: offsets 0 at z = 0 and 200 at z = 200 reproduce the 45-degree cone
+    w2 = oracle.cone_mute(np.ones_like(img), (0.0, 0.0, -100.0), (d, d, d), (400.0, 0.0, 0.0),
+                          mute_map=[[0.0, 0.0], [200.0, 200.0]])
+    assert np.allclose(w2[:, :, 11:], w[:, :, 11:], atol=1e-6)

@@ -41,3 +41,24 @@ def test_illum_damping_matches_oracle(klrtm, smooth):
     di = DeviceArray.from_host(img)
     check(lib().klrtm_apply_damping(d.ptr, di.ptr, img.size))
     assert np.array_equal(di.download(), (img * got).astype(np.float32))
+
+
+@pytest.mark.parametrize("use_map", [False, True])
+def test_cone_mute_matches_oracle(klrtm, use_map):
+    shape = (21, 33, 40)
+    ny, nx, nz = shape
+    img = smooth_field(shape, -1.0, 1.0, 51)
+    origin, spacing, src = (-50.0, 20.0, -30.0), (12.5, 10.0, 7.5), (140.0, 110.0, 5.0)
+    mmap = np.array([[0.0, 10.0], [100.0, 60.0], [250.0, 90.0]], np.float32) if use_map else None
+    want = img.copy()
+    ww = oracle.cone_mute(want, origin, spacing, src, angle_deg=35.0, offset_min=5.0, mute_map=mmap)
+    d, dw = DeviceArray.from_host(img), DeviceArray(shape)
+    mp = None if mmap is None else mmap.ctypes.data_as(__import__("ctypes").POINTER(__import__("ctypes").c_float))
+    check(lib().klrtm_cone_mute(d.ptr, dw.ptr, nx, ny, nz, *origin, *spacing, *src, 35.0, 5.0, mp,
+                                0 if mmap is None else mmap.shape[0]))
+    gw = dw.download()
+    print(f"[postproc] cone mute map={use_map}: muted {np.mean(ww == 0):.2f}, tapered {np.mean((ww > 0) & (ww < 1)):.2f}",
+          flush=True)
+    assert np.allclose(gw, ww, rtol=0, atol=2e-6)
+    assert np.allclose(d.download(), want, rtol=0, atol=2e-6)
+    assert 0 < np.mean(ww == 0) < 1 and np.mean((ww > 0) & (ww < 1)) > 0
