@@ -304,6 +304,13 @@ int klrtm_apply_damping(const float *illum, float *img, size_t n);
  * taper out to 2|r1|, 0 beyond (and wherever r1 <= 0). */
 int klrtm_cone_mute(float *image, float *weight, int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy,
                     float dz, float sx, float sy, float sz, float angle_deg, float offset_min, const float *map, int nmap);
+/* Lagrange::apply (bgprtm/Lagrange.cpp:19-266): separable Lagrange interpolation with npoints (even)
+ * samples from the rectangular input grid (nzi, nxi, nyi; origins and spacings) to the output grid,
+ * along z, then x (when the output is 2D or 3D), then y (when either grid is 3D); in and out are
+ * device volumes [ny][nx][nz].  Output samples before the input range take the first input sample. */
+int klrtm_lagrange_resample(const float *in, int nzi, int nxi, int nyi, float z0i, float dzi, float x0i, float dxi,
+                            float y0i, float dyi, float *out, int nzo, int nxo, int nyo, float z0o, float dzo, float x0o,
+                            float dxo, float y0o, float dyo, int npoints);
 /* avgVolume3D (bgprtm/volumefilter.cpp:59-98) in place: box means of half widths sx, sy, sz along
  * y, then x, then z, edge samples replicated (ny > 1) */
 int klrtm_avg_volume3d(float *v, int nx, int ny, int nz, int sx, int sy, int sz);

@@ -1,8 +1,10 @@
 // postproc.hip -- image post-processing after the imaging condition: the
 // source-illumination compensation of ImagingCondition::computeDamping /
 // applyDamping (bgprtm/ImagingCondition.cpp:600-672), with the box smoothing
-// of avgVolume3D / avgVolume1D (bgprtm/volumefilter.cpp:59-135), and the cone
-// mute of ImagingCondition::imageConeMute (bgprtm/ImagingCondition.cpp:690-756).
+// of avgVolume3D / avgVolume1D (bgprtm/volumefilter.cpp:59-135), the cone
+// mute of ImagingCondition::imageConeMute (bgprtm/ImagingCondition.cpp:690-756)
+// and the Lagrange resampling of the image to the output grid
+// (bgprtm/Lagrange.cpp:19-266).
 //
 // The box filter is a running sum along one axis with edge replication: one
 // lane per line keeps the sum and walks the line in the reference's order
@@ -133,6 +135,40 @@ __global__ void cone_mute_kernel(float *__restrict__ image, float *__restrict__ 
   }
 }
 
+// Lagrange::interpz/x/y (bgprtm/Lagrange.cpp:190-266): out sample = sum_i in[nbeg + i] * coef[i] along one
+// axis, i ascending, in FP32 (as applyCoef).  AXIS 0: z (n_in -> n_out along the contiguous axis,
+// lines over nx*ny); 1: x (lines over ny, nz); 2: y (lines over nx, nz).
+template<int AXIS>
+__global__ void lagrange_kernel(const float *__restrict__ in, float *__restrict__ out, int nin, int nout, int na, int nb,
+                                const float *__restrict__ coef, const int *__restrict__ norder,
+                                const int *__restrict__ nbeg, int np) {
+  // output layout: AXIS 0: [na*nb][nout]; 1: [nb = ny][nout][na = nz]; 2: [nout][nb = nx][na = nz]
+  const size_t total = (size_t)nout * na * nb;
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    int o;
+    size_t ibase, s, obase;
+    if(AXIS == 0) {
+      const size_t line = e / nout;
+      o = (int)(e - line * nout);
+      ibase = line * nin, s = 1, obase = e;
+    } else if(AXIS == 1) {
+      const int iz = (int)(e % na);
+      const size_t t = e / na;
+      o = (int)(t % nout);
+      const int iy = (int)(t / nout);
+      ibase = (size_t)iy * nin * na + iz, s = (size_t)na, obase = e;
+    } else {
+      const size_t xz = e % ((size_t)na * nb);
+      o = (int)(e / ((size_t)na * nb));
+      ibase = xz, s = (size_t)na * nb, obase = e;
+    }
+    const int nb0 = nbeg[o], no = norder[o];
+    float y = 0.0f;
+    for(int i = 0; i < no; i++) y += in[ibase + (size_t)(nb0 + i) * s] * coef[(size_t)o * np + i];
+    out[obase] = y;
+  }
+}
+
 static unsigned grid_of(long n) { return (unsigned)std::max(1L, std::min((n + 255) / 256, 256L * 64)); }
 
 // avgVolume3D: y, then x, then z (bgprtm/volumefilter.cpp:59-98)
@@ -240,6 +276,110 @@ int klrtm_cone_mute(float *image, float *weight, int nx, int ny, int nz, float x
   const size_t n = (size_t)nx * ny * nz;
   hipLaunchKernelGGL(cone_mute_kernel, dim3(grid_of((long)n)), dim3(256), 0, stream(), image, weight, a);
   return check_launch("cone_mute_kernel");
+}
+
+int klrtm_lagrange_resample(const float *in, int nzi, int nxi, int nyi, float z0i, float dzi, float x0i, float dxi,
+                            float y0i, float dyi, float *out, int nzo, int nxo, int nyo, float z0o, float dzo, float x0o,
+                            float dxo, float y0o, float dyo, int npoints) {
+  KL_REQUIRE(in && out && nzi > 1 && nxi > 0 && nyi > 0 && nzo > 0 && nxo > 0 && nyo > 0, KLRTM_EINVAL, "bad grids");
+  KL_REQUIRE(npoints >= 2 && npoints % 2 == 0 && npoints <= 16, KLRTM_EINVAL, "npoints must be even, 2..16");
+  const bool do_y = nyo > 1 || nyi > 1, do_x = do_y || nxo > 1;
+  KL_REQUIRE(!do_x || nxi > 1, KLRTM_EINVAL, "resampling along x needs nx_in > 1");
+  KL_REQUIRE(!do_y || nyi > 1, KLRTM_EINVAL, "resampling along y needs ny_in > 1");
+  KL_REQUIRE(do_x || nxo == nxi, KLRTM_EINVAL, "without x resampling nx_out must equal nx_in");
+  KL_REQUIRE(do_y || nyo == nyi, KLRTM_EINVAL, "without y resampling ny_out must equal ny_in");
+  struct Axis {
+    std::vector<float> coef;
+    std::vector<int> norder, nbeg;
+  };
+  // Lagrange::initCoef / checkGrid / getCoef (bgprtm/Lagrange.cpp:121-175); an output sample before the
+  // input range takes the first input sample (the reference reads index -1 there), and the stencil is
+  // kept inside grids shorter than npoints
+  auto init = [&](int nin, float g0i, float gdi, int nout, float g0o, float gdo, Axis &A) {
+    std::vector<float> tin(nin), tout(nout);
+    for(int i = 0; i < nin; i++) tin[i] = g0i + i * gdi;
+    for(int i = 0; i < nout; i++) tout[i] = g0o + i * gdo;
+    A.coef.assign((size_t)nout * npoints, 0.f), A.norder.resize(nout), A.nbeg.resize(nout);
+    const int mintp = npoints / 2;
+    for(int o = 0; o < nout; o++) {
+      const float z = tout[o];
+      int iz0;
+      if(z < tin[0]) iz0 = -1;
+      else if(z > tin[nin - 1]) iz0 = nin;
+      else {
+        iz0 = 0;
+        while(z > tin[iz0 + 1] && iz0 <= nin - 1) iz0++;
+      }
+      int no, nb;
+      if(iz0 == -1) no = 1, nb = 0;
+      else if(iz0 == nin) no = 1, nb = nin - 1;
+      else if(iz0 < mintp) no = (iz0 + 1) * 2, nb = 0;
+      else if(nin - 1 - iz0 < mintp) no = (nin - 1 - iz0) * 2, nb = nin - no;
+      else no = npoints, nb = iz0 - mintp;
+      nb = std::min(std::max(nb, 0), nin - 1), no = std::max(1, std::min(no, nin - nb));  // stay inside tiny grids
+      float *c = &A.coef[(size_t)o * npoints];
+      for(int i = 0; i < no; i++) {
+        c[i] = 1.0f;
+        for(int j = 0; j < no; j++)
+          if(j != i) c[i] = c[i] * (z - tin[j + nb]) / (tin[i + nb] - tin[j + nb]);
+      }
+      A.norder[o] = no, A.nbeg[o] = nb;
+    }
+  };
+  Axis az, ax, ay;
+  init(nzi, z0i, dzi, nzo, z0o, dzo, az);
+  if(do_x) init(nxi, x0i, dxi, nxo, x0o, dxo, ax);
+  if(do_y) init(nyi, y0i, dyi, nyo, y0o, dyo, ay);
+  std::vector<void *> allocs;
+  auto upload = [&](const void *h, size_t bytes) -> void * {
+    void *d = nullptr;
+    if(hipMallocAsync(&d, std::max<size_t>(bytes, 4), stream()) != hipSuccess) return nullptr;
+    allocs.push_back(d);
+    if(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream()) != hipSuccess) return nullptr;
+    return d;
+  };
+  int rc = KLRTM_OK;
+  auto fail = [&](const char *what) {
+    set_error("resampling: %s failed", what);
+    rc = KLRTM_ENOMEM;
+  };
+  const size_t nwz = (size_t)nzo * nxi * nyi, nwx = (size_t)nzo * nxo * nyi;
+  float *bz = nullptr, *bx = nullptr;
+  if(do_x && hipMallocAsync(&bz, nwz * sizeof(float), stream()) == hipSuccess) allocs.push_back(bz);
+  if(do_y && hipMallocAsync(&bx, nwx * sizeof(float), stream()) == hipSuccess) allocs.push_back(bx);
+  if((do_x && !bz) || (do_y && !bx)) fail("work allocation");
+  const float *cz = rc ? nullptr : (const float *)upload(az.coef.data(), az.coef.size() * 4);
+  const int *oz = rc ? nullptr : (const int *)upload(az.norder.data(), az.norder.size() * 4);
+  const int *gz = rc ? nullptr : (const int *)upload(az.nbeg.data(), az.nbeg.size() * 4);
+  if(!rc && (!cz || !oz || !gz)) fail("coefficient upload");
+  LaunchTimer tm("lagrange");
+  if(!rc) {
+    float *zout = do_x ? bz : out;
+    hipLaunchKernelGGL(lagrange_kernel<0>, dim3(grid_of((long)nzo * nxi * nyi)), dim3(256), 0, stream(), in, zout, nzi,
+                       nzo, nxi, nyi, cz, oz, gz, npoints);
+  }
+  if(!rc && do_x) {
+    const float *cx = (const float *)upload(ax.coef.data(), ax.coef.size() * 4);
+    const int *ox = (const int *)upload(ax.norder.data(), ax.norder.size() * 4);
+    const int *gx = (const int *)upload(ax.nbeg.data(), ax.nbeg.size() * 4);
+    if(!cx || !ox || !gx) fail("coefficient upload");
+    else
+      hipLaunchKernelGGL(lagrange_kernel<1>, dim3(grid_of((long)nxo * nzo * nyi)), dim3(256), 0, stream(), bz,
+                         do_y ? bx : out, nxi, nxo, nzo, nyi, cx, ox, gx, npoints);
+  }
+  if(!rc && do_y) {
+    const float *cy = (const float *)upload(ay.coef.data(), ay.coef.size() * 4);
+    const int *oy = (const int *)upload(ay.norder.data(), ay.norder.size() * 4);
+    const int *gy = (const int *)upload(ay.nbeg.data(), ay.nbeg.size() * 4);
+    if(!cy || !oy || !gy) fail("coefficient upload");
+    else
+      hipLaunchKernelGGL(lagrange_kernel<2>, dim3(grid_of((long)nyo * nxo * nzo)), dim3(256), 0, stream(), bx, out, nyi,
+                         nyo, nzo, nxo, cy, oy, gy, npoints);
+  }
+  if(!rc) rc = check_launch("lagrange_kernel");
+  KL_HIP(hipStreamSynchronize(stream()));  // the host coefficient tables must outlive their copies
+  for(void *d : allocs) (void)hipFreeAsync(d, stream());
+  return rc;
 }
 
 int klrtm_apply_damping(const float *illum, float *img, size_t n) {

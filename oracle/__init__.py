@@ -78,6 +78,8 @@ def lib():
         L.orc_illum_damping.argtypes = [_pf] + [C.c_int] * 6 + [C.c_float, C.c_float, _pd]
         L.orc_avg_volume3d.argtypes = [_pf] + [C.c_int] * 6
         L.orc_cone_mute.argtypes = [_pf, _pf] + [C.c_int] * 3 + [C.c_float] * 11 + [_pf, C.c_int]
+        L.orc_lagrange_resample.argtypes = [_pf] + [C.c_int] * 3 + [C.c_float] * 6 + [_pf] + [C.c_int] * 3 + \
+            [C.c_float] * 6 + [C.c_int]
         L.orc_illum.argtypes = [_pf, _pf, C.c_size_t]
         L.orc_apply_tape.argtypes = [_pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, C.c_int, _pf, C.c_int]
         _lib = L
@@ -302,3 +304,15 @@ def cone_mute(image, origin, spacing, source, angle_deg=30.0, offset_min=0.0, mu
                         angle_deg, offset_min, None if m is None else m.ctypes.data_as(_pf),
                         0 if m is None else m.shape[0])
     return w
+
+
+def lagrange_resample(vol, grid_in, shape_out, grid_out, npoints=8):
+    """Lagrange::apply (bgprtm/Lagrange.cpp:19-266): vol (ny, nx, nz) on grid_in = (z0, dz, x0, dx, y0, dy)
+    resampled to shape_out (ny, nx, nz) on grid_out."""
+    nyi, nxi, nzi = vol.shape
+    nyo, nxo, nzo = shape_out
+    v = np.ascontiguousarray(vol, np.float32)
+    out = np.zeros(shape_out, np.float32)
+    lib().orc_lagrange_resample(v.ctypes.data_as(_pf), nzi, nxi, nyi, *grid_in, out.ctypes.data_as(_pf), nzo, nxo, nyo,
+                                *grid_out, npoints)
+    return out

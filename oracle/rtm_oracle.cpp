@@ -1139,6 +1139,72 @@ void orc_cone_mute(float *image, float *weight, int nx, int ny, int nz, float x0
   }
 }
 
+// Lagrange (bgprtm/Lagrange.cpp:121-266): coefficients per output sample, then z, x, y passes
+struct LagAxis {
+  std::vector<float> coef;
+  std::vector<int> norder, nbeg;
+};
+static LagAxis lag_init(int nin, float g0i, float gdi, int nout, float g0o, float gdo, int nPoints) {
+  LagAxis A;
+  std::vector<float> tblIn(nin), tblOut(nout);
+  for(int i = 0; i < nin; i++) tblIn[i] = g0i + i * gdi;
+  for(int i = 0; i < nout; i++) tblOut[i] = g0o + i * gdo;
+  A.coef.assign((size_t)nout * nPoints, 0.f), A.norder.resize(nout), A.nbeg.resize(nout);
+  int mintp = nPoints / 2;
+  for(int iz = 0; iz < nout; ++iz) {
+    float z = tblOut[iz];
+    int iz0;
+    if(z < tblIn[0]) iz0 = -1;
+    else if(z > tblIn[nin - 1]) iz0 = nin;
+    else {
+      iz0 = 0;
+      while(z > tblIn[iz0 + 1] && iz0 <= nin - 1) iz0++;
+    }
+    int no, nb;
+    if(iz0 == -1) no = 1, nb = 0;  // the reference's nbeg -1 would read before the line
+    else if(iz0 == nin) no = 1, nb = nin - 1;
+    else if(iz0 < mintp) no = (iz0 + 1) * 2, nb = 0;
+    else if(nin - 1 - iz0 < mintp) no = (nin - 1 - iz0) * 2, nb = nin - no;
+    else no = nPoints, nb = iz0 - mintp;
+    nb = std::min(std::max(nb, 0), nin - 1), no = std::max(1, std::min(no, nin - nb));
+    float *c = &A.coef[(size_t)iz * nPoints];
+    for(int i = 0; i < no; ++i) {
+      c[i] = 1.0;
+      for(int j = 0; j < no; ++j)
+        if(j != i) c[i] = c[i] * (z - tblIn[j + nb]) / (tblIn[i + nb] - tblIn[j + nb]);
+    }
+    A.norder[iz] = no, A.nbeg[iz] = nb;
+  }
+  return A;
+}
+static float lag_apply(const LagAxis &A, int o, int np, const float *data, size_t stride) {
+  float yval = 0.0;
+  for(int i = 0; i < A.norder[o]; ++i) yval += data[(size_t)(A.nbeg[o] + i) * stride] * A.coef[(size_t)o * np + i];
+  return yval;
+}
+void orc_lagrange_resample(const float *in, int nzi, int nxi, int nyi, float z0i, float dzi, float x0i, float dxi,
+                           float y0i, float dyi, float *out, int nzo, int nxo, int nyo, float z0o, float dzo, float x0o,
+                           float dxo, float y0o, float dyo, int np) {
+  bool do_y = nyo > 1 || nyi > 1, do_x = do_y || nxo > 1;
+  LagAxis az = lag_init(nzi, z0i, dzi, nzo, z0o, dzo, np);
+  std::vector<float> wz((size_t)nzo * nxi * nyi), wx((size_t)nzo * nxo * nyi);
+  float *zo = do_x ? wz.data() : out;
+  for(size_t l = 0; l < (size_t)nxi * nyi; l++)
+    for(int o = 0; o < nzo; o++) zo[l * nzo + o] = lag_apply(az, o, np, in + l * nzi, 1);
+  if(!do_x) return;
+  LagAxis ax = lag_init(nxi, x0i, dxi, nxo, x0o, dxo, np);
+  float *xo = do_y ? wx.data() : out;
+  for(int iy = 0; iy < nyi; iy++)
+    for(int o = 0; o < nxo; o++)
+      for(int iz = 0; iz < nzo; iz++)
+        xo[((size_t)iy * nxo + o) * nzo + iz] = lag_apply(ax, o, np, wz.data() + (size_t)iy * nxi * nzo + iz, nzo);
+  if(!do_y) return;
+  LagAxis ay = lag_init(nyi, y0i, dyi, nyo, y0o, dyo, np);
+  for(int o = 0; o < nyo; o++)
+    for(size_t xz = 0; xz < (size_t)nxo * nzo; xz++)
+      out[(size_t)o * nxo * nzo + xz] = lag_apply(ay, o, np, wx.data() + xz, (size_t)nxo * nzo);
+}
+
 // avgVolume3D (bgprtm/volumefilter.cpp:59-98): y, then x, then z
 void orc_avg_volume3d(float *v, int nx, int ny, int nz, int sx, int sy, int sz) {
   const size_t nxz = (size_t)nx * nz;

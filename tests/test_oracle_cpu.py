@@ -469,3 +469,19 @@ def test_cone_mute_known_answers():
     w2 = oracle.cone_mute(np.ones_like(img), (0.0, 0.0, -100.0), (d, d, d), (400.0, 0.0, 0.0),
                           mute_map=[[0.0, 0.0], [200.0, 200.0]])
     assert np.allclose(w2[:, :, 11:], w[:, :, 11:], atol=1e-6)
+
+
+def test_lagrange_resample_reproduces_polynomials():
+    """8-point Lagrange interpolation is exact for a cubic (in each axis) away from the grid ends,
+    where the full stencil fits; at the input samples themselves it returns the samples."""
+    ny, nx, nz = 12, 14, 20
+    z, x, y = 10.0 * np.arange(nz), 12.0 * np.arange(nx), 9.0 * np.arange(ny)
+    f = lambda Y, X, Z: 1e-6 * Z ** 3 - 2e-4 * X * X + 0.5 * Y + 3.0
+    v = f(y[:, None, None], x[None, :, None], z[None, None, :]).astype(np.float32)
+    zo, xo, yo = 3.0 + 7.0 * np.arange(24), 5.0 + 9.0 * np.arange(16), 4.0 + 6.0 * np.arange(15)
+    out = oracle.lagrange_resample(v, (0.0, 10.0, 0.0, 12.0, 0.0, 9.0), (15, 16, 24), (3.0, 7.0, 5.0, 9.0, 4.0, 6.0))
+    want = f(yo[:, None, None], xo[None, :, None], zo[None, None, :])
+    iz, ix, iy = (zo >= 40) & (zo <= 150), (xo >= 48) & (xo <= 96), (yo >= 36) & (yo <= 63)
+    assert np.abs(out - want)[np.ix_(iy, ix, iz)].max() <= 1e-6 * np.abs(want).max()
+    same = oracle.lagrange_resample(v, (0.0, 10.0, 0.0, 12.0, 0.0, 9.0), v.shape, (0.0, 10.0, 0.0, 12.0, 0.0, 9.0))
+    assert np.allclose(same, v, rtol=1e-6)

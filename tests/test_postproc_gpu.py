@@ -62,3 +62,24 @@ def test_cone_mute_matches_oracle(klrtm, use_map):
     assert np.allclose(gw, ww, rtol=0, atol=2e-6)
     assert np.allclose(d.download(), want, rtol=0, atol=2e-6)
     assert 0 < np.mean(ww == 0) < 1 and np.mean((ww > 0) & (ww < 1)) > 0
+
+
+@pytest.mark.parametrize("case", ["3d", "2d", "z-only"])
+def test_lagrange_resample_bit_exact(klrtm, case):
+    rng = np.random.default_rng(7)
+    if case == "3d":
+        shp_i, g_i = (12, 14, 20), (0.0, 10.0, 0.0, 12.0, 0.0, 9.0)
+        shp_o, g_o = (15, 16, 24), (3.0, 7.0, 5.0, 9.0, 4.0, 6.0)
+    elif case == "2d":
+        shp_i, g_i = (1, 30, 25), (0.0, 8.0, 100.0, 10.0, 0.0, 1.0)
+        shp_o, g_o = (1, 41, 33), (-4.0, 6.0, 95.0, 7.5, 0.0, 1.0)
+    else:
+        shp_i, g_i = (3, 4, 50), (0.0, 5.0, 0.0, 1.0, 0.0, 1.0)
+        shp_o, g_o = (3, 4, 80), (1.0, 3.1, 0.0, 1.0, 0.0, 1.0)
+    v = rng.standard_normal(shp_i).astype(np.float32)
+    want = oracle.lagrange_resample(v, g_i, shp_o, g_o)
+    d, o = DeviceArray.from_host(v), DeviceArray(shp_o)
+    check(lib().klrtm_lagrange_resample(d.ptr, shp_i[2], shp_i[1], shp_i[0], *g_i, o.ptr, shp_o[2], shp_o[1], shp_o[0],
+                                        *g_o, 8))
+    print(f"[postproc] lagrange {case} {shp_i} -> {shp_o}", flush=True)
+    assert np.array_equal(o.download().view(np.uint32), want.view(np.uint32))
