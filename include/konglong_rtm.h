@@ -320,6 +320,12 @@ int klrtm_avg_volume3d(float *v, int nx, int ny, int nz, int sx, int sy, int sz)
  * (x0, y0, z0) of the nxF*nyF*nzF output grid; img is FP32 (is_double 0) or FP64 (is_double 1). */
 int klrtm_stack_image(const void *img, int is_double, int nx, int ny, int nz, double *full, int nxF, int nyF, int nzF,
                       int x0, int y0, int z0);
+/* ImagingCondition::saveImage to a destination grid (bgprtm/ImagingCondition.cpp:874-945): the shot image
+ * (nz, nx, ny; first crossline/inline numbers ix0s, iy0s, increments incx, incy) is added (stack != 0,
+ * do_stack) to or copied into dest (nz, nxd, nyd from ix0d, iy0d, the same increments) on the
+ * overlapping lines.  dest holds the whole output grid when stacking, else the overlap window only. */
+int klrtm_save_image(const float *img, int nz, int nx, int ny, int ix0s, int iy0s, int incx, int incy, float *dest,
+                     int ix0d, int iy0d, int nxd, int nyd, int stack);
 
 #ifdef __cplusplus
 }

@@ -121,6 +121,7 @@ _SIGS = {
     "klrtm_apply_damping": (_i, [_vp, _vp, _sz]),
     "klrtm_avg_volume3d": (_i, [_vp, _i, _i, _i, _i, _i, _i]),
     "klrtm_cone_mute": (_i, [_vp, _vp, _i, _i, _i] + [_f] * 11 + [_pf, _i]),
+    "klrtm_save_image": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _i, _i, _i, _i, _i]),
     "klrtm_lagrange_resample": (_i, [_vp, _i, _i, _i] + [_f] * 6 + [_vp, _i, _i, _i] + [_f] * 6 + [_i]),
     "klrtm_stream_create": (_i, [C.POINTER(_vp)]),
     "klrtm_stream_destroy": (_i, [_vp]),

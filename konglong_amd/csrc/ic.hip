@@ -115,9 +115,49 @@ __global__ void stack_kernel(const T *__restrict__ img, int nx, int ny, int nz, 
   }
 }
 
+// the inline/crossline window copy of ImagingCondition::saveImage (bgprtm/ImagingCondition.cpp:874-945):
+// rows il = iy0 .. iy1 (step incy), columns xl = ix0 .. ix1 (step incx) of the shot image go to the
+// destination rows and columns, added (stack) or copied
+__global__ void save_window_kernel(const float *__restrict__ in, float *__restrict__ out, int nz, int nyw, int nxw,
+                                   size_t in_row, size_t in_col0, size_t out_row, size_t out_col0, int stack) {
+  const size_t total = (size_t)nyw * nxw * nz;
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(e % nz);
+    const size_t t = e / nz;
+    const int jx = (int)(t % nxw), jy = (int)(t / nxw);
+    const size_t i = ((size_t)jy * in_row + in_col0 + jx) * nz + iz, o = ((size_t)jy * out_row + out_col0 + jx) * nz + iz;
+    out[o] = stack ? out[o] + in[i] : in[i];
+  }
+}
+
 }  // namespace klrtm
 
 using namespace klrtm;
+
+extern "C" int klrtm_save_image(const float *img, int nz, int nx, int ny, int ix0s, int iy0s, int incx, int incy,
+                                float *dest, int ix0d, int iy0d, int nxd, int nyd, int stack) {
+  KL_REQUIRE(img && dest && nz > 0 && nx > 0 && ny > 0 && incx > 0 && incy > 0, KLRTM_EINVAL, "bad image");
+  const int ix1s = ix0s + (nx - 1) * incx, iy1s = iy0s + (ny - 1) * incy;
+  const int ix1d = ix0d + (nxd - 1) * incx, iy1d = iy0d + (nyd - 1) * incy;
+  const int ix0 = std::max(ix0s, ix0d), iy0 = std::max(iy0s, iy0d), ix1 = std::min(ix1s, ix1d), iy1 = std::min(iy1s, iy1d);
+  const int nyp = std::max(0, (iy1 - iy0) / incy + 1), nxp = std::max(0, (ix1 - ix0) / incx + 1);
+  KL_REQUIRE((iy0s - iy0d) % incy == 0 && (ix0s - ix0d) % incx == 0, KLRTM_EINVAL,
+             "input lines (%d, %d) and output lines (%d, %d) do not align on the increments (%d, %d)", iy0s, ix0s, iy0d,
+             ix0d, incy, incx);
+  KL_REQUIRE(!(nx == 1 && nxp == 0), KLRTM_EINVAL, "the crossline range does not meet the output grid");
+  if(iy1 < iy0 || ix1 < ix0) return KLRTM_OK;  // no overlap
+  const int iy0off = stack ? iy0d : iy0;
+  const int ixoff = stack ? (ix0 - ix0d) / incx : 0;
+  const int nxoff = stack ? nxd : nxp;
+  const size_t total = (size_t)nyp * nxp * nz;
+  const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 256 * 64));
+  // first row and column of the window in the input and in the output (rows of nx / nxoff columns)
+  const float *in = img + (size_t)((iy0 - iy0s) / incy) * nx * nz;
+  float *out = dest + (size_t)((iy0 - iy0off) / incy) * nxoff * nz;
+  hipLaunchKernelGGL(save_window_kernel, dim3(nb), dim3(256), 0, stream(), in, out, nz, nyp, nxp, (size_t)nx,
+                     (size_t)((ix0 - ix0s) / incx), (size_t)nxoff, (size_t)ixoff, stack);
+  return check_launch("save_window_kernel");
+}
 
 extern "C" int klrtm_stack_image(const void *img, int is_double, int nx, int ny, int nz, double *full, int nxF, int nyF,
                                  int nzF, int x0, int y0, int z0) {

@@ -316,3 +316,29 @@ def lagrange_resample(vol, grid_in, shape_out, grid_out, npoints=8):
     lib().orc_lagrange_resample(v.ctypes.data_as(_pf), nzi, nxi, nyi, *grid_in, out.ctypes.data_as(_pf), nzo, nxo, nyo,
                                 *grid_out, npoints)
     return out
+
+
+def save_image(regimg, ix0s, iy0s, incx, incy, dest, ix0d, iy0d, nxd, nyd, do_stack=True):
+    """ImagingCondition::saveImage to a destination grid (bgprtm/ImagingCondition.cpp:874-945), the
+    reference loop over inlines and crosslines: regimg (ny, nx, nz) float32 into dest, the whole output
+    grid (nyd, nxd, nz) when stacking, else the overlap window only; in place."""
+    ny, nx, nz = regimg.shape
+    ix1s, iy1s = ix0s + (nx - 1) * incx, iy0s + (ny - 1) * incy
+    ix1d, iy1d = ix0d + (nxd - 1) * incx, iy0d + (nyd - 1) * incy
+    ix0, iy0 = max(ix0s, ix0d), max(iy0s, iy0d)
+    ix1, iy1 = min(ix1s, ix1d), min(iy1s, iy1d)
+    nxpartial = max(0, (ix1 - ix0) // incx + 1)
+    iy0off = iy0d if do_stack else iy0
+    ixoff = (ix0 - ix0d) // incx if do_stack else 0
+    nxoff = nxd if do_stack else nxpartial
+    src = regimg.reshape(-1, nz)
+    dst = dest.reshape(-1, nz)
+    for il in range(iy0, iy1 + 1, incy):
+        i_in = (il - iy0s) // incy * nx + (ix0 - ix0s) // incx
+        i_out = (il - iy0off) // incy * nxoff + ixoff
+        for k, xl in enumerate(range(ix0, ix1 + 1, incx)):
+            if do_stack:
+                dst[i_out + k] += src[i_in + k]
+            else:
+                dst[i_out + k] = src[i_in + k]
+    return dest

@@ -485,3 +485,16 @@ def test_lagrange_resample_reproduces_polynomials():
     assert np.abs(out - want)[np.ix_(iy, ix, iz)].max() <= 1e-6 * np.abs(want).max()
     same = oracle.lagrange_resample(v, (0.0, 10.0, 0.0, 12.0, 0.0, 9.0), v.shape, (0.0, 10.0, 0.0, 12.0, 0.0, 9.0))
     assert np.allclose(same, v, rtol=1e-6)
+
+
+def test_save_image_window():
+    """Stacking twice doubles the overlap, leaves the rest of the output grid alone, and lands each
+    inline/crossline where its number says."""
+    nz, nx, ny = 5, 6, 4
+    img = np.arange(ny * nx * nz, dtype=np.float32).reshape(ny, nx, nz) + 1
+    dest = np.zeros((5, 8, nz), np.float32)
+    # image lines: xl 10, 12, ..., 20; il 100, 101, ...; output xl 6, 8, ..., 20; il 99..103
+    oracle.save_image(img, 10, 100, 2, 1, dest, 6, 99, 8, 5, True)
+    oracle.save_image(img, 10, 100, 2, 1, dest, 6, 99, 8, 5, True)
+    assert np.array_equal(dest[1:5, 2:8], 2 * img)
+    assert not dest[0].any() and not dest[:, :2].any()

@@ -83,3 +83,28 @@ def test_lagrange_resample_bit_exact(klrtm, case):
                                         *g_o, 8))
     print(f"[postproc] lagrange {case} {shp_i} -> {shp_o}", flush=True)
     assert np.array_equal(o.download().view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("stack", [1, 0])
+def test_save_image_matches_oracle(klrtm, stack):
+    """saveImage onto an inline/crossline output grid: offsets, increments and partial overlap."""
+    rng = np.random.default_rng(11)
+    nz, nx, ny = 17, 23, 19
+    img = rng.standard_normal((ny, nx, nz)).astype(np.float32)
+    ix0s, iy0s, incx, incy = 105, 211, 2, 3       # shot image lines
+    ix0d, iy0d, nxd, nyd = 99, 220, 30, 14          # output grid lines (aligned on the increments)
+    if stack:
+        dest0 = rng.standard_normal((nyd, nxd, nz)).astype(np.float32)
+    else:   # the overlap window only
+        iy0, iy1 = max(iy0s, iy0d), min(iy0s + (ny - 1) * incy, iy0d + (nyd - 1) * incy)
+        ix0, ix1 = max(ix0s, ix0d), min(ix0s + (nx - 1) * incx, ix0d + (nxd - 1) * incx)
+        dest0 = np.zeros(((iy1 - iy0) // incy + 1, (ix1 - ix0) // incx + 1, nz), np.float32)
+    want = oracle.save_image(img, ix0s, iy0s, incx, incy, dest0.copy(), ix0d, iy0d, nxd, nyd, bool(stack))
+    dimg, ddest = DeviceArray.from_host(img), DeviceArray.from_host(dest0)
+    check(lib().klrtm_save_image(dimg.ptr, nz, nx, ny, ix0s, iy0s, incx, incy, ddest.ptr, ix0d, iy0d, nxd, nyd, stack))
+    got = ddest.download()
+    assert np.array_equal(got, want)
+    assert not np.array_equal(want, dest0)
+    # misaligned lines are rejected
+    assert lib().klrtm_save_image(dimg.ptr, nz, nx, ny, ix0s + 1, iy0s, incx, incy, ddest.ptr, ix0d, iy0d, nxd, nyd,
+                                  stack) != 0
