@@ -169,12 +169,62 @@ def batched_vs_plain(nshots, nt, batch, seed=505):
             "tiles": {f"{k[0]}x{k[1]}": list(v[0]) if v[0] else None for k, v in tuner.cache.items()}}
 
 
+def tti_full_shot(nt=3000, seed=303):
+    """One whole TTI FD16 RTM shot on the 600 x 600 x 400 grid (dx 12.5 m, PML 40, 15 Hz, snapshots every
+    3 steps), with learned snapshots held in HBM for the whole shot (163 GB for the 1000 snapshots; FP16
+    with block scales would need 289 GB, FP32 577 GB).  Returns the shot's timings."""
+    from konglong_amd import models, learned
+    from konglong_amd._lib import ENGINE_FD, TTI, SNAP_LEARNED, lib, check
+    from konglong_amd.propagator import Propagator, shot_config
+    shape = (600, 600, 400)
+    ny, nx, nz = shape
+    d, nb, fp, k = 12.5, 40, 15.0, 3
+    t0 = time.perf_counter()
+    vel, eps, dl = models.smooth_vti_model(shape, 20.0, seed)
+    pjx, pjy = models.tilt_axes(shape, 20.0, seed + 10)
+    dt = models.cfl_dt(vel, d, ENGINE_FD, 16, TTI, eps, dl)
+    t_model = time.perf_counter() - t0
+    codec, info = learned.train((256, 256, 256), [4011, 4012], d, dt, 300, k, nl=32, nbnd=nb, f_peak=fp,
+                                vmin=1500.0, vmax=5000.0, sigma=20.0)
+    rcv = models.surface_receivers(nx, ny, nb + 2, step=4)
+    keep = (rcv[:, 0] >= nb) & (rcv[:, 0] < nx - nb) & (rcv[:, 1] >= nb) & (rcv[:, 1] < ny - nb)
+    rcv = rcv[keep]
+    common = dict(nbnd=nb, vmax=float(vel.max()), src_xyz=((nx - 1) / 2, (ny - 1) / 2, nb + 2.0), f_peak=fp,
+                  spread=10, nrcv=rcv.shape[0], snap_every=k, tmute_frac=0.1, ic_fp32_only=1)
+    p = Propagator(shot_config(nx, ny, nz, d, dt, nt, ENGINE_FD, 16, TTI, snap_format=SNAP_LEARNED, **common))
+    check(lib().klrtm_propagator_set_lcodec(p.h, codec.h))
+    p.set_model(vel, eps, dl, pjx, pjy)
+    p.set_receivers(rcv)
+    t0 = time.perf_counter()
+    rec = p.model()
+    t_mod = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    out = p.rtm(rec, want=("image",))
+    t_rtm = time.perf_counter() - t0
+    tm = p.timings()
+    p.close()
+    codec.close()
+    img = out["image"]
+    snap_gb = (nx // 8) * (ny // 8) * (nz // 8) * 580.74 * (nt // k) / 1e9
+    return {"metric": METRIC, "value": round(3600.0 / t_rtm, 3), "unit": "shots/hour", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "tti-fd16-600 whole shot", "grid": [nx, ny, nz], "spacing_m": d, "order": 16,
+                       "pml": nb, "dt_s": round(dt, 7), "nt": nt, "snap_every": k, "snap_format": "learned",
+                       "receivers": int(rcv.shape[0])},
+            "rtm_s": round(t_rtm, 2), "modelling_s": round(t_mod, 2), "forward_ms": round(tm["forward_ms"], 1),
+            "backward_ms": round(tm["backward_ms"], 1), "learned_snapshot_store_gb": round(snap_gb, 1),
+            "fp16_snapshot_store_gb": round(nx * ny * nz * 2.0 * (1 + 2.0 / 512) * (nt // k) / 1e9, 1),
+            "image_finite": bool(np.isfinite(img).all()),
+            "image_rms": float(np.sqrt(np.mean(img.astype(np.float64) ** 2))), "model_build_s": round(t_model, 1),
+            "codec_training": info}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="vti-cfd10-400", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="vti-cfd10-400", choices=sorted(WORKLOADS) + ["tti-fd16-600-shot"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--batched-shots", type=int, default=8, help="shots of the batched-vs-plain leg (0: skip)")
@@ -198,6 +248,9 @@ def main():
     if _lib.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "tti-fd16-600-shot":
+        print(json.dumps(tti_full_shot()))
+        return
     w, eng, mdl, vel, eps, dl, dt = build_case(args.workload)
     ny, nx, nz = w["shape"]
     ncell = nx * ny * nz
