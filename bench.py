@@ -229,7 +229,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--batched-shots", type=int, default=8, help="shots of the batched-vs-plain leg (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=200)
-    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--batch", type=int, default=8)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

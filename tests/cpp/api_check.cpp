@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/bgprtm/FdEngine.h"
+#include "../../include/bgprtm/ImagingCondition.h"
 #include "../../include/bgprtm/ImagingCorrelator.h"
 #include "../../include/bgprtm/KernelCPU.h"
 #include "../../include/bgprtm/PML.h"
@@ -25,6 +26,12 @@ void orc_engine_kernel_f(void *h, int model, const float *p0, float *pb, float *
 void orc_engine_laplacian_f(void *h, const float *in, float *out);
 void orc_compress_pair(const float *in, uint16_t *out, int nx, int ny, int nz, int mode);
 void orc_pml_round(float g, float *p, float *q, int n);
+void orc_apply_symmetry(float *wf0, int nx, int ny, int nz, int iz0, int sym, int tprlen, int taper);
+void orc_illum_damping(float *illum, int nx, int ny, int nz, int smooth, int sxy, int sz, float ampDamping,
+                       float dampWeight, double *average_out);
+void orc_lagrange_resample(const float *in, int nzi, int nxi, int nyi, float z0i, float dzi, float x0i, float dxi,
+                           float y0i, float dyi, float *out, int nzo, int nxo, int nyo, float z0o, float dzo, float x0o,
+                           float dxo, float y0o, float dyo, int np);
 }
 
 static int failures = 0;
@@ -122,6 +129,35 @@ int main() {
     std::vector<float> got = img.host(), b = dbb.host(), want(n);
     for(int i = 0; i < n; i++) want[i] = 0.f + p0[i] * b[i];
     report("ImagingCorrelator::crossone", std::memcmp(got.data(), want.data(), n * 4) == 0, rel_l2(got, want));
+  }
+  // Propagator::apply_symmetry, computeDamping, Lagrange
+  {
+    std::vector<float> w(p0), want(p0);
+    konglong::DeviceBuffer<float> dw(n);
+    dw.upload(w.data());
+    ImagingPost::apply_symmetry(dw.get(), nx, ny, nz, 9, -1, 5);
+    orc_apply_symmetry(want.data(), nx, ny, nz, 9, -1, 5, 1);
+    std::vector<float> got = dw.host();
+    report("Propagator::apply_symmetry bit-exact", std::memcmp(got.data(), want.data(), n * 4) == 0, rel_l2(got, want));
+    std::vector<float> ill(n), ref;
+    for(int i = 0; i < n; i++) ill[i] = 1.0f + p0[i] * p0[i];
+    ref = ill;
+    konglong::DeviceBuffer<float> di(n);
+    di.upload(ill.data());
+    ImagingPost::computeDamping(di.get(), nx, ny, nz, true, 3, 2);
+    double avg = 0;
+    orc_illum_damping(ref.data(), nx, ny, nz, 1, 3, 2, 1.0f, 0.07f, &avg);
+    got = di.host();
+    report("ImagingCondition::computeDamping", rel_l2(got, ref) < 1e-6, rel_l2(got, ref));
+    RectGrid gi{nz, nx, ny, 0.f, 10.f, 0.f, 10.f, 0.f, 10.f}, go{40, 20, 15, 2.f, 7.f, 5.f, 13.f, 1.f, 14.f};
+    const int no = go.nz * go.nx * go.ny;
+    konglong::DeviceBuffer<float> dout(no);
+    Lagrange(gi, go, 8).apply(d0.get(), dout.get());
+    std::vector<float> lw(no);
+    orc_lagrange_resample(p0.data(), nz, nx, ny, 0.f, 10.f, 0.f, 10.f, 0.f, 10.f, lw.data(), go.nz, go.nx, go.ny, 2.f, 7.f,
+                          5.f, 13.f, 1.f, 14.f, 8);
+    got = dout.host();
+    report("Lagrange::apply bit-exact", std::memcmp(got.data(), lw.data(), no * 4) == 0, rel_l2(got, lw));
   }
   if(failures) {
     std::printf("API_CHECK FAIL (%d)\n", failures);
