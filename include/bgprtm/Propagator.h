@@ -24,6 +24,12 @@ public:
                 const float *pjy = nullptr) {
     konglong::check(klrtm_propagator_set_model(p_, vel, eps, del, pjx, pjy));
   }
+  // Q attenuation (Model::useQ): 1/Q volume and Q::coeffs [cq0, cq[order], wq[order]]; call before
+  // setModel.  An empty coefficient vector switches Q off.
+  void setQ(const float *invQ, const std::vector<float> &coeffs) {
+    const int order = coeffs.empty() ? 0 : (int)(coeffs.size() - 1) / 2;
+    konglong::check(klrtm_propagator_set_q(p_, invQ, order, coeffs.empty() ? nullptr : coeffs.data()));
+  }
   void setReceivers(const std::vector<float> &xyz) { konglong::check(klrtm_propagator_set_receivers(p_, xyz.data())); }
   void setSourceWavelet(const std::vector<float> &w) {
     konglong::check(klrtm_propagator_set_source_wavelet(p_, w.data()));

@@ -69,6 +69,15 @@ int klrtm_timer_reset(void);
 int klrtm_vel2(float *vel, size_t n, float dt);
 /* w0[i] <- 2*w1[i] - w0[i] (device) */
 int klrtm_update2nd(float *w0, const float *w1, size_t n);
+/* Q (viscoacoustic) branches of the same two functions (bgprtm/Propagator.cpp:725-741, 1789-1816).
+ * coeffs is the reference's Q::coeffs layout [cq0, cq[order], wq[order]] (bgprtm/Q.cpp:311-318),
+ * order in [1, 8].  klrtm_vel2_q: vel <- v^2 dt^2 (1 - invq^2/4 + invq^4/48)/(1 - cq0 invq) and
+ * invq <- invq/(1 - cq0 invq).  klrtm_update2nd_q: the leapfrog with memory variables; d0 and d1 are
+ * device arrays [order][n] (Wavefield::wq[iq0], wq[iq1]); d0 is updated, and the caller swaps d0 and
+ * d1 with w0 and w1 after every step (Wavefield::swap, bgprtm/Wavefield.cpp:132). */
+int klrtm_vel2_q(float *vel, float *invq, size_t n, float dt, float cq0);
+int klrtm_update2nd_q(float *w0, const float *w1, const float *invq, float *d0, const float *d1, size_t n, int order,
+                      const float *coeffs);
 
 /* ---- KernelCPU + FdEngine_fd / FdEngine_cfd + PML (bgprtm/KernelCPU.h:19-35) */
 typedef struct klrtm_kernel klrtm_kernel;
@@ -148,6 +157,12 @@ int klrtm_receiver_set_traces(klrtm_receiver *r, const float *traces_host);
 int klrtm_receiver_apply(klrtm_receiver *r, float *w0, int it, const float *vel, float scaler0);
 /* record[:, jt] = sum kx*ky*kz * w   (device record kept by the receiver object) */
 int klrtm_receiver_extract(klrtm_receiver *r, const float *w, int jt);
+/* the same with ghost mirrors about the plane iz_mirror (Receiver::apply with ghost,
+ * bgprtm/Receiver.cpp:457-507; Receiver::extract_value with mirror_ghost, bgprtm/Receiver.cpp:394-426):
+ * each term at depth z is also subtracted at 2*iz_mirror - z (vel taken there), or read from there and
+ * subtracted on extraction; mirrors above z = 0 are dropped */
+int klrtm_receiver_apply_ghost(klrtm_receiver *r, float *w0, int it, const float *vel, float scaler0, int iz_mirror);
+int klrtm_receiver_extract_ghost(klrtm_receiver *r, const float *w, int jt, int iz_mirror);
 /* copy the record out as [nr][nt] */
 int klrtm_receiver_get_record(klrtm_receiver *r, float *record_host);
 /* the spread tables as built (for inspection): lengths/offsets per axis */
@@ -248,6 +263,18 @@ int klrtm_propagator_destroy(klrtm_propagator *p);
 /* host model volumes: velocity (m/s) and optional eps, del, pjx, pjy (NULL allowed) */
 int klrtm_propagator_set_model(klrtm_propagator *p, const float *vel, const float *eps, const float *del,
                                const float *pjx, const float *pjy);
+/* Q attenuation (Model::useQ with volModel[Q], bgprtm/ModelLoader.cpp:24): host 1/Q volume and the
+ * Q::coeffs [cq0, cq[order], wq[order]] of klrtm_update2nd_q.  Call before klrtm_propagator_set_model,
+ * whose Vel2 then takes the Q branch; every step becomes the kernel (pb -= vel*L) followed by the Q
+ * leapfrog, and the memory variables are cleared with the wavefields (Wavefield::cleanMemory,
+ * bgprtm/Wavefield.cpp:116-118).  order 0 switches Q off again. */
+int klrtm_propagator_set_q(klrtm_propagator *p, const float *invq_host, int order, const float *coeffs);
+/* ghost surfaces (sourceSurfaceType / receiverSurfaceType SOURCE_GHOST, RECEIVER_GHOST, GHOST;
+ * Propagator::determine_ghost, bgprtm/Propagator.cpp:1538-1541): the injection of the chosen sides is
+ * mirrored about iz_mirror (klrtm_receiver_apply_ghost), and with receiver_side the modelling extraction
+ * too; the top PML stays on.  A FREESURFACE config (free_surface_iz >= 0) mirrors both sides' injection
+ * about the surface by itself.  source_side = receiver_side = 0 switches the ghosts off. */
+int klrtm_propagator_set_ghost(klrtm_propagator *p, int iz_mirror, int source_side, int receiver_side);
 /* source wavelet samples (host, nt floats); replaces the built-in Ricker (Source::setupWavelet) */
 int klrtm_propagator_set_source_wavelet(klrtm_propagator *p, const float *wavelet_host);
 /* receivers: rcv_xyz[3*nrcv] grid coordinates */

@@ -65,6 +65,8 @@ _SIGS = {
     "klrtm_pml_apply_trans": (_i, [_f, _vp, _vp, _vp, _i]),
     "klrtm_vel2": (_i, [_vp, _sz, _f]),
     "klrtm_update2nd": (_i, [_vp, _vp, _sz]),
+    "klrtm_vel2_q": (_i, [_vp, _vp, _sz, _f, _f]),
+    "klrtm_update2nd_q": (_i, [_vp, _vp, _vp, _vp, _vp, _sz, _i, _pf]),
     "klrtm_kernel_create": (_i, [C.POINTER(_vp), _i, _i, _i, _f, _f, _f, _i, _i]),
     "klrtm_kernel_destroy": (_i, [_vp]),
     "klrtm_kernel_set_model": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -84,6 +86,8 @@ _SIGS = {
     "klrtm_receiver_set_traces": (_i, [_vp, _pf]),
     "klrtm_receiver_apply": (_i, [_vp, _vp, _i, _vp, _f]),
     "klrtm_receiver_extract": (_i, [_vp, _vp, _i]),
+    "klrtm_receiver_apply_ghost": (_i, [_vp, _vp, _i, _vp, _f, _i]),
+    "klrtm_receiver_extract_ghost": (_i, [_vp, _vp, _i, _i]),
     "klrtm_receiver_get_record": (_i, [_vp, _pf]),
     "klrtm_receiver_get_tables": (_i, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pf, _pf, _pf, _pi]),
     "klrtm_nshort_volume": (_sz, [_i, _i, _i]),
@@ -101,6 +105,8 @@ _SIGS = {
     "klrtm_propagator_create": (_i, [C.POINTER(_vp), C.POINTER(ShotConfig)]),
     "klrtm_propagator_destroy": (_i, [_vp]),
     "klrtm_propagator_set_model": (_i, [_vp, _pf, _pf, _pf, _pf, _pf]),
+    "klrtm_propagator_set_q": (_i, [_vp, _pf, _i, _pf]),
+    "klrtm_propagator_set_ghost": (_i, [_vp, _i, _i, _i]),
     "klrtm_propagator_set_receivers": (_i, [_vp, _pf]),
     "klrtm_propagator_set_source_wavelet": (_i, [_vp, _pf]),
     "klrtm_propagator_model": (_i, [_vp, _pf]),

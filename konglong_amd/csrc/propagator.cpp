@@ -16,6 +16,7 @@
 #include "lcodec.h"
 #include "fd_kernels.h"
 #include "surface.h"
+#include "qatten.h"
 #include <cmath>
 #include <cstring>
 #include <algorithm>
@@ -39,6 +40,20 @@ struct Propagator {
   double *d_img64 = nullptr, *d_ill64 = nullptr;
   float *d_taper_z = nullptr;
   const LCodec *lc = nullptr;  // KLRTM_SNAP_LEARNED codec (borrowed)
+  // Q attenuation: raw 1/Q as uploaded, 1/Q after Vel2's correction, memory variables wq[2][order][n]
+  float *d_invq_raw = nullptr, *d_invq = nullptr, *d_q[2] = {nullptr, nullptr};
+  int q_order = 0, iq0 = 0;
+  bool q_ready = false;  // set_model has run the Q branch of Vel2 since the last set_q
+  float q_coeffs[1 + 2 * Q_MAX_ORDER] = {};
+  // ghost mirror plane of SOURCE_GHOST / RECEIVER_GHOST / GHOST surfaces (-1: none) and its sides
+  int ghost_iz = -1;
+  bool ghost_src = false, ghost_rcv = false;
+  // Propagator::determine_ghost (bgprtm/Propagator.cpp:1538-1541): a FREESURFACE side mirrors its
+  // injection about the surface too; only the ghost types mirror the modelling extraction
+  // (ExtractReceiverData::extract_value, bgprtm/ExtractReceiverData.cpp:230-233)
+  int src_mirror() const { return c.free_surface_iz >= 0 ? c.free_surface_iz : (ghost_src ? ghost_iz : -1); }
+  int rcv_mirror() const { return c.free_surface_iz >= 0 ? c.free_surface_iz : (ghost_rcv ? ghost_iz : -1); }
+  int ext_mirror() const { return ghost_rcv ? ghost_iz : -1; }
   std::vector<float> wtaperT;
   double t_fwd = 0, t_bwd = 0, t_ic = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -58,7 +73,8 @@ Propagator::~Propagator() {
   if(rcv) receiver_destroy(rcv);
   for(void *p : {(void *)d_vel, (void *)d_eps, (void *)d_del, (void *)d_pjx, (void *)d_pjy, (void *)w0, (void *)w1,
                  (void *)wr, (void *)d_snap, (void *)d_scales, (void *)d_smax, (void *)d_img, (void *)d_ill,
-                 (void *)d_img64, (void *)d_ill64, (void *)d_taper_z})
+                 (void *)d_img64, (void *)d_ill64, (void *)d_taper_z, (void *)d_invq_raw, (void *)d_invq,
+                 (void *)d_q[0], (void *)d_q[1]})
     (void)hipFree(p);
   for(auto &e : ev)
     if(e) (void)hipEventDestroy(e);
@@ -123,11 +139,23 @@ int Propagator::clear_images() {
 int Propagator::reset() {
   KL_HIP(hipMemsetAsync(w0, 0, n * sizeof(float), stream()));
   KL_HIP(hipMemsetAsync(w1, 0, n * sizeof(float), stream()));
+  int rc;
+  if(q_order > 0 && ((rc = launch_fill(d_q[0], 0.f, n * q_order)) || (rc = launch_fill(d_q[1], 0.f, n * q_order))))
+    return rc;  // Wavefield::cleanMemory (bgprtm/Wavefield.cpp:116-118)
   return eng->clean();  // KernelCPU::cleanPMLMemory (bgprtm/Propagator.cpp:1165-1166)
 }
 
 // Propagator::kernel() + update2nd (bgprtm/Propagator.cpp:1217-1223)
+// With Q: the kernel, then the Q leapfrog on the memory-variable set iq0, which swaps with w0/w1
+// (Wavefield::swapPointers, bgprtm/Wavefield.cpp:130-133).
 int Propagator::step(int) {
+  if(q_order > 0) {
+    KL_REQUIRE(q_ready, KLRTM_ESTATE, "Q is set: call klrtm_propagator_set_model after klrtm_propagator_set_q");
+    int rc = eng->run(c.model, MODE_KERNEL, w1, w0, wr);
+    if(rc || (rc = launch_update2nd_q(w0, w1, d_invq, d_q[iq0], d_q[iq0 ^ 1], n, q_order, q_coeffs))) return rc;
+    iq0 ^= 1;
+    return KLRTM_OK;
+  }
   if(c.fuse_step) return eng->run(c.model, MODE_STEP, w1, w0, wr);
   int rc = eng->run(c.model, MODE_KERNEL, w1, w0, wr);
   if(rc) return rc;
@@ -262,9 +290,35 @@ int propagator_set_model(Propagator &P, const float *vel, const float *eps, cons
   if((rc = upload_opt(&P.d_vel, vel, P.n)) || (rc = upload_opt(&P.d_eps, eps, P.n)) || (rc = upload_opt(&P.d_del, del, P.n)) ||
      (rc = upload_opt(&P.d_pjx, pjx, P.n)) || (rc = upload_opt(&P.d_pjy, pjy, P.n)))
     return rc;
-  if((rc = launch_vel2(P.d_vel, P.n, P.c.dt))) return rc;  // Propagator::Vel2
+  if(P.q_order > 0) {  // Propagator::Vel2, Q branch: also corrects 1/Q (from the raw upload)
+    KL_HIP(hipMemcpyAsync(P.d_invq, P.d_invq_raw, P.n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    if((rc = launch_vel2_q(P.d_vel, P.d_invq, P.n, P.c.dt, P.q_coeffs[0]))) return rc;
+    P.q_ready = true;
+  } else if((rc = launch_vel2(P.d_vel, P.n, P.c.dt))) return rc;  // Propagator::Vel2
   return klrtm_kernel_set_model(reinterpret_cast<klrtm_kernel *>(P.eng), P.d_vel, nullptr, P.d_del, P.d_eps, P.d_pjx,
                                 P.d_pjy);
+}
+
+int propagator_set_q(Propagator &P, const float *invq, int order, const float *coeffs) {
+  P.q_ready = false;
+  if(order == 0) {
+    P.q_order = 0;
+    return KLRTM_OK;
+  }
+  int rc = q_coeffs_check(order, coeffs);
+  if(rc) return rc;
+  KL_REQUIRE(invq, KLRTM_EINVAL, "null 1/Q volume");
+  if(P.q_order != order) {
+    for(float *&d : P.d_q) (void)hipFree(d), d = nullptr;
+    P.q_order = 0;
+    if((rc = dmalloc(&P.d_q[0], P.n * order)) || (rc = dmalloc(&P.d_q[1], P.n * order))) return rc;
+  }
+  if((rc = upload_opt(&P.d_invq_raw, invq, P.n))) return rc;
+  if(!P.d_invq && (rc = dmalloc(&P.d_invq, P.n))) return rc;
+  std::copy(coeffs, coeffs + 1 + 2 * order, P.q_coeffs);
+  P.q_order = order;
+  P.iq0 = 0;
+  return KLRTM_OK;
 }
 
 int propagator_set_receivers(Propagator &P, const float *xyz) {
@@ -282,8 +336,9 @@ int propagator_model(Propagator &P, float *record) {
   KL_HIP(hipEventRecord(P.ev[0], stream()));
   for(int it = 0; it < P.c.nt; it++) {
     if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f)) || (rc = P.surface())) return rc;  // it_shift_force = -1
-    if((rc = receiver_extract(*P.rcv, P.w0, it))) return rc;
+    // it_shift_force = -1
+    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f, P.src_mirror())) || (rc = P.surface())) return rc;
+    if((rc = receiver_extract(*P.rcv, P.w0, it, P.ext_mirror()))) return rc;
     std::swap(P.w0, P.w1);
   }
   KL_HIP(hipEventRecord(P.ev[1], stream()));
@@ -308,7 +363,7 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
   KL_HIP(hipEventRecord(P.ev[0], stream()));
   for(int it = 0; it < nt; it++) {
     if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f)) || (rc = P.surface())) return rc;
+    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f, P.src_mirror())) || (rc = P.surface())) return rc;
     if(it % k == 0 && it / k < P.ntsav && (rc = P.store(it / k))) return rc;
     std::swap(P.w0, P.w1);
   }
@@ -321,7 +376,7 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
   KL_HIP(hipEventRecord(P.ev[2], stream()));
   for(int it = nt - 1; it >= 0; it--) {
     if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f)) || (rc = P.surface())) return rc;
+    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f, P.rcv_mirror())) || (rc = P.surface())) return rc;
     if(it % k == 0 && it / k < P.ntsav && (rc = P.image(it / k))) return rc;
     std::swap(P.w0, P.w1);
   }
@@ -348,7 +403,7 @@ int propagator_forward_steps(Propagator &P, int it0, int nsteps, int store_snaps
   const int k = std::max(1, P.c.snap_every);
   for(int it = it0; it < it0 + nsteps; it++) {
     if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f)) || (rc = P.surface())) return rc;
+    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f, P.src_mirror())) || (rc = P.surface())) return rc;
     if(store_snapshots && it % k == 0) {
       int s = (it / k) % std::max(1, P.ntsav);
       if((rc = P.store(s))) return rc;
@@ -365,7 +420,7 @@ int propagator_backward_steps(Propagator &P, int it0, int nsteps, int image) {
   const int k = std::max(1, P.c.snap_every);
   for(int it = it0; it > it0 - nsteps; it--) {
     if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f)) || (rc = P.surface())) return rc;
+    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f, P.rcv_mirror())) || (rc = P.surface())) return rc;
     if(image && it >= 0 && it % k == 0 && (rc = P.image((it / k) % std::max(1, P.ntsav)))) return rc;
     std::swap(P.w0, P.w1);
   }
@@ -393,6 +448,20 @@ int klrtm_propagator_set_model(klrtm_propagator *p, const float *vel, const floa
                                const float *pjy) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
   return propagator_set_model(*reinterpret_cast<Propagator *>(p), vel, eps, del, pjx, pjy);
+}
+int klrtm_propagator_set_q(klrtm_propagator *p, const float *invq, int order, const float *coeffs) {
+  KL_REQUIRE(p && order >= 0, KLRTM_EINVAL, "bad argument");
+  return propagator_set_q(*reinterpret_cast<Propagator *>(p), invq, order, coeffs);
+}
+int klrtm_propagator_set_ghost(klrtm_propagator *p, int iz_mirror, int source_side, int receiver_side) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  const bool any = source_side || receiver_side;
+  KL_REQUIRE(!any || (iz_mirror >= 0 && 2 * iz_mirror < P.c.nz), KLRTM_EINVAL,
+             "ghost mirror plane %d needs 0 <= 2*iz < nz (%d)", iz_mirror, P.c.nz);
+  P.ghost_iz = any ? iz_mirror : -1;
+  P.ghost_src = source_side != 0, P.ghost_rcv = receiver_side != 0;
+  return KLRTM_OK;
 }
 int klrtm_propagator_set_source_wavelet(klrtm_propagator *p, const float *wav) {
   KL_REQUIRE(p && wav, KLRTM_EINVAL, "null argument");

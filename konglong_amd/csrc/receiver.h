@@ -10,8 +10,8 @@ int receiver_create(Receiver **out, int nx, int ny, int nz, int nr, const float 
 void receiver_destroy(Receiver *R);
 int receiver_set_traces(Receiver &R, const float *traces_host);
 int receiver_set_traces_device(Receiver &R, const float *tm_device);
-int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scaler0);
-int receiver_extract(Receiver &R, const float *w, int jt);
+int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scaler0, int iz_mirror = -1);
+int receiver_extract(Receiver &R, const float *w, int jt, int iz_mirror = -1);
 int receiver_get_record(Receiver &R, float *out_host);
 int receiver_clear_record(Receiver &R);
 float *receiver_record_device(Receiver &R);

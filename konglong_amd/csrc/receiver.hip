@@ -16,6 +16,14 @@
 //     receiver contracts its ky*kx window of that plane.
 // Contraction is off so a single trace (the source) is injected with exactly
 // the reference's products and sums.
+//
+// Ghost mirrors (Receiver::apply with ghost, bgprtm/Receiver.cpp:457-475,
+// 477-507; Receiver::extract_value, bgprtm/Receiver.cpp:394-426): with a
+// mirror plane iz_mirror, every injected term at depth z is also subtracted at
+// 2*iz_mirror - z (with vel there), after all direct terms of the group, and
+// extraction subtracts the kz-weighted samples mirrored the same way.  Terms
+// whose mirror falls above z = 0 are dropped, as the reference's lz2 limit
+// does.
 #pragma clang fp contract(off)
 #include "klrtm_internal.h"
 #include <algorithm>
@@ -127,6 +135,7 @@ Receiver::~Receiver() {
 struct GroupArgs {
   int nx, ny, nz, nr;
   int offz, lz, bx0, by0, bw, bh, lxmax, lymax, L;
+  int izm;  // ghost mirror plane (-1: none)
 };
 
 // wy[j][k] = (val_r*scaler)*ky_r[j] for the receiver r at CSR position k of
@@ -197,6 +206,24 @@ __global__ void inject_apply_kernel(const GroupArgs a, const float *__restrict__
   w0[o] += (T * kz[k]) * vel[o];
 }
 
+// w0[col, 2*izm - (offz+k)] -= T*kz[k]*vel there (one lane per term; distinct k of a column mirror to
+// distinct cells)
+__global__ void inject_mirror_kernel(const GroupArgs a, const float *__restrict__ plane, const float *__restrict__ kz,
+                                     float *__restrict__ w0, const float *__restrict__ vel) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long tot = (long)a.bw * a.bh * a.lz;
+  if(i >= tot) return;
+  int k = (int)(i % a.lz);
+  int c = (int)(i / a.lz);
+  const int zm = 2 * a.izm - (a.offz + k);
+  if(zm < 0 || zm >= a.nz) return;
+  int ix = a.bx0 + c % a.bw, iy = a.by0 + c / a.bw;
+  float T = plane[c];
+  if(T == 0.f) return;
+  size_t o = ((size_t)iy * a.nx + ix) * a.nz + zm;
+  w0[o] -= (T * kz[k]) * vel[o];
+}
+
 // P[iy][ix] = sum_k kz[k]*w[col, offz+k]
 __global__ void extract_plane_kernel(const GroupArgs a, const float *__restrict__ kz, const float *__restrict__ w,
                                      float *__restrict__ plane) {
@@ -206,6 +233,13 @@ __global__ void extract_plane_kernel(const GroupArgs a, const float *__restrict_
   const float *col = w + ((size_t)iy * a.nx + ix) * a.nz + a.offz;
   float s = 0.f;
   for(int k = 0; k < a.lz; k++) s += kz[k] * col[k];
+  if(a.izm >= 0) {
+    // mirrored samples 2*izm - offz - k for k < lz2 = min(lz, 2*izm - offz + 1), inside the grid
+    const float *colm = w + ((size_t)iy * a.nx + ix) * a.nz;
+    const int top = 2 * a.izm - a.offz;
+    const int lz2 = min(a.lz, top + 1);
+    for(int k = max(0, top - (a.nz - 1)); k < lz2; k++) s -= kz[k] * colm[top - k];
+  }
   plane[c] = s;
 }
 
@@ -231,10 +265,11 @@ static GroupArgs gargs(const Receiver &R, const RcvGroup &g) {
   a.nx = R.nx, a.ny = R.ny, a.nz = R.nz, a.nr = g.nr;
   a.offz = g.offz, a.lz = g.lz, a.bx0 = g.bx0, a.by0 = g.by0, a.bw = g.bw, a.bh = g.bh;
   a.lxmax = g.lxmax, a.lymax = g.lymax, a.L = R.L;
+  a.izm = -1;
   return a;
 }
 
-int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scaler0) {
+int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scaler0, int iz_mirror) {
   if(it < 0 || it > R.nt - 1) return KLRTM_OK;  // bgprtm/Receiver.cpp:1205
   KL_REQUIRE(R.d_traces, KLRTM_ESTATE, "receiver traces not set");
   const float scaler = scaler0 * R.dt_dxyz;
@@ -250,16 +285,22 @@ int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scale
                        g.bin_start, g.corg, g.ckx, R.d_wy, R.d_plane);
     long tot = (long)cells * g.lz;
     hipLaunchKernelGGL(inject_apply_kernel, dim3(div_up(tot, 256)), dim3(256), 0, stream(), a, R.d_plane, g.kz, w0, vel);
+    if(iz_mirror >= 0) {
+      a.izm = iz_mirror;
+      hipLaunchKernelGGL(inject_mirror_kernel, dim3(div_up(tot, 256)), dim3(256), 0, stream(), a, R.d_plane, g.kz, w0,
+                         vel);
+    }
   }
   return check_launch("inject");
 }
 
-int receiver_extract(Receiver &R, const float *w, int jt) {
+int receiver_extract(Receiver &R, const float *w, int jt, int iz_mirror) {
   KL_REQUIRE(jt >= 0 && jt < R.nt, KLRTM_EINVAL, "extract time index %d outside [0,%d)", jt, R.nt);
   float *row = R.d_record + (size_t)jt * R.nr;
   LaunchTimer tm("extract");
   for(auto &g : R.groups) {
     GroupArgs a = gargs(R, g);
+    a.izm = iz_mirror;
     int cells = g.bw * g.bh;
     hipLaunchKernelGGL(extract_plane_kernel, dim3(div_up(cells, 256)), dim3(256), 0, stream(), a, g.kz, w, R.d_plane);
     hipLaunchKernelGGL(extract_rcv_kernel, dim3(div_up(g.nr, 256)), dim3(256), 0, stream(), a, g.bin_rcv, R.d_kx, R.d_ky,
@@ -470,11 +511,23 @@ int klrtm_receiver_set_traces(klrtm_receiver *r, const float *traces) {
 }
 int klrtm_receiver_apply(klrtm_receiver *r, float *w0, int it, const float *vel, float scaler0) {
   KL_REQUIRE(r && w0 && vel, KLRTM_EINVAL, "null argument");
-  return receiver_apply(*reinterpret_cast<Receiver *>(r), w0, it, vel, scaler0);
+  return receiver_apply(*reinterpret_cast<Receiver *>(r), w0, it, vel, scaler0, -1);
 }
 int klrtm_receiver_extract(klrtm_receiver *r, const float *w, int jt) {
   KL_REQUIRE(r && w, KLRTM_EINVAL, "null argument");
-  return receiver_extract(*reinterpret_cast<Receiver *>(r), w, jt);
+  return receiver_extract(*reinterpret_cast<Receiver *>(r), w, jt, -1);
+}
+int klrtm_receiver_apply_ghost(klrtm_receiver *r, float *w0, int it, const float *vel, float scaler0, int iz_mirror) {
+  KL_REQUIRE(r && w0 && vel, KLRTM_EINVAL, "null argument");
+  Receiver &R = *reinterpret_cast<Receiver *>(r);
+  KL_REQUIRE(iz_mirror >= 0 && iz_mirror < R.nz, KLRTM_EINVAL, "mirror plane %d outside [0, %d)", iz_mirror, R.nz);
+  return receiver_apply(R, w0, it, vel, scaler0, iz_mirror);
+}
+int klrtm_receiver_extract_ghost(klrtm_receiver *r, const float *w, int jt, int iz_mirror) {
+  KL_REQUIRE(r && w, KLRTM_EINVAL, "null argument");
+  Receiver &R = *reinterpret_cast<Receiver *>(r);
+  KL_REQUIRE(iz_mirror >= 0 && iz_mirror < R.nz, KLRTM_EINVAL, "mirror plane %d outside [0, %d)", iz_mirror, R.nz);
+  return receiver_extract(R, w, jt, iz_mirror);
 }
 int klrtm_receiver_get_record(klrtm_receiver *r, float *rec) {
   KL_REQUIRE(r && rec, KLRTM_EINVAL, "null argument");

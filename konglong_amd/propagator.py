@@ -47,6 +47,22 @@ class Propagator:
         arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (vel, eps, del_, pjx, pjy)]
         check(lib().klrtm_propagator_set_model(self.h, *[fptr(a) for a in arrs]))
 
+    def set_q(self, invq, coeffs):
+        """Q attenuation (Model::useQ): the 1/Q volume and Q::coeffs [cq0, cq[order], wq[order]]; call
+        before set_model.  invq None switches Q off."""
+        if invq is None:
+            check(lib().klrtm_propagator_set_q(self.h, None, 0, None))
+            return
+        q = np.ascontiguousarray(invq, np.float32)
+        cf = np.ascontiguousarray(coeffs, np.float32)
+        assert q.shape == self.shape and cf.size % 2 == 1
+        check(lib().klrtm_propagator_set_q(self.h, fptr(q), (cf.size - 1) // 2, fptr(cf)))
+
+    def set_ghost(self, iz_mirror, source_side=True, receiver_side=True):
+        """SOURCE_GHOST / RECEIVER_GHOST / GHOST surfaces: mirror the chosen sides' injection (and with
+        receiver_side the modelling extraction) about iz_mirror."""
+        check(lib().klrtm_propagator_set_ghost(self.h, int(iz_mirror), int(source_side), int(receiver_side)))
+
     def set_source_wavelet(self, wavelet):
         w = np.ascontiguousarray(wavelet, np.float32)
         assert w.shape == (self.cfg.nt,)

@@ -29,12 +29,19 @@ class Receiver:
         assert t.shape == (self.nr, self.nt)
         check(lib().klrtm_receiver_set_traces(self.h, fptr(t)))
 
-    def apply(self, w0, it, vel, scaler0=1.0):
-        """Receiver::apply: w0 += trace[:, it]*scaler0*dt_dxyz*kx*ky*kz*vel."""
-        check(lib().klrtm_receiver_apply(self.h, w0.ptr, it, vel.ptr, scaler0))
+    def apply(self, w0, it, vel, scaler0=1.0, iz_mirror=-1):
+        """Receiver::apply: w0 += trace[:, it]*scaler0*dt_dxyz*kx*ky*kz*vel; with iz_mirror >= 0 the
+        ghost mirror about that plane is subtracted too."""
+        if iz_mirror >= 0:
+            check(lib().klrtm_receiver_apply_ghost(self.h, w0.ptr, it, vel.ptr, scaler0, iz_mirror))
+        else:
+            check(lib().klrtm_receiver_apply(self.h, w0.ptr, it, vel.ptr, scaler0))
 
-    def extract(self, w, jt):
-        check(lib().klrtm_receiver_extract(self.h, w.ptr, jt))
+    def extract(self, w, jt, iz_mirror=-1):
+        if iz_mirror >= 0:
+            check(lib().klrtm_receiver_extract_ghost(self.h, w.ptr, jt, iz_mirror))
+        else:
+            check(lib().klrtm_receiver_extract(self.h, w.ptr, jt))
 
     def record(self):
         out = np.zeros((self.nr, self.nt), np.float32)

@@ -46,6 +46,9 @@ def lib():
             getattr(L, f"orc_engine_deriv1_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
                                                             C.c_int]
             getattr(L, f"orc_update2nd_{s}").argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+            getattr(L, f"orc_vel2_q_{s}").argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, T, T]
+            getattr(L, f"orc_update2nd_q_{s}").argtypes = [C.c_void_p] * 5 + [C.c_size_t, C.c_int, C.c_int,
+                                                                             C.c_void_p]
         L.orc_taylor_coef2.argtypes = [C.c_int, _pd]
         L.orc_taylor_coef1.argtypes = [C.c_int, _pd]
         L.orc_pml_profile.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _pf, _pf]
@@ -61,6 +64,8 @@ def lib():
                                  C.c_int, _pi, _pi, _pi, _pi, _pi, _pi]
         L.orc_extract.argtypes = [_pf, C.c_int, C.c_int, C.c_int, C.c_int, _pf, _pf, _pf, _pf, C.c_int, C.c_int,
                                   C.c_int, _pi, _pi, _pi, _pi, _pi, _pi]
+        L.orc_inject_ghost.argtypes = L.orc_inject.argtypes + [C.c_int]
+        L.orc_extract_ghost.argtypes = L.orc_extract.argtypes + [C.c_int]
         L.orc_compress16.argtypes = [_pf, C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_inject_d.argtypes = [_pd, _pd, C.c_int, C.c_int, C.c_int, C.c_int, _pd, _pf, _pf, _pf, C.c_int, C.c_int,
                                    C.c_int, _pi, _pi, _pi, _pi, _pi, _pi]
@@ -261,6 +266,35 @@ def uncompress_pair(code, ny, nx, nz, mode):
 
 def pair_autobit(range1, range2, nbits):
     return lib().orc_pair_autobit(range1, range2, nbits)
+
+
+def _vol(a, dtype):
+    assert a.dtype == dtype and a.flags.c_contiguous, (a.dtype, dtype)
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def vel2_q(vel, invq, dt, coeffs):
+    """Propagator::Vel2, Q branch (bgprtm/Propagator.cpp:725-741), in place on vel (v -> v^2 dt^2 with
+    the Q correction) and invq (1/Q -> corrected), float32 or float64 volumes."""
+    s = "d" if vel.dtype == np.float64 else "f"
+    T = C.c_double if s == "d" else C.c_float
+    getattr(lib(), f"orc_vel2_q_{s}")(_vol(vel, vel.dtype), _vol(invq, vel.dtype), vel.size, T(dt), T(coeffs[0]))
+    return vel, invq
+
+
+def update2nd_q(w0, w1, invq, d0, d1, coeffs):
+    """Propagator::update2nd, Q branch (bgprtm/Propagator.cpp:1789-1816): w0 (ny, nx, nz) and the
+    memory-variable set d0 (order, ny, nx, nz) in place; d1 is the other set (read only).  coeffs is
+    Q::coeffs [cq0, cq[order], wq[order]]."""
+    dt = w0.dtype
+    s = "d" if dt == np.float64 else "f"
+    ny, nx, nz = w0.shape
+    order = d0.shape[0]
+    cf = np.ascontiguousarray(coeffs, dt)
+    assert cf.size == 1 + 2 * order and d1.shape == d0.shape
+    getattr(lib(), f"orc_update2nd_q_{s}")(_vol(w0, dt), _vol(w1, dt), _vol(invq, dt), _vol(d0, dt), _vol(d1, dt),
+                                           nx * nz, ny, order, _vol(cf, dt))
+    return w0
 
 
 def apply_symmetry(wf, iz0, sym, tprlen, taper=1):

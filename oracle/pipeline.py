@@ -15,7 +15,8 @@ import math
 
 import numpy as np
 
-from . import lib, Engine, spread_coeff, taper_kernel, xy_taper, compress16, uncompress16, apply_symmetry, _pd
+from . import (lib, Engine, spread_coeff, taper_kernel, xy_taper, compress16, uncompress16, apply_symmetry, _pd,
+               vel2_q, update2nd_q)
 
 _pf, _pi = C.POINTER(C.c_float), C.POINTER(C.c_int)
 
@@ -65,18 +66,21 @@ def _tab_args(t):
             t["offx"].ctypes.data_as(_pi), t["offy"].ctypes.data_as(_pi), t["offz"].ctypes.data_as(_pi))
 
 
-def inject(w0, vel, shape, t, vals):
+def inject(w0, vel, shape, t, vals, iz_mirror=-1):
+    """Receiver::apply; with iz_mirror >= 0 the ghost mirror about that plane (bgprtm/Receiver.cpp:457-475)."""
     ny, nx, nz = shape
     vals = np.ascontiguousarray(vals, np.float32)
-    lib().orc_inject(w0.ctypes.data_as(_pf), vel.ctypes.data_as(_pf), nx, ny, nz, vals.size, vals.ctypes.data_as(_pf),
-                     *_tab_args(t))
+    lib().orc_inject_ghost(w0.ctypes.data_as(_pf), vel.ctypes.data_as(_pf), nx, ny, nz, vals.size,
+                           vals.ctypes.data_as(_pf), *_tab_args(t), int(iz_mirror))
 
 
-def extract(w, shape, t):
+def extract(w, shape, t, iz_mirror=-1):
+    """Receiver::extract_value; with iz_mirror >= 0 the ghost mirror (bgprtm/Receiver.cpp:394-426)."""
     ny, nx, nz = shape
     nr = t["lx"].size
     out = np.zeros(nr, np.float32)
-    lib().orc_extract(w.ctypes.data_as(_pf), nx, ny, nz, nr, out.ctypes.data_as(_pf), *_tab_args(t))
+    lib().orc_extract_ghost(w.ctypes.data_as(_pf), nx, ny, nz, nr, out.ctypes.data_as(_pf), *_tab_args(t),
+                            int(iz_mirror))
     return out
 
 
@@ -84,9 +88,19 @@ class Shot:
     """One shot: model (record) and rtm (image) in FP32, reference step order."""
 
     def __init__(self, nx, ny, nz, d, dt, nt, engine, order, model, nbnd, vmax, src_xyz, f_peak, spread, k1, k2,
-                 rcv_xyz, snap_every=1, snap_format=0, tmute_frac=0.1, threads=4, free_surface_iz=-1, surface_taper=1):
+                 rcv_xyz, snap_every=1, snap_format=0, tmute_frac=0.1, threads=4, free_surface_iz=-1, surface_taper=1,
+                 q=None, ghost=None):
         self.shape = (ny, nx, nz)
         self.fs, self.fs_taper = free_surface_iz, surface_taper
+        # injection mirrors (Propagator::determine_ghost, bgprtm/Propagator.cpp:1538-1541): FREESURFACE
+        # mirrors both sides' injection about the surface; ghost = (iz_mirror, source side, receiver
+        # side) for SOURCE_GHOST / RECEIVER_GHOST / GHOST, which also mirrors the modelling extraction
+        gz, gs, gr = ghost if ghost is not None else (-1, False, False)
+        self.src_mirror = free_surface_iz if free_surface_iz >= 0 else (gz if gs else -1)
+        self.rcv_mirror = free_surface_iz if free_surface_iz >= 0 else (gz if gr else -1)
+        self.ext_mirror = gz if gr else -1
+        # Q attenuation: (1/Q volume, Q::coeffs [cq0, cq[order], wq[order]])
+        self.q = None if q is None else (np.ascontiguousarray(q[0], np.float32), np.asarray(q[1], np.float32))
         self.nx, self.ny, self.nz, self.dt, self.nt = nx, ny, nz, dt, nt
         self.model_type, self.snap_every, self.snap_format = model, snap_every, snap_format
         self.eng = Engine(nx, ny, nz, d, d, d, engine, order, np.float32, threads=threads)
@@ -101,7 +115,11 @@ class Shot:
     def set_model(self, vel, eps=None, del_=None, pjx=None, pjy=None):
         v = np.asarray(vel, np.float32)
         dt = np.float32(self.dt)
-        self.m = ((v * v) * (dt * dt)).astype(np.float32)  # Propagator::Vel2: dt*dt in FP32
+        if self.q is not None:  # Propagator::Vel2, Q branch (bgprtm/Propagator.cpp:725-741)
+            self.m, self.invq = v.copy(), self.q[0].copy()
+            vel2_q(self.m, self.invq, dt, self.q[1])
+        else:
+            self.m = ((v * v) * (dt * dt)).astype(np.float32)  # Propagator::Vel2: dt*dt in FP32
         self.eng.set_model(self.m, eps, del_, pjx, pjy)
         # FREESURFACE switches the top PML off (bgprtm/FdEngine.cpp:45)
         self.eng.set_boundary(self.nbnd, self.dt, self.vmax, dim_pml=3, top=self.fs < 0)
@@ -116,18 +134,32 @@ class Shot:
     def _fresh(self):
         self.eng.clean_pml()
         z = np.zeros(self.shape, np.float32)
+        if self.q is not None:  # Wavefield::cleanMemory (bgprtm/Wavefield.cpp:116-118)
+            order = (self.q[1].size - 1) // 2
+            self.wq = [np.zeros((order,) + self.shape, np.float32) for _ in range(2)]
         return z.copy(), z.copy(), z.copy()
+
+    def _step(self, w1, w0, wr):
+        """Propagator::kernel() + update2nd; with Q the Q leapfrog on the memory-variable set that
+        swaps with the wavefields (bgprtm/Propagator.cpp:1789-1816, bgprtm/Wavefield.cpp:130-133)."""
+        if self.q is None:
+            self.eng.step(self.model_type, w1, w0, wr)
+            return
+        self.eng.kernel(self.model_type, w1, w0, wr)
+        update2nd_q(w0, w1, self.invq, self.wq[0], self.wq[1], self.q[1])
+        self.wq.reverse()
 
     def model(self):
         w0, w1, wr = self._fresh()
         nr = self.rcv["lx"].size
         rec = np.zeros((nr, self.nt), np.float32)
         for it in range(self.nt):
-            self.eng.step(self.model_type, w1, w0, wr)
+            self._step(w1, w0, wr)
             if 0 <= it - 1 < self.nt:
-                inject(w0, self.m, self.shape, self.src, self.wavelet[it - 1:it] * self.dt_dxyz)
+                inject(w0, self.m, self.shape, self.src, self.wavelet[it - 1:it] * self.dt_dxyz,
+                       self.src_mirror)
             self._surface(w0)
-            rec[:, it] = extract(w0, self.shape, self.rcv)
+            rec[:, it] = extract(w0, self.shape, self.rcv, self.ext_mirror)
             w0, w1 = w1, w0
         self.last = w1
         return rec
@@ -144,9 +176,10 @@ class Shot:
         snaps = [None] * ntsav
         w0, w1, wr = self._fresh()
         for it in range(self.nt):
-            self.eng.step(self.model_type, w1, w0, wr)
+            self._step(w1, w0, wr)
             if 0 <= it - 1 < self.nt:
-                inject(w0, self.m, self.shape, self.src, self.wavelet[it - 1:it] * self.dt_dxyz)
+                inject(w0, self.m, self.shape, self.src, self.wavelet[it - 1:it] * self.dt_dxyz,
+                       self.src_mirror)
             self._surface(w0)
             if it % k == 0 and it // k < ntsav:
                 snaps[it // k] = compress16(w0) if self.snap_format == 1 else w0.copy()
@@ -160,10 +193,10 @@ class Shot:
         w0, w1, wr = self._fresh()
         rec = np.ascontiguousarray(record, np.float32)
         for it in range(self.nt - 1, -1, -1):
-            self.eng.step(self.model_type, w1, w0, wr)
+            self._step(w1, w0, wr)
             ia = it + 1
             if 0 <= ia < self.nt:
-                inject(w0, self.m, self.shape, self.rcv, rec[:, ia] * self.dt_dxyz)
+                inject(w0, self.m, self.shape, self.rcv, rec[:, ia] * self.dt_dxyz, self.rcv_mirror)
             self._surface(w0)
             if it % k == 0 and it // k < ntsav:
                 s = it // k

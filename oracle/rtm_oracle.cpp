@@ -643,6 +643,47 @@ template<class T> void update2nd(T *w0, const T *w1, size_t n, int nthreads) {
   for(size_t i = 0; i < n; i++) w0[i] = (w1[i] + w1[i]) - w0[i];
 }
 
+// Q branches.  coeffs = Q::coeffs [cq0, cq[order], wq[order]]; Q::updateCq (bgprtm/Q.cpp:311-318) sums
+// cqsum in index order.
+// Propagator::Vel2 with Q: bgprtm/Propagator.cpp:725-741
+template<class T> void vel2_q(T *vel, T *invQ, size_t n, T dt, T cq0) {
+  T dt2 = dt * dt;
+  for(size_t i = 0; i < n; i++) {
+    T v = vel[i];
+    T invq = invQ[i];
+    T invq2 = invq * invq;
+    T correction = 1 / (1 - cq0 * invq);
+    vel[i] = v * v * dt2 * (1 - T(0.25) * invq2 + invq2 * invq2 / 48) * correction;
+    invQ[i] = invq * correction;
+  }
+}
+// Propagator::update2nd with Q: bgprtm/Propagator.cpp:1789-1816, one plane (nxz cells) at a time: the
+// leapfrog over the plane, then each memory-variable set l over the plane.  d0 = wq[iq0], d1 = wq[iq1]
+// ([order][n]).
+template<class T> void update2nd_q(T *w0, const T *w1, const T *invQ, T *d0all, const T *d1all, size_t nxz, int ny,
+                                   int order, const T *coeffs) {
+  const T *cq = coeffs + 1, *wq = coeffs + 1 + order;
+  T cqsum = 0;
+  for(int i = 0; i < order; i++) cqsum += cq[i];
+  const size_t n = nxz * ny;
+  for(int iy = 0; iy < ny; iy++) {
+    for(size_t ixz = 0; ixz < nxz; ixz++) {
+      size_t i = iy * nxz + ixz;
+      w0[i] = w1[i] * (2 + invQ[i] * cqsum) - w0[i];
+    }
+    for(int l = 0; l < order; l++) {
+      T *d0 = d0all + (size_t)l * n;
+      const T *d1 = d1all + (size_t)l * n;
+      T cl = cq[l], el = wq[l];
+      for(size_t ixz = 0; ixz < nxz; ixz++) {
+        size_t i = iy * nxz + ixz;
+        w0[i] += invQ[i] * ((1 + el) * d0[i] - d1[i] * 2);
+        d0[i] = cl * w1[i] + el * d0[i];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Spread kernels: bgprtm/Util.cpp:317-397 (CompactOrmsbySpreadCoeff).  The
 // reference takes sincf_unnorm/blackman from unvendored libCommon; this build
@@ -967,6 +1008,11 @@ void orc_cfd_derive_d(int n, double d, int sgn, const double *x, double *b, int 
     e->deriv1(axis, in, out, iacc, isign);                                                                            \
   }                                                                                                                   \
   void orc_update2nd_##S(T *w0, const T *w1, size_t n) { update2nd(w0, w1, n, 1); }                                   \
+  void orc_vel2_q_##S(T *vel, T *invq, size_t n, T dt, T cq0) { vel2_q(vel, invq, n, dt, cq0); }                       \
+  void orc_update2nd_q_##S(T *w0, const T *w1, const T *invq, T *d0, const T *d1, size_t nxz, int ny, int order,      \
+                           const T *coeffs) {                                                                         \
+    update2nd_q(w0, w1, invq, d0, d1, nxz, ny, order, coeffs);                                                        \
+  }                                                                                                                   \
   void orc_engine_step_##S(void *h, int model, const T *w1, T *w0, T *wr) {                                           \
     Engine<T> *e = (Engine<T> *)h;                                                                                    \
     e->kernel(model, w1, w0, wr);                                                                                     \
@@ -991,9 +1037,13 @@ void orc_taper_kernel(float *kernel, int l, int off, const float *taper, int nta
 // extraction (bgprtm/Receiver.cpp:394-426).  Receivers are given as flat
 // tables: kernels kx[nr*Lx], ky[nr*Ly], kz[nr*Lz] with lengths lx/ly/lz and
 // offsets offx/offy/offz; values val[nr] already scaled by dt_dxyz.
-void orc_inject(float *w0, const float *vel, int nx, int ny, int nz, int nr, const float *val, const float *kx, const float *ky,
-                const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz, const int *offx,
-                const int *offy, const int *offz) {
+// Ghost mirrors about the plane iz_mirror (>= 0; -1: none): Receiver::combine_src_omp with mirror_ghost
+// (bgprtm/Receiver.cpp:457-475) subtracts the summed buffer at depth z from 2*iz_mirror - z (vel taken
+// there) after every direct term, for mirrors at or below z = 0; extract_value (bgprtm/Receiver.cpp:
+// 394-426) subtracts the kz-weighted mirrored samples per (x, y) tap after the direct ones.
+void orc_inject_ghost(float *w0, const float *vel, int nx, int ny, int nz, int nr, const float *val, const float *kx,
+                      const float *ky, const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly,
+                      const int *lz, const int *offx, const int *offy, const int *offz, int iz_mirror) {
   size_t nxz = (size_t)nx * nz;
   std::vector<float> b((size_t)nx * ny * nz, 0.f);
   std::vector<char> touched((size_t)nx * ny * nz, 0);
@@ -1009,22 +1059,48 @@ void orc_inject(float *w0, const float *vel, int nx, int ny, int nz, int nr, con
   }
   for(size_t i = 0; i < (size_t)ny * nxz; i++)
     if(touched[i]) w0[i] += b[i] * vel[i];
+  if(iz_mirror < 0) return;
+  for(size_t col = 0; col < (size_t)nx * ny; col++) {
+    for(int z = 0; z < nz; z++) {
+      const size_t i = col * nz + z;
+      const int zm = 2 * iz_mirror - z;
+      if(!touched[i] || zm < 0 || zm >= nz) continue;
+      const size_t m = col * nz + zm;
+      w0[m] -= b[i] * vel[m];
+    }
+  }
 }
-void orc_extract(const float *w, int nx, int ny, int nz, int nr, float *out, const float *kx, const float *ky, const float *kz,
-                 int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz, const int *offx, const int *offy,
-                 const int *offz) {
+void orc_inject(float *w0, const float *vel, int nx, int ny, int nz, int nr, const float *val, const float *kx, const float *ky,
+                const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz, const int *offx,
+                const int *offy, const int *offz) {
+  orc_inject_ghost(w0, vel, nx, ny, nz, nr, val, kx, ky, kz, Lx, Ly, Lz, lx, ly, lz, offx, offy, offz, -1);
+}
+void orc_extract_ghost(const float *w, int nx, int ny, int nz, int nr, float *out, const float *kx, const float *ky,
+                       const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz,
+                       const int *offx, const int *offy, const int *offz, int iz_mirror) {
   for(int ir = 0; ir < nr; ir++) {
     float val = 0;
     for(int iy = 0; iy < ly[ir]; iy++) {
       float cy = ky[(size_t)ir * Ly + iy];
       for(int ix = 0; ix < lx[ir]; ix++) {
         float cx = cy * kx[(size_t)ir * Lx + ix];
-        size_t off = ((size_t)(iy + offy[ir]) * nx + ix + offx[ir]) * nz + offz[ir];
+        size_t col = (size_t)(iy + offy[ir]) * nx + ix + offx[ir];
+        size_t off = col * nz + offz[ir];
         for(int iz = 0; iz < lz[ir]; iz++) val += cx * kz[(size_t)ir * Lz + iz] * w[off + iz];
+        if(iz_mirror >= 0) {
+          const int top = 2 * iz_mirror - offz[ir];
+          const int lz2 = std::min(lz[ir], top + 1);
+          for(int iz = std::max(0, top - (nz - 1)); iz < lz2; iz++) val -= cx * kz[(size_t)ir * Lz + iz] * w[col * nz + top - iz];
+        }
       }
     }
     out[ir] = val;
   }
+}
+void orc_extract(const float *w, int nx, int ny, int nz, int nr, float *out, const float *kx, const float *ky, const float *kz,
+                 int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz, const int *offx, const int *offy,
+                 const int *offz) {
+  orc_extract_ghost(w, nx, ny, nz, nr, out, kx, ky, kz, Lx, Ly, Lz, lx, ly, lz, offx, offy, offz, -1);
 }
 
 // Propagator::apply_symmetry (bgprtm/Propagator.cpp:1857-1918): free-surface mirror above iz0 with
