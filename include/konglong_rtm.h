@@ -275,6 +275,15 @@ int klrtm_propagator_set_q(klrtm_propagator *p, const float *invq_host, int orde
  * too; the top PML stays on.  A FREESURFACE config (free_surface_iz >= 0) mirrors both sides' injection
  * about the surface by itself.  source_side = receiver_side = 0 switches the ghosts off. */
 int klrtm_propagator_set_ghost(klrtm_propagator *p, int iz_mirror, int source_side, int receiver_side);
+/* snapshot cache outside HBM (CacheFile, bgprtm/CacheFile.cpp:32-109): klrtm_propagator_rtm keeps two
+ * device staging slots and streams every snapshot to host memory when all of them fit mem_limit_gb
+ * ("cache_mem_limit"), else to the file at path (SEQ_ZXYT: snapshot s at byte offset s*stride, stride =
+ * the code size, for COMPRESS_16 2*klrtm_nshort_volume; removed on destruction unless keep).  Writes and
+ * reads run on a host thread beside the propagation.  FP32, COMPRESS_16 and learned snapshots; call
+ * before the first rtm call.  The stepping API (forward_steps/backward_steps) keeps snapshots in HBM. */
+int klrtm_propagator_set_snapshot_cache(klrtm_propagator *p, const char *path, double mem_limit_gb, int keep);
+/* after an rtm call with a cache: whether it lives in host memory, its stride and snapshot count */
+int klrtm_propagator_snapshot_cache_info(klrtm_propagator *p, int *in_memory, size_t *stride_bytes, int *nsnap);
 /* source wavelet samples (host, nt floats); replaces the built-in Ricker (Source::setupWavelet) */
 int klrtm_propagator_set_source_wavelet(klrtm_propagator *p, const float *wavelet_host);
 /* receivers: rcv_xyz[3*nrcv] grid coordinates */

@@ -17,6 +17,7 @@
 #include "fd_kernels.h"
 #include "surface.h"
 #include "qatten.h"
+#include "cachefile.h"
 #include <cmath>
 #include <cstring>
 #include <algorithm>
@@ -45,6 +46,15 @@ struct Propagator {
   int q_order = 0, iq0 = 0;
   bool q_ready = false;  // set_model has run the Q branch of Vel2 since the last set_q
   float q_coeffs[1 + 2 * Q_MAX_ORDER] = {};
+  // snapshot cache outside HBM (CacheFile): two device staging slots, the snapshots in a file or in
+  // host memory
+  bool use_cache = false;
+  std::string cache_path;
+  double cache_mem_gb = 0;
+  int cache_keep = 0;
+  SnapCache *cache = nullptr;
+  size_t snap_bytes = 0;  // unpadded code size of one snapshot (the cache stride)
+  char *slot(int itsav) const { return d_snap + snap_stride_bytes * (size_t)(use_cache ? (itsav & 1) : itsav); }
   // ghost mirror plane of SOURCE_GHOST / RECEIVER_GHOST / GHOST surfaces (-1: none) and its sides
   int ghost_iz = -1;
   bool ghost_src = false, ghost_rcv = false;
@@ -68,6 +78,7 @@ struct Propagator {
 };
 
 Propagator::~Propagator() {
+  if(cache) snapcache_destroy(cache);
   delete eng;
   if(src) receiver_destroy(src);
   if(rcv) receiver_destroy(rcv);
@@ -110,10 +121,21 @@ int Propagator::alloc_snapshots() {
     break;
   default: set_error("unknown snapshot format %d", c.snap_format); return KLRTM_EINVAL;
   }
+  snap_bytes = snap_stride_bytes;
   snap_stride_bytes = (snap_stride_bytes + 255) & ~(size_t)255;
-  hipError_t e = hipMalloc(&d_snap, snap_stride_bytes * ntsav);
+  int dev_slots = ntsav;
+  if(use_cache) {
+    KL_REQUIRE(!nscales, KLRTM_EINVAL, "the snapshot cache takes FP32, COMPRESS_16 or learned snapshots");
+    ntsav = (c.nt + k - 1) / k;  // every snapshot of the shot goes to the cache
+    dev_slots = 2;
+    int rc = cache ? KLRTM_OK
+                   : snapcache_create(&cache, cache_path.empty() ? nullptr : cache_path.c_str(), cache_mem_gb, cache_keep,
+                                      snap_bytes, ntsav);
+    if(rc) return rc;
+  }
+  hipError_t e = hipMalloc(&d_snap, snap_stride_bytes * dev_slots);
   if(e != hipSuccess) {
-    set_error("snapshot store of %d x %zu bytes does not fit: %s", ntsav, snap_stride_bytes, hipGetErrorString(e));
+    set_error("snapshot store of %d x %zu bytes does not fit: %s", dev_slots, snap_stride_bytes, hipGetErrorString(e));
     d_snap = nullptr;
     return KLRTM_ENOMEM;
   }
@@ -163,15 +185,18 @@ int Propagator::step(int) {
 }
 
 int Propagator::store(int itsav) {
-  char *dst = d_snap + snap_stride_bytes * (size_t)itsav;
+  char *dst = slot(itsav);
+  int rc = KLRTM_OK;
   switch(c.snap_format) {
   case KLRTM_SNAP_FP32:
     KL_HIP(hipMemcpyAsync(dst, w0, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
-    return KLRTM_OK;
-  case KLRTM_SNAP_C16: return c16_encode(w0, (uint16_t *)dst, c.nx, c.ny, c.nz, d_smax);
-  case KLRTM_SNAP_LEARNED: return lcodec_encode(*lc, w0, dst, c.nx, c.ny, c.nz);
-  default: return blockq_encode(w0, (uint16_t *)dst, d_scales + nscales * (size_t)itsav, c.nx, c.ny, c.nz, c.snap_format);
+    break;
+  case KLRTM_SNAP_C16: rc = c16_encode(w0, (uint16_t *)dst, c.nx, c.ny, c.nz, d_smax); break;
+  case KLRTM_SNAP_LEARNED: rc = lcodec_encode(*lc, w0, dst, c.nx, c.ny, c.nz); break;
+  default: rc = blockq_encode(w0, (uint16_t *)dst, d_scales + nscales * (size_t)itsav, c.nx, c.ny, c.nz, c.snap_format);
   }
+  if(rc || !use_cache) return rc;
+  return snapcache_store(*cache, itsav, dst);  // CacheFile::pwrite at itsav*stride (bgprtm/Propagator.cpp:1233-1243)
 }
 
 int Propagator::image(int itsav) {
@@ -187,7 +212,11 @@ int Propagator::image(int itsav) {
     int rc = c16_slice_max(w0, c.nx, c.ny, c.nz, d_smax);
     if(rc) return rc;
   }
-  const char *snap = d_snap + snap_stride_bytes * (size_t)itsav;
+  if(use_cache) {  // the snapshot from the cache, the next one (in decreasing order) read meanwhile
+    int rc = snapcache_load(*cache, itsav, slot(itsav));
+    if(rc || (rc = snapcache_prefetch(*cache, itsav - 1))) return rc;
+  }
+  const char *snap = slot(itsav);
   double *i64 = c.ic_fp32_only ? nullptr : d_img64, *l64 = c.ic_fp32_only ? nullptr : d_ill64;
   if(c.snap_format == KLRTM_SNAP_LEARNED) return lcodec_image(*lc, a, snap, w0, d_img, d_ill, i64, l64);
   const float *sc = nscales ? d_scales + nscales * (size_t)itsav : nullptr;
@@ -368,6 +397,7 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
     std::swap(P.w0, P.w1);
   }
   KL_HIP(hipEventRecord(P.ev[1], stream()));
+  if(P.use_cache && (rc = snapcache_flush(*P.cache))) return rc;
   // backward: receiver wavefield from the record, imaging at the stored steps
   if(record && (rc = receiver_set_traces(*P.rcv, record))) return rc;
   if(!record && (rc = receiver_set_traces_device(*P.rcv, receiver_record_device(*P.rcv)))) return rc;
@@ -398,6 +428,7 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
 }
 
 int propagator_forward_steps(Propagator &P, int it0, int nsteps, int store_snapshots) {
+  KL_REQUIRE(!P.use_cache, KLRTM_ESTATE, "the stepping API keeps its snapshots in HBM (no snapshot cache)");
   int rc;
   if(store_snapshots && (rc = P.alloc_snapshots())) return rc;
   const int k = std::max(1, P.c.snap_every);
@@ -415,6 +446,7 @@ int propagator_forward_steps(Propagator &P, int it0, int nsteps, int store_snaps
 
 int propagator_backward_steps(Propagator &P, int it0, int nsteps, int image) {
   KL_REQUIRE(P.rcv, KLRTM_ESTATE, "receivers not set");
+  KL_REQUIRE(!P.use_cache, KLRTM_ESTATE, "the stepping API keeps its snapshots in HBM (no snapshot cache)");
   int rc;
   if(image && (rc = P.alloc_snapshots())) return rc;
   const int k = std::max(1, P.c.snap_every);
@@ -461,6 +493,26 @@ int klrtm_propagator_set_ghost(klrtm_propagator *p, int iz_mirror, int source_si
              "ghost mirror plane %d needs 0 <= 2*iz < nz (%d)", iz_mirror, P.c.nz);
   P.ghost_iz = any ? iz_mirror : -1;
   P.ghost_src = source_side != 0, P.ghost_rcv = receiver_side != 0;
+  return KLRTM_OK;
+}
+int klrtm_propagator_set_snapshot_cache(klrtm_propagator *p, const char *path, double mem_limit_gb, int keep) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  KL_REQUIRE(!P.d_snap, KLRTM_ESTATE, "snapshots are already allocated: set the cache before the first rtm call");
+  KL_REQUIRE(mem_limit_gb >= 0, KLRTM_EINVAL, "negative memory limit");
+  P.use_cache = true;
+  P.cache_path = path ? path : "";
+  P.cache_mem_gb = mem_limit_gb;
+  P.cache_keep = keep;
+  return KLRTM_OK;
+}
+int klrtm_propagator_snapshot_cache_info(klrtm_propagator *p, int *in_memory, size_t *stride_bytes, int *nsnap) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  KL_REQUIRE(P.cache, KLRTM_ESTATE, "no snapshot cache in use (set one, then run rtm)");
+  if(in_memory) *in_memory = snapcache_in_memory(*P.cache) ? 1 : 0;
+  if(stride_bytes) *stride_bytes = P.snap_bytes;
+  if(nsnap) *nsnap = P.ntsav;
   return KLRTM_OK;
 }
 int klrtm_propagator_set_source_wavelet(klrtm_propagator *p, const float *wav) {

@@ -63,6 +63,17 @@ class Propagator:
         receiver_side the modelling extraction) about iz_mirror."""
         check(lib().klrtm_propagator_set_ghost(self.h, int(iz_mirror), int(source_side), int(receiver_side)))
 
+    def set_snapshot_cache(self, path=None, mem_limit_gb=0.0, keep=False):
+        """CacheFile: stream the snapshots of rtm to host memory (when they fit mem_limit_gb) or to the
+        file at path instead of holding them in HBM."""
+        check(lib().klrtm_propagator_set_snapshot_cache(self.h, None if path is None else str(path).encode(),
+                                                        float(mem_limit_gb), int(keep)))
+
+    def snapshot_cache_info(self):
+        m, st, ns = C.c_int(), C.c_size_t(), C.c_int()
+        check(lib().klrtm_propagator_snapshot_cache_info(self.h, C.byref(m), C.byref(st), C.byref(ns)))
+        return dict(in_memory=bool(m.value), stride=st.value, nsnap=ns.value)
+
     def set_source_wavelet(self, wavelet):
         w = np.ascontiguousarray(wavelet, np.float32)
         assert w.shape == (self.cfg.nt,)

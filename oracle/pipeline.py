@@ -207,6 +207,7 @@ class Shot:
                 img += S * R
                 ill += S * S
             w0, w1 = w1, w0
+        self.snaps = snaps  # the stored snapshots (COMPRESS_16 codes or FP32 volumes), for cache checks
         return img, ill
 
 
