@@ -30,6 +30,14 @@ public:
     const int order = coeffs.empty() ? 0 : (int)(coeffs.size() - 1) / 2;
     konglong::check(klrtm_propagator_set_q(p_, invQ, order, coeffs.empty() ? nullptr : coeffs.data()));
   }
+  // SOURCE_GHOST / RECEIVER_GHOST / GHOST: mirror the chosen sides' injection about iz_mirror
+  void setGhost(int izMirror, bool sourceSide, bool receiverSide) {
+    konglong::check(klrtm_propagator_set_ghost(p_, izMirror, sourceSide ? 1 : 0, receiverSide ? 1 : 0));
+  }
+  // CacheFile: snapshots to host memory (when they fit memLimitGB) or to the file at path
+  void setSnapshotCache(const char *path, double memLimitGB, bool keep = false) {
+    konglong::check(klrtm_propagator_set_snapshot_cache(p_, path, memLimitGB, keep ? 1 : 0));
+  }
   void setReceivers(const std::vector<float> &xyz) { konglong::check(klrtm_propagator_set_receivers(p_, xyz.data())); }
   void setSourceWavelet(const std::vector<float> &w) {
     konglong::check(klrtm_propagator_set_source_wavelet(p_, w.data()));
