@@ -106,6 +106,7 @@ _SIGS = {
     "klrtm_propagator_destroy": (_i, [_vp]),
     "klrtm_propagator_set_model": (_i, [_vp, _pf, _pf, _pf, _pf, _pf]),
     "klrtm_propagator_set_q": (_i, [_vp, _pf, _i, _pf]),
+    "klrtm_propagator_set_rho": (_i, [_vp, _pf]),
     "klrtm_propagator_set_ghost": (_i, [_vp, _i, _i, _i]),
     "klrtm_propagator_set_snapshot_cache": (_i, [_vp, C.c_char_p, C.c_double, _i]),
     "klrtm_propagator_snapshot_cache_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_sz), C.POINTER(_i)]),

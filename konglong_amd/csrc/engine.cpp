@@ -193,6 +193,7 @@ void Engine::fill_args(IsoArgs &a, int deriv_order) {
   } else {
     float zpi = 1 / dz, dxi = 1 / dx, dyi = 1 / dy;
     for(int m = 0; m <= ne; m++) a.cz[m] = zpi * c1[m], a.cx[m] = c1[m] * dxi, a.cy[m] = c1[m] * dyi;
+    a.rho = rho;
   }
 }
 
@@ -217,7 +218,9 @@ int Engine::run(int model, int mode, const float *p0, float *pb, float *pr) {
   int rc = check_model(model);
   if(rc) return rc;
   KL_REQUIRE(p0 && pb, KLRTM_EINVAL, "null wavefield pointer");
-  bool laplace = engine == KLRTM_ENGINE_FD && model == KLRTM_ISO;  // bgprtm/KernelCPU.cpp:128-134
+  // the Laplacian form needs no density: rho forces the first-derivative form (force_tti,
+  // bgprtm/ModelLoader.cpp:63, bgprtm/KernelCPU.cpp:128-134)
+  bool laplace = engine == KLRTM_ENGINE_FD && model == KLRTM_ISO && !rho;
   if(laplace) {
     KL_REQUIRE(mode == MODE_STEP || pr, KLRTM_EINVAL, "ISO needs the pr scratch volume");
     IsoArgs a;
@@ -280,8 +283,9 @@ int klrtm_kernel_set_model(klrtm_kernel *kk, const float *vel, const float *rho,
                            const float *pjx, const float *pjy) {
   Engine *k = reinterpret_cast<Engine *>(kk);
   KL_REQUIRE(k && vel, KLRTM_EINVAL, "null engine or vel");
-  KL_REQUIRE(!rho, KLRTM_EINVAL, "density (rho) is not supported by this build");
-  k->vel = vel, k->del = del, k->eps = eps, k->pjx = pjx, k->pjy = pjy;
+  KL_REQUIRE(!rho || k->engine == KLRTM_ENGINE_FD, KLRTM_EINVAL,
+             "density (rho) runs with the FD engines; the compact (CFD) engine does not take it");
+  k->vel = vel, k->rho = rho, k->del = del, k->eps = eps, k->pjx = pjx, k->pjy = pjy;
   return KLRTM_OK;
 }
 int klrtm_kernel_set_pml_options(klrtm_kernel *kk, float pml_eps, int pml3d, float scaler, int top) {

@@ -15,6 +15,7 @@ struct IsoArgs {
   int has_pml;
   float cz[9], cx[9], cy[9];  // [0..R]: 2nd-derivative (Laplacian) or 1st-derivative (grad/div) weights
   PmlDesc pml;
+  const float *rho;  // density (first-derivative form only): gradient / rho, divergence terms * vel * rho
 };
 
 int launch_iso(int R, int mode, const float *u, float *w0, const float *vel, float *pr, const IsoArgs &a);

@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(512) iso_step_kernel(const float *__restrict__
 
 // ---------------------------------------------------------------------------
 // Gradient + PML round 1 + scalar operator.  Writes px, py (= pr), pz.
-template<int R, int MODEL>
+template<int R, int MODEL, bool RHO>
 __global__ void __launch_bounds__(1024) grad_kernel(const float *__restrict__ u, float *__restrict__ px,
                                                     float *__restrict__ py, float *__restrict__ pz,
                                                     const float *__restrict__ eps, const float *__restrict__ del,
@@ -360,6 +360,12 @@ __global__ void __launch_bounds__(1024) grad_kernel(const float *__restrict__ u,
         if(do_y) gy = pml1_y<true>(a.pml, 0, gy, ix, iy, iz, nx, ny, nz);
       }
       const size_t i = pl + c.col;
+      if(RHO) {  // KernelCPU::divideRho before the scalar operator (bgprtm/KernelCPU.cpp:103, 340-349)
+        const float r = a.rho[i];
+        gx /= r;
+        if(do_y) gy /= r;
+        gz /= r;
+      }
       if(MODEL == KLRTM_VTI) scalar_vti(e, d, gx, gy, gz);
       else if(MODEL == KLRTM_TTI) scalar_tti(e, d, ax, ay, gx, gy, gz);
       px[i] = gx;
@@ -375,7 +381,7 @@ __global__ void __launch_bounds__(1024) grad_kernel(const float *__restrict__ u,
 // Divergence + PML round 2 + pb -= vel*div (x, z, then y as the reference)
 // (+ update2nd in MODE_STEP).  Dx needs x-neighbours of px, Dz z-neighbours of
 // pz, Dy y-neighbours of py: two LDS tiles and one register queue.
-template<int R, int MODE>
+template<int R, int MODE, bool RHO>
 __global__ void __launch_bounds__(1024) div_kernel(const float *__restrict__ px, const float *__restrict__ py,
                                                    const float *__restrict__ pz, const float *__restrict__ w1,
                                                    float *__restrict__ w0, const float *__restrict__ vel,
@@ -398,7 +404,8 @@ __global__ void __launch_bounds__(1024) div_kernel(const float *__restrict__ px,
   }
   float *zrow = sz + c.tx * SZ + c.tz + R;
   float *xrow = sx + (c.tx + R) * TZ + c.tz;
-  auto fetch = [&](size_t p, float &qv, float &cz, float &cx, Halo &hz, Halo &hx, float &wv, float &vv, float &uv, int iyp) {
+  auto fetch = [&](size_t p, float &qv, float &cz, float &cx, Halo &hz, Halo &hx, float &wv, float &vv, float &uv,
+                   float &rv, int iyp) {
     hz = load_halo<R>(pz, p, c, a, true, false);
     hx = load_halo<R>(px, p, c, a, false, true);
     if(c.act) {
@@ -406,17 +413,18 @@ __global__ void __launch_bounds__(1024) div_kernel(const float *__restrict__ px,
       cz = pz[p + c.col], cx = px[p + c.col];
       wv = w0[p + c.col], vv = vel[p + c.col];
       if(MODE == MODE_STEP) uv = w1[p + c.col];
+      if(RHO) rv = a.rho[p + c.col];
     }
   };
-  float qn = 0.f, czn = 0.f, cxn = 0.f, wn = 0.f, vn = 0.f, un = 0.f;
+  float qn = 0.f, czn = 0.f, cxn = 0.f, wn = 0.f, vn = 0.f, un = 0.f, rn = 1.f;
   Halo hzn, hxn;
-  fetch((size_t)c.y0 * nxz, qn, czn, cxn, hzn, hxn, wn, vn, un, c.y0);
+  fetch((size_t)c.y0 * nxz, qn, czn, cxn, hzn, hxn, wn, vn, un, rn, c.y0);
   for(int iy = c.y0; iy < c.y1; iy++) {
     const size_t pl = (size_t)iy * nxz;
     q[2 * R] = qn;
-    const float cz = czn, cx = cxn, wv = wn, vv = vn, uv = un;
+    const float cz = czn, cx = cxn, wv = wn, vv = vn, uv = un, rv = rn;
     const Halo hz = hzn, hx = hxn;
-    if(iy + 1 < c.y1) fetch(pl + nxz, qn, czn, cxn, hzn, hxn, wn, vn, un, iy + 1);
+    if(iy + 1 < c.y1) fetch(pl + nxz, qn, czn, cxn, hzn, hxn, wn, vn, un, rn, iy + 1);
     lds_barrier();
     zrow[0] = cz;
     if(c.tz < R) zrow[-R] = hz.z0;
@@ -450,9 +458,16 @@ __global__ void __launch_bounds__(1024) div_kernel(const float *__restrict__ px,
         dzv = pml1_z<true>(a.pml, 1, dzv, ix, iy, iz, nx, ny, nz);
         if(do_y) dyv = pml1_y<true>(a.pml, 1, dyv, ix, iy, iz, nx, ny, nz);
       }
-      float w = fmaf(-dxv, vv, wv);
-      w = fmaf(-dzv, vv, w);
-      if(do_y) w = fmaf(-dyv, vv, w);
+      float w;
+      if(RHO) {  // out -= d*vel*rho (bgprtm/FdEngine_fd.cpp:409-412, 435-438, 461-464)
+        w = fmaf(-(dxv * vv), rv, wv);
+        w = fmaf(-(dzv * vv), rv, w);
+        if(do_y) w = fmaf(-(dyv * vv), rv, w);
+      } else {
+        w = fmaf(-dxv, vv, wv);
+        w = fmaf(-dzv, vv, w);
+        if(do_y) w = fmaf(-dyv, vv, w);
+      }
       if(MODE == MODE_STEP) w = (uv + uv) - w;
       w0[pl + c.col] = w;
     }
@@ -490,7 +505,7 @@ __global__ void __launch_bounds__(256) fd_deriv1_kernel(const float *__restrict_
       else d = pml1_y<true>(a.pml, round, d, ix, iy, iz, nx, ny, nz);
     }
   }
-  if(IACC) out[i] = fmaf(-d, vel[i], out[i]);
+  if(IACC) out[i] = a.rho ? fmaf(-(d * vel[i]), a.rho[i], out[i]) : fmaf(-d, vel[i], out[i]);
   else out[i] = d;
 }
 
@@ -563,8 +578,12 @@ static int launch_grad_rm(const float *u, float *px, float *py, float *pz, const
   int nb = a.nbz * a.nbx * a.nby;
   size_t lds = sizeof(float) * (size_t)(a.tx + 2 * R) * (a.tz + 2 * R);
   LaunchTimer tm("fd_grad");
-  hipLaunchKernelGGL((grad_kernel<R, MODEL>), dim3(nb), dim3(a.tz * a.tx), lds, stream(), u, px, py, pz, eps, del, pjx,
-                     pjy, a);
+  if(a.rho)
+    hipLaunchKernelGGL((grad_kernel<R, MODEL, true>), dim3(nb), dim3(a.tz * a.tx), lds, stream(), u, px, py, pz, eps, del,
+                       pjx, pjy, a);
+  else
+    hipLaunchKernelGGL((grad_kernel<R, MODEL, false>), dim3(nb), dim3(a.tz * a.tx), lds, stream(), u, px, py, pz, eps, del,
+                       pjx, pjy, a);
   return check_launch("grad_kernel");
 }
 template<int R>
@@ -593,11 +612,15 @@ static int launch_div_r(int mode, const float *px, const float *py, const float 
   int nb = a.nbz * a.nbx * a.nby;
   size_t lds = sizeof(float) * ((size_t)a.tx * (a.tz + 2 * R) + (size_t)(a.tx + 2 * R) * a.tz);
   LaunchTimer tm(mode == MODE_STEP ? "fd_div_step" : "fd_div");
-  if(mode == MODE_STEP)
-    hipLaunchKernelGGL((div_kernel<R, MODE_STEP>), dim3(nb), dim3(a.tz * a.tx), lds, stream(), px, py, pz, w1, w0, vel, a);
+  const dim3 g(nb), b(a.tz * a.tx);
+  if(mode == MODE_STEP && a.rho)
+    hipLaunchKernelGGL((div_kernel<R, MODE_STEP, true>), g, b, lds, stream(), px, py, pz, w1, w0, vel, a);
+  else if(mode == MODE_STEP)
+    hipLaunchKernelGGL((div_kernel<R, MODE_STEP, false>), g, b, lds, stream(), px, py, pz, w1, w0, vel, a);
+  else if(a.rho)
+    hipLaunchKernelGGL((div_kernel<R, MODE_KERNEL, true>), g, b, lds, stream(), px, py, pz, w1, w0, vel, a);
   else
-    hipLaunchKernelGGL((div_kernel<R, MODE_KERNEL>), dim3(nb), dim3(a.tz * a.tx), lds, stream(), px, py, pz, w1, w0, vel,
-                       a);
+    hipLaunchKernelGGL((div_kernel<R, MODE_KERNEL, false>), g, b, lds, stream(), px, py, pz, w1, w0, vel, a);
   return check_launch("div_kernel");
 }
 int launch_div(int R, int mode, const float *px, const float *py, const float *pz, const float *w1, float *w0,

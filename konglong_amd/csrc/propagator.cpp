@@ -29,6 +29,8 @@ struct Propagator {
   Engine *eng = nullptr;
   size_t n = 0;
   float *d_vel = nullptr, *d_eps = nullptr, *d_del = nullptr, *d_pjx = nullptr, *d_pjy = nullptr;
+  float *d_rho = nullptr;
+  bool use_rho = false;
   float *w0 = nullptr, *w1 = nullptr, *wr = nullptr;
   Receiver *src = nullptr, *rcv = nullptr;
   int ntsav = 0;
@@ -84,7 +86,7 @@ Propagator::~Propagator() {
   if(rcv) receiver_destroy(rcv);
   for(void *p : {(void *)d_vel, (void *)d_eps, (void *)d_del, (void *)d_pjx, (void *)d_pjy, (void *)w0, (void *)w1,
                  (void *)wr, (void *)d_snap, (void *)d_scales, (void *)d_smax, (void *)d_img, (void *)d_ill,
-                 (void *)d_img64, (void *)d_ill64, (void *)d_taper_z, (void *)d_invq_raw, (void *)d_invq,
+                 (void *)d_img64, (void *)d_ill64, (void *)d_taper_z, (void *)d_invq_raw, (void *)d_invq, (void *)d_rho,
                  (void *)d_q[0], (void *)d_q[1]})
     (void)hipFree(p);
   for(auto &e : ev)
@@ -324,8 +326,8 @@ int propagator_set_model(Propagator &P, const float *vel, const float *eps, cons
     if((rc = launch_vel2_q(P.d_vel, P.d_invq, P.n, P.c.dt, P.q_coeffs[0]))) return rc;
     P.q_ready = true;
   } else if((rc = launch_vel2(P.d_vel, P.n, P.c.dt))) return rc;  // Propagator::Vel2
-  return klrtm_kernel_set_model(reinterpret_cast<klrtm_kernel *>(P.eng), P.d_vel, nullptr, P.d_del, P.d_eps, P.d_pjx,
-                                P.d_pjy);
+  return klrtm_kernel_set_model(reinterpret_cast<klrtm_kernel *>(P.eng), P.d_vel, P.use_rho ? P.d_rho : nullptr, P.d_del,
+                                P.d_eps, P.d_pjx, P.d_pjy);
 }
 
 int propagator_set_q(Propagator &P, const float *invq, int order, const float *coeffs) {
@@ -480,6 +482,19 @@ int klrtm_propagator_set_model(klrtm_propagator *p, const float *vel, const floa
                                const float *pjy) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
   return propagator_set_model(*reinterpret_cast<Propagator *>(p), vel, eps, del, pjx, pjy);
+}
+int klrtm_propagator_set_rho(klrtm_propagator *p, const float *rho) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  KL_REQUIRE(!rho || P.c.engine == KLRTM_ENGINE_FD, KLRTM_EINVAL,
+             "density (rho) runs with the FD engines; the compact (CFD) engine does not take it");
+  P.use_rho = rho != nullptr;
+  int rc;
+  if(rho && (rc = upload_opt(&P.d_rho, rho, P.n))) return rc;
+  if(!P.eng->vel) return KLRTM_OK;  // set_model registers it
+  // the model is already on the device: re-register its volumes with the new density
+  return klrtm_kernel_set_model(reinterpret_cast<klrtm_kernel *>(P.eng), P.d_vel, P.use_rho ? P.d_rho : nullptr, P.d_del,
+                                P.d_eps, P.d_pjx, P.d_pjy);
 }
 int klrtm_propagator_set_q(klrtm_propagator *p, const float *invq, int order, const float *coeffs) {
   KL_REQUIRE(p && order >= 0, KLRTM_EINVAL, "bad argument");

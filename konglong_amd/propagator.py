@@ -47,6 +47,13 @@ class Propagator:
         arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (vel, eps, del_, pjx, pjy)]
         check(lib().klrtm_propagator_set_model(self.h, *[fptr(a) for a in arrs]))
 
+    def set_rho(self, rho):
+        """Density (Model::useRho, FD engines): the gradient is divided by rho and each divergence term
+        scaled by vel*rho; ISO takes the first-derivative form.  None switches it off."""
+        r = None if rho is None else np.ascontiguousarray(rho, np.float32)
+        assert r is None or r.shape == self.shape
+        check(lib().klrtm_propagator_set_rho(self.h, fptr(r)))
+
     def set_q(self, invq, coeffs):
         """Q attenuation (Model::useQ): the 1/Q volume and Q::coeffs [cq0, cq[order], wq[order]]; call
         before set_model.  invq None switches Q off."""

@@ -39,6 +39,7 @@ def lib():
                 getattr(L, f"orc_engine_{n}_{s}").argtypes = [C.c_void_p]
             getattr(L, f"orc_engine_set_threads_{s}").argtypes = [C.c_void_p, C.c_int]
             getattr(L, f"orc_engine_set_model_{s}").argtypes = [C.c_void_p] + [C.c_void_p] * 5
+            getattr(L, f"orc_engine_set_rho_{s}").argtypes = [C.c_void_p, C.c_void_p]
             getattr(L, f"orc_engine_set_boundary_{s}").argtypes = [C.c_void_p, _pi, T, T, T, C.c_int, T, C.c_int, C.c_int]
             getattr(L, f"orc_engine_kernel_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
             getattr(L, f"orc_engine_step_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -119,6 +120,11 @@ class Engine:
         arrs = [self._c(a) for a in (vel, eps, del_, pjx, pjy)]
         self._keep = arrs
         getattr(lib(), f"orc_engine_set_model_{self.s}")(self.h, *[None if a is None else a.ctypes.data for a in arrs])
+
+    def set_rho(self, rho):
+        """Density (KernelCPU::setModel's rho): None switches it off."""
+        self._rho = None if rho is None else self._c(rho)
+        getattr(lib(), f"orc_engine_set_rho_{self.s}")(self.h, None if self._rho is None else self._rho.ctypes.data)
 
     def set_boundary(self, nb, dt, vmax, pml_eps=1e-3, pml3d=True, scaler=0.3, top=True, dim_pml=3):
         nb = np.ascontiguousarray(nb, np.int32)

@@ -89,7 +89,7 @@ class Shot:
 
     def __init__(self, nx, ny, nz, d, dt, nt, engine, order, model, nbnd, vmax, src_xyz, f_peak, spread, k1, k2,
                  rcv_xyz, snap_every=1, snap_format=0, tmute_frac=0.1, threads=4, free_surface_iz=-1, surface_taper=1,
-                 q=None, ghost=None):
+                 q=None, ghost=None, rho=None):
         self.shape = (ny, nx, nz)
         self.fs, self.fs_taper = free_surface_iz, surface_taper
         # injection mirrors (Propagator::determine_ghost, bgprtm/Propagator.cpp:1538-1541): FREESURFACE
@@ -101,6 +101,7 @@ class Shot:
         self.ext_mirror = gz if gr else -1
         # Q attenuation: (1/Q volume, Q::coeffs [cq0, cq[order], wq[order]])
         self.q = None if q is None else (np.ascontiguousarray(q[0], np.float32), np.asarray(q[1], np.float32))
+        self.rho = None if rho is None else np.ascontiguousarray(rho, np.float32)
         self.nx, self.ny, self.nz, self.dt, self.nt = nx, ny, nz, dt, nt
         self.model_type, self.snap_every, self.snap_format = model, snap_every, snap_format
         self.eng = Engine(nx, ny, nz, d, d, d, engine, order, np.float32, threads=threads)
@@ -121,6 +122,7 @@ class Shot:
         else:
             self.m = ((v * v) * (dt * dt)).astype(np.float32)  # Propagator::Vel2: dt*dt in FP32
         self.eng.set_model(self.m, eps, del_, pjx, pjy)
+        self.eng.set_rho(self.rho)
         # FREESURFACE switches the top PML off (bgprtm/FdEngine.cpp:45)
         self.eng.set_boundary(self.nbnd, self.dt, self.vmax, dim_pml=3, top=self.fs < 0)
 

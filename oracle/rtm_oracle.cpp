@@ -228,6 +228,7 @@ template<class T> struct Engine {
   std::unique_ptr<Pml<T>> pml[3][2];
   std::vector<T> buf[3][2][2][3];
   const T *vel = nullptr, *eps = nullptr, *del = nullptr, *pjx = nullptr, *pjy = nullptr;
+  const T *rho = nullptr;  // density: forces the first-derivative form (force_tti, bgprtm/ModelLoader.cpp:63)
   CfdPlan<T> cfd[3];  // [AX_Z, AX_X, AX_Y]
   std::vector<T> px2d, pz2d, wrk;
   int nthreads = 1;
@@ -436,6 +437,10 @@ template<class T> struct Engine {
       for(size_t i = (size_t)iy * nxz; i < (size_t)(iy + 1) * nxz; i++) pb[i] = pb[i] - pr[i] * vel[i];
   }
 
+  // density at the cells of a column or slice of vel (nullptr without density): the accumulates
+  // multiply by vel*rho (bgprtm/FdEngine_fd.cpp:409-412, bgprtm/FdEngine_cfd.cpp:42, 66, 91)
+  const T *rho_of(const T *v) const { return (rho && v) ? rho + (v - vel) : nullptr; }
+
   // --- FD first derivatives, one column: bgprtm/FdEngine_fd.cpp:388-470 ----
   // iacc=0: out = PML(D u); iacc=1: out -= PML(D u) * vel
   void fd_dz_col(const T *u, T *out, int ix, int iy, int iacc, int isign, const T *vcol) {
@@ -450,8 +455,13 @@ template<class T> struct Engine {
       for(int iz = iz0; iz < iz1; iz++) vz[iz] += c * u[iz + m];
     }
     pml1_zbnd(vz, ix, iy, isign);
-    if(iacc)
-      for(int iz = 0; iz < nz; iz++) out[iz] -= vz[iz] * vcol[iz];
+    if(iacc) {
+      const T *rcol = rho_of(vcol);
+      if(rcol)
+        for(int iz = 0; iz < nz; iz++) out[iz] -= vz[iz] * vcol[iz] * rcol[iz];
+      else
+        for(int iz = 0; iz < nz; iz++) out[iz] -= vz[iz] * vcol[iz];
+    }
   }
   void fd_dx_col(const T *u, T *out, int ix, int iy, int iacc, int isign, const T *vcol) {
     thread_local std::vector<T> tmp;
@@ -467,8 +477,13 @@ template<class T> struct Engine {
         for(int iz = 0; iz < nz; iz++) vx[iz] += c * u[iz + (ptrdiff_t)m * nz];
     }
     pml1_xbnd(vx, ix, iy, isign);
-    if(iacc)
-      for(int iz = 0; iz < nz; iz++) out[iz] -= vx[iz] * vcol[iz];
+    if(iacc) {
+      const T *rcol = rho_of(vcol);
+      if(rcol)
+        for(int iz = 0; iz < nz; iz++) out[iz] -= vx[iz] * vcol[iz] * rcol[iz];
+      else
+        for(int iz = 0; iz < nz; iz++) out[iz] -= vx[iz] * vcol[iz];
+    }
   }
   void fd_dy_col(const T *u, T *out, int ix, int iy, int iacc, int isign, const T *vcol) {
     thread_local std::vector<T> tmp;
@@ -484,8 +499,13 @@ template<class T> struct Engine {
         for(int iz = 0; iz < nz; iz++) vy[iz] += c * u[iz + (ptrdiff_t)m * nxz];
     }
     pml1_ybnd(vy, ix, iy, isign);
-    if(iacc)
-      for(int iz = 0; iz < nz; iz++) out[iz] -= vy[iz] * vcol[iz];
+    if(iacc) {
+      const T *rcol = rho_of(vcol);
+      if(rcol)
+        for(int iz = 0; iz < nz; iz++) out[iz] -= vy[iz] * vcol[iz] * rcol[iz];
+      else
+        for(int iz = 0; iz < nz; iz++) out[iz] -= vy[iz] * vcol[iz];
+    }
   }
 
   // --- slice / volume derivative drivers (FD or CFD) ------------------------
@@ -502,8 +522,10 @@ template<class T> struct Engine {
     T *dst = iacc ? w.data() : out;
     for(int ix = 0; ix < nx; ix++) cfd[AX_Z].derive(sg, in + (size_t)ix * nz, dst + (size_t)ix * nz, 1);
     for(int ix = 0; ix < nx; ix++) pml1_zbnd(dst + (size_t)ix * nz, ix, iy, isign);
-    if(iacc)
-      for(size_t i = 0; i < nxz; i++) out[i] = out[i] - dst[i] * vslice[i];
+    if(iacc) {
+      const T *rs = rho_of(vslice);
+      for(size_t i = 0; i < nxz; i++) out[i] = rs ? out[i] - dst[i] * vslice[i] * rs[i] : out[i] - dst[i] * vslice[i];
+    }
   }
   // x-derivative of one y-slice: bgprtm/FdEngine_cfd.cpp:83-105
   void dx_slice(const T *in, T *out, int iy, int iacc, int isign, const T *vslice) {
@@ -518,8 +540,10 @@ template<class T> struct Engine {
     T *dst = iacc ? w.data() : out;
     for(int iz = 0; iz < nz; iz++) cfd[AX_X].derive(sg, in + iz, dst + iz, (size_t)nz);
     for(int ix = 0; ix < nx; ix++) pml1_xbnd(dst + (size_t)ix * nz, ix, iy, isign);
-    if(iacc)
-      for(size_t i = 0; i < nxz; i++) out[i] = out[i] - dst[i] * vslice[i];
+    if(iacc) {
+      const T *rs = rho_of(vslice);
+      for(size_t i = 0; i < nxz; i++) out[i] = rs ? out[i] - dst[i] * vslice[i] * rs[i] : out[i] - dst[i] * vslice[i];
+    }
   }
   // one axis over the volume (0 z, 1 x, 2 y), as the slice drivers below apply it
   void deriv1(int axis, const T *in, T *out, int iacc, int isign) {
@@ -564,7 +588,8 @@ template<class T> struct Engine {
           pml1_ybnd(col, ix, iy, isign);
           T *o = out + (size_t)iy * nxz + (size_t)ix * nz;
           const T *vc = v + (size_t)iy * nxz + (size_t)ix * nz;
-          for(int iz = 0; iz < nz; iz++) o[iz] = o[iz] - col[iz] * vc[iz];
+          const T *rc = rho_of(vc);
+          for(int iz = 0; iz < nz; iz++) o[iz] = rc ? o[iz] - col[iz] * vc[iz] * rc[iz] : o[iz] - col[iz] * vc[iz];
         } else {
           pml1_ybnd(out + (size_t)iy * nxz + (size_t)ix * nz, ix, iy, isign);
         }
@@ -620,6 +645,15 @@ template<class T> struct Engine {
         const T *ps = p0 + (size_t)iy * nxz;
         dz_slice(ps, pz.data(), iy, 0, 1, nullptr);
         dx_slice(ps, px.data(), iy, 0, 1, nullptr);
+        if(rho) {  // KernelCPU::divideRho: bgprtm/KernelCPU.cpp:103, 340-349
+          const T *r = rho + (size_t)iy * nxz;
+          T *py = ny > 1 ? pr + (size_t)iy * nxz : nullptr;
+          for(size_t i = 0; i < nxz; i++) {
+            if(nx > 1) px[i] /= r[i];
+            if(py) py[i] /= r[i];
+            pz[i] /= r[i];
+          }
+        }
         if(modelType == TTI || modelType == VTI) scalar(modelType, px.data(), ny > 1 ? pr + (size_t)iy * nxz : nullptr, pz.data(), iy);
         dx_slice(px.data(), pb + (size_t)iy * nxz, iy, 1, -1, vel + (size_t)iy * nxz);
         dz_slice(pz.data(), pb + (size_t)iy * nxz, iy, 1, -1, vel + (size_t)iy * nxz);
@@ -630,7 +664,7 @@ template<class T> struct Engine {
 
   // KernelCPU::ISO (Laplacian branch) / TTI-alike: bgprtm/KernelCPU.cpp:125-174
   void kernel(int modelType, const T *p0, T *pb, T *pr) {
-    if(modelType == ISO && engine == ENGINE_FD) {
+    if(modelType == ISO && engine == ENGINE_FD && !rho) {
       laplacian(p0, pr);
       applyVel_minus(pr, pb);
     } else tti(p0, pb, pr, modelType);
@@ -999,6 +1033,7 @@ void orc_cfd_derive_d(int n, double d, int sgn, const double *x, double *b, int 
     e->setBoundary(nb, dt, vmax, pml_eps, dimPML);                                                                    \
   }                                                                                                                   \
   void orc_engine_clean_pml_##S(void *h) { ((Engine<T> *)h)->cleanPML(); }                                            \
+  void orc_engine_set_rho_##S(void *h, const T *rho) { ((Engine<T> *)h)->rho = rho; }                                  \
   void orc_engine_kernel_##S(void *h, int model, const T *p0, T *pb, T *pr) {                                         \
     ((Engine<T> *)h)->kernel(model, p0, pb, pr);                                                                      \
   }                                                                                                                   \
