@@ -25,7 +25,7 @@ public:
   KernelCPU(const KernelCPU &) = delete;
   KernelCPU &operator=(const KernelCPU &) = delete;
 
-  // borrowed device pointers; vel holds v^2 dt^2 (Propagator::Vel2); rho may be null (FD engines only)
+  // borrowed device pointers; vel holds v^2 dt^2 (Propagator::Vel2); rho (density) may be null
   void setModel(float *vel, float *rho, float *del, float *eps, float *pjx, float *pjy) {
     konglong::check(klrtm_kernel_set_model(k_, vel, rho, del, eps, pjx, pjy));
   }

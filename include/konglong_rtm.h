@@ -85,10 +85,11 @@ typedef struct klrtm_kernel klrtm_kernel;
  * even) or KLRTM_ENGINE_CFD (order ignored, 10th-order compact).  Requires nx,nz > 2. */
 int klrtm_kernel_create(klrtm_kernel **k, int nx, int ny, int nz, float dx, float dy, float dz, int engine, int order);
 int klrtm_kernel_destroy(klrtm_kernel *k);
-/* KernelCPU::setModel: device pointers, borrowed; vel holds v^2 dt^2; rho (density, may be NULL; FD
- * engines only) divides the gradient and scales each divergence term by vel*rho, and moves ISO to the
+/* KernelCPU::setModel: device pointers, borrowed; vel holds v^2 dt^2; rho (density, may be NULL)
+ * divides the gradient and scales each divergence term by vel*rho, and moves ISO to the
  * first-derivative form as the reference's force_tti does (bgprtm/KernelCPU.cpp:93-134, 340-349);
- * del/eps needed for VTI/TTI, pjx/pjy for TTI */
+ * the FdEngine accumulates (klrtm_kernel_deriv1 with iacc) take it too; del/eps needed for VTI/TTI,
+ * pjx/pjy for TTI */
 int klrtm_kernel_set_model(klrtm_kernel *k, const float *vel, const float *rho, const float *del, const float *eps,
                            const float *pjx, const float *pjy);
 /* global_pars keys pml_eps (1e-3), pml3d (1), pml3d_scaler (0.3), top_pml_flag (1);
@@ -270,7 +271,7 @@ int klrtm_propagator_set_model(klrtm_propagator *p, const float *vel, const floa
  * whose Vel2 then takes the Q branch; every step becomes the kernel (pb -= vel*L) followed by the Q
  * leapfrog, and the memory variables are cleared with the wavefields (Wavefield::cleanMemory,
  * bgprtm/Wavefield.cpp:116-118).  order 0 switches Q off again. */
-/* density (Model::useRho, bgprtm/ModelLoader.cpp:63): host volume, FD engines, before or after
+/* density (Model::useRho, bgprtm/ModelLoader.cpp:63): host volume, before or after
  * klrtm_propagator_set_model.  NULL switches it off. */
 int klrtm_propagator_set_rho(klrtm_propagator *p, const float *rho_host);
 int klrtm_propagator_set_q(klrtm_propagator *p, const float *invq_host, int order, const float *coeffs);

@@ -955,12 +955,81 @@ static int cfd_run_m(Engine &e, int mode, const float *p0, float *pb, float *pr)
   return launch_zline<P_DIV>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z");
 }
 
+// ---------------------------------------------------------------------------
+// Density (KernelCPU::setModel with rho): every derivative pass stores its derivative, and two pointwise
+// passes do the rest in the reference's order.  After the gradient: divideRho, then the scalar operator
+// (bgprtm/KernelCPU.cpp:101-106, 340-349).  After the divergence derivatives: the accumulates
+// pb - (Dx*vel)*rho, - (Dz*vel)*rho (bgprtm/FdEngine_cfd.cpp:42, 91) and pb - Dy*(vel*rho) (the y form
+// multiplies vel*rho first, bgprtm/FdEngine_cfd.cpp:64-67), then the leapfrog.
+template<int MODEL>
+__global__ void cfd_rho_grad_kernel(float *__restrict__ px, float *__restrict__ py, float *__restrict__ pz,
+                                    const float *__restrict__ rho, const float *__restrict__ eps,
+                                    const float *__restrict__ del, const float *__restrict__ pjx,
+                                    const float *__restrict__ pjy, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const float r = rho[i];
+  float gx = px[i] / r, gy = py ? py[i] / r : 0.f, gz = pz[i] / r;
+  if(MODEL == KLRTM_VTI) scalar_vti(eps[i], del[i], gx, gy, gz);
+  else if(MODEL == KLRTM_TTI) scalar_tti(eps[i], del[i], pjx[i], py ? pjy[i] : 0.f, gx, gy, gz);
+  px[i] = gx;
+  if(py) py[i] = gy;
+  pz[i] = gz;
+}
+// the x and z accumulates and/or the y accumulate of the derivatives given, + leapfrog if w1
+__global__ void cfd_rho_div_kernel(float *__restrict__ pb, const float *__restrict__ dx, const float *__restrict__ dz,
+                                   const float *__restrict__ dy, const float *__restrict__ vel,
+                                   const float *__restrict__ rho, const float *__restrict__ w1, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const float v = vel[i], r = rho[i];
+  float w = pb[i];
+  if(dx) w = w - dx[i] * v * r;
+  if(dz) w = w - dz[i] * v * r;
+  if(dy) w = w - dy[i] * (v * r);
+  if(w1) w = (w1[i] + w1[i]) - w;
+  pb[i] = w;
+}
+
+template<int MODEL>
+static int cfd_run_rho(Engine &e, int mode, const float *p0, float *pb, float *pr) {
+  int rc;
+  const bool y = e.ny > 1;
+  float *px = e.d_px, *pz = e.d_pz;
+  const size_t n = (size_t)e.nx * e.ny * e.nz;
+  const dim3 g((unsigned)div_up((long)n, 256)), b(256);
+  if(y && (rc = launch_line<AXY, P_GRAD, MODEL>(e, 0, 0, p0, pr, pr, pz, pb, p0, "cfd_grad_y"))) return rc;
+  if((rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z"))) return rc;
+  if((rc = launch_line<AXX, P_GRAD, MODEL>(e, 0, 0, p0, px, pr, pz, pb, p0, "cfd_grad_x"))) return rc;
+  hipLaunchKernelGGL((cfd_rho_grad_kernel<MODEL>), g, b, 0, stream(), px, y ? pr : nullptr, pz, e.rho, e.eps, e.del, e.pjx,
+                     e.pjy, n);
+  if((rc = check_launch("cfd_rho_grad_kernel"))) return rc;
+  // Dx, Dz in place; pb takes their accumulates; Dy goes to px, so pr keeps the operated y gradient
+  if((rc = launch_line<AXX, P_GRAD, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x"))) return rc;
+  if((rc = launch_zline<P_GRAD>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z"))) return rc;
+  const float *lf = mode == MODE_STEP ? p0 : nullptr;
+  hipLaunchKernelGGL(cfd_rho_div_kernel, g, b, 0, stream(), pb, px, pz, nullptr, e.vel, e.rho, y ? nullptr : lf, n);
+  if((rc = check_launch("cfd_rho_div_kernel")) || !y) return rc;
+  if((rc = launch_line<AXY, P_GRAD, MODEL>(e, 1, 1, pr, px, pr, pz, pb, p0, "cfd_div_y"))) return rc;
+  hipLaunchKernelGGL(cfd_rho_div_kernel, g, b, 0, stream(), pb, nullptr, nullptr, px, e.vel, e.rho, lf, n);
+  return check_launch("cfd_rho_div_kernel");
+}
+
 // FdEngine_cfd::d{x,y,z}1 over the whole volume (bgprtm/FdEngine_cfd.cpp:24-297): out = D(in) with the
 // PML of that component (isign > 0: gradient plan, PML round 1; else divergence plan, round 2), or with
 // iacc out -= vel*D(in).  tmp holds b' when accumulating (in is left untouched).
 int cfd_deriv1(Engine &e, int axis, int iacc, int isign, const float *in, float *out, float *tmp) {
   const int sg = isign > 0 ? 0 : 1, rnd = sg;
   float *t = iacc ? tmp : out;
+  if(iacc && e.rho) {  // out -= vel*rho*D(in) in the axis' operation order
+    int rc = cfd_deriv1(e, axis, 0, isign, in, t, nullptr);
+    if(rc) return rc;
+    const size_t n = (size_t)e.nx * e.ny * e.nz;
+    hipLaunchKernelGGL(cfd_rho_div_kernel, dim3((unsigned)div_up((long)n, 256)), dim3(256), 0, stream(), out,
+                       axis == 1 ? t : nullptr, axis == 0 ? t : nullptr, axis == 2 ? t : nullptr, e.vel, e.rho, nullptr,
+                       n);
+    return check_launch("cfd_rho_div_kernel");
+  }
   if(axis == 0) {
     if(iacc) return launch_zline<P_DIV>(e, sg, rnd, in, t, out, nullptr, "cfd_deriv1_z");
     return launch_zline<P_GRAD>(e, sg, rnd, in, t, nullptr, nullptr, "cfd_deriv1_z");
@@ -974,6 +1043,11 @@ int cfd_deriv1(Engine &e, int axis, int iacc, int isign, const float *in, float 
 }
 
 int cfd_run(Engine &e, int model, int mode, const float *p0, float *pb, float *pr) {
+  if(e.rho) {
+    if(model == KLRTM_VTI) return cfd_run_rho<KLRTM_VTI>(e, mode, p0, pb, pr);
+    if(model == KLRTM_TTI) return cfd_run_rho<KLRTM_TTI>(e, mode, p0, pb, pr);
+    return cfd_run_rho<KLRTM_ISO>(e, mode, p0, pb, pr);
+  }
   if(model == KLRTM_VTI) return cfd_run_m<KLRTM_VTI>(e, mode, p0, pb, pr);
   if(model == KLRTM_TTI) return cfd_run_m<KLRTM_TTI>(e, mode, p0, pb, pr);
   return cfd_run_m<KLRTM_ISO>(e, mode, p0, pb, pr);

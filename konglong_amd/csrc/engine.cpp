@@ -283,8 +283,6 @@ int klrtm_kernel_set_model(klrtm_kernel *kk, const float *vel, const float *rho,
                            const float *pjx, const float *pjy) {
   Engine *k = reinterpret_cast<Engine *>(kk);
   KL_REQUIRE(k && vel, KLRTM_EINVAL, "null engine or vel");
-  KL_REQUIRE(!rho || k->engine == KLRTM_ENGINE_FD, KLRTM_EINVAL,
-             "density (rho) runs with the FD engines; the compact (CFD) engine does not take it");
   k->vel = vel, k->rho = rho, k->del = del, k->eps = eps, k->pjx = pjx, k->pjy = pjy;
   return KLRTM_OK;
 }

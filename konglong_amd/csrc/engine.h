@@ -13,7 +13,7 @@ struct Engine {
   int engine = KLRTM_ENGINE_FD, order = 8, ne = 4;
   float c1[9] = {0}, c2[9] = {0};
   const float *vel = nullptr, *eps = nullptr, *del = nullptr, *pjx = nullptr, *pjy = nullptr;
-  const float *rho = nullptr;  // density (FD engines; forces the first-derivative form)
+  const float *rho = nullptr;  // density (forces the first-derivative form)
   // PML options (global_pars keys pml_eps, pml3d, pml3d_scaler, top_pml_flag)
   float pml_eps = 1e-3f, pml3d_scaler = 0.3f;
   int pml3d = 1, top_pml = 1;

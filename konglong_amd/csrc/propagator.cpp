@@ -486,8 +486,6 @@ int klrtm_propagator_set_model(klrtm_propagator *p, const float *vel, const floa
 int klrtm_propagator_set_rho(klrtm_propagator *p, const float *rho) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
   Propagator &P = *reinterpret_cast<Propagator *>(p);
-  KL_REQUIRE(!rho || P.c.engine == KLRTM_ENGINE_FD, KLRTM_EINVAL,
-             "density (rho) runs with the FD engines; the compact (CFD) engine does not take it");
   P.use_rho = rho != nullptr;
   int rc;
   if(rho && (rc = upload_opt(&P.d_rho, rho, P.n))) return rc;

@@ -48,8 +48,8 @@ class Propagator:
         check(lib().klrtm_propagator_set_model(self.h, *[fptr(a) for a in arrs]))
 
     def set_rho(self, rho):
-        """Density (Model::useRho, FD engines): the gradient is divided by rho and each divergence term
-        scaled by vel*rho; ISO takes the first-derivative form.  None switches it off."""
+        """Density (Model::useRho): the gradient is divided by rho and each divergence term scaled by
+        vel*rho; ISO takes the first-derivative form.  None switches it off."""
         r = None if rho is None else np.ascontiguousarray(rho, np.float32)
         assert r is None or r.shape == self.shape
         check(lib().klrtm_propagator_set_rho(self.h, fptr(r)))

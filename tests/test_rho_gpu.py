@@ -8,7 +8,7 @@ import pytest
 import oracle
 from oracle import pipeline
 from konglong_amd._lib import DeviceArray, KlrtmError, ENGINE_FD, ENGINE_CFD, ISO, VTI, TTI, SNAP_FP32
-from konglong_amd.kernel import KernelCPU, FdEngine_fd
+from konglong_amd.kernel import KernelCPU, FdEngine_fd, FdEngine_cfd
 from konglong_amd.propagator import Propagator, shot_config
 from tests.helpers import rel_l2, smooth_field
 from tests.test_engine_gpu import _inputs, NX, NY, NZ, D, DT, NB
@@ -51,14 +51,15 @@ def test_density_step_parity(klrtm, order, model):
     assert rel_l2(gr2.download(), opr) <= 1e-5
 
 
-def test_density_deriv1_accumulate(klrtm):
+@pytest.mark.parametrize("engine", [ENGINE_FD, ENGINE_CFD], ids=["FD8", "CFD10"])
+def test_density_deriv1_accumulate(klrtm, engine):
     m, _, _, _, _, w1, w0 = _inputs(ISO, seed=17)
     rho = smooth_field((NY, NX, NZ), 1.0, 2.6, 82)
-    o = oracle.Engine(NX, NY, NZ, D, D, D, ENGINE_FD, 8, np.float32)
+    o = oracle.Engine(NX, NY, NZ, D, D, D, engine, 8 if engine == ENGINE_FD else 10, np.float32)
     o.set_model(m)
     o.set_rho(rho)
     o.set_boundary(NB, DT, 2500.0, dim_pml=3)
-    f = FdEngine_fd(NX, NY, NZ, D, D, D, 8)
+    f = FdEngine_fd(NX, NY, NZ, D, D, D, 8) if engine == ENGINE_FD else FdEngine_cfd(NX, NY, NZ, D, D, D)
     dm, dr = DeviceArray.from_host(m), DeviceArray.from_host(rho)
     f.setModel(dm, dr)
     f.setBoundary(*NB, DT, 2500.0)
@@ -67,7 +68,9 @@ def test_density_deriv1_accumulate(klrtm):
         o.deriv1(axis, w1, want, 1, -1)
         got = DeviceArray.from_host(w0)
         f.deriv1(axis, DeviceArray.from_host(w1), got, 1, -1)
-        # out -= vel*rho*D(u): compare the increments
+        # out -= vel*rho*D(u): compare the increments (the compact engine bit for bit)
+        if engine == ENGINE_CFD:
+            assert np.array_equal(got.download().view(np.uint32), want.view(np.uint32)), axis
         assert rel_l2(got.download() - w0, want - w0) <= 1e-5, axis
 
 
@@ -104,12 +107,37 @@ def test_density_shot_matches_oracle(klrtm):
     assert rel_l2(p.model(), want) > 1e-3
 
 
-def test_density_rejected_by_cfd(klrtm):
+@pytest.mark.parametrize("model", [ISO, VTI, TTI], ids=["ISO", "VTI", "TTI"])
+def test_density_cfd_bit_exact(klrtm, model):
+    """The compact engine with density keeps the reference's operation order: bit for bit."""
+    print(f"[rho-cfd] model={model}", flush=True)
+    m, eps, dl, pjx, pjy, w1, w0 = _inputs(model, seed=19)
+    rho = smooth_field((NY, NX, NZ), 1.0, 2.6, 85)
+    o = oracle.Engine(NX, NY, NZ, D, D, D, ENGINE_CFD, 10, np.float32)
+    o.set_model(m, eps, dl, pjx, pjy)
+    o.set_rho(rho)
+    o.set_boundary(NB, DT, 2500.0, dim_pml=3)
     k = KernelCPU(NX, NY, NZ, D, D, D, ENGINE_CFD, 10)
-    dm = DeviceArray.from_host(np.ones((NY, NX, NZ), np.float32))
-    with pytest.raises(KlrtmError, match="CFD"):
-        k.setModel(dm, dm)
-    cfg = shot_config(16, 16, 20, 10.0, 1e-3, 4, ENGINE_CFD, 10, VTI, nbnd=4, vmax=2000.0, nrcv=1, spread=2)
-    p = Propagator(cfg)
-    with pytest.raises(KlrtmError, match="CFD"):
-        p.set_rho(np.ones((16, 16, 20), np.float32))
+    dev = {n: (DeviceArray.from_host(a) if a is not None else None)
+           for n, a in (("m", m), ("e", eps), ("d", dl), ("x", pjx), ("y", pjy), ("r", rho))}
+    k.setModel(dev["m"], dev["r"], dev["d"], dev["e"], dev["x"], dev["y"])
+    k.setBoundary(*NB, DT, 2500.0)
+    ow1, ow0, owr = w1.copy(), w0.copy(), np.zeros_like(w1)
+    g1, g0, gr = DeviceArray.from_host(w1), DeviceArray.from_host(w0), DeviceArray((NY, NX, NZ))
+    nsteps = 3 if model == TTI else 8   # CFD with TTI grows without bound, also in the oracle
+    for it in range(nsteps):
+        o.step(model, ow1, ow0, owr)
+        ow0, ow1 = ow1, ow0
+        k.step(model, g1, g0, gr)
+        g0, g1 = g1, g0
+    got = g1.download()
+    assert np.array_equal(got.view(np.uint32), ow1.view(np.uint32)), rel_l2(got, ow1)
+    # the unfused call leaves the operated, density-divided y gradient in pr
+    o.clean_pml()
+    k.cleanPMLMemory()
+    opb, opr = w0.copy(), np.zeros_like(w1)
+    o.kernel(model, w1, opb, opr)
+    gb, gr2 = DeviceArray.from_host(w0), DeviceArray((NY, NX, NZ))
+    {ISO: k.ISO, VTI: k.VTI, TTI: k.TTI}[model](DeviceArray.from_host(w1), gb, gr2)
+    assert np.array_equal(gb.download().view(np.uint32), opb.view(np.uint32))
+    assert np.array_equal(gr2.download().view(np.uint32), opr.view(np.uint32))
