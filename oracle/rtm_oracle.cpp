@@ -589,7 +589,8 @@ template<class T> struct Engine {
           T *o = out + (size_t)iy * nxz + (size_t)ix * nz;
           const T *vc = v + (size_t)iy * nxz + (size_t)ix * nz;
           const T *rc = rho_of(vc);
-          for(int iz = 0; iz < nz; iz++) o[iz] = rc ? o[iz] - col[iz] * vc[iz] * rc[iz] : o[iz] - col[iz] * vc[iz];
+          // dy1_2D multiplies by vel*rho as one factor (bgprtm/FdEngine_cfd.cpp:64-67)
+          for(int iz = 0; iz < nz; iz++) o[iz] = rc ? o[iz] - col[iz] * (vc[iz] * rc[iz]) : o[iz] - col[iz] * vc[iz];
         } else {
           pml1_ybnd(out + (size_t)iy * nxz + (size_t)ix * nz, ix, iy, isign);
         }
