@@ -26,7 +26,7 @@ class Autotuner:
     forward steps with the library's kernel timers, and the fastest is kept per (grid, engine, order,
     model).  The propagator's wavefields are reset afterwards."""
 
-    TILES = ((64, 8), (64, 4), (32, 8), (128, 4))
+    TILES = ((64, 8), (64, 4), (32, 8), (128, 4), (16, 16))
     YCHUNKS = (0, 16, 32, 64)   # 0: the engine's default chunking
 
     def __init__(self, steps=4):
@@ -55,7 +55,7 @@ class Autotuner:
 
     def _search(self, p, k):
         L = lib()
-        name = b"iso_step" if p.cfg.model == ISO else b"fd_grad"
+        names = (b"iso_step",) if p.cfg.model == ISO else (b"fd_grad", b"fd_div_step")
         R = p.cfg.order // 2
         table = {}
         p.reset()
@@ -71,10 +71,13 @@ class Autotuner:
                 p.forward_steps(1, self.steps, False)
                 L.klrtm_synchronize()
                 L.klrtm_timer_enable(0)
-                tot, cnt = C.c_double(), C.c_long()
-                L.klrtm_timer_read(name, C.byref(tot), C.byref(cnt))
-                if cnt.value:
-                    table[(tz, tx, yc)] = tot.value / cnt.value
+                ms = 0.0
+                for name in names:
+                    tot, cnt = C.c_double(), C.c_long()
+                    L.klrtm_timer_read(name, C.byref(tot), C.byref(cnt))
+                    ms = ms + tot.value / cnt.value if cnt.value and ms is not None else None
+                if ms is not None:
+                    table[(tz, tx, yc)] = ms
         L.klrtm_timer_reset()
         p.reset()
         best = min(table, key=table.get) if table else None

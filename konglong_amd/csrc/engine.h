@@ -40,7 +40,7 @@ struct Engine {
   int ensure_comps(int comps);
   int clean();
   int set_boundary(const int *nbnd, float dt, float vmax);
-  void default_tile(int R, long want_blocks, int &tz_out, int &tx_out, int &ychunk_out) const;
+  void default_tile(int R, long want_blocks, int &tz_out, int &tx_out, int &ychunk_out, bool first_deriv = false) const;
   void fill_args(IsoArgs &a, int deriv_order);
   int check_model(int model);
   int scratch();

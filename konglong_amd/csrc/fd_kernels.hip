@@ -313,17 +313,23 @@ __global__ void __launch_bounds__(1024) grad_kernel(const float *__restrict__ u,
   float qn = 0.f;
   Halo hn = load_halo<R>(u, (size_t)c.y0 * nxz, c, a, true, true);
   if(c.act && c.y0 + R < ny) qn = u[(size_t)(c.y0 + R) * nxz + c.col];
+  // the model parameters of a plane are loaded one plane ahead, with its halos
+  float en = 0.f, dn = 0.f, axn = 0.f, ayn = 0.f;
+  auto model_ld = [&](size_t p) {
+    if(c.act && MODEL != KLRTM_ISO) en = eps[p + c.col], dn = del[p + c.col];
+    if(c.act && MODEL == KLRTM_TTI) axn = pjx[p + c.col], ayn = do_y ? pjy[p + c.col] : 0.f;
+  };
+  model_ld((size_t)c.y0 * nxz);
   for(int iy = c.y0; iy < c.y1; iy++) {
     const size_t pl = (size_t)iy * nxz;
     q[2 * R] = qn;
     const Halo h = hn;
-    float e = 0.f, d = 0.f, ax = 0.f, ay = 0.f;
-    if(c.act && MODEL != KLRTM_ISO) e = eps[pl + c.col], d = del[pl + c.col];
-    if(c.act && MODEL == KLRTM_TTI) ax = pjx[pl + c.col], ay = do_y ? pjy[pl + c.col] : 0.f;
+    const float e = en, d = dn, ax = axn, ay = ayn;
     if(iy + 1 < c.y1) {
       const size_t pn = pl + nxz;
       hn = load_halo<R>(u, pn, c, a, true, true);
       if(c.act) qn = (iy + 1 + R < ny) ? u[pn + (size_t)R * nxz + c.col] : 0.f;
+      model_ld(pn);
     }
     lds_barrier();
     srow[0] = q[R];
