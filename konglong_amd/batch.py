@@ -26,7 +26,8 @@ class Autotuner:
     forward steps with the library's kernel timers, and the fastest is kept per (grid, engine, order,
     model).  The propagator's wavefields are reset afterwards."""
 
-    TILES = ((64, 8), (64, 4), (32, 8), (128, 4), (16, 16))
+    TILES = ((64, 8), (64, 4), (32, 8), (128, 4))
+    TILES_FIRST_DERIV = TILES + ((16, 16),)  # the gradient/divergence kernels also take 16 x 16
     YCHUNKS = (0, 16, 32, 64)   # 0: the engine's default chunking
 
     def __init__(self, steps=4):
@@ -59,7 +60,7 @@ class Autotuner:
         R = p.cfg.order // 2
         table = {}
         p.reset()
-        for tz, tx in self.TILES:
+        for tz, tx in (self.TILES if p.cfg.model == ISO else self.TILES_FIRST_DERIV):
             if tz < R or tx < R or tz * tx > 512:
                 continue
             for yc in self.YCHUNKS:
