@@ -79,6 +79,14 @@ int klrtm_vel2_q(float *vel, float *invq, size_t n, float dt, float cq0);
 int klrtm_update2nd_q(float *w0, const float *w1, const float *invq, float *d0, const float *d1, size_t n, int order,
                       const float *coeffs);
 
+/* ---- FreqUtil time shift (bgprtm/FreqUtil.cpp:24-39), hipFFT on the device ------------------
+ * out[s][0..nt) = the nt-sample device trace in, zero-padded to ntfft, shifted by shift_over_dt[s]
+ * samples (host array, may be fractional; positive delays) through exp(-i w t) in the frequency domain:
+ * a circular shift over ntfft samples.  klrtm_fft_size gives the smallest 2^a 3^b 5^c >= n, the padded
+ * length used by klrtm_propagator_set_source_array (the reference's libCommon::padfft is unvendored). */
+int klrtm_fft_size(int n);
+int klrtm_timeshift(const float *in, int nt, int ntfft, int ntr, const float *shift_over_dt, float *out);
+
 /* ---- KernelCPU + FdEngine_fd / FdEngine_cfd + PML (bgprtm/KernelCPU.h:19-35) */
 typedef struct klrtm_kernel klrtm_kernel;
 /* KernelCPU(nx,ny,nz,dx,dy,dz,engine,nThreads): engine KLRTM_ENGINE_FD (order 8..16,
@@ -292,6 +300,12 @@ int klrtm_propagator_set_snapshot_cache(klrtm_propagator *p, const char *path, d
 int klrtm_propagator_snapshot_cache_info(klrtm_propagator *p, int *in_memory, size_t *stride_bytes, int *nsnap);
 /* source wavelet samples (host, nt floats); replaces the built-in Ricker (Source::setupWavelet) */
 int klrtm_propagator_set_source_wavelet(klrtm_propagator *p, const float *wavelet_host);
+/* source array (Propagator::setupReceiver with several source positions and delays,
+ * bgprtm/Propagator.cpp:783-811): nsrc positions xyz[3*nsrc] (grid coordinates) each injecting the
+ * current wavelet delayed by delay_s[nsrc] seconds (klrtm_timeshift with ntfft = klrtm_fft_size(nt));
+ * replaces the single source of the shot config.  A later klrtm_propagator_set_source_wavelet
+ * replaces the wavelet of every source of the array (delays kept). */
+int klrtm_propagator_set_source_array(klrtm_propagator *p, int nsrc, const float *xyz, const float *delay_s);
 /* receivers: rcv_xyz[3*nrcv] grid coordinates */
 int klrtm_propagator_set_receivers(klrtm_propagator *p, const float *rcv_xyz);
 /* forward modelling with extraction every step: record_host [nrcv][nt] (may be NULL) */

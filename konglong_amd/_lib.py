@@ -66,6 +66,8 @@ _SIGS = {
     "klrtm_vel2": (_i, [_vp, _sz, _f]),
     "klrtm_update2nd": (_i, [_vp, _vp, _sz]),
     "klrtm_vel2_q": (_i, [_vp, _vp, _sz, _f, _f]),
+    "klrtm_fft_size": (_i, [_i]),
+    "klrtm_timeshift": (_i, [_vp, _i, _i, _i, _pf, _vp]),
     "klrtm_update2nd_q": (_i, [_vp, _vp, _vp, _vp, _vp, _sz, _i, _pf]),
     "klrtm_kernel_create": (_i, [C.POINTER(_vp), _i, _i, _i, _f, _f, _f, _i, _i]),
     "klrtm_kernel_destroy": (_i, [_vp]),
@@ -107,6 +109,7 @@ _SIGS = {
     "klrtm_propagator_set_model": (_i, [_vp, _pf, _pf, _pf, _pf, _pf]),
     "klrtm_propagator_set_q": (_i, [_vp, _pf, _i, _pf]),
     "klrtm_propagator_set_rho": (_i, [_vp, _pf]),
+    "klrtm_propagator_set_source_array": (_i, [_vp, _i, _pf, _pf]),
     "klrtm_propagator_set_ghost": (_i, [_vp, _i, _i, _i]),
     "klrtm_propagator_set_snapshot_cache": (_i, [_vp, C.c_char_p, C.c_double, _i]),
     "klrtm_propagator_snapshot_cache_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_sz), C.POINTER(_i)]),

@@ -18,6 +18,7 @@
 #include "surface.h"
 #include "qatten.h"
 #include "cachefile.h"
+#include "timeshift.h"
 #include <cmath>
 #include <cstring>
 #include <algorithm>
@@ -67,6 +68,9 @@ struct Propagator {
   int rcv_mirror() const { return c.free_surface_iz >= 0 ? c.free_surface_iz : (ghost_rcv ? ghost_iz : -1); }
   int ext_mirror() const { return ghost_rcv ? ghost_iz : -1; }
   std::vector<float> wtaperT;
+  std::vector<float> wavelet;    // the source wavelet (host, nt samples)
+  std::vector<float> src_delay;  // source array: delay per source in seconds (empty: the single source)
+  int set_source_traces();
   double t_fwd = 0, t_bwd = 0, t_ic = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ~Propagator();
@@ -279,9 +283,9 @@ int propagator_create(Propagator **out, const klrtm_shot_config *cfg) {
     delete P;
     return rc;
   }
-  std::vector<float> wav(c.nt);
-  for(int it = 0; it < c.nt; it++) wav[it] = ricker(it * c.dt, c.f_peak);
-  if((rc = receiver_set_traces(*P->src, wav.data()))) {
+  P->wavelet.resize(c.nt);
+  for(int it = 0; it < c.nt; it++) P->wavelet[it] = ricker(it * c.dt, c.f_peak);
+  if((rc = receiver_set_traces(*P->src, P->wavelet.data()))) {
     delete P;
     return rc;
   }
@@ -350,6 +354,29 @@ int propagator_set_q(Propagator &P, const float *invq, int order, const float *c
   P.q_order = order;
   P.iq0 = 0;
   return KLRTM_OK;
+}
+
+// the source traces: the wavelet, or for a source array the wavelet delayed per source
+// (Propagator::setupReceiver: FreqUtil::timeshift_in / timeshift_out(delay/dt), bgprtm/Propagator.cpp:797-811)
+int Propagator::set_source_traces() {
+  if(src_delay.empty()) return receiver_set_traces(*src, wavelet.data());
+  const int ns = (int)src_delay.size(), nt = c.nt;
+  std::vector<float> sh(ns), tr((size_t)ns * nt);
+  for(int s = 0; s < ns; s++) sh[s] = src_delay[s] / c.dt;
+  float *d_in = nullptr, *d_out = nullptr;
+  int rc = dmalloc(&d_in, nt);
+  if(!rc) rc = dmalloc(&d_out, (size_t)ns * nt);
+  if(!rc && hipMemcpy(d_in, wavelet.data(), sizeof(float) * nt, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("wavelet upload failed");
+    rc = KLRTM_EDEVICE;
+  }
+  if(!rc) rc = timeshift(d_in, nt, fft_size(nt), ns, sh.data(), d_out);
+  if(!rc && hipMemcpy(tr.data(), d_out, sizeof(float) * tr.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("source traces download failed");
+    rc = KLRTM_EDEVICE;
+  }
+  (void)hipFree(d_in), (void)hipFree(d_out);
+  return rc ? rc : receiver_set_traces(*src, tr.data());
 }
 
 int propagator_set_receivers(Propagator &P, const float *xyz) {
@@ -530,7 +557,22 @@ int klrtm_propagator_snapshot_cache_info(klrtm_propagator *p, int *in_memory, si
 }
 int klrtm_propagator_set_source_wavelet(klrtm_propagator *p, const float *wav) {
   KL_REQUIRE(p && wav, KLRTM_EINVAL, "null argument");
-  return receiver_set_traces(*reinterpret_cast<Propagator *>(p)->src, wav);
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  P.wavelet.assign(wav, wav + P.c.nt);
+  return P.set_source_traces();
+}
+int klrtm_propagator_set_source_array(klrtm_propagator *p, int nsrc, const float *xyz, const float *delay_s) {
+  KL_REQUIRE(p && xyz && delay_s && nsrc > 0, KLRTM_EINVAL, "bad source array");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  const klrtm_shot_config &c = P.c;
+  Receiver *r = nullptr;
+  int rc = receiver_create(&r, c.nx, c.ny, c.nz, nsrc, xyz, c.src_spread, c.khpass, c.khcut, c.nt,
+                           1.0f / (c.dx * c.dy * c.dz));
+  if(rc) return rc;
+  receiver_destroy(P.src);
+  P.src = r;
+  P.src_delay.assign(delay_s, delay_s + nsrc);
+  return P.set_source_traces();
 }
 int klrtm_propagator_set_receivers(klrtm_propagator *p, const float *xyz) {
   KL_REQUIRE(p && xyz, KLRTM_EINVAL, "null argument");

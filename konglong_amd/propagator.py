@@ -81,6 +81,14 @@ class Propagator:
         check(lib().klrtm_propagator_snapshot_cache_info(self.h, C.byref(m), C.byref(st), C.byref(ns)))
         return dict(in_memory=bool(m.value), stride=st.value, nsnap=ns.value)
 
+    def set_source_array(self, xyz, delays_s):
+        """Source array (Propagator::setupReceiver): the wavelet injected at each position, delayed by
+        its delay in seconds (FFT time shift)."""
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        dl = np.ascontiguousarray(delays_s, np.float32).reshape(-1)
+        assert dl.size == xyz.shape[0]
+        check(lib().klrtm_propagator_set_source_array(self.h, xyz.shape[0], fptr(xyz), fptr(dl)))
+
     def set_source_wavelet(self, wavelet):
         w = np.ascontiguousarray(wavelet, np.float32)
         assert w.shape == (self.cfg.nt,)

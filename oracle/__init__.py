@@ -303,6 +303,31 @@ def update2nd_q(w0, w1, invq, d0, d1, coeffs):
     return w0
 
 
+def fft_size(n):
+    """Smallest 2^a 3^b 5^c >= n: the padded FFT length this build uses where the reference calls the
+    unvendored libCommon::padfft (parity unpinned for that choice)."""
+    m = max(int(n), 1)
+    while True:
+        r = m
+        for f in (2, 3, 5):
+            while r % f == 0:
+                r //= f
+        if r == 1:
+            return m
+        m += 1
+
+
+def timeshift(trace, shift_over_dt, ntfft):
+    """FreqUtil::timeshift_in/out (bgprtm/FreqUtil.cpp:24-39) in FP64: the trace zero-padded to ntfft,
+    its real spectrum times exp(-i iw (2 pi / ntfft) t), the inverse real transform divided by ntfft
+    (numpy's irfft includes it), the first nt samples kept."""
+    x = np.asarray(trace, np.float64)
+    nt = x.size
+    W = np.fft.rfft(x, ntfft)
+    iw = np.arange(W.size)
+    return np.fft.irfft(W * np.exp(-1j * iw * (2 * np.pi / ntfft) * shift_over_dt), ntfft)[:nt]
+
+
 def apply_symmetry(wf, iz0, sym, tprlen, taper=1):
     """Propagator::apply_symmetry (bgprtm/Propagator.cpp:1857-1918) in place on a (ny, nx, nz)
     float32 or float64 volume."""

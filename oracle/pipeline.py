@@ -16,7 +16,7 @@ import math
 import numpy as np
 
 from . import (lib, Engine, spread_coeff, taper_kernel, xy_taper, compress16, uncompress16, apply_symmetry, _pd,
-               vel2_q, update2nd_q)
+               vel2_q, update2nd_q, fft_size, timeshift)
 
 _pf, _pi = C.POINTER(C.c_float), C.POINTER(C.c_int)
 
@@ -89,7 +89,7 @@ class Shot:
 
     def __init__(self, nx, ny, nz, d, dt, nt, engine, order, model, nbnd, vmax, src_xyz, f_peak, spread, k1, k2,
                  rcv_xyz, snap_every=1, snap_format=0, tmute_frac=0.1, threads=4, free_surface_iz=-1, surface_taper=1,
-                 q=None, ghost=None, rho=None):
+                 q=None, ghost=None, rho=None, src_array=None):
         self.shape = (ny, nx, nz)
         self.fs, self.fs_taper = free_surface_iz, surface_taper
         # injection mirrors (Propagator::determine_ghost, bgprtm/Propagator.cpp:1538-1541): FREESURFACE
@@ -112,6 +112,12 @@ class Shot:
         self.rcv = spread_tables(nx, ny, nz, rcv_xyz, spread, k1, k2)
         self.wavelet = ricker(nt, dt, f_peak)
         self.tmute_frac = tmute_frac
+        # source array (Propagator::setupReceiver, bgprtm/Propagator.cpp:783-811): positions and delays
+        # in seconds; each source injects the wavelet time-shifted by its delay
+        self.src_delays = None
+        if src_array is not None:
+            self.src = spread_tables(nx, ny, nz, np.asarray(src_array[0], np.float32).reshape(-1, 3), spread, k1, k2)
+            self.src_delays = np.asarray(src_array[1], np.float64)
 
     def set_model(self, vel, eps=None, del_=None, pjx=None, pjy=None):
         v = np.asarray(vel, np.float32)
@@ -132,6 +138,16 @@ class Shot:
         if self.fs >= 0:
             nzb = self.nbnd[4] if not np.isscalar(self.nbnd) else self.nbnd
             apply_symmetry(w0, self.fs, -1, nzb, self.fs_taper)
+
+    def _src_vals(self, it):
+        """Source values of step it (already scaled by dt_dxyz)."""
+        if self.src_delays is None:
+            return self.wavelet[it:it + 1] * self.dt_dxyz
+        if getattr(self, "_src_tr", None) is None or self._src_tr[0] is not self.wavelet:
+            nfft = fft_size(self.nt)
+            tr = np.array([timeshift(self.wavelet, d / self.dt, nfft) for d in self.src_delays], np.float32)
+            self._src_tr = (self.wavelet, tr)
+        return self._src_tr[1][:, it] * self.dt_dxyz
 
     def _fresh(self):
         self.eng.clean_pml()
@@ -158,7 +174,7 @@ class Shot:
         for it in range(self.nt):
             self._step(w1, w0, wr)
             if 0 <= it - 1 < self.nt:
-                inject(w0, self.m, self.shape, self.src, self.wavelet[it - 1:it] * self.dt_dxyz,
+                inject(w0, self.m, self.shape, self.src, self._src_vals(it - 1),
                        self.src_mirror)
             self._surface(w0)
             rec[:, it] = extract(w0, self.shape, self.rcv, self.ext_mirror)
@@ -180,7 +196,7 @@ class Shot:
         for it in range(self.nt):
             self._step(w1, w0, wr)
             if 0 <= it - 1 < self.nt:
-                inject(w0, self.m, self.shape, self.src, self.wavelet[it - 1:it] * self.dt_dxyz,
+                inject(w0, self.m, self.shape, self.src, self._src_vals(it - 1),
                        self.src_mirror)
             self._surface(w0)
             if it % k == 0 and it // k < ntsav:
@@ -188,8 +204,9 @@ class Shot:
             w0, w1 = w1, w0
         img = np.zeros(self.shape, np.float32)
         ill = np.zeros(self.shape, np.float32)
-        tmax = self.nt * self.dt
-        ntaperT = int(np.rint(np.float32(self.tmute_frac * tmax / (k * self.dt))))
+        # ImagingCondition::calTapes rounds an FP32 quotient (bgprtm/ImagingCondition.cpp:185)
+        f32 = np.float32
+        ntaperT = int(np.rint(f32(self.tmute_frac) * (f32(self.nt) * f32(self.dt)) / (f32(k) * f32(self.dt))))
         wtT = [np.float32(0.5) * (np.float32(1) - np.cos(np.float32(np.pi) * np.float32(i) / np.float32(ntaperT)))
                for i in range(ntaperT)]
         w0, w1, wr = self._fresh()
@@ -279,7 +296,8 @@ class Shot64:
             w0, w1 = w1, w0
         img = np.zeros(self.shape, np.float64)
         ill = np.zeros(self.shape, np.float64)
-        ntaperT = int(np.rint(self.tmute_frac * self.nt * self.dt / (k * self.dt)))
+        f32 = np.float32  # the taper length rounds an FP32 quotient, as in Shot (a count, not a precision)
+        ntaperT = int(np.rint(f32(self.tmute_frac) * (f32(self.nt) * f32(self.dt)) / (f32(k) * f32(self.dt))))
         w0, w1, wr = self._fresh()
         rec = np.asarray(record, np.float64)
         for it in range(self.nt - 1, -1, -1):
