@@ -34,6 +34,8 @@ WORKLOADS = {
                         snap_every=4, record_s=2.0, seed=505),
     "iso-fd8-128": dict(shape=(128, 128, 128), d=10.0, engine="FD", order=8, model="ISO", nbnd=20, fpeak=15.0,
                         snap_every=1, record_s=0.6, seed=101, dt=1e-3),
+    "tti-fd16-600": dict(shape=(600, 600, 400), d=12.5, engine="FD", order=16, model="TTI", nbnd=40, fpeak=15.0,
+                         snap_every=3, record_s=2.325, seed=303),
 }
 # algorithmic HBM bytes per cell update of the step (state + model arrays read once, new field written once)
 VOL_BYTES = {"ISO": 16, "VTI": 24, "TTI": 32}
@@ -53,6 +55,8 @@ def build_case(name):
         vel = models.smooth_vti_model(w["shape"], 20.0, w["seed"])[0]
     else:
         vel, eps, dl = models.smooth_vti_model(w["shape"], 20.0, w["seed"])
+    if mdl == TTI:  # tilt 0-45 deg, azimuth 0-360 deg, smoothed like the other fields
+        w["pj"] = models.tilt_axes(w["shape"], 20.0, w["seed"] + 10)
     dt = w.get("dt") or models.cfl_dt(vel, w["d"], eng, w["order"], mdl, eps, dl)
     return w, eng, mdl, vel, eps, dl, dt
 
@@ -82,7 +86,7 @@ def cpu_baseline(vel, eps, dl, w, eng, mdl, dt, budget_s):
     threads = max(1, min(16, os.cpu_count() or 1))
     m = (vel * vel * (np.float32(dt) * np.float32(dt))).astype(np.float32)
     o = oracle.Engine(nx, ny, nz, w["d"], w["d"], w["d"], eng, w["order"], np.float32, threads=threads)
-    o.set_model(m, eps, dl)
+    o.set_model(m, eps, dl, *w.get("pj", (None, None)))
     o.set_boundary([w["nbnd"]] * 6, dt, float(vel.max()), dim_pml=3)
     rng = np.random.default_rng(1)
     w1 = (1e-3 * rng.standard_normal(w["shape"])).astype(np.float32)
@@ -266,7 +270,7 @@ def main():
                       khpass=kband[0], khcut=kband[1], nrcv=rcv.shape[0], snap_every=w["snap_every"],
                       snap_format=SNAP_FP32, tmute_frac=0.1, fuse_step=1, snap_slots=16)
     p = Propagator(cfg)
-    p.set_model(vel, eps, dl)
+    p.set_model(vel, eps, dl, *w.get("pj", (None, None)))
     p.set_receivers(rcv)
     rng = np.random.default_rng(7)
     p.set_record((1e-3 * rng.standard_normal((rcv.shape[0], nt))).astype(np.float32))
