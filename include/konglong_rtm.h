@@ -293,8 +293,9 @@ int klrtm_propagator_set_ghost(klrtm_propagator *p, int iz_mirror, int source_si
  * device staging slots and streams every snapshot to host memory when all of them fit mem_limit_gb
  * ("cache_mem_limit"), else to the file at path (SEQ_ZXYT: snapshot s at byte offset s*stride, stride =
  * the code size, for COMPRESS_16 2*klrtm_nshort_volume; removed on destruction unless keep).  Writes and
- * reads run on a host thread beside the propagation.  FP32, COMPRESS_16 and learned snapshots; call
- * before the first rtm call.  The stepping API (forward_steps/backward_steps) keeps snapshots in HBM. */
+ * reads run on a host thread beside the propagation.  Every snapshot format (a block-scaled FP16/BF16
+ * snapshot is its payload padded to 256 bytes followed by its scales); call before the first rtm call.
+ * The stepping API (forward_steps/backward_steps) keeps snapshots in HBM. */
 int klrtm_propagator_set_snapshot_cache(klrtm_propagator *p, const char *path, double mem_limit_gb, int keep);
 /* after an rtm call with a cache: whether it lives in host memory, its stride and snapshot count */
 int klrtm_propagator_snapshot_cache_info(klrtm_propagator *p, int *in_memory, size_t *stride_bytes, int *nsnap);

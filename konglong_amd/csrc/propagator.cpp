@@ -58,6 +58,11 @@ struct Propagator {
   SnapCache *cache = nullptr;
   size_t snap_bytes = 0;  // unpadded code size of one snapshot (the cache stride)
   char *slot(int itsav) const { return d_snap + snap_stride_bytes * (size_t)(use_cache ? (itsav & 1) : itsav); }
+  // block scales of snapshot itsav: in HBM beside the payloads, or inside the staging slot with a cache
+  float *scales(int itsav) const {
+    if(!use_cache) return d_scales + nscales * (size_t)itsav;
+    return (float *)(slot(itsav) + (((size_t)c.nx * c.ny * c.nz * sizeof(uint16_t) + 255) & ~(size_t)255));
+  }
   // ghost mirror plane of SOURCE_GHOST / RECEIVER_GHOST / GHOST surfaces (-1: none) and its sides
   int ghost_iz = -1;
   bool ghost_src = false, ghost_rcv = false;
@@ -131,7 +136,9 @@ int Propagator::alloc_snapshots() {
   snap_stride_bytes = (snap_stride_bytes + 255) & ~(size_t)255;
   int dev_slots = ntsav;
   if(use_cache) {
-    KL_REQUIRE(!nscales, KLRTM_EINVAL, "the snapshot cache takes FP32, COMPRESS_16 or learned snapshots");
+    // a cached block-scaled snapshot carries its scales in its slot, after the padded payload
+    if(nscales) snap_bytes = snap_stride_bytes + sizeof(float) * nscales;
+    snap_stride_bytes = (snap_bytes + 255) & ~(size_t)255;
     ntsav = (c.nt + k - 1) / k;  // every snapshot of the shot goes to the cache
     dev_slots = 2;
     int rc = cache ? KLRTM_OK
@@ -145,7 +152,7 @@ int Propagator::alloc_snapshots() {
     d_snap = nullptr;
     return KLRTM_ENOMEM;
   }
-  if(nscales) KL_HIP(hipMalloc(&d_scales, sizeof(float) * nscales * ntsav));
+  if(nscales && !use_cache) KL_HIP(hipMalloc(&d_scales, sizeof(float) * nscales * ntsav));
   return KLRTM_OK;
 }
 
@@ -199,7 +206,7 @@ int Propagator::store(int itsav) {
     break;
   case KLRTM_SNAP_C16: rc = c16_encode(w0, (uint16_t *)dst, c.nx, c.ny, c.nz, d_smax); break;
   case KLRTM_SNAP_LEARNED: rc = lcodec_encode(*lc, w0, dst, c.nx, c.ny, c.nz); break;
-  default: rc = blockq_encode(w0, (uint16_t *)dst, d_scales + nscales * (size_t)itsav, c.nx, c.ny, c.nz, c.snap_format);
+  default: rc = blockq_encode(w0, (uint16_t *)dst, scales(itsav), c.nx, c.ny, c.nz, c.snap_format);
   }
   if(rc || !use_cache) return rc;
   return snapcache_store(*cache, itsav, dst);  // CacheFile::pwrite at itsav*stride (bgprtm/Propagator.cpp:1233-1243)
@@ -225,7 +232,7 @@ int Propagator::image(int itsav) {
   const char *snap = slot(itsav);
   double *i64 = c.ic_fp32_only ? nullptr : d_img64, *l64 = c.ic_fp32_only ? nullptr : d_ill64;
   if(c.snap_format == KLRTM_SNAP_LEARNED) return lcodec_image(*lc, a, snap, w0, d_img, d_ill, i64, l64);
-  const float *sc = nscales ? d_scales + nscales * (size_t)itsav : nullptr;
+  const float *sc = nscales ? scales(itsav) : nullptr;
   return launch_ic(c.snap_format, rq, a, snap, sc, w0, d_smax, d_img, d_ill, i64, l64);
 }
 

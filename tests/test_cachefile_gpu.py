@@ -67,7 +67,7 @@ def test_file_cache_matches_hbm_and_reference_layout(klrtm, tmp_path):
     assert os.path.exists(path)  # keep
 
 
-@pytest.mark.parametrize("fmt", [SNAP_FP32, SNAP_C16], ids=["fp32", "c16"])
+@pytest.mark.parametrize("fmt", [SNAP_FP32, SNAP_C16, SNAP_FP16], ids=["fp32", "c16", "fp16-block-scaled"])
 def test_memory_cache_matches_hbm(klrtm, fmt):
     print(f"[cachefile] FD8-ISO memory cache fmt={fmt}", flush=True)
     make, o = _shot(ENGINE_FD, 8, ISO, fmt)
@@ -85,14 +85,25 @@ def test_memory_cache_matches_hbm(klrtm, fmt):
     assert np.array_equal(got2["image"], ref["image"])
 
 
-def test_cache_rejects_misuse(klrtm, tmp_path):
-    make, o = _shot(ENGINE_FD, 8, ISO, SNAP_FP16, nt=12)
+def test_file_cache_block_scaled(klrtm, tmp_path):
+    """FP16 snapshots with their block scales through a file: the image of the in-HBM run."""
+    make, o = _shot(ENGINE_FD, 8, ISO, SNAP_FP16)
     rec = o.model()
+    ref = make().rtm(rec)
     p = make()
-    p.set_snapshot_cache(tmp_path / "x.cache")
-    with pytest.raises(KlrtmError, match="cache"):   # block-scaled snapshots keep their scales apart
-        p.rtm(rec)
-    make2, _ = _shot(ENGINE_FD, 8, ISO, SNAP_FP32, nt=12)
+    p.set_snapshot_cache(tmp_path / "h.cache", mem_limit_gb=0.0)
+    got = p.rtm(rec)
+    info = p.snapshot_cache_info()
+    ny, nx, nz = o.shape
+    pad = (nx * ny * nz * 2 + 255) // 256 * 256
+    assert not info["in_memory"] and info["stride"] > pad
+    assert np.array_equal(got["image"], ref["image"]) and np.array_equal(got["illum"], ref["illum"])
+
+
+def test_cache_rejects_misuse(klrtm, tmp_path):
+    make, o = _shot(ENGINE_FD, 8, ISO, SNAP_FP32, nt=12)
+    rec = o.model()
+    make2 = make
     p2 = make2()
     p2.set_snapshot_cache(None, mem_limit_gb=0.0)   # does not fit memory and has no file
     with pytest.raises(KlrtmError, match="file"):
