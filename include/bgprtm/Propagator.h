@@ -30,6 +30,13 @@ public:
     const int order = coeffs.empty() ? 0 : (int)(coeffs.size() - 1) / 2;
     konglong::check(klrtm_propagator_set_q(p_, invQ, order, coeffs.empty() ? nullptr : coeffs.data()));
   }
+  // density (Model::useRho): host volume, nullptr switches it off
+  void setRho(const float *rho) { konglong::check(klrtm_propagator_set_rho(p_, rho)); }
+  // source array (Propagator::setupReceiver): positions xyz[3*n] in grid coordinates, delays in seconds
+  void setSourceArray(const std::vector<float> &xyz, const std::vector<float> &delay) {
+    if(xyz.size() != 3 * delay.size()) throw konglong::Error(KLRTM_EINVAL, "konglong: xyz needs 3 values per delay");
+    konglong::check(klrtm_propagator_set_source_array(p_, (int)delay.size(), xyz.data(), delay.data()));
+  }
   // SOURCE_GHOST / RECEIVER_GHOST / GHOST: mirror the chosen sides' injection about iz_mirror
   void setGhost(int izMirror, bool sourceSide, bool receiverSide) {
     konglong::check(klrtm_propagator_set_ghost(p_, izMirror, sourceSide ? 1 : 0, receiverSide ? 1 : 0));
