@@ -13,6 +13,8 @@
 #include "../../include/bgprtm/ImagingCorrelator.h"
 #include "../../include/bgprtm/KernelCPU.h"
 #include "../../include/bgprtm/PML.h"
+#include "../../include/bgprtm/Propagator.h"
+#include "../../include/bgprtm/Receiver.h"
 #include "../../include/bgprtm/WaveFieldCompress.h"
 
 extern "C" {  // oracle (oracle/rtm_oracle.cpp)
