@@ -44,10 +44,10 @@ public:
   }
   void VTI(float *p0, float *pb, float *pr) { konglong::check(klrtm_kernel_vti(k_, p0, pb, pr)); }
   // bndType/iz0/spread select the free-surface mirror in the reference; ABSORB (0) only here
+  // bndType, iz0 and the taper length are unused, as in the reference (bgprtm/KernelCPU.cpp:125-174): a
+  // free surface is Propagator::apply_symmetry after the injection (klrtm_apply_symmetry)
   void ISO(float *p0, float *pb, float *pr, int bndType = 0, int iz0 = 0, int spread = 0) {
-    if(bndType != 0)
-      throw konglong::Error(KLRTM_EINVAL, "konglong: only the absorbing surface (bndType 0) is supported");
-    (void)iz0, (void)spread;
+    (void)bndType, (void)iz0, (void)spread;
     konglong::check(klrtm_kernel_iso(k_, p0, pb, pr));
   }
   void cleanPMLMemory() { konglong::check(klrtm_kernel_clean_pml(k_)); }
