@@ -29,8 +29,17 @@ public:
   void apply(float *w0, int it, const float *vel, float scaler = 1.0f) {
     konglong::check(klrtm_receiver_apply(r_, w0, it, vel, scaler));
   }
-  // record[:, jt] = sum kx*ky*kz * w
-  void extract(const float *w, int jt) { konglong::check(klrtm_receiver_extract(r_, w, jt)); }
+  // Receiver::apply(wf, it, ghost, vdt2, scaler0): with ghost the injection is mirrored about the surface
+  // plane izMirror (grid->getIDz(zsurf))
+  void apply(float *w0, int it, bool ghost, int izMirror, const float *vel, float scaler = 1.0f) {
+    if(ghost) konglong::check(klrtm_receiver_apply_ghost(r_, w0, it, vel, scaler, izMirror));
+    else konglong::check(klrtm_receiver_apply(r_, w0, it, vel, scaler));
+  }
+  // record[:, jt] = sum kx*ky*kz * w (minus the mirrored samples with a ghost plane izMirror >= 0)
+  void extract(const float *w, int jt, int izMirror = -1) {
+    if(izMirror >= 0) konglong::check(klrtm_receiver_extract_ghost(r_, w, jt, izMirror));
+    else konglong::check(klrtm_receiver_extract(r_, w, jt));
+  }
   std::vector<float> record() const {
     std::vector<float> rec((size_t)nr_ * nt_);
     konglong::check(klrtm_receiver_get_record(r_, rec.data()));
