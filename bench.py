@@ -237,9 +237,9 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
-              f"(requested --gpus {args.gpus}, WORLD_SIZE {world}); multi-GPU shot sharding is described "
-              f"in DESIGN.md but not run.", file=sys.stderr)
+        print(f"bench.py measures one GPU: the hot path runs on a single MI355X "
+              f"(requested --gpus {args.gpus}, WORLD_SIZE {world}); sharding shots over several GPUs is a "
+              f"design, not something this benchmark runs.", file=sys.stderr)
         sys.exit(2)
 
     import ctypes as C
