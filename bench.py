@@ -107,6 +107,112 @@ def cpu_baseline(vel, eps, dl, w, eng, mdl, dt, budget_s):
                       f"{ny}x{nx}x{nz} model, FP32, {threads} OpenMP threads"}
 
 
+def adjacent_physics(steps=6):
+    """Kernel timings of the paths either side of the step on the 400^3 grid (dx 12.5 m, PML 32): the Q
+    leapfrog, density on the first-derivative form, ghost-mirrored injection, the snapshot cache in host
+    memory and the FFT time shift.  Times are HIP-event means per launch (ms) or wall times (s)."""
+    import ctypes as C
+    from konglong_amd import models
+    from konglong_amd._lib import ENGINE_FD, ISO, VTI, SNAP_FP32, DeviceArray, lib, check, fptr
+    from konglong_amd.propagator import Propagator, shot_config
+    from konglong_amd.receiver import Receiver
+    L = lib()
+    shape = (400, 400, 400)
+    ny, nx, nz = shape
+    n = nx * ny * nz
+    vel, eps, dl = models.smooth_vti_model(shape, 20.0, 606)
+    dt = models.cfl_dt(vel, 12.5, ENGINE_FD, 8, VTI, eps, dl)
+    rcv = models.surface_receivers(nx, ny, 34)
+
+    def timed(p, names, nsteps=steps):
+        p.reset()
+        p.forward_steps(0, 1, False)
+        L.klrtm_synchronize()
+        check(L.klrtm_timer_reset())
+        check(L.klrtm_timer_enable(1))
+        p.forward_steps(1, nsteps, False)
+        L.klrtm_synchronize()
+        check(L.klrtm_timer_enable(0))
+        out = {}
+        for nm in names:
+            t, c = C.c_double(), C.c_long()
+            L.klrtm_timer_read(nm.encode(), C.byref(t), C.byref(c))
+            if c.value:
+                out[nm] = round(t.value / c.value, 4)
+        return out
+
+    def shot(model, nt=2 + steps, **kw):
+        cfg = shot_config(nx, ny, nz, 12.5, dt, nt, ENGINE_FD, 8, model, nbnd=32, vmax=float(vel.max()),
+                          src_xyz=(199.5, 199.5, 34.0), f_peak=15.0, spread=10, nrcv=rcv.shape[0], snap_every=4,
+                          snap_format=SNAP_FP32, **kw)
+        p = Propagator(cfg)
+        p.set_receivers(rcv)
+        return p
+
+    res = {"grid": list(shape)}
+    # Q (order 3) on the ISO FD8 step: kernel + Q leapfrog against the fused step
+    p = shot(ISO)
+    p.set_model(vel)
+    base = timed(p, ["iso_step"])
+    p.set_q(np.full(shape, 0.02, np.float32), np.array([0.04, 0.3, 0.2, 0.1, 0.9, 0.7, 0.5], np.float32))
+    p.set_model(vel)
+    q = timed(p, ["iso_kernel", "update2nd_q"])
+    res["q_order3"] = {"iso_step_ms": base.get("iso_step"), **{k + "_ms": v for k, v in q.items()},
+                       "update2nd_q_GBps": round(n * 52 / (q["update2nd_q"] * 1e-3) / 1e9, 1)}
+    p.close()
+    # density on the VTI FD8 step (gradient + divergence kernels), against no density
+    p = shot(VTI)
+    p.set_model(vel, eps, dl)
+    a = timed(p, ["fd_grad", "fd_div_step"])
+    p.set_rho(models.smooth_vti_model(shape, 20.0, 607)[0] / np.float32(2000.0))
+    b = timed(p, ["fd_grad", "fd_div_step"])
+    res["density_vti_fd8"] = {"without_ms": round(sum(a.values()), 4), "with_ms": round(sum(b.values()), 4)}
+    p.close()
+    # ghost-mirrored injection of the 160 000-receiver surface (spread 10)
+    r = Receiver(nx, ny, nz, rcv, 10, 0.3, 0.3, 4, 1.0 / 12.5 ** 3)
+    r.set_traces(np.random.default_rng(1).standard_normal((rcv.shape[0], 4)).astype(np.float32))
+    w, dv = DeviceArray(shape), DeviceArray.from_host(vel)
+    gh = {}
+    for mirror in (-1, 24):
+        r.apply(w, 0, dv, 1.0, mirror)
+        L.klrtm_synchronize()
+        check(L.klrtm_timer_reset())
+        check(L.klrtm_timer_enable(1))
+        for it in range(4):
+            r.apply(w, it, dv, 1.0, mirror)
+        L.klrtm_synchronize()
+        check(L.klrtm_timer_enable(0))
+        t, c = C.c_double(), C.c_long()
+        L.klrtm_timer_read(b"inject", C.byref(t), C.byref(c))
+        gh["mirror" if mirror >= 0 else "plain"] = round(t.value / c.value, 4)
+    res["injection_ms"] = gh
+    del w, dv, r
+    # snapshot cache in host memory: a 40-step RTM with 10 FP32 snapshots (2.56 GB) against HBM
+    times = {}
+    for cached in (False, True):
+        p = shot(ISO, nt=40)
+        p.set_model(vel)
+        if cached:
+            p.set_snapshot_cache(None, mem_limit_gb=8.0)
+        rec = np.zeros((rcv.shape[0], 40), np.float32)
+        p.rtm(rec, want=("image",))  # warm (allocations, pinned staging)
+        t0 = time.perf_counter()
+        p.rtm(rec, want=("image",))
+        times["host_cache" if cached else "hbm"] = round(time.perf_counter() - t0, 3)
+        p.close()
+    res["rtm_40_steps_s"] = times
+    # FFT time shift: 1000 traces of 3000 samples
+    tr = DeviceArray.from_host(np.random.default_rng(2).standard_normal(3000).astype(np.float32))
+    out = DeviceArray((1000, 3000))
+    sh = np.linspace(-50, 50, 1000).astype(np.float32)
+    nfft = int(L.klrtm_fft_size(3000))
+    check(L.klrtm_timeshift(tr.ptr, 3000, nfft, 1000, fptr(sh), out.ptr))
+    t0 = time.perf_counter()
+    check(L.klrtm_timeshift(tr.ptr, 3000, nfft, 1000, fptr(sh), out.ptr))
+    res["timeshift_1000x3000_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    return res
+
+
 def batched_vs_plain(nshots, nt, batch, seed=505):
     """Batched multi-shot ISO RTM against the plain per-shot FP32 path on the same shots.
 
@@ -228,7 +334,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="vti-cfd10-400", choices=sorted(WORKLOADS) + ["tti-fd16-600-shot"])
+    ap.add_argument("--workload", default="vti-cfd10-400",
+                    choices=sorted(WORKLOADS) + ["tti-fd16-600-shot", "adjacent-400"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--batched-shots", type=int, default=8, help="shots of the batched-vs-plain leg (0: skip)")
@@ -254,6 +361,9 @@ def main():
         sys.exit(1)
     if args.workload == "tti-fd16-600-shot":
         print(json.dumps(tti_full_shot()))
+        return
+    if args.workload == "adjacent-400":
+        print(json.dumps({"workload": "adjacent-400", **adjacent_physics()}))
         return
     w, eng, mdl, vel, eps, dl, dt = build_case(args.workload)
     ny, nx, nz = w["shape"]
