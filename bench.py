@@ -156,7 +156,7 @@ def adjacent_physics(steps=6):
     base = timed(p, ["iso_step"])
     p.set_q(np.full(shape, 0.02, np.float32), np.array([0.04, 0.3, 0.2, 0.1, 0.9, 0.7, 0.5], np.float32))
     p.set_model(vel)
-    q = timed(p, ["iso_kernel", "update2nd_q"])
+    q = timed(p, ["iso_pb", "update2nd_q"])
     res["q_order3"] = {"iso_step_ms": base.get("iso_step"), **{k + "_ms": v for k, v in q.items()},
                        "update2nd_q_GBps": round(n * 52 / (q["update2nd_q"] * 1e-3) / 1e9, 1)}
     p.close()

@@ -4,7 +4,8 @@
 
 namespace klrtm {
 
-enum { MODE_KERNEL = 0, MODE_STEP = 1, MODE_LAP = 2 };  // pb -= vel*L (+pr = L) | + leapfrog | pr = L only
+// pb -= vel*L (+pr = L) | + leapfrog | pr = L only | pb -= vel*L only (the Laplacian form; pr untouched)
+enum { MODE_KERNEL = 0, MODE_STEP = 1, MODE_LAP = 2, MODE_PB = 3 };
 
 // Arguments of the y-streaming stencil kernels (passed by value).
 struct IsoArgs {

@@ -253,6 +253,8 @@ __global__ void __launch_bounds__(512) iso_step_kernel(const float *__restrict__
       } else if(MODE == MODE_KERNEL) {
         buf_st_nt(L, prs(pr, y), oc, 0);
         buf_st_nt(fmaf(-L, C.v, C.w), prs(w0, y), oc, 0);
+      } else if(MODE == MODE_PB) {
+        buf_st_nt(fmaf(-L, C.v, C.w), prs(w0, y), oc, 0);
       } else {
         buf_st_nt(L, prs(pr, y), oc, 0);
       }
@@ -560,9 +562,11 @@ static int launch_iso_r(int mode, const float *u, float *w0, const float *vel, f
   grid_of(a);
   int nb = a.nbz * a.nbx * a.nby;
   size_t lds = 2 * sizeof(float) * (size_t)(a.tx + 2 * R) * (a.tz + 2 * R);  // two plane tiles
-  LaunchTimer tm(mode == MODE_STEP ? "iso_step" : mode == MODE_KERNEL ? "iso_kernel" : "laplacian");
+  LaunchTimer tm(mode == MODE_STEP ? "iso_step" : mode == MODE_KERNEL ? "iso_kernel" : mode == MODE_PB ? "iso_pb"
+                                                                                                : "laplacian");
   if(mode == MODE_STEP) return launch_iso_rm<R, MODE_STEP>(u, w0, vel, pr, a, nb, lds);
   if(mode == MODE_KERNEL) return launch_iso_rm<R, MODE_KERNEL>(u, w0, vel, pr, a, nb, lds);
+  if(mode == MODE_PB) return launch_iso_rm<R, MODE_PB>(u, w0, vel, pr, a, nb, lds);
   return launch_iso_rm<R, MODE_LAP>(u, w0, vel, pr, a, nb, lds);
 }
 

@@ -186,7 +186,7 @@ int Propagator::reset() {
 int Propagator::step(int) {
   if(q_order > 0) {
     KL_REQUIRE(q_ready, KLRTM_ESTATE, "Q is set: call klrtm_propagator_set_model after klrtm_propagator_set_q");
-    int rc = eng->run(c.model, MODE_KERNEL, w1, w0, wr);
+    int rc = eng->run(c.model, MODE_PB, w1, w0, wr);  // pb -= vel*L (no Laplacian copy in wr)
     if(rc || (rc = launch_update2nd_q(w0, w1, d_invq, d_q[iq0], d_q[iq0 ^ 1], n, q_order, q_coeffs))) return rc;
     iq0 ^= 1;
     return KLRTM_OK;
