@@ -46,6 +46,7 @@ def lib():
             getattr(L, f"orc_engine_laplacian_{s}").argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
             getattr(L, f"orc_engine_deriv1_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
                                                             C.c_int]
+            getattr(L, f"orc_engine_scalar_{s}").argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
             getattr(L, f"orc_update2nd_{s}").argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
             getattr(L, f"orc_vel2_q_{s}").argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, T, T]
             getattr(L, f"orc_update2nd_q_{s}").argtypes = [C.c_void_p] * 5 + [C.c_size_t, C.c_int, C.c_int,
@@ -147,6 +148,12 @@ class Engine:
     def deriv1(self, axis, u, out, iacc, isign):
         """One first derivative over the volume (axis 0 z, 1 x, 2 y) with its PML; iacc: out -= vel*D(u)."""
         getattr(lib(), f"orc_engine_deriv1_{self.s}")(self.h, axis, u.ctypes.data, out.ctypes.data, iacc, isign)
+
+    def scalar(self, model, wx, wy, wz):
+        """KernelCPU::ScalarVTI / ScalarTTI (model 2 / 3) in place on gradient volumes (wy may be None),
+        with the model parameters given to set_model."""
+        getattr(lib(), f"orc_engine_scalar_{self.s}")(self.h, model, wx.ctypes.data,
+                                                      None if wy is None else wy.ctypes.data, wz.ctypes.data)
 
     def close(self):
         if getattr(self, "h", None):

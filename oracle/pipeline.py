@@ -167,19 +167,43 @@ class Shot:
         update2nd_q(w0, w1, self.invq, self.wq[0], self.wq[1], self.q[1])
         self.wq.reverse()
 
-    def model(self):
+    def forward(self, visit):
+        """The forward time loop of Propagator::modelling / migration (bgprtm/Propagator.cpp:1101-1153,
+        1210-1243): the step, the source injection of trace sample it-1, the free surface; visit(it, w0)
+        then sees the new field (where the record is extracted and the snapshot stored)."""
         w0, w1, wr = self._fresh()
-        nr = self.rcv["lx"].size
-        rec = np.zeros((nr, self.nt), np.float32)
         for it in range(self.nt):
             self._step(w1, w0, wr)
             if 0 <= it - 1 < self.nt:
                 inject(w0, self.m, self.shape, self.src, self._src_vals(it - 1),
                        self.src_mirror)
             self._surface(w0)
-            rec[:, it] = extract(w0, self.shape, self.rcv, self.ext_mirror)
+            visit(it, w0)
             w0, w1 = w1, w0
-        self.last = w1
+        return w1
+
+    def backward(self, record, visit):
+        """The backward loop of Propagator::migration (bgprtm/Propagator.cpp:1199-1255): it runs from
+        nt-1 down; the step, the receiver injection of record sample it+1, the free surface; visit(it, w0)
+        then sees the new field (where it is imaged)."""
+        w0, w1, wr = self._fresh()
+        rec = np.ascontiguousarray(record, np.float32)
+        for it in range(self.nt - 1, -1, -1):
+            self._step(w1, w0, wr)
+            ia = it + 1
+            if 0 <= ia < self.nt:
+                inject(w0, self.m, self.shape, self.rcv, rec[:, ia] * self.dt_dxyz, self.rcv_mirror)
+            self._surface(w0)
+            visit(it, w0)
+            w0, w1 = w1, w0
+
+    def model(self):
+        nr = self.rcv["lx"].size
+        rec = np.zeros((nr, self.nt), np.float32)
+
+        def visit(it, w0):
+            rec[:, it] = extract(w0, self.shape, self.rcv, self.ext_mirror)
+        self.last = self.forward(visit)
         return rec
 
     def _decode(self, snap):
@@ -192,16 +216,11 @@ class Shot:
         k = self.snap_every
         ntsav = (self.nt + k - 1) // k
         snaps = [None] * ntsav
-        w0, w1, wr = self._fresh()
-        for it in range(self.nt):
-            self._step(w1, w0, wr)
-            if 0 <= it - 1 < self.nt:
-                inject(w0, self.m, self.shape, self.src, self._src_vals(it - 1),
-                       self.src_mirror)
-            self._surface(w0)
+
+        def store(it, w0):
             if it % k == 0 and it // k < ntsav:
                 snaps[it // k] = compress16(w0) if self.snap_format == 1 else w0.copy()
-            w0, w1 = w1, w0
+        self.forward(store)
         img = np.zeros(self.shape, np.float32)
         ill = np.zeros(self.shape, np.float32)
         # ImagingCondition::calTapes rounds an FP32 quotient (bgprtm/ImagingCondition.cpp:185)
@@ -209,23 +228,17 @@ class Shot:
         ntaperT = int(np.rint(f32(self.tmute_frac) * (f32(self.nt) * f32(self.dt)) / (f32(k) * f32(self.dt))))
         wtT = [np.float32(0.5) * (np.float32(1) - np.cos(np.float32(np.pi) * np.float32(i) / np.float32(ntaperT)))
                for i in range(ntaperT)]
-        w0, w1, wr = self._fresh()
-        rec = np.ascontiguousarray(record, np.float32)
-        for it in range(self.nt - 1, -1, -1):
-            self._step(w1, w0, wr)
-            ia = it + 1
-            if 0 <= ia < self.nt:
-                inject(w0, self.m, self.shape, self.rcv, rec[:, ia] * self.dt_dxyz, self.rcv_mirror)
-            self._surface(w0)
+
+        def image(it, w0):
             if it % k == 0 and it // k < ntsav:
                 s = it // k
                 S = self._decode(snaps[s])
                 R = uncompress16(compress16(w0), *self.shape) if self.snap_format == 1 else _edge_taper(w0)
                 if s < ntaperT:
                     S = (S * wtT[s]).astype(np.float32)
-                img += S * R
-                ill += S * S
-            w0, w1 = w1, w0
+                img[...] += S * R
+                ill[...] += S * S
+        self.backward(record, image)
         self.snaps = snaps  # the stored snapshots (COMPRESS_16 codes or FP32 volumes), for cache checks
         return img, ill
 
@@ -271,33 +284,19 @@ class Shot64:
         z = np.zeros(self.shape, np.float64)
         return z.copy(), z.copy(), z.copy()
 
-    def model(self):
+    def forward(self, visit):
+        """Shot.forward in FP64."""
         w0, w1, wr = self._fresh()
-        rec = np.zeros((self.rcv["lx"].size, self.nt), np.float64)
         for it in range(self.nt):
             self.eng.step(self.model_type, w1, w0, wr)
             if 0 <= it - 1 < self.nt:
                 self._inject(w0, self.src, self.wavelet[it - 1:it] * self.dt_dxyz)
-            rec[:, it] = self._extract(w0, self.rcv)
+            visit(it, w0)
             w0, w1 = w1, w0
-        return rec
+        return w1
 
-    def rtm(self, record):
-        k = self.snap_every
-        ntsav = (self.nt + k - 1) // k
-        snaps = [None] * ntsav
-        w0, w1, wr = self._fresh()
-        for it in range(self.nt):
-            self.eng.step(self.model_type, w1, w0, wr)
-            if 0 <= it - 1 < self.nt:
-                self._inject(w0, self.src, self.wavelet[it - 1:it] * self.dt_dxyz)
-            if it % k == 0 and it // k < ntsav:
-                snaps[it // k] = w0.copy()
-            w0, w1 = w1, w0
-        img = np.zeros(self.shape, np.float64)
-        ill = np.zeros(self.shape, np.float64)
-        f32 = np.float32  # the taper length rounds an FP32 quotient, as in Shot (a count, not a precision)
-        ntaperT = int(np.rint(f32(self.tmute_frac) * (f32(self.nt) * f32(self.dt)) / (f32(k) * f32(self.dt))))
+    def backward(self, record, visit):
+        """Shot.backward in FP64."""
         w0, w1, wr = self._fresh()
         rec = np.asarray(record, np.float64)
         for it in range(self.nt - 1, -1, -1):
@@ -305,15 +304,41 @@ class Shot64:
             ia = it + 1
             if 0 <= ia < self.nt:
                 self._inject(w0, self.rcv, rec[:, ia] * self.dt_dxyz)
+            visit(it, w0)
+            w0, w1 = w1, w0
+
+    def model(self):
+        rec = np.zeros((self.rcv["lx"].size, self.nt), np.float64)
+
+        def visit(it, w0):
+            rec[:, it] = self._extract(w0, self.rcv)
+        self.forward(visit)
+        return rec
+
+    def rtm(self, record):
+        k = self.snap_every
+        ntsav = (self.nt + k - 1) // k
+        snaps = [None] * ntsav
+
+        def store(it, w0):
+            if it % k == 0 and it // k < ntsav:
+                snaps[it // k] = w0.copy()
+        self.forward(store)
+        img = np.zeros(self.shape, np.float64)
+        ill = np.zeros(self.shape, np.float64)
+        f32 = np.float32  # the taper length rounds an FP32 quotient, as in Shot (a count, not a precision)
+        ntaperT = int(np.rint(f32(self.tmute_frac) * (f32(self.nt) * f32(self.dt)) / (f32(k) * f32(self.dt))))
+
+        def image(it, w0):
             if it % k == 0 and it // k < ntsav:
                 s = it // k
                 S = _edge_taper(snaps[s], np.float64)
                 R = _edge_taper(w0, np.float64)
                 if s < ntaperT:
                     S = S * (0.5 * (1 - math.cos(math.pi * s / ntaperT)))
-                img += S * R
-                ill += S * S
-            w0, w1 = w1, w0
+                img[...] += S * R
+                ill[...] += S * S
+        self.backward(record, image)
         return img, ill
 
 

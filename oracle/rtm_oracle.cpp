@@ -1043,6 +1043,13 @@ void orc_cfd_derive_d(int n, double d, int sgn, const double *x, double *b, int 
     Engine<T> *e = (Engine<T> *)h;                                                                                    \
     e->deriv1(axis, in, out, iacc, isign);                                                                            \
   }                                                                                                                   \
+  void orc_engine_scalar_##S(void *h, int model, T *wx, T *wy, T *wz) {                                              \
+    Engine<T> *e = (Engine<T> *)h;                                                                                    \
+    for(int iy = 0; iy < e->ny; iy++) {                                                                               \
+      const size_t o = (size_t)iy * e->nxz;                                                                           \
+      e->scalar(model, wx + o, wy ? wy + o : nullptr, wz + o, iy);                                                    \
+    }                                                                                                                 \
+  }                                                                                                                   \
   void orc_update2nd_##S(T *w0, const T *w1, size_t n) { update2nd(w0, w1, n, 1); }                                   \
   void orc_vel2_q_##S(T *vel, T *invq, size_t n, T dt, T cq0) { vel2_q(vel, invq, n, dt, cq0); }                       \
   void orc_update2nd_q_##S(T *w0, const T *w1, const T *invq, T *d0, const T *d1, size_t nxz, int ny, int order,      \

@@ -1,4 +1,6 @@
 """Shared input generators and comparison helpers for the tests."""
+import math
+
 import numpy as np
 
 
@@ -58,3 +60,35 @@ def bf16_round(a):
 
 def bf16_bits_to_float(h):
     return (np.asarray(h, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def dense_cfd(n, d, sgn):
+    """Assemble B (9-band) and A (11-band) from the literals and folding rules, dense."""
+    ca5 = np.array([0.861085998189822, 0.753782983214427, 0.257072513289755, 0.031326722562826, 0.000410456919800])
+    cb5 = np.array([1.0, 0.753205475977546, 0.312203852849021, 0.063580085296350, 0.004597763362256])
+    fold = -1.0 if sgn == 0 else 1.0  # gradient: odd extension, divergence: even
+    B = np.zeros((n, n))
+    A = np.zeros((n, n))
+    for i in range(n):
+        for k in range(-4, 5):
+            c = cb5[abs(k)]
+            j = i + k
+            if j < 0:
+                B[i, -j] += fold * c
+            elif j > n - 1:
+                B[i, 2 * (n - 1) - j] += fold * c
+            else:
+                B[i, j] += c
+    for i in range(n):
+        for k in range(-5, 6):
+            c = 0.0 if k == 0 else math.copysign(ca5[abs(k) - 1], k) / (2 * d)
+            j = i + k
+            if 5 <= i < n - 5:
+                A[i, j] += c
+            elif j < 0:
+                A[i, -j] += -fold * c
+            elif j > n - 1:
+                A[i, 2 * (n - 1) - j] += -fold * c
+            else:
+                A[i, j] += c
+    return B, A
