@@ -1,7 +1,7 @@
 """Seeded synthetic earth models and the CFL time step.
 
 The reference loads models from files (ModelLoader) and derives dt from the
-CFL rule of ModelPrepare::calcDeltaTime (bgprtm/ModelPrepare.cpp:647-760);
+CFL rule of ModelPrepare::calcDeltaTime (bgprtm/ModelPrepare.cpp:647-870);
 these generators build the benchmark workloads' models directly.
 """
 import math
@@ -87,8 +87,10 @@ def fd_k1max(order):
     return math.sqrt(sym) * 1.01
 
 
-def cfl_dt(vel, d, engine=ENGINE_FD, order=8, model=ISO, eps=None, dl=None):
-    """dt from the CFL rule of ModelPrepare::calcDeltaTime (ISO and VTI branches; TTI uses the VTI bound)."""
+def cfl_dt(vel, d, engine=ENGINE_FD, order=8, model=ISO, eps=None, dl=None, pjx=None, pjy=None):
+    """dt from the CFL rule of ModelPrepare::calcDeltaTime without density (bgprtm/ModelPrepare.cpp:701-718 ISO,
+    :747-765 VTI, :826-870 TTI).  TTI needs the tilt axes pjx, pjy: the grid wavenumber vector is rotated into
+    the symmetry frame and the bound is the largest qP rate over the 7 non-empty axis subsets."""
     dx, dy, dz = (d, d, d) if np.isscalar(d) else d
     k1max = fd_k1max(order) if engine == ENGINE_FD else math.pi
     rdx2, rdy2, rdz2 = 1.0 / dx ** 2, 1.0 / dy ** 2, 1.0 / dz ** 2
@@ -96,11 +98,42 @@ def cfl_dt(vel, d, engine=ENGINE_FD, order=8, model=ISO, eps=None, dl=None):
     if model == ISO:
         lam = v * math.sqrt(rdx2 + rdy2 + rdz2)
         return float(np.min(2.0 / k1max / lam))
+    if model == TTI:
+        if pjx is None or pjy is None:
+            raise ValueError("the TTI CFL rule needs the tilt axes pjx, pjy")
+        dt = np.inf
+        for j in range(v.shape[0]):   # y slices: the rotated bound needs several temporaries per cell
+            dt = min(dt, _tti_dt_slice(v[j], np.asarray(eps[j], np.float64), np.asarray(dl[j], np.float64),
+                                       np.asarray(pjx[j], np.float32), np.asarray(pjy[j], np.float32),
+                                       1.0 / dx, 1.0 / dy, 1.0 / dz, k1max))
+        return float(dt)
     e = np.asarray(eps, np.float64)
     dd = np.asarray(dl, np.float64)
     tmp = (1 + 2 * e) * (rdx2 + rdy2) + rdz2
     lam = v * np.sqrt((tmp + np.sqrt(np.maximum(tmp * tmp - 8 * (e - dd) * (rdx2 + rdy2) * rdz2, 0))) / 2)
     return float(np.min(2.0 / k1max / lam / np.sqrt(1 + 2 * e)))
+
+
+def _tti_dt_slice(v, e, dd, pjx, pjy, akx, aky, akz, k1max):
+    """The TTI branch of bgprtm/ModelPrepare.cpp:826-870 on one slice: sin/cos of tilt and azimuth from the
+    axis (pjx, pjy), the rotated squared components, and the max over the axis subsets ic = 1..7."""
+    st = np.sqrt(pjx.astype(np.float64) ** 2 + pjy.astype(np.float64) ** 2)
+    ct = np.sqrt(np.maximum(1.0 - st * st, 0.0))
+    big = np.abs(st) > 1e-15
+    safe = np.where(big, st, 1.0)
+    cp = np.where(big, -pjx / safe, 1.0)
+    sp = np.where(big, -pjy / safe, 0.0)
+    rx = (cp * ct * akx + sp * ct * aky - st * akz) ** 2
+    ry = (-sp * akx + cp * aky) ** 2
+    rz = (cp * st * akx + sp * st * aky + ct * akz) ** 2
+    lam = np.zeros_like(v)
+    for ic in range(1, 8):
+        icz, icx, icy = ic & 1, (ic >> 1) & 1, (ic >> 2) & 1
+        h = rx * icx + ry * icy
+        t1 = (1.0 + 2.0 * e) * h + rz * icz
+        t2 = v * np.sqrt((t1 + np.sqrt(np.maximum(t1 * t1 - 8.0 * (e - dd) * h * rz * icz, 0.0))) / 2.0)
+        lam = np.maximum(lam, t2)
+    return float(np.min(2.0 / k1max / lam / np.sqrt(1.0 + 2.0 * e)))
 
 
 def surface_receivers(nx, ny, iz, step=1):

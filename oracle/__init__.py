@@ -26,12 +26,14 @@ def build(ref=True):
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
 
 
-def lib():
+def lib(path=None):
+    """The oracle library; `path` selects another build of the same source (the CPU-baseline timing builds)
+    when nothing is loaded yet."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if path is None and not os.path.exists(LIB_PATH):
             build(ref=False)
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path or LIB_PATH)
         for s, T in (("f", C.c_float), ("d", C.c_double)):
             getattr(L, f"orc_engine_create_{s}").restype = C.c_void_p
             getattr(L, f"orc_engine_create_{s}").argtypes = [C.c_int] * 3 + [T] * 3 + [C.c_int] * 2
