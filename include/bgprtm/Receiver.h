@@ -29,13 +29,17 @@ public:
   void apply(float *w0, int it, const float *vel, float scaler = 1.0f) {
     konglong::check(klrtm_receiver_apply(r_, w0, it, vel, scaler));
   }
-  // Receiver::apply(wf, it, ghost, vdt2, scaler0): with ghost the injection is mirrored about the surface
-  // plane izMirror (grid->getIDz(zsurf))
+  // Receiver::apply(wf, it, ghost, vdt2, scaler0) (bgprtm/Receiver.cpp:1204-1279): with ghost the
+  // injection is mirrored about the surface plane.  The reference finds that plane itself from its grid
+  // and zsurf (iz_mirror = grid->getIDz(zsurf), bgprtm/Receiver.cpp:1204); this build takes it as
+  // izMirror, because it has no Grid object
   void apply(float *w0, int it, bool ghost, int izMirror, const float *vel, float scaler = 1.0f) {
     if(ghost) konglong::check(klrtm_receiver_apply_ghost(r_, w0, it, vel, scaler, izMirror));
     else konglong::check(klrtm_receiver_apply(r_, w0, it, vel, scaler));
   }
-  // record[:, jt] = sum kx*ky*kz * w (minus the mirrored samples with a ghost plane izMirror >= 0)
+  // record[:, jt] = sum kx*ky*kz * w (minus the mirrored samples with a ghost plane izMirror >= 0): the
+  // reference's per-receiver extract_value(wf, ir, mirror_ghost) (bgprtm/Receiver.cpp:394-426) for every
+  // receiver at once, with the plane passed in as for apply
   void extract(const float *w, int jt, int izMirror = -1) {
     if(izMirror >= 0) konglong::check(klrtm_receiver_extract_ghost(r_, w, jt, izMirror));
     else konglong::check(klrtm_receiver_extract(r_, w, jt));
@@ -46,6 +50,7 @@ public:
     return rec;
   }
   int size() const { return nr_; }
+  klrtm_receiver *handle() const { return r_; }
 
 private:
   klrtm_receiver *r_ = nullptr;

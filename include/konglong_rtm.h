@@ -87,6 +87,22 @@ int klrtm_update2nd_q(float *w0, const float *w1, const float *invq, float *d0, 
 int klrtm_fft_size(int n);
 int klrtm_timeshift(const float *in, int nt, int ntfft, int ntr, const float *shift_over_dt, float *out);
 
+/* ---- PhaseCorrection, backward (bgprtm/PhaseCorrection.cpp:12-192) ----------------------------
+ * PhaseCorrection(nt, dt, Dt, t0, fmax, BWD, w_pow, phase_deg, do_tdsp, ntOut, dtOut, t0Out) with the
+ * FFT length ntfft given (the reference pads nt*(1 + tpad_ratio) with the unvendored libfftv::padfftv;
+ * klrtm_fft_size is a 2^a 3^b 5^c choice) and tpad_ratio, tdsp_acos_ratio, skip_tdsp_correction at their
+ * defaults.  applyBackward(trace, ntr, trcOut, scaler) on device traces [ntr][nt] -> [ntr][nt_out]
+ * (nt_out = ntOut when resampling, else nt): the time-dispersion correction of a propagation step Dt,
+ * resampling to dtOut with the output origin t0Out, the optional w^w_pow filter and phase rotation;
+ * scaler multiplies the output only when resampling, as in the reference.  Only the table form
+ * (do_tdsp != 0, the default tdsp_correction 2) is built. */
+typedef struct klrtm_phasecorr klrtm_phasecorr;
+int klrtm_phasecorr_create(klrtm_phasecorr **p, int nt, float dt, float Dt, float t0, float fmax, int ntfft, float w_pow,
+                           float phase_deg, int do_tdsp, int ntOut, float dtOut, float t0Out);
+int klrtm_phasecorr_destroy(klrtm_phasecorr *p);
+int klrtm_phasecorr_info(klrtm_phasecorr *p, int *nt_out, int *nw1);
+int klrtm_phasecorr_backward(klrtm_phasecorr *p, const float *in, int ntr, float *out, float scaler);
+
 /* ---- KernelCPU + FdEngine_fd / FdEngine_cfd + PML (bgprtm/KernelCPU.h:19-35) */
 typedef struct klrtm_kernel klrtm_kernel;
 /* KernelCPU(nx,ny,nz,dx,dy,dz,engine,nThreads): engine KLRTM_ENGINE_FD (order 8..16,
@@ -174,6 +190,19 @@ int klrtm_receiver_extract(klrtm_receiver *r, const float *w, int jt);
  * subtracted on extraction; mirrors above z = 0 are dropped */
 int klrtm_receiver_apply_ghost(klrtm_receiver *r, float *w0, int it, const float *vel, float scaler0, int iz_mirror);
 int klrtm_receiver_extract_ghost(klrtm_receiver *r, const float *w, int jt, int iz_mirror);
+/* FK pre-processing of the traces, Receiver::spread_sdomain (bgprtm/Receiver.cpp:851-1010): the traces
+ * (klrtm_receiver_set_traces first) are spread in x/y onto the nx*ny surface for every time sample,
+ * transformed over (y, x, t) with t zero-padded to ntfft (the reference pads nt*(1 + tpad_ratio) with the
+ * unvendored libfftv::padfftv), filtered per (ky, kx, w) with k = w/vsurf (true amplitude: the obliquity
+ * cos; deghost: 0.5/sin(kz z_depth) with zerofreq_threshold (ghost_zerofreq_threshold, 0.1) and the low
+ * cut flcut/flpass in Hz (souLowCut, souLowPass), times i*sign with sign -1 on the receiver side, +1 on the
+ * source side), and transformed back.  Afterwards klrtm_receiver_apply injects that surface wavefield with
+ * receiver 0's z kernel, as the reference does (bgprtm/Receiver.cpp:1216-1225).  dx, dy: grid spacing;
+ * dt: the traces' sampling.  klrtm_receiver_fk_data gives the device array [nt][ny][nx] (NULL before).
+ * Setting new traces ends the FK mode (call klrtm_receiver_fk_spread again). */
+int klrtm_receiver_fk_spread(klrtm_receiver *r, float dx, float dy, float dt, int ntfft, float vsurf, int trueamp,
+                             float z_depth, int deghost, int sign, float flcut, float flpass, float zerofreq_threshold);
+int klrtm_receiver_fk_data(klrtm_receiver *r, float **fk);
 /* copy the record out as [nr][nt] */
 int klrtm_receiver_get_record(klrtm_receiver *r, float *record_host);
 /* the spread tables as built (for inspection): lengths/offsets per axis */
@@ -236,6 +265,28 @@ int klrtm_lcodec_correlate(const klrtm_lcodec *c, const void *code, const float 
  * products in double. */
 int klrtm_ic_correlate(const float *src, const float *rec, int nx, int ny, int nz, float taper_t, const float *taper_z,
                        int ntaper_z, float *image, float *illum, double *image64, double *illum64);
+
+/* ---- ImagingCorrelator FFT imaging conditions (bgprtm/ImagingCorrelator.cpp:17-431), nDbl = 1 ----
+ * The LowcutFilter options of ImagingCondition::getImagingOption (bgprtm/ImagingCondition.cpp:151-160,
+ * enum IMAGE_OPTION of bgprtm/ImagingCorrelator.h:17-19), hipFFT on the device.  klrtm_correlator_create
+ * replaces ImagingCorrelator(nx,ny,nz,dx,dy,dz,nx2,ny2,nz2,kmin,nThreads) with nx2 = nx, ny2 = ny,
+ * nz2 = nz (nz even: the reference's in-place layout pads each trace to nz+2).  klrtm_correlator_run
+ * replaces run(fftv, image, imgFWI, imgRAW, srcWave, recWave, option) on device volumes; any image may be
+ * NULL, image64 (optional) accumulates the image's terms in FP64.  src and rec are work buffers: like the
+ * reference's srcWave/recWave they may be overwritten (INVERSION2 filters them in place).  costerm and
+ * sinterm are the post-loop steps of INVERSION2 (bgprtm/ImagingCorrelator.cpp:308-353). */
+enum { KLRTM_IC_IMAGE1 = 0, KLRTM_IC_INVERSION = 1, KLRTM_IC_INVERSION2 = 2, KLRTM_IC_ALLIN3 = 3 };
+typedef struct klrtm_correlator klrtm_correlator;
+int klrtm_correlator_create(klrtm_correlator **c, int nx, int ny, int nz, float dx, float dy, float dz, float kmin);
+int klrtm_correlator_destroy(klrtm_correlator *c);
+int klrtm_correlator_run(klrtm_correlator *c, float *image, float *img_fwi, float *img_raw, double *image64, float *src,
+                         float *rec, int option);
+int klrtm_correlator_costerm(klrtm_correlator *c, float *image);
+int klrtm_correlator_sinterm(klrtm_correlator *c, float *imgfull, const float *imgcos);
+/* the pieces, for inspection: H_axis(in) (axis 0 z, 1 x, 2 y) as derive + buffPad2(-1) leave it
+ * (bgprtm/ImagingCorrelator.cpp:195-283), and kfilter in place (bgprtm/ImagingCorrelator.cpp:285-306) */
+int klrtm_correlator_unit_derivative(klrtm_correlator *c, const float *in, int axis, float *out);
+int klrtm_correlator_kfilter(klrtm_correlator *c, float *wave);
 
 /* ---- whole-shot driver (Propagator::rtm / Propagator::mod) ------------------ */
 typedef struct {
@@ -335,6 +386,21 @@ int klrtm_propagator_kernel(klrtm_propagator *p, klrtm_kernel **k);
 int klrtm_propagator_images(klrtm_propagator *p, float **img, float **ill, double **img64, double **ill64);
 /* zero the image accumulators (the step API accumulates into them; klrtm_propagator_rtm clears them itself) */
 int klrtm_propagator_clear_images(klrtm_propagator *p);
+/* the imaging condition of ImagingCondition::getImagingOption (LowcutFilter, bgprtm/ImagingCondition.cpp:
+ * 151-160): KLRTM_IC_IMAGE1 (the default zero-lag product), KLRTM_IC_INVERSION, KLRTM_IC_INVERSION2 or
+ * KLRTM_IC_ALLIN3, with kmin = minFreq / vMin (bgprtm/ImagingCondition.cpp:393-396) for INVERSION2.  The FFT
+ * options also fill an FWI and a raw image (device pointers from klrtm_propagator_ic_images); the FP64
+ * image takes the per-step terms (none under INVERSION2, whose post-loop cos term is FP32).
+ * klrtm_propagator_rtm runs the post-loop costerm/sinterm of INVERSION2 itself; the step API calls
+ * klrtm_propagator_finish_images after its last imaging step.  Learned snapshots take IMAGE1 only. */
+int klrtm_propagator_set_imaging(klrtm_propagator *p, int option, float kmin);
+/* FK true amplitude / deghosting of the receiver traces before the backward run (Receiver::spread with
+ * true_amplitude RECEIVER and fk_domain SDOMAIN, bgprtm/Receiver.cpp:1012-1040): klrtm_receiver_fk_spread
+ * with sign -1, the shot's spacing and dt, ntfft = klrtm_fft_size(rint(1.5 nt)).  enable 0 switches it off. */
+int klrtm_propagator_set_fk(klrtm_propagator *p, int enable, float vsurf, int trueamp, float z_depth, int deghost,
+                            float flcut, float flpass, float zerofreq_threshold);
+int klrtm_propagator_ic_images(klrtm_propagator *p, float **fwi, float **raw);
+int klrtm_propagator_finish_images(klrtm_propagator *p);
 
 /* ---- Propagator::apply_symmetry (bgprtm/Propagator.cpp:1857-1918) --------------
  * Free-surface mirror in place on a device volume: above z index iz0 each column takes

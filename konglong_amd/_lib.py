@@ -15,6 +15,7 @@ KLRTM_OK = 0
 ENGINE_CFD, ENGINE_FD = 4, 7
 ISO, VTI, TTI = 1, 2, 3
 SNAP_FP32, SNAP_C16, SNAP_FP16, SNAP_BF16, SNAP_LEARNED = 0, 1, 16, 17, 32
+IC_IMAGE1, IC_INVERSION, IC_INVERSION2, IC_ALLIN3 = 0, 1, 2, 3
 
 _vp, _i, _f, _d, _sz, _l = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t, C.c_long
 _pi, _pf, _pd, _pl = C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_long)
@@ -104,6 +105,17 @@ _SIGS = {
     "klrtm_blockq_encode": (_i, [_vp, _vp, _vp, _i, _i, _i, _i]),
     "klrtm_blockq_decode": (_i, [_vp, _vp, _vp, _i, _i, _i, _i]),
     "klrtm_ic_correlate": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _vp, _vp, _vp]),
+    "klrtm_phasecorr_create": (_i, [C.POINTER(_vp), _i, _f, _f, _f, _f, _i, _f, _f, _i, _i, _f, _f]),
+    "klrtm_phasecorr_destroy": (_i, [_vp]),
+    "klrtm_phasecorr_info": (_i, [_vp, _pi, _pi]),
+    "klrtm_phasecorr_backward": (_i, [_vp, _vp, _i, _vp, _f]),
+    "klrtm_correlator_create": (_i, [C.POINTER(_vp), _i, _i, _i, _f, _f, _f, _f]),
+    "klrtm_correlator_destroy": (_i, [_vp]),
+    "klrtm_correlator_run": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "klrtm_correlator_costerm": (_i, [_vp, _vp]),
+    "klrtm_correlator_sinterm": (_i, [_vp, _vp, _vp]),
+    "klrtm_correlator_unit_derivative": (_i, [_vp, _vp, _i, _vp]),
+    "klrtm_correlator_kfilter": (_i, [_vp, _vp]),
     "klrtm_propagator_create": (_i, [C.POINTER(_vp), C.POINTER(ShotConfig)]),
     "klrtm_propagator_destroy": (_i, [_vp]),
     "klrtm_propagator_set_model": (_i, [_vp, _pf, _pf, _pf, _pf, _pf]),
@@ -127,6 +139,12 @@ _SIGS = {
     "klrtm_propagator_kernel": (_i, [_vp, C.POINTER(_vp)]),
     "klrtm_propagator_images": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "klrtm_propagator_clear_images": (_i, [_vp]),
+    "klrtm_propagator_set_imaging": (_i, [_vp, _i, _f]),
+    "klrtm_propagator_set_fk": (_i, [_vp, _i, _f, _i, _f, _i, _f, _f, _f]),
+    "klrtm_receiver_fk_spread": (_i, [_vp, _f, _f, _f, _i, _f, _i, _f, _i, _i, _f, _f, _f]),
+    "klrtm_receiver_fk_data": (_i, [_vp, C.POINTER(_vp)]),
+    "klrtm_propagator_ic_images": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    "klrtm_propagator_finish_images": (_i, [_vp]),
     "klrtm_stack_image": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _i, _i, _i, _i, _i]),
     "klrtm_apply_symmetry": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i]),
     "klrtm_illum_damping": (_i, [_vp, _i, _i, _i, _i, _i, _i, _f, _f, _pd]),

@@ -42,7 +42,7 @@ def build(verbose=False):
         raise RuntimeError("hipcc failed for: " + ", ".join(os.path.basename(b[0]) for b in bad))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + [r[0] for r in results] + \
-        ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
+        ["-L/opt/rocm/lib", "-lhipfft", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stdout + p.stderr)

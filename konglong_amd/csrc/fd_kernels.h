@@ -4,7 +4,9 @@
 
 namespace klrtm {
 
-// pb -= vel*L (+pr = L) | + leapfrog | pr = L only | pb -= vel*L only (the Laplacian form; pr untouched)
+// pb -= vel*L (+pr = L) | + leapfrog | pr = L only | pb -= vel*L only.  MODE_PB leaves pr untouched on the
+// Laplacian form (FD ISO without density) only; the first-derivative form (FD VTI/TTI, FD ISO with
+// density) and the CFD engine run it as MODE_KERNEL, which uses pr as scratch.
 enum { MODE_KERNEL = 0, MODE_STEP = 1, MODE_LAP = 2, MODE_PB = 3 };
 
 // Arguments of the y-streaming stencil kernels (passed by value).

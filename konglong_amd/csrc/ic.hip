@@ -19,7 +19,9 @@
 
 namespace klrtm {
 
-template<int FMT, bool RQ>
+// PREP: instead of correlating, write the tapered S and R to a.out_s / a.out_r (the operands of the
+// FFT imaging conditions) and accumulate only the illumination
+template<int FMT, bool RQ, bool PREP>
 __global__ void ic_fused_kernel(const IcArgs a, const void *__restrict__ snap, const float *__restrict__ scales,
                                 const float *__restrict__ w, const unsigned *__restrict__ rmax, float *__restrict__ img,
                                 float *__restrict__ ill, double *__restrict__ img64, double *__restrict__ ill64) {
@@ -48,10 +50,15 @@ __global__ void ic_fused_kernel(const IcArgs a, const void *__restrict__ snap, c
       R = vm > 1e-25f ? vm * c16_value(c16_shift(R * (1.0f / vm))) : 0.f;
     }
     R = edge_taper(a.T, R, ix, iy, iz, a.nx, a.ny, a.nz);
-    const float d = S * R;
-    if(img) img[i] = img[i] + d;
+    if(PREP) {
+      a.out_s[i] = S;
+      a.out_r[i] = R;
+    } else {
+      const float d = S * R;
+      if(img) img[i] = img[i] + d;
+      if(img64) img64[i] = img64[i] + (double)S * (double)R;
+    }
     if(ill) ill[i] = ill[i] + S * S;
-    if(img64) img64[i] = img64[i] + (double)S * (double)R;
     if(ill64) ill64[i] = ill64[i] + (double)S * (double)S;
   }
 }
@@ -62,8 +69,15 @@ int launch_ic(int fmt, bool rq, const IcArgs &a, const void *snap, const float *
   const size_t nxz = (size_t)a.nx * a.nz;
   const dim3 grid((unsigned)std::min<size_t>((nxz + 255) / 256, 1024), (unsigned)a.ny);
   LaunchTimer tm("ic_correlate");
-#define KL_IC(F, Q) \
-  hipLaunchKernelGGL((ic_fused_kernel<F, Q>), grid, dim3(256), 0, stream(), a, snap, scales, w, rmax, img, ill, img64, ill64)
+  const bool prep = a.out_s != nullptr;
+  KL_REQUIRE(!prep || a.out_r, KLRTM_EINVAL, "imaging operands need both outputs");
+#define KL_IC(F, Q)                                                                                                       \
+  if(prep)                                                                                                                \
+    hipLaunchKernelGGL((ic_fused_kernel<F, Q, true>), grid, dim3(256), 0, stream(), a, snap, scales, w, rmax, img, ill,   \
+                       img64, ill64);                                                                                     \
+  else                                                                                                                    \
+    hipLaunchKernelGGL((ic_fused_kernel<F, Q, false>), grid, dim3(256), 0, stream(), a, snap, scales, w, rmax, img, ill,  \
+                       img64, ill64)
   if(fmt == KLRTM_SNAP_FP32) {
     if(rq) KL_IC(KLRTM_SNAP_FP32, true);
     else KL_IC(KLRTM_SNAP_FP32, false);

@@ -48,6 +48,9 @@ struct Propagator {
   float *d_invq_raw = nullptr, *d_invq = nullptr, *d_q[2] = {nullptr, nullptr};
   int q_order = 0, iq0 = 0;
   bool q_ready = false;  // set_model has run the Q branch of Vel2 since the last set_q
+  // set_q has changed the Q set-up since the last set_model: d_vel holds m for the old set-up
+  // (a Q-corrected Vel2, or a plain one), so stepping needs set_model first
+  bool vel_stale = false;
   float q_coeffs[1 + 2 * Q_MAX_ORDER] = {};
   // snapshot cache outside HBM (CacheFile): two device staging slots, the snapshots in a file or in
   // host memory
@@ -72,10 +75,25 @@ struct Propagator {
   int src_mirror() const { return c.free_surface_iz >= 0 ? c.free_surface_iz : (ghost_src ? ghost_iz : -1); }
   int rcv_mirror() const { return c.free_surface_iz >= 0 ? c.free_surface_iz : (ghost_rcv ? ghost_iz : -1); }
   int ext_mirror() const { return ghost_rcv ? ghost_iz : -1; }
+  // FFT imaging conditions (ImagingCorrelator::run options other than IMAGE1): the correlator, the FWI
+  // and raw images, and the tapered S and R operands of each imaging step
+  // FK true amplitude / deghosting of the receiver side (Receiver::spread_sdomain)
+  bool fk = false;
+  float fk_vsurf = 0, fk_z = 0, fk_flcut = 0, fk_flpass = 0, fk_thr = 0.1f;
+  int fk_trueamp = 0, fk_deghost = 0;
+  int ic_option = KLRTM_IC_IMAGE1;
+  Correlator *corr = nullptr;
+  float *d_fwi = nullptr, *d_raw = nullptr, *d_ics = nullptr, *d_icr = nullptr;
   std::vector<float> wtaperT;
   std::vector<float> wavelet;    // the source wavelet (host, nt samples)
   std::vector<float> src_delay;  // source array: delay per source in seconds (empty: the single source)
   int set_source_traces();
+  // Receiver::spread_sdomain on the receiver traces (sign -1), when switched on
+  int fk_receivers() {
+    if(!fk) return KLRTM_OK;
+    return receiver_fk_spread(*rcv, c.dx, c.dy, c.dt, fft_size((int)nearbyintf(c.nt * 1.5f)), fk_vsurf, fk_trueamp, fk_z,
+                              fk_deghost, -1, fk_flcut, fk_flpass, fk_thr);
+  }
   double t_fwd = 0, t_bwd = 0, t_ic = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ~Propagator();
@@ -90,6 +108,8 @@ struct Propagator {
 
 Propagator::~Propagator() {
   if(cache) snapcache_destroy(cache);
+  if(corr) correlator_destroy(corr);
+  for(void *p : {(void *)d_fwi, (void *)d_raw, (void *)d_ics, (void *)d_icr}) (void)hipFree(p);
   delete eng;
   if(src) receiver_destroy(src);
   if(rcv) receiver_destroy(rcv);
@@ -168,6 +188,7 @@ int Propagator::clear_images() {
   int rc;
   if((rc = launch_fill(d_img, 0.f, n)) || (rc = launch_fill(d_ill, 0.f, n))) return rc;
   if((rc = launch_fill((float *)d_img64, 0.f, 2 * n)) || (rc = launch_fill((float *)d_ill64, 0.f, 2 * n))) return rc;
+  if(d_fwi && ((rc = launch_fill(d_fwi, 0.f, n)) || (rc = launch_fill(d_raw, 0.f, n)))) return rc;
   return KLRTM_OK;
 }
 
@@ -184,6 +205,7 @@ int Propagator::reset() {
 // With Q: the kernel, then the Q leapfrog on the memory-variable set iq0, which swaps with w0/w1
 // (Wavefield::swapPointers, bgprtm/Wavefield.cpp:130-133).
 int Propagator::step(int) {
+  KL_REQUIRE(!vel_stale, KLRTM_ESTATE, "Q was changed: call klrtm_propagator_set_model after klrtm_propagator_set_q");
   if(q_order > 0) {
     KL_REQUIRE(q_ready, KLRTM_ESTATE, "Q is set: call klrtm_propagator_set_model after klrtm_propagator_set_q");
     int rc = eng->run(c.model, MODE_PB, w1, w0, wr);  // pb -= vel*L (no Laplacian copy in wr)
@@ -231,9 +253,43 @@ int Propagator::image(int itsav) {
   }
   const char *snap = slot(itsav);
   double *i64 = c.ic_fp32_only ? nullptr : d_img64, *l64 = c.ic_fp32_only ? nullptr : d_ill64;
-  if(c.snap_format == KLRTM_SNAP_LEARNED) return lcodec_image(*lc, a, snap, w0, d_img, d_ill, i64, l64);
   const float *sc = nscales ? scales(itsav) : nullptr;
+  if(ic_option != KLRTM_IC_IMAGE1) {
+    // ImagingCondition::stackImageInterp with an FFT option: the tapered S and R (and the illumination),
+    // then ImagingCorrelator::run (bgprtm/ImagingCondition.cpp:104-141); the FP64 image takes the same
+    // terms, except under INVERSION2, whose post-loop cos term is FP32 only
+    a.out_s = d_ics, a.out_r = d_icr;
+    int rc = launch_ic(c.snap_format, rq, a, snap, sc, w0, d_smax, nullptr, d_ill, nullptr, l64);
+    if(rc) return rc;
+    return correlator_run(*corr, d_img, d_fwi, d_raw, ic_option == KLRTM_IC_INVERSION2 ? nullptr : i64, d_ics, d_icr,
+                          ic_option);
+  }
+  if(c.snap_format == KLRTM_SNAP_LEARNED) return lcodec_image(*lc, a, snap, w0, d_img, d_ill, i64, l64);
   return launch_ic(c.snap_format, rq, a, snap, sc, w0, d_smax, d_img, d_ill, i64, l64);
+}
+
+// the post-loop steps of ImagingCondition::genImage for INVERSION2 (bgprtm/ImagingCondition.cpp:427-430)
+static int finish_images(Propagator &P) {
+  if(P.ic_option != KLRTM_IC_INVERSION2) return KLRTM_OK;
+  int rc = correlator_costerm(*P.corr, P.d_img);
+  return rc ? rc : correlator_sinterm(*P.corr, P.d_fwi, P.d_img);
+}
+
+int propagator_set_imaging(Propagator &P, int option, float kmin) {
+  KL_REQUIRE(option >= KLRTM_IC_IMAGE1 && option <= KLRTM_IC_ALLIN3, KLRTM_EINVAL, "unknown imaging option %d", option);
+  if(option == KLRTM_IC_IMAGE1) {
+    P.ic_option = option;
+    return KLRTM_OK;
+  }
+  KL_REQUIRE(P.c.snap_format != KLRTM_SNAP_LEARNED, KLRTM_EINVAL,
+             "the FFT imaging conditions take FP32, COMPRESS_16 or block-scaled snapshots");
+  if(P.corr) correlator_destroy(P.corr), P.corr = nullptr;
+  int rc = correlator_create(&P.corr, P.c.nx, P.c.ny, P.c.nz, P.c.dx, P.c.dy, P.c.dz, kmin);
+  if(rc) return rc;
+  for(float **d : {&P.d_fwi, &P.d_raw, &P.d_ics, &P.d_icr})
+    if(!*d && (rc = dmalloc(d, P.n))) return rc;
+  P.ic_option = option;
+  return P.clear_images();
 }
 
 static float ricker(float t, float fp) {
@@ -337,6 +393,7 @@ int propagator_set_model(Propagator &P, const float *vel, const float *eps, cons
     if((rc = launch_vel2_q(P.d_vel, P.d_invq, P.n, P.c.dt, P.q_coeffs[0]))) return rc;
     P.q_ready = true;
   } else if((rc = launch_vel2(P.d_vel, P.n, P.c.dt))) return rc;  // Propagator::Vel2
+  P.vel_stale = false;
   return klrtm_kernel_set_model(reinterpret_cast<klrtm_kernel *>(P.eng), P.d_vel, P.use_rho ? P.d_rho : nullptr, P.d_del,
                                 P.d_eps, P.d_pjx, P.d_pjy);
 }
@@ -344,9 +401,11 @@ int propagator_set_model(Propagator &P, const float *vel, const float *eps, cons
 int propagator_set_q(Propagator &P, const float *invq, int order, const float *coeffs) {
   P.q_ready = false;
   if(order == 0) {
+    if(P.q_order > 0) P.vel_stale = true;  // d_vel still carries the Q correction of Vel2
     P.q_order = 0;
     return KLRTM_OK;
   }
+  P.vel_stale = true;
   int rc = q_coeffs_check(order, coeffs);
   if(rc) return rc;
   KL_REQUIRE(invq, KLRTM_EINVAL, "null 1/Q volume");
@@ -437,6 +496,7 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
   // backward: receiver wavefield from the record, imaging at the stored steps
   if(record && (rc = receiver_set_traces(*P.rcv, record))) return rc;
   if(!record && (rc = receiver_set_traces_device(*P.rcv, receiver_record_device(*P.rcv)))) return rc;
+  if((rc = P.fk_receivers())) return rc;
   if((rc = P.clear_images())) return rc;
   if((rc = P.reset())) return rc;
   KL_HIP(hipEventRecord(P.ev[2], stream()));
@@ -446,6 +506,7 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
     if(it % k == 0 && it / k < P.ntsav && (rc = P.image(it / k))) return rc;
     std::swap(P.w0, P.w1);
   }
+  if((rc = finish_images(P))) return rc;
   KL_HIP(hipEventRecord(P.ev[3], stream()));
   KL_HIP(hipEventSynchronize(P.ev[3]));
   float f = 0, b = 0;
@@ -620,7 +681,8 @@ int klrtm_propagator_set_record(klrtm_propagator *p, const float *record) {
   KL_REQUIRE(p && record, KLRTM_EINVAL, "null argument");
   Propagator &P = *reinterpret_cast<Propagator *>(p);
   KL_REQUIRE(P.rcv, KLRTM_ESTATE, "receivers not set");
-  return receiver_set_traces(*P.rcv, record);
+  int rc = receiver_set_traces(*P.rcv, record);
+  return rc ? rc : P.fk_receivers();
 }
 int klrtm_propagator_reset(klrtm_propagator *p) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
@@ -643,6 +705,34 @@ int klrtm_propagator_images(klrtm_propagator *p, float **img, float **ill, doubl
   return KLRTM_OK;
 }
 
+int klrtm_propagator_set_imaging(klrtm_propagator *p, int option, float kmin) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  return propagator_set_imaging(*reinterpret_cast<Propagator *>(p), option, kmin);
+}
+int klrtm_propagator_set_fk(klrtm_propagator *p, int enable, float vsurf, int trueamp, float z_depth, int deghost,
+                            float flcut, float flpass, float zerofreq_threshold) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  if(!enable) {
+    P.fk = false;
+    return KLRTM_OK;
+  }
+  KL_REQUIRE(vsurf > 0 && (trueamp || deghost), KLRTM_EINVAL, "FK needs vsurf > 0 and true amplitude or deghosting");
+  P.fk = true, P.fk_vsurf = vsurf, P.fk_trueamp = trueamp, P.fk_z = z_depth, P.fk_deghost = deghost;
+  P.fk_flcut = flcut, P.fk_flpass = flpass, P.fk_thr = zerofreq_threshold;
+  return KLRTM_OK;
+}
+int klrtm_propagator_ic_images(klrtm_propagator *p, float **fwi, float **raw) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  if(fwi) *fwi = P.d_fwi;
+  if(raw) *raw = P.d_raw;
+  return KLRTM_OK;
+}
+int klrtm_propagator_finish_images(klrtm_propagator *p) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  return finish_images(*reinterpret_cast<Propagator *>(p));
+}
 int klrtm_propagator_clear_images(klrtm_propagator *p) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
   return reinterpret_cast<Propagator *>(p)->clear_images();

@@ -15,4 +15,6 @@ int receiver_extract(Receiver &R, const float *w, int jt, int iz_mirror = -1);
 int receiver_get_record(Receiver &R, float *out_host);
 int receiver_clear_record(Receiver &R);
 float *receiver_record_device(Receiver &R);
+int receiver_fk_spread(Receiver &R, float dx, float dy, float dt, int ntfft, float vsurf, int trueamp, float z,
+                       int deghost, int sign, float flcut, float flpass, float thr);
 }  // namespace klrtm

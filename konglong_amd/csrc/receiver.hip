@@ -26,10 +26,12 @@
 // does.
 #pragma clang fp contract(off)
 #include "klrtm_internal.h"
+#include "fd_kernels.h"
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <hipfft/hipfft.h>
 
 namespace klrtm {
 
@@ -117,6 +119,10 @@ struct Receiver {
   float *d_plane = nullptr;   // scratch plane (largest group bbox)
   float *d_wy = nullptr;      // scratch y factors of the injection terms (largest group, [L][nr])
   size_t plane_cells = 0;
+  // FK pre-processing (spread_sdomain): the surface wavefield [nt][ny][nx] injected with receiver 0's
+  // z kernel instead of the traces
+  float *d_fk = nullptr, *d_fk_kz = nullptr;
+  int fk_offz = 0, fk_lz = 0;
   ~Receiver();
 };
 
@@ -127,7 +133,7 @@ Receiver::~Receiver() {
   }
   (void)hipFree(d_kx), (void)hipFree(d_ky), (void)hipFree(d_lx), (void)hipFree(d_ly);
   (void)hipFree(d_offx), (void)hipFree(d_offy), (void)hipFree(d_traces), (void)hipFree(d_record), (void)hipFree(d_plane);
-  (void)hipFree(d_wy);
+  (void)hipFree(d_wy), (void)hipFree(d_fk), (void)hipFree(d_fk_kz);
 }
 
 // ---------------------------------------------------------------------------
@@ -269,10 +275,209 @@ static GroupArgs gargs(const Receiver &R, const RcvGroup &g) {
   return a;
 }
 
+// ---------------------------------------------------------------------------
+// FK pre-processing: Receiver::spread_sdomain (bgprtm/Receiver.cpp:851-1010) and the injection of its
+// result (bgprtm/Receiver.cpp:1216-1225).  The traces are spread in x/y onto the surface grid for every
+// time sample (apply_src_coefxy), transformed R2C over (y, x, t) with t zero-padded to ntfft, filtered per
+// (ky, kx, w) and transformed back C2R without normalisation; each step then injects sample it of every
+// surface column with receiver 0's z kernel: w0[col, offz+k] += (val*scaler)*kz[k]*vel (apply_src_coefz_lz).
+
+// data[((by0+py)*nx + bx0+px)*ntfft + it] += plane[c]
+__global__ void fk_gather_kernel(const float *__restrict__ plane, float *__restrict__ data, int bx0, int by0, int bw, int bh,
+                                 int nx, int ntfft, int it) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if(c >= bw * bh) return;
+  const size_t o = ((size_t)(by0 + c / bw) * nx + bx0 + c % bw) * ntfft + it;
+  data[o] = data[o] + plane[c];
+}
+
+// the (ky, kx, w) filter of spread_sdomain (bgprtm/Receiver.cpp:957-982): obliquity cos (true amplitude),
+// deghost 0.5 / sin(kz z) with the zero-frequency threshold and the low-cut taper, times i*sign; zero at
+// w = 0 and for evanescent waves; the scalar sqrt(1/(nkx nky ntfft)) on every propagating sample
+__global__ void fk_filter_kernel(float2 *__restrict__ c, int nkx, int nky, int nw, float dkx, float dky, float dwv,
+                                 int trueamp, int deghost, float z, float thr, int iw_lc, int iw_lp, float scalar,
+                                 int sign) {
+  const size_t tot = (size_t)nkx * nky * nw;
+  const float half_pi = (float)M_PI / 2, pi = (float)M_PI;
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const int iw = (int)(e % nw);
+    const size_t t = e / nw;
+    const int ix = (int)(t % nkx), iy = (int)(t / nkx);
+    // Receiver::get_wavenumber (bgprtm/Receiver.cpp:605-617)
+    const float kx = nkx == 1 ? 0.f : (ix < nkx / 2 + 1 ? ix * dkx : (ix - nkx) * dkx);
+    const float ky = nky == 1 ? 0.f : (iy < nky / 2 + 1 ? iy * dky : (iy - nky) * dky);
+    float2 v = c[e];
+    if(iw == 0) {
+      c[e] = make_float2(0.f, 0.f);
+      continue;
+    }
+    const float k = iw * dwv;
+    const float tmp = 1.0f - (kx * kx + ky * ky) / (k * k);
+    if(tmp > 0.0f) {
+      const float cosv = sqrtf(tmp);
+      float factor = trueamp ? cosv : 1;
+      factor *= scalar;
+      if(deghost) {
+        const float kzdz = k * cosv * z;
+        const float sinv = sinf(kzdz);
+        factor *= kzdz >= half_pi ? 1 : sinv <= thr ? 1 / thr : 1 / sinv;
+        factor *= 0.5f;
+        if(iw <= iw_lc) factor = 0;
+        else if(iw <= iw_lp) factor *= 0.5f * (1 - cosf(pi * (iw - iw_lc) / (iw_lp - iw_lc + 1)));
+        const float f = factor * sign;
+        v = make_float2(-(v.y * f), v.x * f);
+      } else {
+        v = make_float2(v.x * factor, v.y * factor);
+      }
+    } else {
+      v = make_float2(0.f, 0.f);
+    }
+    c[e] = v;
+  }
+}
+
+// fk[it][cell] = data[cell][it]
+__global__ void fk_transpose_kernel(const float *__restrict__ data, float *__restrict__ fk, size_t ncell, int nt, int ntfft) {
+  const size_t tot = ncell * nt;
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t cell = e % ncell;
+    const int it = (int)(e / ncell);
+    fk[e] = data[cell * ntfft + it];
+  }
+}
+
+// w0[col, offz+k] += (val*scaler)*kz[k]*vel, and the ghost mirror (apply_src_coefz_lz)
+__global__ void fk_apply_kernel(const float *__restrict__ row, float scaler, const float *__restrict__ kz, int offz, int lz,
+                                int nx, int ny, int nz, int izm, float *__restrict__ w0, const float *__restrict__ vel) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= (long)nx * ny * lz) return;
+  const int k = (int)(i % lz);
+  const long col = i / lz;
+  const float T = row[col] * scaler;
+  const size_t o = (size_t)col * nz + offz + k;
+  w0[o] += T * kz[k] * vel[o];
+}
+__global__ void fk_mirror_kernel(const float *__restrict__ row, float scaler, const float *__restrict__ kz, int offz, int lz,
+                                 int nx, int ny, int nz, int izm, float *__restrict__ w0, const float *__restrict__ vel) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= (long)nx * ny * lz) return;
+  const int k = (int)(i % lz);
+  const long col = i / lz;
+  const int zm = 2 * izm - offz - k;
+  if(k >= min(lz, 2 * izm - offz + 1) || zm >= nz) return;
+  const float T = row[col] * scaler;
+  const size_t o = (size_t)col * nz + zm;
+  w0[o] -= T * kz[k] * vel[o];
+}
+
+int receiver_fk_spread(Receiver &R, float dx, float dy, float dt, int ntfft, float vsurf, int trueamp, float z,
+                       int deghost, int sign, float flcut, float flpass, float thr) {
+  KL_REQUIRE(R.d_traces, KLRTM_ESTATE, "receiver traces not set");
+  KL_REQUIRE(R.nx > 1 && ntfft >= R.nt && vsurf > 0 && dt > 0, KLRTM_EINVAL, "bad FK parameters");
+  KL_REQUIRE(trueamp || deghost, KLRTM_EINVAL, "the FK operation needs true amplitude or deghosting");
+  const int nx = R.nx, ny = R.ny, nt = R.nt, nw = ntfft / 2 + 1;
+  const size_t ncell = (size_t)nx * ny;
+  float *data = nullptr;
+  float2 *spec = nullptr;
+  hipfftHandle fwd = 0, bwd = 0;
+  bool hf = false, hb = false;
+  int rc = KLRTM_OK;
+  auto fail = [&](int code, const char *msg) {
+    set_error("%s", msg);
+    rc = code;
+  };
+  if(hipMalloc(&data, sizeof(float) * ncell * ntfft) != hipSuccess || hipMalloc(&spec, sizeof(float2) * ncell * nw) != hipSuccess) {
+    fail(KLRTM_ENOMEM, "FK work space does not fit");
+  } else if(!R.d_fk && hipMalloc(&R.d_fk, sizeof(float) * ncell * nt) != hipSuccess) {
+    R.d_fk = nullptr;
+    fail(KLRTM_ENOMEM, "FK surface wavefield does not fit");
+  }
+  if(!rc) rc = launch_fill(data, 0.f, ncell * ntfft);
+  // apply_src_coefxy for every time sample (bgprtm/Receiver.cpp:509-529): the plane of each group
+  for(int it = 0; it < nt && !rc; it++) {
+    const float *val = R.d_traces + (size_t)it * R.nr;
+    for(auto &g : R.groups) {
+      GroupArgs a = gargs(R, g);
+      const int cells = g.bw * g.bh, nwy = g.nr * R.L;
+      hipLaunchKernelGGL(inject_wy_kernel, dim3(div_up(nwy, 256)), dim3(256), 0, stream(), nwy, g.nr, g.bin_rcv, g.cky, val,
+                         1.0f, R.d_wy);
+      hipLaunchKernelGGL(inject_plane_kernel<IG>, dim3(div_up((long)cells * IG, 256)), dim3(256), 0, stream(), a,
+                         g.bin_start, g.corg, g.ckx, R.d_wy, R.d_plane);
+      hipLaunchKernelGGL(fk_gather_kernel, dim3(div_up(cells, 256)), dim3(256), 0, stream(), R.d_plane, data, g.bx0, g.by0,
+                         g.bw, g.bh, nx, ntfft, it);
+    }
+    if(it % 256 == 0 && (rc = check_launch("fk spread"))) break;
+  }
+  if(!rc) {
+    int n3[3] = {ny, nx, ntfft}, ce[3] = {ny, nx, nw};
+    const int rank = ny > 1 ? 3 : 2;
+    int *pn = rank == 3 ? n3 : n3 + 1, *pce = rank == 3 ? ce : ce + 1;
+    if(hipfftPlanMany(&fwd, rank, pn, pn, 1, (int)(ncell * ntfft), pce, 1, (int)(ncell * nw), HIPFFT_R2C, 1) != HIPFFT_SUCCESS)
+      fail(KLRTM_EDEVICE, "hipfftPlanMany (FK forward) failed");
+    else hf = true;
+    if(!rc && hipfftPlanMany(&bwd, rank, pn, pce, 1, (int)(ncell * nw), pn, 1, (int)(ncell * ntfft), HIPFFT_C2R, 1) !=
+                  HIPFFT_SUCCESS)
+      fail(KLRTM_EDEVICE, "hipfftPlanMany (FK backward) failed");
+    else if(!rc) hb = true;
+  }
+  if(!rc && (hipfftSetStream(fwd, stream()) != HIPFFT_SUCCESS ||
+             hipfftExecR2C(fwd, data, (hipfftComplex *)spec) != HIPFFT_SUCCESS))
+    fail(KLRTM_EDEVICE, "FK forward transform failed");
+  if(!rc) {
+    // bgprtm/Receiver.cpp:856-905: spacings, wavenumber steps and the scalar, in the reference's types
+    const int nkx = nx, nky = ny;
+    const int iw_lc = (int)nearbyintf(flcut * dt * ntfft), iw_lp = (int)nearbyintf(flpass * dt * ntfft);
+    const float dkx = nkx == 1 ? 0.f : (float)(2.0f * M_PI / (dx * nkx));
+    const float dky = nky == 1 ? 0.f : (float)(2.0f * M_PI / (dy * nky));
+    const float dwv = (float)(2.0f * M_PI / (dt * ntfft * vsurf));
+    const float scalar = (float)sqrt(1.0 / nkx / nky / ntfft);
+    const size_t tot = ncell * nw;
+    hipLaunchKernelGGL(fk_filter_kernel, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 256 * 64)), dim3(256), 0, stream(),
+                       spec, nkx, nky, nw, dkx, dky, dwv, trueamp, deghost, z, thr, iw_lc, iw_lp, scalar, sign);
+    rc = check_launch("fk_filter_kernel");
+  }
+  if(!rc && (hipfftSetStream(bwd, stream()) != HIPFFT_SUCCESS ||
+             hipfftExecC2R(bwd, (hipfftComplex *)spec, data) != HIPFFT_SUCCESS))
+    fail(KLRTM_EDEVICE, "FK backward transform failed");
+  if(!rc) {
+    const size_t tot = ncell * nt;
+    hipLaunchKernelGGL(fk_transpose_kernel, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 256 * 64)), dim3(256), 0,
+                       stream(), data, R.d_fk, ncell, nt, ntfft);
+    rc = check_launch("fk_transpose_kernel");
+  }
+  // receiver 0's z kernel (kernels_z[0], bgprtm/Receiver.cpp:1220-1222)
+  if(!rc) {
+    R.fk_offz = R.offz[0], R.fk_lz = R.lz[0];
+    if(!R.d_fk_kz && hipMalloc(&R.d_fk_kz, sizeof(float) * std::max(1, R.Lz)) != hipSuccess) fail(KLRTM_ENOMEM, "FK kz");
+    else if(hipMemcpy(R.d_fk_kz, R.kz.data(), sizeof(float) * R.lz[0], hipMemcpyHostToDevice) != hipSuccess)
+      fail(KLRTM_EDEVICE, "FK kz upload failed");
+  }
+  if(hipStreamSynchronize(stream()) != hipSuccess && !rc) fail(KLRTM_EDEVICE, "FK pre-processing failed on the device");
+  if(hf) (void)hipfftDestroy(fwd);
+  if(hb) (void)hipfftDestroy(bwd);
+  (void)hipFree(data), (void)hipFree(spec);
+  if(rc) {
+    (void)hipFree(R.d_fk);
+    R.d_fk = nullptr;
+  }
+  return rc;
+}
+
 int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scaler0, int iz_mirror) {
   if(it < 0 || it > R.nt - 1) return KLRTM_OK;  // bgprtm/Receiver.cpp:1205
-  KL_REQUIRE(R.d_traces, KLRTM_ESTATE, "receiver traces not set");
+  KL_REQUIRE(R.d_traces || R.d_fk, KLRTM_ESTATE, "receiver traces not set");
   const float scaler = scaler0 * R.dt_dxyz;
+  if(R.d_fk) {  // the FK surface wavefield (bgprtm/Receiver.cpp:1216-1225)
+    LaunchTimer tm("inject");
+    const float *row = R.d_fk + (size_t)it * R.nx * R.ny;
+    const long tot = (long)R.nx * R.ny * R.fk_lz;
+    hipLaunchKernelGGL(fk_apply_kernel, dim3(div_up(tot, 256)), dim3(256), 0, stream(), row, scaler, R.d_fk_kz, R.fk_offz,
+                       R.fk_lz, R.nx, R.ny, R.nz, iz_mirror, w0, vel);
+    if(iz_mirror >= 0)
+      hipLaunchKernelGGL(fk_mirror_kernel, dim3(div_up(tot, 256)), dim3(256), 0, stream(), row, scaler, R.d_fk_kz, R.fk_offz,
+                         R.fk_lz, R.nx, R.ny, R.nz, iz_mirror, w0, vel);
+    return check_launch("fk_apply_kernel");
+  }
   const float *val = R.d_traces + (size_t)it * R.nr;
   LaunchTimer tm("inject");
   for(auto &g : R.groups) {
@@ -447,7 +652,14 @@ int receiver_create(Receiver **out, int nx, int ny, int nz, int nr, const float 
   return KLRTM_OK;
 }
 
+// new traces end the FK mode: its surface wavefield belonged to the old ones
+static void fk_drop(Receiver &R) {
+  (void)hipFree(R.d_fk);
+  R.d_fk = nullptr;
+}
+
 int receiver_set_traces(Receiver &R, const float *traces) {
+  fk_drop(R);
   std::vector<float> t((size_t)R.nt * R.nr);
   for(int r = 0; r < R.nr; r++)
     for(int it = 0; it < R.nt; it++) t[(size_t)it * R.nr + r] = traces[(size_t)r * R.nt + it];
@@ -457,6 +669,7 @@ int receiver_set_traces(Receiver &R, const float *traces) {
 }
 
 int receiver_set_traces_device(Receiver &R, const float *tm_device) {
+  fk_drop(R);
   if(!R.d_traces) KL_HIP(hipMalloc(&R.d_traces, (size_t)R.nt * R.nr * sizeof(float)));
   KL_HIP(hipMemcpyAsync(R.d_traces, tm_device, (size_t)R.nt * R.nr * sizeof(float), hipMemcpyDeviceToDevice, stream()));
   return KLRTM_OK;
@@ -528,6 +741,18 @@ int klrtm_receiver_extract_ghost(klrtm_receiver *r, const float *w, int jt, int 
   Receiver &R = *reinterpret_cast<Receiver *>(r);
   KL_REQUIRE(iz_mirror >= 0 && iz_mirror < R.nz, KLRTM_EINVAL, "mirror plane %d outside [0, %d)", iz_mirror, R.nz);
   return receiver_extract(R, w, jt, iz_mirror);
+}
+int klrtm_receiver_fk_spread(klrtm_receiver *r, float dx, float dy, float dt, int ntfft, float vsurf, int trueamp,
+                             float z_depth, int deghost, int sign, float flcut, float flpass, float zerofreq_threshold) {
+  KL_REQUIRE(r, KLRTM_EINVAL, "null receiver");
+  KL_REQUIRE(sign == 1 || sign == -1, KLRTM_EINVAL, "sign must be +1 (source) or -1 (receiver)");
+  return receiver_fk_spread(*reinterpret_cast<Receiver *>(r), dx, dy, dt, ntfft, vsurf, trueamp, z_depth, deghost, sign,
+                            flcut, flpass, zerofreq_threshold);
+}
+int klrtm_receiver_fk_data(klrtm_receiver *r, float **fk) {
+  KL_REQUIRE(r && fk, KLRTM_EINVAL, "null argument");
+  *fk = reinterpret_cast<Receiver *>(r)->d_fk;
+  return KLRTM_OK;
 }
 int klrtm_receiver_get_record(klrtm_receiver *r, float *rec) {
   KL_REQUIRE(r && rec, KLRTM_EINVAL, "null argument");

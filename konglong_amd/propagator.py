@@ -104,6 +104,12 @@ class Propagator:
         check(lib().klrtm_propagator_model(self.h, fptr(rec)))
         return rec
 
+    def set_imaging(self, option, kmin=1.0 / 1492.0):
+        """The imaging condition (LowcutFilter): IC_IMAGE1, IC_INVERSION, IC_INVERSION2 or IC_ALLIN3; kmin is
+        minFreq / vMin (bgprtm/ImagingCondition.cpp:393-396)."""
+        check(lib().klrtm_propagator_set_imaging(self.h, int(option), float(kmin)))
+        self.ic_option = int(option)
+
     def rtm(self, record=None, want=("image", "illum", "image64", "illum64")):
         n = self.shape
         img = np.zeros(n, np.float32) if "image" in want else None
@@ -112,7 +118,15 @@ class Propagator:
         ill64 = np.zeros(n, np.float64) if "illum64" in want else None
         r = None if record is None else np.ascontiguousarray(record, np.float32)
         check(lib().klrtm_propagator_rtm(self.h, fptr(r), fptr(img), fptr(ill), dptr(img64), dptr(ill64)))
-        return dict(image=img, illum=ill, image64=img64, illum64=ill64)
+        out = dict(image=img, illum=ill, image64=img64, illum64=ill64)
+        if getattr(self, "ic_option", 0):   # the FWI and raw images of the FFT imaging conditions
+            fwi, raw = C.c_void_p(), C.c_void_p()
+            check(lib().klrtm_propagator_ic_images(self.h, C.byref(fwi), C.byref(raw)))
+            for name, ptr in (("fwi", fwi), ("raw", raw)):
+                a = np.empty(n, np.float32)
+                check(lib().klrtm_memcpy_d2h(a.ctypes.data, ptr.value, a.nbytes))
+                out[name] = a
+        return out
 
     def timings(self):
         f, b, i = C.c_double(), C.c_double(), C.c_double()

@@ -43,6 +43,24 @@ class Receiver:
         else:
             check(lib().klrtm_receiver_extract(self.h, w.ptr, jt))
 
+    def fk_spread(self, dx, dy, dt, vsurf, trueamp=1, z_depth=0.0, deghost=0, sign=-1, ntfft=None, flcut=0.0,
+                  flpass=0.0, zerofreq_threshold=0.1):
+        """Receiver::spread_sdomain: FK true amplitude / deghost of the traces set last; apply() then injects
+        the surface wavefield it leaves (sign -1 receiver side, +1 source side)."""
+        if ntfft is None:
+            ntfft = int(lib().klrtm_fft_size(int(round(self.nt * 1.5))))
+        check(lib().klrtm_receiver_fk_spread(self.h, dx, dy, dt, ntfft, vsurf, int(trueamp), z_depth, int(deghost),
+                                             int(sign), flcut, flpass, zerofreq_threshold))
+        return ntfft
+
+    def fk_data(self, shape):
+        """The FK surface wavefield [nt][ny][nx] (shape = (ny, nx)), downloaded."""
+        p = C.c_void_p()
+        check(lib().klrtm_receiver_fk_data(self.h, C.byref(p)))
+        out = np.empty((self.nt,) + tuple(shape), np.float32)
+        check(lib().klrtm_memcpy_d2h(out.ctypes.data, p.value, out.nbytes))
+        return out
+
     def record(self):
         out = np.zeros((self.nr, self.nt), np.float32)
         check(lib().klrtm_receiver_get_record(self.h, fptr(out)))

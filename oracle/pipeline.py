@@ -8,6 +8,7 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).  Restates
   * Propagator::migration time loop          bgprtm/Propagator.cpp:1210-1255
   * ImagingCondition tapers + crossone       bgprtm/ImagingCondition.cpp:180-226,
                                              bgprtm/ImagingCorrelator.cpp:180-194
+  * ImagingCorrelator::run (FFT options)     bgprtm/ImagingCorrelator.cpp:355-439 (oracle/correlator.py)
 with the oracle engine for KernelCPU/FdEngine/PML.
 """
 import ctypes as C
@@ -15,6 +16,7 @@ import math
 
 import numpy as np
 
+from . import correlator
 from . import (lib, Engine, spread_coeff, taper_kernel, xy_taper, compress16, uncompress16, apply_symmetry, _pd,
                vel2_q, update2nd_q, fft_size, timeshift)
 
@@ -103,6 +105,7 @@ class Shot:
         self.q = None if q is None else (np.ascontiguousarray(q[0], np.float32), np.asarray(q[1], np.float32))
         self.rho = None if rho is None else np.ascontiguousarray(rho, np.float32)
         self.nx, self.ny, self.nz, self.dt, self.nt = nx, ny, nz, dt, nt
+        self.d = d
         self.model_type, self.snap_every, self.snap_format = model, snap_every, snap_format
         self.eng = Engine(nx, ny, nz, d, d, d, engine, order, np.float32, threads=threads)
         self.nbnd, self.vmax = nbnd, vmax
@@ -212,7 +215,10 @@ class Shot:
         # the reference's decoder applies the same edge taper to every codec
         return _edge_taper(snap.reshape(self.shape))
 
-    def rtm(self, record):
+    def rtm(self, record, ic_option=0, kmin=1.0 / 1492.0):
+        """Image and illumination of the record.  ic_option selects ImagingCondition::getImagingOption's
+        imaging condition (bgprtm/ImagingCondition.cpp:151-160): 0 the zero-lag product (FP32 sums), 1-3 the
+        FFT conditions of oracle.correlator (FP64 sums), which also return the FWI and raw images."""
         k = self.snap_every
         ntsav = (self.nt + k - 1) // k
         snaps = [None] * ntsav
@@ -236,11 +242,20 @@ class Shot:
                 R = uncompress16(compress16(w0), *self.shape) if self.snap_format == 1 else _edge_taper(w0)
                 if s < ntaperT:
                     S = (S * wtT[s]).astype(np.float32)
-                img[...] += S * R
                 ill[...] += S * S
+                if ic_option == 0:
+                    img[...] += S * R
+                else:
+                    acc[0], acc[1], acc[2] = correlator.run(ic_option, S, R, (self.d, self.d, self.d), kmin, *acc)
+        acc = [np.zeros(self.shape), np.zeros(self.shape), np.zeros(self.shape)]
         self.backward(record, image)
         self.snaps = snaps  # the stored snapshots (COMPRESS_16 codes or FP32 volumes), for cache checks
-        return img, ill
+        if ic_option == 0:
+            return img, ill
+        if ic_option == 2:   # the post-loop cos and sin terms of INVERSION2 (bgprtm/ImagingCondition.cpp:427-430)
+            acc[0] = correlator.costerm(acc[0], (self.d, self.d, self.d))
+            acc[1] = correlator.sinterm(acc[1], acc[0])
+        return acc[0], ill, acc[1], acc[2]
 
 
 class Shot64:
@@ -315,7 +330,10 @@ class Shot64:
         self.forward(visit)
         return rec
 
-    def rtm(self, record):
+    def rtm(self, record, ic_option=0, kmin=1.0 / 1492.0):
+        """Image and illumination of the record.  ic_option selects ImagingCondition::getImagingOption's
+        imaging condition (bgprtm/ImagingCondition.cpp:151-160): 0 the zero-lag product (FP32 sums), 1-3 the
+        FFT conditions of oracle.correlator (FP64 sums), which also return the FWI and raw images."""
         k = self.snap_every
         ntsav = (self.nt + k - 1) // k
         snaps = [None] * ntsav

@@ -1090,8 +1090,12 @@ void orc_inject_ghost(float *w0, const float *vel, int nx, int ny, int nz, int n
   size_t nxz = (size_t)nx * nz;
   std::vector<float> b((size_t)nx * ny * nz, 0.f);
   std::vector<char> touched((size_t)nx * ny * nz, 0);
-  for(int ir = 0; ir < nr; ir++) {
-    for(int iy = 0; iy < ly[ir]; iy++) {
+  // threads own y planes; each cell still takes its terms in receiver order, as the serial loop
+#pragma omp parallel for schedule(dynamic, 1)
+  for(int py = 0; py < ny; py++) {
+    for(int ir = 0; ir < nr; ir++) {
+      const int iy = py - offy[ir];
+      if(iy < 0 || iy >= ly[ir]) continue;
       float cy = val[ir] * ky[(size_t)ir * Ly + iy];
       for(int ix = 0; ix < lx[ir]; ix++) {
         float cx = cy * kx[(size_t)ir * Lx + ix];
@@ -1121,6 +1125,7 @@ void orc_inject(float *w0, const float *vel, int nx, int ny, int nz, int nr, con
 void orc_extract_ghost(const float *w, int nx, int ny, int nz, int nr, float *out, const float *kx, const float *ky,
                        const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz,
                        const int *offx, const int *offy, const int *offz, int iz_mirror) {
+#pragma omp parallel for schedule(static)
   for(int ir = 0; ir < nr; ir++) {
     float val = 0;
     for(int iy = 0; iy < ly[ir]; iy++) {

@@ -26,6 +26,12 @@ void orc_engine_set_boundary_f(void *h, const int *nb, float dt, float vmax, flo
                                int top, int dimPML);
 void orc_engine_kernel_f(void *h, int model, const float *p0, float *pb, float *pr);
 void orc_engine_laplacian_f(void *h, const float *in, float *out);
+void orc_inject_ghost(float *w0, const float *vel, int nx, int ny, int nz, int nr, const float *val, const float *kx,
+                      const float *ky, const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly,
+                      const int *lz, const int *offx, const int *offy, const int *offz, int iz_mirror);
+void orc_extract_ghost(const float *w, int nx, int ny, int nz, int nr, float *out, const float *kx, const float *ky,
+                       const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz,
+                       const int *offx, const int *offy, const int *offz, int iz_mirror);
 void orc_compress_pair(const float *in, uint16_t *out, int nx, int ny, int nz, int mode);
 void orc_pml_round(float g, float *p, float *q, int n);
 void orc_apply_symmetry(float *wf0, int nx, int ny, int nz, int iz0, int sym, int tprlen, int taper);
@@ -131,6 +137,57 @@ int main() {
     std::vector<float> got = img.host(), b = dbb.host(), want(n);
     for(int i = 0; i < n; i++) want[i] = 0.f + p0[i] * b[i];
     report("ImagingCorrelator::crossone", std::memcmp(got.data(), want.data(), n * 4) == 0, rel_l2(got, want));
+  }
+  // Receiver::apply with the ghost mirror (one trace: bit-exact) and extraction with the mirror
+  {
+    const std::vector<float> xyz = {14.3f, 11.6f, 12.2f};
+    const int ntr = 3, izm = 9;
+    Receiver r(nx, ny, nz, xyz, 4, 0.3f, 0.3f, ntr, d, d, d);
+    const std::vector<float> tr = {0.5f, -1.25f, 2.0f};
+    r.setTraces(tr);
+    konglong::DeviceBuffer<float> w(n);
+    w.upload(p0.data());
+    r.apply(w.get(), 1, true, izm, dv.get(), 1.0f);
+    int L[2] = {0, 0}, lx, ly, lz, ox, oy, oz;
+    konglong::check(klrtm_receiver_get_tables(r.handle(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                              nullptr, nullptr, nullptr, L));
+    std::vector<float> kx(L[0]), ky(L[0]), kz(L[1]);
+    konglong::check(klrtm_receiver_get_tables(r.handle(), &lx, &ly, &lz, &ox, &oy, &oz, kx.data(), ky.data(),
+                                              kz.data(), L));
+    std::vector<float> want(p0);
+    const float val = tr[1] * (1.0f / (d * d * d));
+    orc_inject_ghost(want.data(), vel.data(), nx, ny, nz, 1, &val, kx.data(), ky.data(), kz.data(), L[0], L[0], L[1],
+                     &lx, &ly, &lz, &ox, &oy, &oz, izm);
+    std::vector<float> got = w.host();
+    report("Receiver::apply ghost bit-exact", std::memcmp(got.data(), want.data(), n * 4) == 0, rel_l2(got, want));
+    r.extract(w.get(), 0, izm);
+    float ex = 0.f;
+    orc_extract_ghost(got.data(), nx, ny, nz, 1, &ex, kx.data(), ky.data(), kz.data(), L[0], L[0], L[1], &lx, &ly, &lz,
+                      &ox, &oy, &oz, izm);
+    const float gx = r.record()[0];
+    const double e = std::fabs((double)gx - ex) / std::max(1e-30, std::fabs((double)ex));
+    report("Receiver::extract_value ghost", e < 1e-6, e);
+  }
+  // ImagingCorrelator(nx,ny,nz,dx,dy,dz,nx2,ny2,nz2,kmin,nThreads)::run(INVERSION) on a plane wave:
+  // S R - sum_a H_a S H_a R = cos(2 theta) (closed form)
+  {
+    const int my = 8, mx = 10, mz = 12, m = my * mx * mz;
+    std::vector<float> cs(m), want(m);
+    for(int iy = 0; iy < my; iy++)
+      for(int ix = 0; ix < mx; ix++)
+        for(int iz = 0; iz < mz; iz++) {
+          const double th = 2 * M_PI * (1.0 * iy / my + 2.0 * ix / mx + 3.0 * iz / mz);
+          cs[(iy * mx + ix) * mz + iz] = (float)std::cos(th), want[(iy * mx + ix) * mz + iz] = (float)std::cos(2 * th);
+        }
+    ImagingCorrelator ic(mx, my, mz, 10.f, 12.5f, 7.5f, mx, my, mz, 1.0f / 1492.0f, 1);
+    konglong::DeviceBuffer<float> s(m), r(m), img(m), fwi(m);
+    s.upload(cs.data()), r.upload(cs.data());
+    float *sb[1] = {s.get()}, *rb[1] = {r.get()};
+    ic.run(nullptr, img.get(), fwi.get(), nullptr, sb, rb, INVERSION);
+    std::vector<float> got = img.host();
+    double e = 0;
+    for(int i = 0; i < m; i++) e = std::max(e, (double)std::fabs(got[i] - want[i]));
+    report("ImagingCorrelator::run INVERSION", e < 1e-5, e);
   }
   // Propagator::apply_symmetry, computeDamping, Lagrange
   {

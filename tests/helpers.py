@@ -22,6 +22,17 @@ def smooth_field(shape, lo, hi, seed, passes=3):
     return (lo + (hi - lo) * f).astype(np.float32)
 
 
+def bandlimited_field(shape, seed, cut=0.3):
+    """A random (ny, nx, nz) field with no energy beyond `cut` of Nyquist on any axis and none at k = 0."""
+    rng = np.random.default_rng(seed)
+    spec = np.fft.rfftn(rng.standard_normal(shape))
+    f = [np.abs(np.fft.fftfreq(n)) for n in shape[:2]] + [np.abs(np.fft.rfftfreq(shape[2]))]
+    mask = (f[0][:, None, None] < cut / 2) & (f[1][None, :, None] < cut / 2) & (f[2][None, None, :] < cut / 2)
+    spec = np.where(mask, spec, 0.0)
+    spec[0, 0, 0] = 0.0
+    return np.fft.irfftn(spec, s=shape, axes=(0, 1, 2))
+
+
 def gaussian_blob(ny, nx, nz, cy, cx, cz, sigma):
     y, x, z = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
     r2 = (y - cy) ** 2 + (x - cx) ** 2 + (z - cz) ** 2

@@ -64,7 +64,8 @@ def test_update2nd_q_rejects_bad_order(klrtm):
     assert lib().klrtm_update2nd_q(d.ptr, d.ptr, d.ptr, d.ptr, d.ptr, 64, 9, fptr(c)) != 0
 
 
-@pytest.mark.parametrize("case", [(ENGINE_FD, 8, ISO), (ENGINE_CFD, 10, VTI)], ids=["FD8-ISO", "CFD10-VTI"])
+@pytest.mark.parametrize("case", [(ENGINE_FD, 8, ISO), (ENGINE_CFD, 10, VTI), (ENGINE_FD, 8, VTI)],
+                         ids=["FD8-ISO", "CFD10-VTI", "FD8-VTI"])
 def test_q_shot_matches_oracle(klrtm, case):
     engine, order, model = case
     print(f"[q-shot] {case}", flush=True)
@@ -113,6 +114,12 @@ def test_q_needs_set_model_after_set_q(klrtm):
     p.set_model(vel)
     p.set_receivers(np.array([[8.0, 8.0, 6.0]], np.float32))
     p.set_q(np.full((ny, nx, nz), 0.02, np.float32), _coeffs(2))
+    with pytest.raises(KlrtmError, match="set_model"):
+        p.model()
+    p.set_model(vel)
+    p.model()
+    # switching Q off also needs set_model: the velocity still carries the Q correction of Vel2
+    p.set_q(None, None)
     with pytest.raises(KlrtmError, match="set_model"):
         p.model()
     p.set_model(vel)
