@@ -8,9 +8,104 @@
 #ifndef KONGLONG_PROPAGATOR_H_
 #define KONGLONG_PROPAGATOR_H_
 
+#include <algorithm>
+#include <cmath>
+#include <memory>
 #include <vector>
 
+#include "ExtractReceiverData.h"
 #include "konglong_gpu.h"
+
+// ---- the reference types the static entry points take (only the members they use) -------------------
+#ifndef KONGLONG_NO_VECTOR3
+// bgprtm/Vector3.h
+class vector3 {
+public:
+  float x = 0.f, y = 0.f, z = 0.f;
+  vector3() = default;
+  vector3(float x_, float y_, float z_) : x(x_), y(y_), z(z_) {}
+};
+#endif
+
+#ifndef KONGLONG_NO_TRACES
+// bgprtm/Traces.h: receiver coordinates (world units) and one trace of nt samples at dt per receiver
+class Traces {
+public:
+  explicit Traces(int traceLength = 0, float * /*extern_mem*/ = nullptr, int /*trc_limit*/ = 0, float dt_ = 0)
+      : nt(traceLength), dt(dt_) {}
+  virtual ~Traces() = default;
+  void addReceiver(vector3 location) {
+    coord.push_back(location);
+    store_.emplace_back((size_t)std::max(nt, 0), 0.f);
+    data.clear();
+    for(auto &t : store_) data.push_back(t.data());
+  }
+  float *getTrace(int ir) { return data[ir]; }
+  int getNReceivers() const { return (int)coord.size(); }
+  vector3 coord_src;
+  std::vector<vector3> coord;
+  std::vector<float *> data;
+  vector3 xyzMin, xyzMax;
+  int nt;
+  float dt = 0;
+
+private:
+  std::vector<std::vector<float>> store_;
+};
+#endif
+
+#ifndef KONGLONG_NO_GRID
+// bgprtm/Grid.h, rectangular grids only: sizes, spacings, origin (z0 is the depth of index 0) and, for the
+// grid Propagator::rtm returns, the shot image on it ((iy*nx + ix)*nz + iz)
+class Grid {
+public:
+  int nx = 0, ny = 0, nz = 0;
+  float dx = 0, dy = 0, dz = 0, x0 = 0, y0 = 0, z0 = 0;
+  std::vector<float> image;
+};
+#endif
+
+#ifndef KONGLONG_NO_OUTPUTGRID
+// bgprtm/OutputGrid.h
+class OutputGrid {
+public:
+  float x0{}, y0{}, z0{};
+  double dx{}, dy{}, dz{};
+  int nx{}, ny{}, nz{};
+  int ix0{}, iy0{}, incx{}, incy{};
+};
+#endif
+
+namespace konglong {
+// The model on a regular grid (host arrays (iy*nx + ix)*nz + iz, index 0 at world (x0, y0) and depth 0),
+// in place of the reference's ModelLoader / model files.
+struct ModelVolume {
+  int nx = 0, ny = 0, nz = 0;
+  float dx = 0, dy = 0, dz = 0, x0 = 0, y0 = 0;
+  int model_type = ISO;  // ISO, VTI, TTI
+  const float *vel = nullptr, *eps = nullptr, *del = nullptr, *pjx = nullptr, *pjy = nullptr;
+};
+// The global_pars keys Propagator::rtm / mod read (bgprtm/Propagator.cpp:2045-2280, DomainRange.cpp:25-80),
+// with the reference's defaults; dt is the propagation step (calcDeltaTime's value, e.g. from
+// konglong_amd.models.cfl_dt), which the reference derives from the model itself.
+struct JobDeck {
+  ModelVolume model;
+  float zMax = 0, maxFreq = 0, tmax = 0, tmaxratio = 0.7f, dt = 0;
+  float padX = 1000.f, padY = 1000.f;     // aperture padding (DomainRange::getComputingRange)
+  int engine = FD::FD, fd_order = 8;      // Engine
+  int nbnd = 32;                          // absorbing layer on every face, outside the aperture
+  int sourceSpreadSize = 10, receiverSpreadSize = 10;
+  float khpass = 0.3f, khcut = 0.3f;
+  int wf_compress = KLRTM_SNAP_FP32;
+  int LowcutFilter = KLRTM_IC_IMAGE1;     // "none"/"laplacian": IMAGE1 (the reference's default is INVERSION)
+  float minFreq = 1.0f, vMin = 1492.f;    // kmin = minFreq / vMin
+  float tmutePerc = 0.1f;
+};
+inline JobDeck &global_pars() {
+  static JobDeck d;
+  return d;
+}
+}  // namespace konglong
 
 class Propagator {
 public:
@@ -63,6 +158,149 @@ public:
   void timings(double &fwd_ms, double &bwd_ms, double &ic_ms) {
     konglong::check(klrtm_propagator_timings(p_, &fwd_ms, &bwd_ms, &ic_ms));
   }
+  void setImaging(int option, float kmin) { konglong::check(klrtm_propagator_set_imaging(p_, option, kmin)); }
+
+  // ---- Propagator::mod / Propagator::rtm (bgprtm/Propagator.h:100-104) over konglong::global_pars() ----
+  // The aperture is the sources' and receivers' x/y range padded by padX/padY (DomainRange::
+  // getComputingRange) and clipped to the model, with nbnd absorbing cells added on every side (model
+  // values replicated outward) and nbnd above depth 0; z runs to zMax.  One shot: the first source of
+  // sourceX/Y/Z with a Ricker of peak maxFreq/3 (bgprtm/Wavelet.cpp:101), nt = rint(tmax/dt) + 1, receivers
+  // at the traces' coordinates.  Traces sampled at another dt are interpolated linearly to dt.
+  static std::unique_ptr<ExtractReceiverData> mod(std::vector<float> &sourceX, std::vector<float> &sourceY,
+                                                  std::vector<float> &sourceZ, int /*sourceID*/, vector3 &recvMin,
+                                                  vector3 &recvMax, Traces *traces = nullptr) {
+    if(!traces || traces->getNReceivers() == 0)
+      throw konglong::Error(KLRTM_EINVAL, "konglong: Propagator::mod needs the receivers (Traces)");
+    Shot sh(sourceX, sourceY, sourceZ, recvMin, recvMax, *traces, false);
+    Propagator p(sh.cfg);
+    sh.setup(p);
+    std::vector<float> rec = p.mod();
+    const konglong::JobDeck &g = konglong::global_pars();
+    auto erd = std::unique_ptr<ExtractReceiverData>(new ExtractReceiverData(sh.nr, sh.cfg.nt, g.dt, 0.f, g.tmax));
+    erd->recDataRaw = rec;
+    return erd;
+  }
+  // the image of the shot on its propagation grid (returned), stacked into destImages[0] on destGrid when
+  // given: every image column that falls on a destination node (inline / crossline numbering from ix0,
+  // iy0 with incx, incy; depth sampling shared from depth 0) is added (ImagingCondition::saveImage)
+  static std::shared_ptr<Grid> rtm(std::vector<float> &sourceX, std::vector<float> &sourceY, std::vector<float> &sourceZ,
+                                   int /*sourceID*/, vector3 &recvMin, vector3 &recvMax,
+                                   std::vector<Traces *> vec_traces = std::vector<Traces *>{nullptr},
+                                   std::vector<float *> destImages = std::vector<float *>{},
+                                   OutputGrid *destGrid = nullptr) {
+    Traces *tr = vec_traces.empty() ? nullptr : vec_traces.back();
+    if(!tr || tr->getNReceivers() == 0)
+      throw konglong::Error(KLRTM_EINVAL, "konglong: Propagator::rtm needs the receiver traces");
+    Shot sh(sourceX, sourceY, sourceZ, recvMin, recvMax, *tr, true);
+    Propagator p(sh.cfg);
+    sh.setup(p);
+    const konglong::JobDeck &g = konglong::global_pars();
+    if(g.LowcutFilter != KLRTM_IC_IMAGE1) p.setImaging(g.LowcutFilter, g.minFreq / g.vMin);
+    auto grid = std::make_shared<Grid>(sh.grid);
+    grid->image.assign((size_t)sh.cfg.nx * sh.cfg.ny * sh.cfg.nz, 0.f);
+    p.rtm(sh.record, grid->image.data());
+    if(destGrid && !destImages.empty() && destImages[0]) stack(*grid, *destGrid, destImages[0]);
+    return grid;
+  }
+
+private:
+  // the shot set-up shared by mod and rtm
+  struct Shot {
+    klrtm_shot_config cfg{};
+    Grid grid;
+    int nr = 0;
+    std::vector<float> vel, eps, del, pjx, pjy, rxyz, record;
+    Shot(const std::vector<float> &sx, const std::vector<float> &sy, const std::vector<float> &sz, const vector3 &rmin,
+         const vector3 &rmax, const Traces &tr, bool need_traces) {
+      const konglong::JobDeck &g = konglong::global_pars();
+      const konglong::ModelVolume &M = g.model;
+      if(!M.vel || M.nx < 1 || M.ny < 1 || M.nz < 1 || g.dt <= 0 || g.tmax <= 0 || g.maxFreq <= 0 || g.zMax <= 0 ||
+         sx.empty() || sy.empty() || sz.empty())
+        throw konglong::Error(KLRTM_EINVAL, "konglong: global_pars needs model, dt, tmax, maxFreq, zMax and a source");
+      // DomainRange::getComputingRange: sources and receivers, padded, in model index units
+      float xlo = std::min(*std::min_element(sx.begin(), sx.end()), rmin.x) - g.padX;
+      float xhi = std::max(*std::max_element(sx.begin(), sx.end()), rmax.x) + g.padX;
+      float ylo = std::min(*std::min_element(sy.begin(), sy.end()), rmin.y) - g.padY;
+      float yhi = std::max(*std::max_element(sy.begin(), sy.end()), rmax.y) + g.padY;
+      const int ix0 = std::max(0, (int)std::floor((xlo - M.x0) / M.dx)), ix1 = std::min(M.nx - 1, (int)std::ceil((xhi - M.x0) / M.dx));
+      const int iy0 = M.ny == 1 ? 0 : std::max(0, (int)std::floor((ylo - M.y0) / M.dy));
+      const int iy1 = M.ny == 1 ? 0 : std::min(M.ny - 1, (int)std::ceil((yhi - M.y0) / M.dy));
+      const int nzm = std::min(M.nz, (int)std::lround(g.zMax / M.dz) + 1);
+      const int b = g.nbnd, by = M.ny == 1 ? 0 : b;
+      const int nx = ix1 - ix0 + 1 + 2 * b, ny = iy1 - iy0 + 1 + 2 * by, nz = nzm + 2 * b;
+      grid.nx = nx, grid.ny = ny, grid.nz = nz, grid.dx = M.dx, grid.dy = M.dy, grid.dz = M.dz;
+      grid.x0 = M.x0 + (ix0 - b) * M.dx, grid.y0 = M.y0 + (iy0 - by) * M.dy, grid.z0 = -b * M.dz;
+      // model values of the sub-volume, replicated outward at the model's edges
+      const size_t n = (size_t)nx * ny * nz;
+      auto cut = [&](const float *src, std::vector<float> &dst) {
+        if(!src) return;
+        dst.resize(n);
+        for(int iy = 0; iy < ny; iy++)
+          for(int ix = 0; ix < nx; ix++)
+            for(int iz = 0; iz < nz; iz++) {
+              const int mx = std::min(std::max(ix0 - b + ix, 0), M.nx - 1), my = std::min(std::max(iy0 - by + iy, 0), M.ny - 1);
+              const int mz = std::min(std::max(iz - b, 0), M.nz - 1);
+              dst[((size_t)iy * nx + ix) * nz + iz] = src[((size_t)my * M.nx + mx) * M.nz + mz];
+            }
+      };
+      cut(M.vel, vel), cut(M.eps, eps), cut(M.del, del), cut(M.pjx, pjx), cut(M.pjy, pjy);
+      float vmax = 0.f;
+      for(float v : vel) vmax = std::max(vmax, v);
+      auto gx = [&](float x) { return (x - grid.x0) / grid.dx; };
+      auto gy = [&](float y) { return M.ny == 1 ? 0.f : (y - grid.y0) / grid.dy; };
+      auto gz = [&](float z) { return (z - grid.z0) / grid.dz; };
+      nr = tr.getNReceivers();
+      rxyz.resize((size_t)nr * 3);
+      for(int r = 0; r < nr; r++)
+        rxyz[3 * r] = gx(tr.coord[r].x), rxyz[3 * r + 1] = gy(tr.coord[r].y), rxyz[3 * r + 2] = gz(tr.coord[r].z);
+      const int nt = (int)std::lround(g.tmax / g.dt) + 1;
+      // Propagator::getDtSave: imaging every int(0.5/maxFreq / dt) steps
+      const int ratio = std::max(1, (int)((0.5f / g.maxFreq) / g.dt));
+      klrtm_shot_config &c = cfg;
+      c.nx = nx, c.ny = ny, c.nz = nz, c.dx = M.dx, c.dy = M.dy, c.dz = M.dz, c.dt = g.dt, c.nt = nt;
+      c.engine = g.engine, c.order = g.fd_order, c.model = M.model_type;
+      for(int f = 0; f < 6; f++) c.nbnd[f] = (f == 2 || f == 3) && ny == 1 ? 0 : b;
+      c.vmax = vmax, c.pml_eps = 1e-3f, c.pml3d_scaler = 0.3f, c.pml3d = 1, c.top_pml = 1;
+      c.src_xyz[0] = gx(sx[0]), c.src_xyz[1] = gy(sy[0]), c.src_xyz[2] = gz(sz[0]);
+      c.f_peak = g.maxFreq / 3.0f;
+      c.src_spread = g.sourceSpreadSize, c.rcv_spread = g.receiverSpreadSize, c.khpass = g.khpass, c.khcut = g.khcut;
+      c.nrcv = nr, c.snap_every = ratio, c.snap_format = g.wf_compress, c.rec_through_codec = 0;
+      c.tmute_frac = g.tmutePerc, c.ntaper_z = 0, c.fuse_step = 1, c.snap_slots = 0, c.ic_fp32_only = 1;
+      c.free_surface_iz = -1, c.surface_taper = 1;
+      if(need_traces) {  // the traces at the propagation step
+        record.assign((size_t)nr * nt, 0.f);
+        const float tdt = tr.dt > 0 ? tr.dt : g.dt;
+        for(int r = 0; r < nr; r++)
+          for(int it = 0; it < nt; it++) {
+            const float t = it * g.dt / tdt;
+            const int i0 = (int)std::floor(t);
+            const float w = t - i0;
+            auto at = [&](int i) { return i >= 0 && i < tr.nt ? tr.data[r][i] : 0.f; };
+            record[(size_t)r * nt + it] = (1 - w) * at(i0) + w * at(i0 + 1);
+          }
+      }
+    }
+    void setup(Propagator &p) {
+      p.setModel(vel.data(), eps.empty() ? nullptr : eps.data(), del.empty() ? nullptr : del.data(),
+                 pjx.empty() ? nullptr : pjx.data(), pjy.empty() ? nullptr : pjy.data());
+      p.setReceivers(rxyz);
+    }
+  };
+  static void stack(const Grid &img, const OutputGrid &og, float *dest) {
+    const int b = (int)std::lround(-img.z0 / img.dz);  // image depth index of depth 0
+    for(int iy = 0; iy < img.ny; iy++)
+      for(int ix = 0; ix < img.nx; ix++) {
+        const double fx = (img.x0 + ix * img.dx - og.x0) / og.dx, fy = og.ny > 1 ? (img.y0 + iy * img.dy - og.y0) / og.dy : 0.0;
+        const long jx = std::lround(fx), jy = std::lround(fy);
+        if(std::fabs(fx - jx) > 1e-3 || std::fabs(fy - jy) > 1e-3 || jx < 0 || jy < 0 || jx >= og.nx || jy >= og.ny)
+          continue;
+        float *col = dest + ((size_t)jy * og.nx + jx) * og.nz;
+        const float *src = img.image.data() + ((size_t)iy * img.nx + ix) * img.nz + b;
+        for(int iz = 0; iz < og.nz && b + iz < img.nz; iz++) col[iz] += src[iz];
+      }
+  }
+
+public:
 
 private:
   klrtm_shot_config cfg_;

@@ -103,6 +103,15 @@ int klrtm_phasecorr_destroy(klrtm_phasecorr *p);
 int klrtm_phasecorr_info(klrtm_phasecorr *p, int *nt_out, int *nw1);
 int klrtm_phasecorr_backward(klrtm_phasecorr *p, const float *in, int ntr, float *out, float scaler);
 
+/* ---- shot record files (ExtractReceiverData::saveRecFile, bgprtm/ExtractReceiverData.cpp:263-381) ----
+ * The reference writes through the unvendored jseisIO / libSWIO; this build writes one little-endian file:
+ * "KLREC001", int32 nr, int32 nt, float32 dt, float32 t0, float32 xyz[nr][3], float32 data[nr][nt]
+ * (host pointers; xyz may be NULL when writing: zeros).  klrtm_record_read fills xyz [nr][3] and data
+ * [nr][nt] (either may be NULL). */
+int klrtm_record_write(const char *path, int nr, int nt, float dt, float t0, const float *xyz, const float *data);
+int klrtm_record_read_header(const char *path, int *nr, int *nt, float *dt, float *t0);
+int klrtm_record_read(const char *path, float *xyz, float *data);
+
 /* ---- KernelCPU + FdEngine_fd / FdEngine_cfd + PML (bgprtm/KernelCPU.h:19-35) */
 typedef struct klrtm_kernel klrtm_kernel;
 /* KernelCPU(nx,ny,nz,dx,dy,dz,engine,nThreads): engine KLRTM_ENGINE_FD (order 8..16,
@@ -144,8 +153,9 @@ int klrtm_kernel_laplacian(klrtm_kernel *k, const float *in, float *out);
 int klrtm_kernel_deriv1(klrtm_kernel *k, int axis, const float *in, float *out, int iacc, int isign);
 /* tile shape of the stencil kernels: tz (z lanes), tx (x rows), ychunk (y planes per
  * block), tz*tx <= 512; 0 keeps the current value, ychunk < 0 returns to the default
- * chunking.  get_tile reports the shape the Laplacian step runs with.  Used by the
- * autotuner. */
+ * chunking.  get_tile reports the shape the Laplacian step runs with.  On the compact (CFD)
+ * engine tz is the number of lines per block of the x and y line passes (64, 128 or 256;
+ * default 256) and tx, ychunk are ignored.  Used by the autotuner. */
 int klrtm_kernel_set_tile(klrtm_kernel *k, int tz, int tx, int ychunk);
 int klrtm_kernel_get_tile(klrtm_kernel *k, int *tz, int *tx, int *ychunk);
 

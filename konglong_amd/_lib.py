@@ -105,6 +105,9 @@ _SIGS = {
     "klrtm_blockq_encode": (_i, [_vp, _vp, _vp, _i, _i, _i, _i]),
     "klrtm_blockq_decode": (_i, [_vp, _vp, _vp, _i, _i, _i, _i]),
     "klrtm_ic_correlate": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _vp, _vp, _vp]),
+    "klrtm_record_write": (_i, [C.c_char_p, _i, _i, _f, _f, _vp, _vp]),
+    "klrtm_record_read_header": (_i, [C.c_char_p, _pi, _pi, _pf, _pf]),
+    "klrtm_record_read": (_i, [C.c_char_p, _vp, _vp]),
     "klrtm_phasecorr_create": (_i, [C.POINTER(_vp), _i, _f, _f, _f, _f, _i, _f, _f, _i, _i, _f, _f]),
     "klrtm_phasecorr_destroy": (_i, [_vp]),
     "klrtm_phasecorr_info": (_i, [_vp, _pi, _pi]),

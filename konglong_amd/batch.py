@@ -27,6 +27,8 @@ class Autotuner:
     model).  The propagator's wavefields are reset afterwards."""
 
     TILES = ((64, 8), (64, 4), (32, 8), (128, 4))
+    CFD_BLOCKS = (64, 128, 256)  # lanes per block of the compact engine's x/y line passes
+    CFD_TIMERS = (b"cfd_grad_y", b"cfd_grad_z", b"cfd_grad_x", b"cfd_div_x", b"cfd_div_z", b"cfd_div_y")
     TILES_FIRST_DERIV = TILES + ((16, 16),)  # the gradient/divergence kernels also take 16 x 16
     YCHUNKS = (0, 16, 32, 64)   # 0: the engine's default chunking
 
@@ -45,40 +47,52 @@ class Autotuner:
         key = (cfg.nx, cfg.ny, cfg.nz, cfg.engine, cfg.order, cfg.model)
         k = self._kernel(p)
         if key not in self.cache:
-            if cfg.engine != ENGINE_FD:
-                self.cache[key] = (None, {})
-            else:
-                self.cache[key] = self._search(p, k)
+            self.cache[key] = self._search(p, k)
         best = self.cache[key][0]
         if best is not None:
             check(lib().klrtm_kernel_set_tile(k, best[0], best[1], best[2] if best[2] > 0 else -1))
         return best
 
+    def _time(self, p, names):
+        """Mean summed kernel time (ms) of one forward step over `self.steps` steps: the named kernels that ran
+        (None if none did)."""
+        L = lib()
+        p.forward_steps(0, 1, False)   # warm
+        L.klrtm_synchronize()
+        L.klrtm_timer_reset()
+        L.klrtm_timer_enable(1)
+        p.forward_steps(1, self.steps, False)
+        L.klrtm_synchronize()
+        L.klrtm_timer_enable(0)
+        ms, seen = 0.0, False
+        for name in names:
+            tot, cnt = C.c_double(), C.c_long()
+            L.klrtm_timer_read(name, C.byref(tot), C.byref(cnt))
+            if cnt.value:
+                ms, seen = ms + tot.value / cnt.value, True
+        return ms if seen else None
+
     def _search(self, p, k):
         L = lib()
-        names = (b"iso_step",) if p.cfg.model == ISO else (b"fd_grad", b"fd_div_step")
-        R = p.cfg.order // 2
         table = {}
         p.reset()
-        for tz, tx in (self.TILES if p.cfg.model == ISO else self.TILES_FIRST_DERIV):
-            if tz < R or tx < R or tz * tx > 512:
-                continue
-            for yc in self.YCHUNKS:
-                check(L.klrtm_kernel_set_tile(k, tz, tx, yc if yc > 0 else -1))
-                p.forward_steps(0, 1, False)   # warm
-                L.klrtm_synchronize()
-                L.klrtm_timer_reset()
-                L.klrtm_timer_enable(1)
-                p.forward_steps(1, self.steps, False)
-                L.klrtm_synchronize()
-                L.klrtm_timer_enable(0)
-                ms = 0.0
-                for name in names:
-                    tot, cnt = C.c_double(), C.c_long()
-                    L.klrtm_timer_read(name, C.byref(tot), C.byref(cnt))
-                    ms = ms + tot.value / cnt.value if cnt.value and ms is not None else None
+        if p.cfg.engine != ENGINE_FD:   # compact engine: lanes per block of the line passes
+            for bs in self.CFD_BLOCKS:
+                check(L.klrtm_kernel_set_tile(k, bs, 0, 0))
+                ms = self._time(p, self.CFD_TIMERS)
                 if ms is not None:
-                    table[(tz, tx, yc)] = ms
+                    table[(bs, 0, 0)] = ms
+        else:
+            names = (b"iso_step",) if p.cfg.model == ISO else (b"fd_grad", b"fd_div_step")
+            R = p.cfg.order // 2
+            for tz, tx in (self.TILES if p.cfg.model == ISO else self.TILES_FIRST_DERIV):
+                if tz < R or tx < R or tz * tx > 512:
+                    continue
+                for yc in self.YCHUNKS:
+                    check(L.klrtm_kernel_set_tile(k, tz, tx, yc if yc > 0 else -1))
+                    ms = self._time(p, names)
+                    if ms is not None:
+                        table[(tz, tx, yc)] = ms
         L.klrtm_timer_reset()
         p.reset()
         best = min(table, key=table.get) if table else None

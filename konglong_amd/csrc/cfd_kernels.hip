@@ -904,8 +904,10 @@ static int launch_line(Engine &e, int sg, int round, const float *in, float *tmp
   a.round = round;
   a.has_pml = e.has_pml;
   a.pml = e.desc;
+  // lanes per block: 256 by default, or the tile tz set for the compact engine (64, 128 or 256)
+  const int bs = e.tz == 64 || e.tz == 128 ? e.tz : 256;
   LaunchTimer tm(name);
-  hipLaunchKernelGGL((cfd_line_kernel<AXIS, PASS, MODEL>), dim3((unsigned)div_up(a.nlines, 256)), dim3(256), 0, stream(),
+  hipLaunchKernelGGL((cfd_line_kernel<AXIS, PASS, MODEL>), dim3((unsigned)div_up(a.nlines, bs)), dim3(bs), 0, stream(),
                      in, tmp, pr, pz, pb, e.vel, w1, e.eps, e.del, e.pjx, e.pjy, a);
   return check_launch(name);
 }

@@ -358,6 +358,12 @@ int klrtm_kernel_deriv1(klrtm_kernel *kk, int axis, const float *in, float *out,
 int klrtm_kernel_set_tile(klrtm_kernel *kk, int tz, int tx, int ychunk) {
   Engine *k = reinterpret_cast<Engine *>(kk);
   KL_REQUIRE(k, KLRTM_EINVAL, "null engine");
+  if(k->engine == KLRTM_ENGINE_CFD) {  // the compact engine: tz = lanes per block of the x/y line passes
+    KL_REQUIRE(tz <= 0 || tz == 64 || tz == 128 || tz == 256, KLRTM_EINVAL,
+               "the compact engine takes 64, 128 or 256 lanes per line block");
+    if(tz > 0) k->tz = tz;
+    return KLRTM_OK;
+  }
   int R = k->ne;
   if(tz > 0) KL_REQUIRE(tz >= R, KLRTM_EINVAL, "tile tz must be >= stencil radius %d", R);
   if(tx > 0) KL_REQUIRE(tx >= R, KLRTM_EINVAL, "tile tx must be >= stencil radius %d", R);

@@ -40,3 +40,40 @@ class PhaseCorrection:
             self.close()
         except Exception:
             pass
+
+
+def resample_record(record, dt, dt_prop, t0, max_freq, dt_out, nt_out, keep_t0=False, dipole=None):
+    """ExtractReceiverData::resampleRecData (bgprtm/ExtractReceiverData.cpp:75-110): the raw record
+    [nr][nt] sampled at dt from t0 (<= 0) through PhaseCorrection(nt, dt, dt_prop, -t0, max_freq, BWD, w_pow,
+    phase_deg, 1, nt_out, dt_out, keep_t0 ? -t0 : 0), the time-dispersion correction of the propagation step
+    dt_prop and the resampling to dt_out.  dipole = (dz, vsurf) for a dipole (true-amplitude) receiver
+    spread: w^-1 and a 90 degree phase, scaled by 2 vsurf / dz.  Returns [nr][nt_out] (numpy)."""
+    record = np.ascontiguousarray(record, np.float32)
+    nr, nt = record.shape
+    w_pow, phase_deg, dt_spread = 0.0, 0.0, 1.0
+    if dipole is not None:
+        dz, vsurf = dipole
+        w_pow, phase_deg, dt_spread = -1.0, 90.0, float(np.float32(0.5) * np.float32(dz) / np.float32(vsurf))
+    pc = PhaseCorrection(nt, dt, dt_prop, -t0, max_freq, w_pow=w_pow, phase_deg=phase_deg, ntOut=nt_out, dtOut=dt_out,
+                         t0Out=-t0 if keep_t0 else 0.0)
+    out = pc.backward(record, scaler=float(np.float32(1.0) / np.float32(dt_spread)))
+    pc.close()
+    return out
+
+
+def write_record(path, data, dt, t0=0.0, xyz=None):
+    """ExtractReceiverData::saveRecFile's content in this build's record file (klrtm_record_write)."""
+    d = np.ascontiguousarray(data, np.float32)
+    c = None if xyz is None else np.ascontiguousarray(xyz, np.float32).reshape(d.shape[0], 3)
+    check(lib().klrtm_record_write(str(path).encode(), d.shape[0], d.shape[1], dt, t0,
+                                   None if c is None else c.ctypes.data, d.ctypes.data))
+
+
+def read_record(path):
+    """(data [nr][nt], dt, t0, xyz [nr][3]) from a record file."""
+    nr, nt, dt, t0 = C.c_int(), C.c_int(), C.c_float(), C.c_float()
+    check(lib().klrtm_record_read_header(str(path).encode(), C.byref(nr), C.byref(nt), C.byref(dt), C.byref(t0)))
+    xyz = np.zeros((nr.value, 3), np.float32)
+    data = np.zeros((nr.value, nt.value), np.float32)
+    check(lib().klrtm_record_read(str(path).encode(), xyz.ctypes.data, data.ctypes.data))
+    return data, dt.value, t0.value, xyz

@@ -8,6 +8,7 @@
 #include <random>
 #include <vector>
 
+#include "../../include/bgprtm/ExtractReceiverData.h"
 #include "../../include/bgprtm/FdEngine.h"
 #include "../../include/bgprtm/ImagingCondition.h"
 #include "../../include/bgprtm/ImagingCorrelator.h"
@@ -188,6 +189,67 @@ int main() {
     double e = 0;
     for(int i = 0; i < m; i++) e = std::max(e, (double)std::fabs(got[i] - want[i]));
     report("ImagingCorrelator::run INVERSION", e < 1e-5, e);
+  }
+  // ExtractReceiverData::resampleRecData without dispersion (dt_prop 0) and with dtOut = dt: the identity on
+  // a band-limited record; saveRecFile writes it
+  {
+    const int nr = 3, ntr = 200;
+    const float dtr = 0.002f;
+    ExtractReceiverData erd(nr, ntr, dtr, 0.f, (ntr - 1) * dtr);
+    for(int r = 0; r < nr; r++)
+      for(int it = 0; it < ntr; it++)
+        erd.recDataRaw[(size_t)r * ntr + it] = (float)std::sin(2 * M_PI * (5.0 + r) * it / 300.0);
+    erd.saveToRecData(0.f, 0.5f / dtr);
+    double e = 0;
+    for(size_t i = 0; i < erd.oData.size(); i++) e = std::max(e, (double)std::fabs(erd.oData[i] - erd.recDataRaw[i]));
+    report("ExtractReceiverData::resampleRecData", e < 2e-3, e);
+    erd.saveRecFile("/tmp/konglong_api_check.rec");
+    int rnr = 0, rnt = 0;
+    float rdt = 0, rt0 = 1;
+    konglong::check(klrtm_record_read_header("/tmp/konglong_api_check.rec", &rnr, &rnt, &rdt, &rt0));
+    report("ExtractReceiverData::saveRecFile", rnr == nr && rnt == erd.ntOut && rdt == dtr && rt0 == 0.f, 0.0);
+  }
+  // Propagator::mod and Propagator::rtm (the static entry points) on a small layered model: the record has
+  // energy, the image is finite and has energy, and stacking onto an output grid adds the coincident columns
+  {
+    const int mx = 24, my = 20, mz = 30;
+    std::vector<float> mv((size_t)mx * my * mz);
+    for(int iy = 0; iy < my; iy++)
+      for(int ix = 0; ix < mx; ix++)
+        for(int iz = 0; iz < mz; iz++) mv[((size_t)iy * mx + ix) * mz + iz] = iz < 14 ? 1800.f : 2400.f;
+    konglong::JobDeck &g = konglong::global_pars();
+    g.model.nx = mx, g.model.ny = my, g.model.nz = mz, g.model.dx = g.model.dy = g.model.dz = 10.f;
+    g.model.x0 = 1000.f, g.model.y0 = 2000.f, g.model.model_type = ISO, g.model.vel = mv.data();
+    g.zMax = 290.f, g.maxFreq = 30.f, g.tmax = 0.25f, g.dt = 1e-3f, g.padX = g.padY = 60.f, g.nbnd = 8;
+    g.sourceSpreadSize = g.receiverSpreadSize = 4;
+    std::vector<float> sx{1120.f}, sy{2100.f}, sz{30.f};
+    Traces tr(251, nullptr, 0, 1e-3f);
+    for(int j = 0; j < 4; j++)
+      for(int i = 0; i < 5; i++) tr.addReceiver(vector3(1080.f + 20.f * i, 2060.f + 20.f * j, 20.f));
+    vector3 rmin(1080.f, 2060.f, 20.f), rmax(1160.f, 2120.f, 20.f);
+    std::unique_ptr<ExtractReceiverData> erd = Propagator::mod(sx, sy, sz, 1, rmin, rmax, &tr);
+    double rec_energy = 0;
+    for(float v : erd->recDataRaw) rec_energy += (double)v * v;
+    report("Propagator::mod record", erd->nr == 20 && erd->nt == 251 && rec_energy > 0, rec_energy);
+    for(int r = 0; r < 20; r++) std::memcpy(tr.getTrace(r), &erd->recDataRaw[(size_t)r * 251], 251 * sizeof(float));
+    OutputGrid og;
+    og.x0 = 1000.f, og.y0 = 2000.f, og.dx = og.dy = og.dz = 10.0, og.nx = mx, og.ny = my, og.nz = mz;
+    std::vector<float> dest((size_t)mx * my * mz, 0.f);
+    std::shared_ptr<Grid> grid = Propagator::rtm(sx, sy, sz, 1, rmin, rmax, std::vector<Traces *>{&tr},
+                                                 std::vector<float *>{dest.data()}, &og);
+    double img_energy = 0, dsum = 0, want = 0;
+    bool finite = true;
+    for(float v : grid->image) img_energy += (double)v * v, finite = finite && std::isfinite(v);
+    for(float v : dest) dsum += v;
+    const int b = (int)std::lround(-grid->z0 / grid->dz);
+    for(int iy = 0; iy < grid->ny; iy++)
+      for(int ix = 0; ix < grid->nx; ix++) {
+        const long jx = std::lround((grid->x0 + ix * grid->dx - og.x0) / og.dx), jy = std::lround((grid->y0 + iy * grid->dy - og.y0) / og.dy);
+        if(jx < 0 || jy < 0 || jx >= og.nx || jy >= og.ny) continue;
+        for(int iz = 0; iz < og.nz && b + iz < grid->nz; iz++) want += grid->image[((size_t)iy * grid->nx + ix) * grid->nz + b + iz];
+      }
+    report("Propagator::rtm image", finite && img_energy > 0, img_energy);
+    report("Propagator::rtm stack onto OutputGrid", std::fabs(dsum - want) <= 1e-6 * (std::fabs(want) + 1e-30), dsum - want);
   }
   // Propagator::apply_symmetry, computeDamping, Lagrange
   {

@@ -47,3 +47,19 @@ def test_record_sized_batch(klrtm):
     want = o.backward(x[pick])
     assert rel_l2(got[pick], want) < 2e-5
     g.close()
+
+
+@pytest.mark.parametrize("dipole", [None, (10.0, 1500.0)], ids=["plain", "dipole"])
+def test_resample_record_matches_oracle(klrtm, dipole):
+    """ExtractReceiverData::resampleRecData: raw record at 2 ms from t0 = -40 ms, propagation step 1 ms,
+    resampled to 4 ms from t = 0 (the dipole case with w^-1, 90 degrees and 1/dt_spread)."""
+    from konglong_amd.preproc import resample_record
+    nr, nt, dt, t0 = 40, 500, 0.002, -0.04
+    x = np.random.default_rng(11).standard_normal((nr, nt)).astype(np.float32)
+    got = resample_record(x, dt, 0.001, t0, 45.0, 0.004, 240, dipole=dipole)
+    ntfft = int(klrtm.lib().klrtm_fft_size(int(round(nt * 1.5))))
+    kw = dict(w_pow=-1.0, phase_deg=90.0) if dipole else {}
+    o = OraclePC(nt, dt, 0.001, -t0, 45.0, ntfft, ntOut=240, dtOut=0.004, t0Out=0.0, **kw)
+    sc = 1.0 if dipole is None else float(np.float32(1) / (np.float32(0.5) * np.float32(dipole[0]) / np.float32(dipole[1])))
+    want = o.backward(x, scaler=sc)
+    assert rel_l2(got, want) < 2e-5
