@@ -10,9 +10,128 @@
 #ifndef KONGLONG_IMAGINGCONDITION_H_
 #define KONGLONG_IMAGINGCONDITION_H_
 
+#include <cmath>
+#include <memory>
 #include <vector>
 
+#include "CacheFile.h"
+#include "ImagingCorrelator.h"
+#include "Propagator.h"
+#include "WaveFieldCompress.h"
 #include "konglong_gpu.h"
+
+// bgprtm/ImagingCondition.h:32-34
+#ifndef KONGLONG_NO_IMAGETYPE
+enum ImageType { STACK = 0, OFCIG = 1, DTCIG = 2, ADCIG = 3, DECON = 4 };
+#endif
+
+namespace konglong {
+// global_pars keys ImagingCondition reads (bgprtm/ImagingCondition.cpp:38-70, 151-198, 393-396): the
+// imaging option (LowcutFilter), the water-depth taper length (grid->getIDz(waterDepth)), minFreq / vMin
+struct ImagingParams {
+  int LowcutFilter = KLRTM_IC_IMAGE1;
+  int ntaperZ = 0;
+  float minFreq = 1.0f, vMin = 1492.f;
+};
+inline ImagingParams &imaging_params() {
+  static ImagingParams p;
+  return p;
+}
+}  // namespace konglong
+
+// ImagingCondition(srcWaveField, recWaveField, imgType)::genImage (bgprtm/ImagingCondition.cpp:322-470) over two
+// snapshot caches: for every stored step the source and receiver snapshots are read from their caches,
+// decoded with WaveFieldCompress::compression (with its edge taper), the source one tapered in time
+// (wtaperT over ntaperT = rint((t1 - t0)/dt) steps) or by the water-depth taper, the illumination (AMP)
+// accumulated and ImagingCorrelator::run applied with the LowcutFilter option; INVERSION2 then takes its
+// cos and sin terms.  Zero lag only (MaxTimeLag 0, STACK), nDbl = 1.  The image (and AMP, FWI, RAW) come
+// back on the host in the returned Grid and the accessors; with destImages and an OutputGrid the image is
+// added into destImages[0] where its columns fall on output nodes.  Cone mute, illumination damping and
+// resampling are the separate ImagingPost / Lagrange calls.
+class ImagingCondition {
+public:
+  ImagingCondition(CacheFile *srcWaveField, CacheFile *recWaveField, ImageType imgType = STACK)
+      : src_(srcWaveField), rec_(recWaveField) {
+    if(imgType != STACK) throw konglong::Error(KLRTM_EINVAL, "konglong: only the zero-lag STACK image is built");
+    if(!src_ || !rec_ || src_->nx != rec_->nx || src_->ny != rec_->ny || src_->nz != rec_->nz || src_->nt != rec_->nt)
+      throw konglong::Error(KLRTM_EINVAL, "konglong: source and receiver caches must have the same shape");
+  }
+  virtual ~ImagingCondition() = default;
+  void getVel(float *vel, float /*dt*/) { vel_ = vel; }
+
+  std::shared_ptr<Grid> genImage(Grid *grid, WaveFieldCompress *wfComp, float /*xMinValid*/, float /*xMaxValid*/,
+                                 float /*yMinValid*/, float /*yMaxValid*/, float /*dtsav*/,
+                                 std::vector<float *> destImages = std::vector<float *>{}, OutputGrid *destGrid = nullptr) {
+    const int nx = src_->nx, ny = src_->ny, nz = src_->nz, nt = src_->nt;
+    const size_t n = (size_t)nx * ny * nz, ns = WaveFieldCompress::nshort_volume(nz, nx, ny);
+    const konglong::ImagingParams &P = konglong::imaging_params();
+    const int opt = P.LowcutFilter;
+    // ImagingCondition::calTapes (bgprtm/ImagingCondition.cpp:180-198)
+    std::vector<float> wtaperT;
+    const int ntaperT = (int)nearbyintf((src_->t1 - src_->t0) / src_->dt);
+    for(int i = 0; i < ntaperT; i++) wtaperT.push_back(0.5f * (1.0f - cosf((float)M_PI * float(i) / float(ntaperT))));
+    std::vector<float> wtaperZ(std::max(P.ntaperZ, 0));
+    for(int i = 0; i < P.ntaperZ; i++) wtaperZ[i] = 0.5f * (1.0f - cosf((float)M_PI * float(i) / float(P.ntaperZ)));
+    konglong::DeviceBuffer<float> dtz(std::max<size_t>(1, wtaperZ.size()));
+    if(!wtaperZ.empty()) dtz.upload(wtaperZ.data());
+    konglong::DeviceBuffer<ushort> cs(ns), cr(ns);
+    konglong::DeviceBuffer<float> S(n), R(n), img(n), amp(n), fwi(n), raw(n);
+    std::vector<ushort> hs(ns), hr(ns);
+    std::unique_ptr<ImagingCorrelator> corr;
+    if(opt != KLRTM_IC_IMAGE1)
+      corr.reset(new ImagingCorrelator(nx, ny, nz, src_->dx, src_->dy, src_->dz, nx, ny, nz, P.minFreq / P.vMin));
+    for(int it = 0; it < nt; it++) {
+      const off_t off = (off_t)it * (off_t)(ns * sizeof(ushort));
+      if(src_->pread(hs.data(), ns * sizeof(ushort), off) < 0 || rec_->pread(hr.data(), ns * sizeof(ushort), off) < 0)
+        throw konglong::Error(KLRTM_EINVAL, "konglong: short read from a snapshot cache");
+      cs.upload(hs.data()), cr.upload(hr.data());
+      wfComp->uncompress(cs.get(), S.get(), nx, ny, nz);
+      wfComp->uncompress(cr.get(), R.get(), nx, ny, nz);
+      // applyTape, then calIllum (amp += S*S) and ImagingCorrelator::run
+      konglong::check(klrtm_apply_tape(S.get(), nx, ny, nz, it < ntaperT ? wtaperT[it] : -1.0f, dtz.get(),
+                                       it < ntaperT ? 0 : P.ntaperZ));
+      konglong::check(klrtm_ic_correlate(S.get(), S.get(), nx, ny, nz, 1.0f, nullptr, 0, amp.get(), nullptr, nullptr,
+                                         nullptr));
+      if(opt == KLRTM_IC_IMAGE1) {
+        konglong::check(klrtm_ic_correlate(S.get(), R.get(), nx, ny, nz, 1.0f, nullptr, 0, img.get(), nullptr, nullptr,
+                                           nullptr));
+      } else {
+        float *sb[1] = {S.get()}, *rb[1] = {R.get()};
+        corr->run(nullptr, img.get(), fwi.get(), raw.get(), sb, rb, opt);
+      }
+    }
+    if(opt == KLRTM_IC_INVERSION2) corr->costerm(img.get()), corr->sinterm(fwi.get(), img.get());
+    image_ = img.host(), amp_ = amp.host(), fwi_ = fwi.host(), raw_ = raw.host();
+    auto out = std::make_shared<Grid>();
+    if(grid) *out = *grid;
+    out->nx = nx, out->ny = ny, out->nz = nz, out->image = image_;
+    if(destGrid && !destImages.empty() && destImages[0]) {
+      // columns on output nodes (the grid's origin and spacing place them), depth sampling shared
+      for(int iy = 0; iy < ny; iy++)
+        for(int ix = 0; ix < nx; ix++) {
+          const double fx = (out->x0 + ix * out->dx - destGrid->x0) / destGrid->dx;
+          const double fy = destGrid->ny > 1 ? (out->y0 + iy * out->dy - destGrid->y0) / destGrid->dy : 0.0;
+          const long jx = std::lround(fx), jy = std::lround(fy);
+          if(std::fabs(fx - jx) > 1e-3 || std::fabs(fy - jy) > 1e-3 || jx < 0 || jy < 0 || jx >= destGrid->nx ||
+             jy >= destGrid->ny)
+            continue;
+          float *col = destImages[0] + ((size_t)jy * destGrid->nx + jx) * destGrid->nz;
+          const float *srcc = image_.data() + ((size_t)iy * nx + ix) * nz;
+          for(int iz = 0; iz < std::min(nz, destGrid->nz); iz++) col[iz] += srcc[iz];
+        }
+    }
+    return out;
+  }
+  const std::vector<float> &image() const { return image_; }
+  const std::vector<float> &illum() const { return amp_; }
+  const std::vector<float> &fwi() const { return fwi_; }
+  const std::vector<float> &raw() const { return raw_; }
+
+private:
+  CacheFile *src_, *rec_;
+  float *vel_ = nullptr;
+  std::vector<float> image_, amp_, fwi_, raw_;
+};
 
 namespace ImagingPost {
 

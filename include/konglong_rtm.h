@@ -276,6 +276,10 @@ int klrtm_lcodec_correlate(const klrtm_lcodec *c, const void *code, const float 
 int klrtm_ic_correlate(const float *src, const float *rec, int nx, int ny, int nz, float taper_t, const float *taper_z,
                        int ntaper_z, float *image, float *illum, double *image64, double *illum64);
 
+/* ImagingCondition::applyTape in place (device wf): taper_t >= 0 scales every cell (time taper wtaperT[it]);
+ * taper_t < 0 scales the first ntaper_z cells of each column by taper_z[iz] (water-depth taper, device) */
+int klrtm_apply_tape(float *wf, int nx, int ny, int nz, float taper_t, const float *taper_z, int ntaper_z);
+
 /* ---- ImagingCorrelator FFT imaging conditions (bgprtm/ImagingCorrelator.cpp:17-431), nDbl = 1 ----
  * The LowcutFilter options of ImagingCondition::getImagingOption (bgprtm/ImagingCondition.cpp:151-160,
  * enum IMAGE_OPTION of bgprtm/ImagingCorrelator.h:17-19), hipFFT on the device.  klrtm_correlator_create

@@ -105,6 +105,7 @@ _SIGS = {
     "klrtm_blockq_encode": (_i, [_vp, _vp, _vp, _i, _i, _i, _i]),
     "klrtm_blockq_decode": (_i, [_vp, _vp, _vp, _i, _i, _i, _i]),
     "klrtm_ic_correlate": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _vp, _vp, _vp]),
+    "klrtm_apply_tape": (_i, [_vp, _i, _i, _i, _f, _vp, _i]),
     "klrtm_record_write": (_i, [C.c_char_p, _i, _i, _f, _f, _vp, _vp]),
     "klrtm_record_read_header": (_i, [C.c_char_p, _pi, _pi, _pf, _pf]),
     "klrtm_record_read": (_i, [C.c_char_p, _vp, _vp]),

@@ -192,6 +192,29 @@ extern "C" int klrtm_stack_image(const void *img, int is_double, int nx, int ny,
   return check_launch("stack_kernel");
 }
 
+// ImagingCondition::applyTape (bgprtm/ImagingCondition.cpp:200-226) in place
+__global__ void apply_tape_kernel(float *__restrict__ wf, size_t nxy, int nz, float taper_t, const float *__restrict__ tz,
+                                  int ntz) {
+  const size_t tot = nxy * (size_t)(taper_t >= 0.f ? nz : ntz);
+  const int lz = taper_t >= 0.f ? nz : ntz;
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t col = e / lz;
+    const int iz = (int)(e - col * lz);
+    const size_t i = col * nz + iz;
+    wf[i] = wf[i] * (taper_t >= 0.f ? taper_t : tz[iz]);
+  }
+}
+
+extern "C" int klrtm_apply_tape(float *wf, int nx, int ny, int nz, float taper_t, const float *taper_z, int ntaper_z) {
+  KL_REQUIRE(wf && nx > 0 && ny > 0 && nz > 0, KLRTM_EINVAL, "bad wavefield");
+  KL_REQUIRE(taper_t >= 0.f || ntaper_z <= 0 || taper_z, KLRTM_EINVAL, "z taper requested without taper values");
+  if(taper_t < 0.f && ntaper_z <= 0) return KLRTM_OK;
+  const size_t tot = (size_t)nx * ny * (taper_t >= 0.f ? nz : std::min(ntaper_z, nz));
+  hipLaunchKernelGGL(apply_tape_kernel, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 256 * 64)), dim3(256), 0, stream(),
+                     wf, (size_t)nx * ny, nz, taper_t, taper_z, std::min(ntaper_z, nz));
+  return check_launch("apply_tape_kernel");
+}
+
 extern "C" int klrtm_ic_correlate(const float *src, const float *rec, int nx, int ny, int nz, float taper_t,
                                   const float *taper_z, int ntaper_z, float *image, float *illum, double *image64,
                                   double *illum64) {

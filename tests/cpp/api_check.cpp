@@ -33,6 +33,8 @@ void orc_inject_ghost(float *w0, const float *vel, int nx, int ny, int nz, int n
 void orc_extract_ghost(const float *w, int nx, int ny, int nz, int nr, float *out, const float *kx, const float *ky,
                        const float *kz, int Lx, int Ly, int Lz, const int *lx, const int *ly, const int *lz,
                        const int *offx, const int *offy, const int *offz, int iz_mirror);
+void orc_compress16(const float *in, uint16_t *out, int nx, int ny, int nz);
+void orc_uncompress16(const uint16_t *in, float *out, int nx, int ny, int nz);
 void orc_compress_pair(const float *in, uint16_t *out, int nx, int ny, int nz, int mode);
 void orc_pml_round(float g, float *p, float *q, int n);
 void orc_apply_symmetry(float *wf0, int nx, int ny, int nz, int iz0, int sym, int tprlen, int taper);
@@ -250,6 +252,46 @@ int main() {
       }
     report("Propagator::rtm image", finite && img_energy > 0, img_energy);
     report("Propagator::rtm stack onto OutputGrid", std::fabs(dsum - want) <= 1e-6 * (std::fabs(want) + 1e-30), dsum - want);
+  }
+  // ImagingCondition(CacheFile*, CacheFile*, STACK)::genImage over two in-memory SEQ_ZXYT caches of
+  // COMPRESS_16 snapshots: image and illumination bit for bit against the oracle's decode, taper and sums
+  {
+    const int cx = 30, cy = 22, cz = 28, cn = cx * cy * cz, nts = 3;
+    const size_t ns = klrtm_nshort_volume(cz, cx, cy);
+    CacheFile src(cx, cy, cz, nts, SEQ_ZXYT, nullptr), rec(cx, cy, cz, nts, SEQ_ZXYT, nullptr);
+    src.setparas(0.f, 0.f, 0.f, 0.f, 2e-3f, 10.f, 10.f, 10.f, 1e-3f);  // ntaperT = 2
+    std::vector<float> wimg(cn, 0.f), wamp(cn, 0.f), a(cn), b(cn), da(cn), db(cn);
+    std::vector<uint16_t> ca(ns), cb(ns);
+    for(int it = 0; it < nts; it++) {
+      for(int i = 0; i < cn; i++) a[i] = U(gen), b[i] = U(gen);
+      orc_compress16(a.data(), ca.data(), cx, cy, cz), orc_compress16(b.data(), cb.data(), cx, cy, cz);
+      src.pwrite(ca.data(), ns * 2, (off_t)it * ns * 2), rec.pwrite(cb.data(), ns * 2, (off_t)it * ns * 2);
+      orc_uncompress16(ca.data(), da.data(), cx, cy, cz), orc_uncompress16(cb.data(), db.data(), cx, cy, cz);
+      const float tt = it < 2 ? 0.5f * (1.0f - cosf((float)M_PI * float(it) / float(2))) : 1.0f;
+      for(int i = 0; i < cn; i++) {
+        const float sv = it < 2 ? da[i] * tt : da[i];
+        wamp[i] = wamp[i] + (sv * 1.0f) * sv;
+        wimg[i] = wimg[i] + (sv * 1.0f) * db[i];
+      }
+    }
+    WaveFieldCompress::compression_ref() = WaveFieldCompress::COMPRESS_16;
+    WaveFieldCompress wfc(1);
+    ImagingCondition ic(&src, &rec, STACK);
+    Grid g0;
+    g0.dx = g0.dy = g0.dz = 10.f;
+    std::shared_ptr<Grid> gi = ic.genImage(&g0, &wfc, 0.f, 0.f, 0.f, 0.f, 1e-3f);
+    report("ImagingCondition::genImage image bit-exact", std::memcmp(gi->image.data(), wimg.data(), cn * 4) == 0,
+           rel_l2(gi->image, wimg));
+    report("ImagingCondition::genImage illumination", std::memcmp(ic.illum().data(), wamp.data(), cn * 4) == 0,
+           rel_l2(ic.illum(), wamp));
+    konglong::imaging_params().LowcutFilter = INVERSION;
+    ImagingCondition ic2(&src, &rec, STACK);
+    std::shared_ptr<Grid> g2 = ic2.genImage(&g0, &wfc, 0.f, 0.f, 0.f, 0.f, 1e-3f);
+    bool fin = true;
+    double efwi = 0;
+    for(size_t i = 0; i < g2->image.size(); i++) fin = fin && std::isfinite(g2->image[i]), efwi += ic2.fwi()[i] * ic2.fwi()[i];
+    report("ImagingCondition::genImage INVERSION", fin && efwi > 0, efwi);
+    konglong::imaging_params().LowcutFilter = IMAGE1;
   }
   // Propagator::apply_symmetry, computeDamping, Lagrange
   {
