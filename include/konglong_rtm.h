@@ -225,6 +225,13 @@ size_t klrtm_nshort_volume(int nz, int nx, int ny);
 int klrtm_compress16(const float *in, uint16_t *out, int nx, int ny, int nz);
 /* decode with the reference's edge taper (20 cells x/y, 10 cells z) */
 int klrtm_uncompress16(const uint16_t *in, float *out, int nx, int ny, int nz);
+/* klrtm_compress16 without any allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops).  The per-slice
+ * maxima live in a fixed device array of KLRTM_C16_NOALLOC_MAX_NY slices, one per GPU, held until the
+ * process ends; ny above that returns KLRTM_EINVAL.  The array is cleared by a fill kernel at every
+ * call; calls that overlap on one GPU (several streams or threads) share it and race. */
+#define KLRTM_C16_NOALLOC_MAX_NY 32768
+int klrtm_compress16_noalloc(const float *in, uint16_t *out, int nx, int ny, int nz);
 
 /* ---- WaveFieldCompress COMPRESS_16P / COMPRESS_8P / COMPRESS_NONE (bgprtm/WaveFieldCompress.cpp:
  * 30-44, 209-530): per z trace (nz floats) of the nx*ny traces, ushort stride nshort_pack(nz, mode).

@@ -95,6 +95,7 @@ _SIGS = {
     "klrtm_receiver_get_tables": (_i, [_vp, _pi, _pi, _pi, _pi, _pi, _pi, _pf, _pf, _pf, _pi]),
     "klrtm_nshort_volume": (_sz, [_i, _i, _i]),
     "klrtm_compress16": (_i, [_vp, _vp, _i, _i, _i]),
+    "klrtm_compress16_noalloc": (_i, [_vp, _vp, _i, _i, _i]),
     "klrtm_nshort_pack": (_sz, [_sz, _i]),
     "klrtm_compress_pair": (_i, [_vp, _vp, _i, _i, _i, _i]),
     "klrtm_uncompress_pair": (_i, [_vp, _vp, _i, _i, _i, _i]),

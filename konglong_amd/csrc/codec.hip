@@ -14,6 +14,7 @@
 #include "codec_common.h"
 #include "codec.h"
 #include <cmath>
+#include <mutex>
 
 namespace klrtm {
 
@@ -312,6 +313,35 @@ static int smax_buffer(int ny, unsigned **p) {
   return KLRTM_OK;
 }
 
+// Scratch of the allocation-free COMPRESS_16 entry point: the per-slice maxima
+// live in a fixed device array, one copy per GPU, held until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host the name is
+// a stand-in, not a device address): its device address is looked up once per
+// device with hipGetSymbolAddress and kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned scratch[KLRTM_C16_NOALLOC_MAX_NY];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned *g_scratch_ptr[kMaxDevices] = {};
+
+static int noalloc_scratch(unsigned **p) {
+  int dev = 0;
+  KL_HIP(hipGetDevice(&dev));
+  KL_REQUIRE(dev >= 0 && dev < kMaxDevices, KLRTM_EINVAL, "device %d beyond the %d devices the scratch table holds",
+             dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if(!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if(e != hipSuccess || !ptr) {
+      set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+      return KLRTM_EDEVICE;
+    }
+    g_scratch_ptr[dev] = (unsigned *)ptr;
+  }
+  *p = g_scratch_ptr[dev];
+  return KLRTM_OK;
+}
+
 }  // namespace klrtm
 
 using namespace klrtm;
@@ -327,6 +357,23 @@ int klrtm_compress16(const float *in, uint16_t *out, int nx, int ny, int nz) {
   int rc = smax_buffer(ny, &smax);
   if(rc) return rc;
   return c16_encode(in, out, nx, ny, nz, smax);
+}
+int klrtm_compress16_noalloc(const float *in, uint16_t *out, int nx, int ny, int nz) {
+  KL_REQUIRE(in && out, KLRTM_EINVAL, "null pointer");
+  KL_REQUIRE(nx > 0 && ny > 0 && nz > 0, KLRTM_EINVAL, "bad dimensions");
+  KL_REQUIRE(ny <= KLRTM_C16_NOALLOC_MAX_NY, KLRTM_EINVAL,
+             "ny = %d slices exceeds the allocation-free scratch of %d slices; use klrtm_compress16", ny,
+             KLRTM_C16_NOALLOC_MAX_NY);
+  unsigned *smax = nullptr;
+  int rc = noalloc_scratch(&smax);
+  if(rc) return rc;
+  // no launch timer here: it would create events on first use
+  rc = c16_slice_max(in, nx, ny, nz, smax);
+  if(rc) return rc;
+  const size_t n = (size_t)nx * ny * nz;
+  hipLaunchKernelGGL(c16_encode_kernel, dim3((unsigned)div_up((long)n, 256)), dim3(256), 0, stream(), in, out, smax,
+                     (size_t)nx * nz, ny);
+  return check_launch("c16_encode_kernel");
 }
 int klrtm_uncompress16(const uint16_t *in, float *out, int nx, int ny, int nz) {
   KL_REQUIRE(in && out, KLRTM_EINVAL, "null pointer");
