@@ -43,8 +43,7 @@ public:
     konglong::check(klrtm_kernel_tti(k_, p0, pb, pr));
   }
   void VTI(float *p0, float *pb, float *pr) { konglong::check(klrtm_kernel_vti(k_, p0, pb, pr)); }
-  // bndType/iz0/spread select the free-surface mirror in the reference; ABSORB (0) only here
-  // bndType, iz0 and the taper length are unused, as in the reference (bgprtm/KernelCPU.cpp:125-174): a
+  // bndType, iz0 and spread are accepted and unused, as in the reference (bgprtm/KernelCPU.cpp:125-174): a
   // free surface is Propagator::apply_symmetry after the injection (klrtm_apply_symmetry)
   void ISO(float *p0, float *pb, float *pr, int bndType = 0, int iz0 = 0, int spread = 0) {
     (void)bndType, (void)iz0, (void)spread;
