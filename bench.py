@@ -255,12 +255,14 @@ def batched_vs_plain(nshots, nt, batch, seed=505):
 
     Workload: the first `nshots` of 64 seeded apertures (200-450 cells square, multiples of 8, x 400
     deep) on a 1000 x 1000 x 400 layered model with a 4500 m/s salt body, dx 12.5 m, FD8, PML 32,
-    15 Hz Ricker, receivers every 4th surface cell, snapshots every 4 steps, nt steps per shot
-    (shots/hour is scaled to the full 2500-step record).  Plain: one shot at a time, FP32
-    snapshots, default tiles.  Batched: `batch` shots side by side on their own streams, learned
-    snapshots (trained beforehand, untimed, on two seeded 256^3 random models), autotuned tiles.
-    Both stack FP32 shot images into an FP64 image; the batched stack's NRMSE is taken against the
-    plain one."""
+    15 Hz Ricker, receivers every 4th surface cell, snapshots every 4 steps kept in HBM, nt steps per
+    shot (shots/hour is scaled to 2500 steps when nt is shorter).  Plain: one shot at a time, FP32
+    snapshots, default tiles, launches issued one by one.  Batched: up to `batch` shots side by side
+    on their own streams (as many as fit a 200 GB snapshot budget), each shot's time loop replayed as
+    captured hipGraphs, autotuned tiles; FP32 snapshots, then learned snapshots (trained beforehand,
+    untimed, on two seeded 256^3 random models).  All stack FP32 shot images into an FP64 image; the
+    batched stacks' NRMSE is taken against the plain one (the learned path's NRMSE against an FP64
+    oracle is measured by the batched-shot GPU test on small apertures)."""
     from konglong_amd import models, learned
     from konglong_amd.batch import BatchRTM, Shot, Autotuner
     from konglong_amd._lib import ENGINE_FD, ISO, SNAP_FP32, SNAP_LEARNED
@@ -292,10 +294,13 @@ def batched_vs_plain(nshots, nt, batch, seed=505):
     t_train = time.perf_counter() - t0
     plain = BatchRTM((nyF, nxF, nz), vel, common, batch=1, snap_format=SNAP_FP32)
     records = plain.run(shots)
+    print(f"[bench] batched leg: plain {nshots} shots x {nt} steps done", file=sys.stderr, flush=True)
     tuner = Autotuner()
-    fast32 = BatchRTM((nyF, nxF, nz), vel, common, batch=batch, snap_format=SNAP_FP32, tuner=tuner)
+    fast32 = BatchRTM((nyF, nxF, nz), vel, common, batch=batch, snap_format=SNAP_FP32, tuner=tuner, graph=True)
     fast32.run(shots, records)
-    fast = BatchRTM((nyF, nxF, nz), vel, common, batch=batch, snap_format=SNAP_LEARNED, lcodec=codec, tuner=tuner)
+    print("[bench] batched leg: batched FP32 done", file=sys.stderr, flush=True)
+    fast = BatchRTM((nyF, nxF, nz), vel, common, batch=batch, snap_format=SNAP_LEARNED, lcodec=codec, tuner=tuner,
+                    graph=True)
     fast.run(shots, records)
     a64 = plain.stack.download()
     na = max(np.linalg.norm(a64), 1e-300)
@@ -307,6 +312,10 @@ def batched_vs_plain(nshots, nt, batch, seed=505):
         b.close()
     return {"workload": "batched ISO FD8 RTM, 1000x1000x400 salt model, apertures 200-450 x 400",
             "shots": nshots, "steps_per_shot_measured": nt, "steps_per_shot_scaled_to": 2500, "batch": batch,
+            "apertures": [[sh.nx, sh.ny] for sh in shots], "snapshot_gb_per_shot_fp32":
+                [round(plain.snapshot_bytes(sh) / 1e9, 1) for sh in shots],
+            "groups_fp32": fast32.times["groups"], "groups_learned": fast.times["groups"],
+            "time_loop": "plain: launches issued one by one; batched: hipGraph replays of snapshot-interval groups",
             "plain_shots_per_hour": round(sph[0], 2),
             "batched_fp32_shots_per_hour": round(sph[1], 2), "batched_fp32_speedup": round(sph[1] / sph[0], 3),
             "batched_fp32_stack_nrmse_vs_plain": err32,
@@ -366,26 +375,8 @@ def tti_full_shot(nt=3000, seed=303):
             "codec_training": info}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="vti-cfd10-400",
-                    choices=sorted(WORKLOADS) + ["tti-fd16-600-shot", "adjacent-400"])
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--batched-shots", type=int, default=8, help="shots of the batched-vs-plain leg (0: skip)")
-    ap.add_argument("--batched-steps", type=int, default=200)
-    ap.add_argument("--batch", type=int, default=8)
-    args = ap.parse_args()
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print(f"bench.py measures one GPU: the hot path runs on a single MI355X "
-              f"(requested --gpus {args.gpus}, WORLD_SIZE {world}); sharding shots over several GPUs is a "
-              f"design, not something this benchmark runs.", file=sys.stderr)
-        sys.exit(2)
-
+def measure(workload, K, W, cpu=False, cpu_budget=15.0):
+    """One workload: K timed forward+backward RTM steps after W warm-up steps; the bench line's fields."""
     import ctypes as C
     from konglong_amd import _lib
     from konglong_amd import models
@@ -393,21 +384,11 @@ def main():
     from konglong_amd._lib import SNAP_FP32
 
     L = _lib.lib()
-    if _lib.device_count() < 1:
-        print("no HIP device visible", file=sys.stderr)
-        sys.exit(1)
-    if args.workload == "tti-fd16-600-shot":
-        print(json.dumps(tti_full_shot()))
-        return
-    if args.workload == "adjacent-400":
-        print(json.dumps({"workload": "adjacent-400", **adjacent_physics()}))
-        return
-    w, eng, mdl, vel, eps, dl, dt = build_case(args.workload)
+    w, eng, mdl, vel, eps, dl, dt = build_case(workload)
     ny, nx, nz = w["shape"]
     ncell = nx * ny * nz
     nb = w["nbnd"]
     nt_shot = int(round(w["record_s"] / dt))
-    K, W = args.steps, args.warmup
     nt = W + K + 2
     iz_rcv = nb + 2 if nb > 0 else 2
     rcv = models.surface_receivers(nx, ny, iz_rcv)
@@ -466,29 +447,84 @@ def main():
     pml_b = (16 if (eng == _lib.ENGINE_FD and mdl == _lib.ISO) else 48) * face_cells
     alg_bytes = VOL_BYTES[w["model"]] * ncell + pml_b
     achieved = alg_bytes / (step_ms * 1e-3) / 1e9 if step_ms > 0 else 0.0
-    traffic = parse_traffic(args.workload)
+    traffic = parse_traffic(workload)
     roofline = {"kernel": kname, "launches_per_step": len(group), "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes_per_step": alg_bytes, "step_kernel_ms": round(step_ms, 4)}
     shots_per_hour = 3600.0 / (nt_shot * el / K)
-    cpu = None
-    if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
-        cpu = cpu_baseline(vel, eps, dl, w, eng, mdl, dt, args.cpu_budget, nt_shot)
+    p.close()
+    cpu_line = cpu_baseline(vel, eps, dl, w, eng, mdl, dt, cpu_budget, nt_shot) if cpu else None
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": args.workload, "grid": [nx, ny, nz], "spacing_m": w["d"], "engine": w["engine"],
+        "config": {"workload": workload, "grid": [nx, ny, nz], "spacing_m": w["d"], "engine": w["engine"],
                    "order": w["order"], "model": w["model"], "pml": nb, "dt_s": round(dt, 7), "nt_shot": nt_shot,
                    "receivers": int(rcv.shape[0]), "snap_every": w["snap_every"], "snap_format": "fp32",
                    "step": "1 forward + 1 backward RTM time step (injection, snapshot, imaging on cadence)"},
         "shots_per_hour": round(shots_per_hour, 3),
         "roofline": roofline,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_line,
         "kernels": kern,
     }
+    return out
+
+
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--workload", default="vti-cfd10-400",
+                    choices=sorted(WORKLOADS) + ["tti-fd16-600-shot", "adjacent-400"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--batched-shots", type=int, default=8, help="shots of the batched-vs-plain leg (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=2500)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--legs", default="iso-fd8-400,tti-fd16-600",
+                    help="further workloads timed after the headline (comma list; empty: none)")
+    ap.add_argument("--leg-steps", type=int, default=10)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: the hot path runs on a single MI355X "
+              f"(requested --gpus {args.gpus}, WORLD_SIZE {world}); sharding shots over several GPUs is a "
+              f"design, not something this benchmark runs.", file=sys.stderr)
+        sys.exit(2)
+
+    from konglong_amd import _lib
+
+    _lib.lib()
+    if _lib.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    if args.workload == "tti-fd16-600-shot":
+        print(json.dumps(tti_full_shot()))
+        return
+    if args.workload == "adjacent-400":
+        print(json.dumps({"workload": "adjacent-400", **adjacent_physics()}))
+        return
+    out = measure(args.workload, args.steps, args.warmup, cpu=not args.no_cpu_baseline and
+                  int(os.environ.get("RANK", "0")) == 0, cpu_budget=args.cpu_budget)
+    _progress(f"{args.workload}: {out['value']} Gcell-updates/s, frac {out['roofline']['frac']}")
+    # the other two plain-FP32 stencil workloads, timed the same way on the same GPU (fewer steps)
+    legs = [x for x in args.legs.split(",") if x and x != args.workload]
+    if legs:
+        out["legs"] = {}
+        for name in legs:
+            try:
+                leg = measure(name, args.leg_steps, 2, cpu=False)
+                _progress(f"{name}: {leg['value']} Gcell-updates/s, frac {leg['roofline']['frac']}")
+                out["legs"][name] = {k: leg[k] for k in ("value", "unit", "ms_per_step", "shots_per_hour", "roofline",
+                                                          "config", "kernels")}
+            except Exception as e:  # the headline line is still printed
+                out["legs"][name] = {"error": f"{type(e).__name__}: {e}"}
     if args.batched_shots > 0:
-        p.close()
         try:
             out["batched"] = batched_vs_plain(args.batched_shots, args.batched_steps, args.batch)
         except Exception as e:  # the headline line is still printed

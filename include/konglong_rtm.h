@@ -401,6 +401,20 @@ int klrtm_propagator_set_record(klrtm_propagator *p, const float *record_host);
 int klrtm_propagator_reset(klrtm_propagator *p);
 /* learned codec of KLRTM_SNAP_LEARNED snapshots (borrowed: it must outlive the propagator's use) */
 int klrtm_propagator_set_lcodec(klrtm_propagator *p, const klrtm_lcodec *c);
+/* Captured step sequences for the stepping API (the time loop of bgprtm/Propagator.cpp:1210-1255 as
+ * hipGraph replays).  With on != 0, klrtm_propagator_forward_steps / backward_steps run every whole
+ * group of G steps (G = snap_every, doubled when odd; forward groups start on the snapshot cadence,
+ * backward groups one step above it) as one replay of a graph captured on first use; the steps
+ * before the first and after the last whole group run as before.  The captured launches read the
+ * time index from a device step counter that each captured step advances: the injected trace
+ * sample, the snapshot slot (stored through a staging buffer) and the imaging time taper.  Results
+ * are identical to the uncaptured calls.  Needs snapshots in HBM (no snapshot cache), the zero-lag
+ * imaging condition and traces rather than the FK surface wavefield.  Every klrtm_propagator_set_*
+ * call drops the captured graphs (they hold the old set-up); engine tiles set through
+ * klrtm_propagator_kernel are baked into a graph once captured, so set them first or call
+ * klrtm_propagator_set_graph again.  graph_stats: graphs held and replays launched. */
+int klrtm_propagator_set_graph(klrtm_propagator *p, int on);
+int klrtm_propagator_graph_stats(klrtm_propagator *p, int *ngraphs, long *replays);
 /* the propagator's engine (for klrtm_kernel_set_tile) */
 int klrtm_propagator_kernel(klrtm_propagator *p, klrtm_kernel **k);
 /* device pointers of the shot's image accumulators (FP32 image, illumination; FP64 image, illumination) */

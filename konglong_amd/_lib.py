@@ -141,6 +141,8 @@ _SIGS = {
     "klrtm_propagator_backward_steps": (_i, [_vp, _i, _i, _i]),
     "klrtm_propagator_set_record": (_i, [_vp, _pf]),
     "klrtm_propagator_set_lcodec": (_i, [_vp, _vp]),
+    "klrtm_propagator_set_graph": (_i, [_vp, _i]),
+    "klrtm_propagator_graph_stats": (_i, [_vp, _pi, _pl]),
     "klrtm_propagator_kernel": (_i, [_vp, C.POINTER(_vp)]),
     "klrtm_propagator_images": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "klrtm_propagator_clear_images": (_i, [_vp]),

@@ -3,11 +3,14 @@
 The reference migrates one shot per process and stacks images on the output
 grid (bgprtm/ImagingCondition.cpp:874-945).  Here a batch of shots with their
 own aperture sub-grids runs side by side on one GPU: every shot is a
-Propagator on its own HIP stream, the host issues the shots' time steps in
-chunks of the snapshot cadence, round-robin, so the kernels of different
-shots overlap (small apertures and the short injection / imaging kernels no
-longer leave the GPU idle), and each finished shot image is added in FP64
-into the stack on the full output grid.  Stencil tile shapes are chosen per
+Propagator on its own HIP stream, so the kernels of different shots overlap
+(small apertures and the short injection / imaging kernels no longer leave
+the GPU idle), and each finished shot image is added in FP64 into the stack
+on the full output grid.  A batch holds as many shots as fit the snapshot
+budget in HBM.  With ``graph=True`` each shot's time loop is a sequence of
+hipGraph replays (one snapshot-interval group per replay, the time index on
+a device counter); otherwise the host issues the shots' steps in chunks of
+the snapshot cadence, round-robin.  Stencil tile shapes are chosen per
 aperture shape by ``Autotuner`` (timed candidates, cached).
 """
 import ctypes as C
@@ -115,7 +118,8 @@ class BatchRTM:
     velocity(x0, y0, nx, ny) returns the shot's (ny, nx, nz) velocity sub-volume; common holds the
     shot_config keywords shared by all shots (d, dt, nt, engine, order, model, nbnd, ...)."""
 
-    def __init__(self, out_shape, velocity, common, batch=4, snap_format=SNAP_FP32, lcodec=None, tuner=None):
+    def __init__(self, out_shape, velocity, common, batch=4, snap_format=SNAP_FP32, lcodec=None, tuner=None,
+                 graph=False, snap_budget_gb=200.0):
         from ._lib import DeviceArray
         self.nyF, self.nxF, self.nz = out_shape
         self.velocity = velocity
@@ -124,6 +128,8 @@ class BatchRTM:
         self.snap_format = snap_format
         self.lcodec = lcodec
         self.tuner = tuner
+        self.graph = bool(graph)
+        self.snap_budget = snap_budget_gb * 1e9
         self.stack = DeviceArray(out_shape, np.float64)
         check(lib().klrtm_synchronize())
         self.streams = []
@@ -147,13 +153,42 @@ class BatchRTM:
         p.set_receivers(shot.rcv_xyz)
         return p
 
+    def snapshot_bytes(self, shot):
+        """HBM bytes of one shot's snapshots (every snap_every-th of nt steps, slots padded to 256 B)."""
+        nt, k = self.common["nt"], max(1, self.common.get("snap_every", 1))
+        shape = (shot.ny, shot.nx, self.nz)
+        if self.snap_format == SNAP_LEARNED:
+            per = self.lcodec.code_bytes(shape)
+        elif self.snap_format == SNAP_FP32:
+            per = 4 * shot.nx * shot.ny * self.nz
+        else:
+            per = 2 * shot.nx * shot.ny * self.nz
+        return ((nt + k - 1) // k) * ((per + 255) // 256 * 256)
+
+    def groups(self, shots):
+        """Consecutive shots, at most `batch` per group and within the snapshot budget (a shot over the
+        budget on its own still forms a group)."""
+        out, cur, used = [], [], 0
+        for sh in shots:
+            b = self.snapshot_bytes(sh)
+            if cur and (len(cur) == self.batch or used + b > self.snap_budget):
+                out.append(cur)
+                cur, used = [], 0
+            cur.append(sh)
+            used += b
+        if cur:
+            out.append(cur)
+        return out
+
     def run(self, shots, records=None):
         """Migrate the shots; records[i] is shot i's [nrcv][nt] record (None: modelled here, untimed).
         Returns the per-shot records used."""
         L = lib()
         used = []
-        for g0 in range(0, len(shots), self.batch):
-            group = shots[g0:g0 + self.batch]
+        self.times.setdefault("groups", [])
+        g0 = 0
+        for group in self.groups(shots):
+            self.times["groups"].append(len(group))
             props = []
             for j, sh in enumerate(group):
                 L.klrtm_set_stream(self.streams[j])
@@ -164,6 +199,8 @@ class BatchRTM:
                 p.set_record(rec)
                 used.append(rec)
                 p.forward_steps(0, 1, True)   # allocates the snapshot store outside the timed region
+                if self.graph:
+                    p.set_graph(True)
                 props.append(p)
             for j in range(len(group)):
                 L.klrtm_stream_synchronize(self.streams[j])
@@ -173,21 +210,31 @@ class BatchRTM:
             for j, p in enumerate(props):
                 L.klrtm_set_stream(self.streams[j])
                 p.reset()
-            for it0 in range(0, nt, k):
+            if self.graph:  # each shot's whole forward run, then its backward run, as graph replays
                 for j, p in enumerate(props):
                     L.klrtm_set_stream(self.streams[j])
-                    p.forward_steps(it0, min(k, nt - it0), True)
-            for j, p in enumerate(props):
-                L.klrtm_set_stream(self.streams[j])
-                p.reset()
-                check(L.klrtm_propagator_clear_images(p.h))
-            it = nt - 1
-            while it >= 0:
-                n = it % k + 1  # down to the next imaged step
+                    p.forward_steps(0, nt, True)
                 for j, p in enumerate(props):
                     L.klrtm_set_stream(self.streams[j])
-                    p.backward_steps(it, n, True)
-                it -= n
+                    p.reset()
+                    check(L.klrtm_propagator_clear_images(p.h))
+                    p.backward_steps(nt - 1, nt, True)
+            else:
+                for it0 in range(0, nt, k):
+                    for j, p in enumerate(props):
+                        L.klrtm_set_stream(self.streams[j])
+                        p.forward_steps(it0, min(k, nt - it0), True)
+                for j, p in enumerate(props):
+                    L.klrtm_set_stream(self.streams[j])
+                    p.reset()
+                    check(L.klrtm_propagator_clear_images(p.h))
+                it = nt - 1
+                while it >= 0:
+                    n = it % k + 1  # down to the next imaged step
+                    for j, p in enumerate(props):
+                        L.klrtm_set_stream(self.streams[j])
+                        p.backward_steps(it, n, True)
+                    it -= n
             # apertures overlap: the stack is updated shot after shot on one stream, in shot order
             for j in range(len(group)):
                 check(L.klrtm_stream_synchronize(self.streams[j]))
@@ -203,6 +250,7 @@ class BatchRTM:
             self.times["steps"] += len(group) * nt
             for p in props:
                 p.close()
+            g0 += len(group)
         L.klrtm_set_stream(None)
         return used
 

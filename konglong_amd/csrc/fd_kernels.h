@@ -31,5 +31,12 @@ int launch_vel2(float *v, size_t n, float dt);
 int launch_update2nd(float *w0, const float *w1, size_t n);
 // p[0..n) = v (a kernel, so that it can sit in a captured graph)
 int launch_fill(float *p, float v, size_t n);
+// device step counter of captured step sequences: *s = v, *s += d
+int launch_step_set(int *s, int v);
+int launch_step_add(int *s, int d);
+// words 4-byte words between buf and snapshot slot (*step / k) % nslots of slots (stride in words);
+// to_slot 1: buf -> slot, 0: slot -> buf
+int launch_slot_copy(void *buf, void *slots, size_t words, size_t stride_words, const int *step, int k, int nslots,
+                     int to_slot);
 
 }  // namespace klrtm

@@ -670,6 +670,44 @@ int launch_fill(float *p, float v, size_t n) {
   hipLaunchKernelGGL(fill_kernel, dim3(nb), dim3(256), 0, stream(), p, v, n);
   return check_launch("fill_kernel");
 }
+// Device step counter of captured step sequences: set once before a run of replays, advanced by every
+// captured step (single-lane kernels, so that they can sit in a graph).
+__global__ void step_set_kernel(int *s, int v) { *s = v; }
+__global__ void step_add_kernel(int *s, int d) { *s = *s + d; }
+int launch_step_set(int *s, int v) {
+  hipLaunchKernelGGL(step_set_kernel, dim3(1), dim3(1), 0, stream(), s, v);
+  return check_launch("step_set_kernel");
+}
+int launch_step_add(int *s, int d) {
+  hipLaunchKernelGGL(step_add_kernel, dim3(1), dim3(1), 0, stream(), s, d);
+  return check_launch("step_add_kernel");
+}
+// copy n 4-byte words between a fixed buffer and snapshot slot itsav = (*step / k) % nslots of a slot
+// array with the given stride (in words): to_slot 1 writes the slot, 0 reads it
+template<class V>
+__global__ void slot_copy_kernel(V *__restrict__ buf, V *__restrict__ slots, size_t n, size_t stride, const int *step,
+                                 int k, int nslots, int to_slot) {
+  const size_t off = stride * (size_t)((*step / k) % nslots);
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    if(to_slot) slots[off + i] = buf[i];
+    else buf[i] = slots[off + i];
+  }
+}
+int launch_slot_copy(void *buf, void *slots, size_t words, size_t stride_words, const int *step, int k, int nslots,
+                     int to_slot) {
+  if(words == 0) return KLRTM_OK;
+  LaunchTimer tm("slot_copy");
+  if(words % 4 == 0 && stride_words % 4 == 0) {
+    const unsigned nb = (unsigned)std::min<size_t>((words / 4 + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(slot_copy_kernel<uint4>, dim3(nb), dim3(256), 0, stream(), (uint4 *)buf, (uint4 *)slots, words / 4,
+                       stride_words / 4, step, k, nslots, to_slot);
+  } else {
+    const unsigned nb = (unsigned)std::min<size_t>((words + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(slot_copy_kernel<uint32_t>, dim3(nb), dim3(256), 0, stream(), (uint32_t *)buf, (uint32_t *)slots,
+                       words, stride_words, step, k, nslots, to_slot);
+  }
+  return check_launch("slot_copy_kernel");
+}
 int launch_update2nd(float *w0, const float *w1, size_t n) {
   if(n == 0) return KLRTM_OK;
   LaunchTimer tm("update2nd");

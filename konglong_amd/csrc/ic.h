@@ -11,7 +11,23 @@ struct IcArgs {
   const float *taper_z;  // device, ntaper_z values
   int ntaper_z;
   float *out_s = nullptr, *out_r = nullptr;  // set: write the tapered S and R here instead of correlating
+  // set (captured step sequences): the snapshot index itsav = (*step / k) % nslots is read on the device,
+  // which picks the time taper taper_tab[itsav] (itsav < ntaper_t, else the z taper) and, for the
+  // launchers that take them, the snapshot snap_base + itsav*snap_stride and its block scales
+  const int *step = nullptr;
+  int k = 1, nslots = 1, ntaper_t = 0;
+  const float *taper_tab = nullptr;
+  const char *snap_base = nullptr;
+  size_t snap_stride = 0;
+  const float *scales_base = nullptr;
+  size_t nscales = 0;
 };
+// the time taper factor of an imaging launch (< 0: use the z taper)
+__device__ __forceinline__ float ic_taper_t(const IcArgs &a) {
+  if(!a.step) return a.taper_t;
+  const int itsav = (*a.step / a.k) % a.nslots;
+  return itsav < a.ntaper_t ? a.taper_tab[itsav] : -1.f;
+}
 struct Correlator;
 int correlator_run(Correlator &C, float *image, float *img_fwi, float *img_raw, double *image64, float *src, float *rec,
                    int option);

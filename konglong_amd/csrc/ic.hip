@@ -28,6 +28,12 @@ __global__ void ic_fused_kernel(const IcArgs a, const void *__restrict__ snap, c
   // one y plane per blockIdx.y; a plane (nx*nz < 2^31 cells) is indexed with 32-bit arithmetic
   const unsigned nxz = (unsigned)a.nx * (unsigned)a.nz;
   const int iy = blockIdx.y;
+  const float taper_t = ic_taper_t(a);
+  if(a.step) {  // the snapshot of this imaging step, picked on the device
+    const int itsav = (*a.step / a.k) % a.nslots;
+    snap = a.snap_base + a.snap_stride * (size_t)itsav;
+    if(a.scales_base) scales = a.scales_base + a.nscales * (size_t)itsav;
+  }
   for(unsigned j = blockIdx.x * blockDim.x + threadIdx.x; j < nxz; j += gridDim.x * blockDim.x) {
     const size_t i = (size_t)iy * nxz + j;
     const int ix = (int)(j / (unsigned)a.nz), iz = (int)(j - (unsigned)ix * (unsigned)a.nz);
@@ -42,7 +48,7 @@ __global__ void ic_fused_kernel(const IcArgs a, const void *__restrict__ snap, c
       S = half_decode<FMT>(((const uint16_t *)snap)[i]) * scales[block_index(ix, iy, iz, a.nx, a.nz)];
     }
     S = edge_taper(a.T, S, ix, iy, iz, a.nx, a.ny, a.nz);
-    if(a.taper_t >= 0.f) S = S * a.taper_t;
+    if(taper_t >= 0.f) S = S * taper_t;
     else if(iz < a.ntaper_z) S = S * a.taper_z[iz];
     float R = w[i];
     if(RQ) {

@@ -33,6 +33,16 @@ const char *get_error();
 
 // ---- stream and launch timing -------------------------------------------------
 hipStream_t stream();
+// set while this thread captures a step sequence into a graph (the launch timers skip their events)
+void set_capturing(bool on);
+bool capturing();
+// Time index of a per-step launch: t = *step + off when step is set (a device counter that captured
+// step sequences advance on the device), else off itself
+struct StepRef {
+  const int *step = nullptr;
+  int off = 0;
+};
+__device__ __forceinline__ int step_index(const StepRef &r) { return r.step ? *r.step + r.off : r.off; }
 // Event pair around one named launch when the timer is enabled; the pair is
 // resolved lazily by klrtm_timer_read.
 struct LaunchTimer {

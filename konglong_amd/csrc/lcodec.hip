@@ -137,6 +137,7 @@ __global__ void __launch_bounds__(256)
   float *T = tiles[wv];
   const long b0 = (long)blockIdx.x * 32;  // the grid holds exactly the tiles: every wave has work
   const int nt0 = wv * 4, nt1 = nt0 + 4;
+  const float taper_t = MODE == LC_IC || MODE == LC_IC64 ? ic_taper_t(a) : -1.f;
   const long b = b0 + (lane & 31);
   bf16x8 za[KS];
 #pragma unroll
@@ -273,7 +274,7 @@ __global__ void __launch_bounds__(256)
           if(ok) vol[i] = xv[e];
         } else {
           float S = edge_taper(a.T, xv[e], ix, iy, iz, g.nx, g.ny, g.nz);
-          if(a.taper_t >= 0.f) S = S * a.taper_t;
+          if(taper_t >= 0.f) S = S * taper_t;
           else if(iz < a.ntaper_z) S = S * a.taper_z[iz];
           const float R = edge_taper(a.T, wv[e], ix, iy, iz, g.nx, g.ny, g.nz);
           const float d = S * R;

@@ -96,6 +96,33 @@ struct Propagator {
   }
   double t_fwd = 0, t_bwd = 0, t_ic = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // Captured step sequences (klrtm_propagator_set_graph): the stepping API replays groups of G steps
+  // (the snapshot interval, doubled when odd, so the wavefield pointers come back to where they were)
+  // as hipGraphs.  Every captured launch that depends on the time index reads the device counter
+  // d_step, which each captured step advances; snapshots go through a fixed staging buffer to the slot
+  // that the counter picks, and the imaging kernels pick their slot and time taper the same way.
+  bool use_graph = false;
+  int *d_step = nullptr;
+  char *d_stage = nullptr;  // one snapshot slot (payload, then block scales)
+  float *d_taper_tab = nullptr;
+  struct Graph {
+    int dir, flag;
+    float *w0;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+  long graph_replays = 0;
+  int graph_group() const {
+    const int k = std::max(1, c.snap_every);
+    return k % 2 ? 2 * k : k;
+  }
+  int graph_ready();
+  int fwd_captured(int j, bool store);
+  int bwd_captured(int j, bool img);
+  int store_captured();
+  int image_captured();
+  int replay(int dir, int flag, int it_start, int nrep);
+  void drop_graphs();
   ~Propagator();
   int alloc_snapshots();
   int reset();
@@ -106,7 +133,14 @@ struct Propagator {
   int image(int itsav);
 };
 
+void Propagator::drop_graphs() {
+  for(auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+  graphs.clear();
+}
+
 Propagator::~Propagator() {
+  drop_graphs();
+  for(void *p : {(void *)d_step, (void *)d_stage, (void *)d_taper_tab}) (void)hipFree(p);
   if(cache) snapcache_destroy(cache);
   if(corr) correlator_destroy(corr);
   for(void *p : {(void *)d_fwi, (void *)d_raw, (void *)d_ics, (void *)d_icr}) (void)hipFree(p);
@@ -266,6 +300,139 @@ int Propagator::image(int itsav) {
   }
   if(c.snap_format == KLRTM_SNAP_LEARNED) return lcodec_image(*lc, a, snap, w0, d_img, d_ill, i64, l64);
   return launch_ic(c.snap_format, rq, a, snap, sc, w0, d_smax, d_img, d_ill, i64, l64);
+}
+
+// ---- captured step sequences ---------------------------------------------------------------------
+
+int Propagator::graph_ready() {
+  KL_REQUIRE(!use_cache, KLRTM_ESTATE, "captured step sequences keep the snapshots in HBM (no snapshot cache)");
+  KL_REQUIRE(ic_option == KLRTM_IC_IMAGE1, KLRTM_ESTATE, "captured step sequences image with the zero-lag condition");
+  KL_REQUIRE(!fk, KLRTM_ESTATE, "captured step sequences inject traces, not the FK surface wavefield");
+  int rc;
+  if((rc = alloc_snapshots())) return rc;
+  // the engine's work volumes and PML state of the first-derivative form, allocated on first use
+  // otherwise (no allocation may happen while a sequence is captured)
+  if(!(eng->engine == KLRTM_ENGINE_FD && c.model == KLRTM_ISO && !eng->rho) &&
+     ((rc = eng->ensure_comps(eng->pml3d ? 3 : 1)) || (rc = eng->scratch())))
+    return rc;
+  if(!d_step) KL_HIP(hipMalloc(&d_step, sizeof(int)));
+  if(!d_stage && c.snap_format != KLRTM_SNAP_FP32) {
+    size_t bytes = snap_stride_bytes + ((sizeof(float) * nscales + 255) & ~(size_t)255);
+    KL_HIP(hipMalloc(&d_stage, bytes));
+  }
+  if(!d_taper_tab && !wtaperT.empty()) {
+    KL_HIP(hipMalloc(&d_taper_tab, sizeof(float) * wtaperT.size()));
+    KL_HIP(hipMemcpy(d_taper_tab, wtaperT.data(), sizeof(float) * wtaperT.size(), hipMemcpyHostToDevice));
+  }
+  return KLRTM_OK;
+}
+
+// Propagator::store for the snapshot slot the device counter picks
+int Propagator::store_captured() {
+  const int k = std::max(1, c.snap_every);
+  const size_t stride_w = snap_stride_bytes / 4;
+  if(c.snap_format == KLRTM_SNAP_FP32) return launch_slot_copy(w0, d_snap, n, stride_w, d_step, k, ntsav, 1);
+  int rc;
+  float *stage_scales = (float *)(d_stage + snap_stride_bytes);
+  switch(c.snap_format) {
+  case KLRTM_SNAP_C16: rc = c16_encode(w0, (uint16_t *)d_stage, c.nx, c.ny, c.nz, d_smax); break;
+  case KLRTM_SNAP_LEARNED: rc = lcodec_encode(*lc, w0, d_stage, c.nx, c.ny, c.nz); break;
+  default: rc = blockq_encode(w0, (uint16_t *)d_stage, stage_scales, c.nx, c.ny, c.nz, c.snap_format);
+  }
+  if(rc || (rc = launch_slot_copy(d_stage, d_snap, stride_w, stride_w, d_step, k, ntsav, 1))) return rc;
+  if(nscales) rc = launch_slot_copy(stage_scales, d_scales, nscales, nscales, d_step, k, ntsav, 1);
+  return rc;
+}
+
+// Propagator::image for the snapshot slot and time taper the device counter picks
+int Propagator::image_captured() {
+  IcArgs a;
+  a.nx = c.nx, a.ny = c.ny, a.nz = c.nz;
+  a.T = make_edge_taper();
+  a.taper_t = -1.f;
+  a.taper_z = d_taper_z;
+  a.ntaper_z = d_taper_z ? c.ntaper_z : 0;
+  a.step = d_step;
+  a.k = std::max(1, c.snap_every);
+  a.nslots = ntsav;
+  a.ntaper_t = (int)wtaperT.size();
+  a.taper_tab = d_taper_tab;
+  a.snap_base = d_snap;
+  a.snap_stride = snap_stride_bytes;
+  a.scales_base = nscales ? d_scales : nullptr;
+  a.nscales = nscales;
+  const bool rq = c.rec_through_codec && c.snap_format == KLRTM_SNAP_C16;
+  int rc;
+  if(rq && (rc = c16_slice_max(w0, c.nx, c.ny, c.nz, d_smax))) return rc;
+  double *i64 = c.ic_fp32_only ? nullptr : d_img64, *l64 = c.ic_fp32_only ? nullptr : d_ill64;
+  if(c.snap_format == KLRTM_SNAP_LEARNED) {
+    const size_t stride_w = snap_stride_bytes / 4;
+    if((rc = launch_slot_copy(d_stage, d_snap, stride_w, stride_w, d_step, a.k, ntsav, 0))) return rc;
+    return lcodec_image(*lc, a, d_stage, w0, d_img, d_ill, i64, l64);
+  }
+  return launch_ic(c.snap_format, rq, a, nullptr, nullptr, w0, d_smax, d_img, d_ill, i64, l64);
+}
+
+// step j of a captured forward group that starts on the snapshot cadence (the counter holds it)
+int Propagator::fwd_captured(int j, bool store_snap) {
+  int rc;
+  StepRef ts;
+  ts.step = d_step, ts.off = -1;
+  if((rc = step(0)) || (rc = receiver_apply_at(*src, w0, ts, d_vel, 1.0f, src_mirror())) || (rc = surface())) return rc;
+  if(store_snap && j % std::max(1, c.snap_every) == 0 && (rc = store_captured())) return rc;
+  std::swap(w0, w1);
+  return launch_step_add(d_step, 1);
+}
+
+// step j of a captured backward group that starts one step above the snapshot cadence
+int Propagator::bwd_captured(int j, bool img) {
+  const int k = std::max(1, c.snap_every);
+  int rc;
+  StepRef ts;
+  ts.step = d_step, ts.off = 1;
+  if((rc = step(0)) || (rc = receiver_apply_at(*rcv, w0, ts, d_vel, 1.0f, rcv_mirror())) || (rc = surface())) return rc;
+  if(img && j % k == k - 1 && (rc = image_captured())) return rc;
+  std::swap(w0, w1);
+  return launch_step_add(d_step, -1);
+}
+
+// nrep replays of the group graph of this direction, flag and wavefield pointers, the counter starting
+// at it_start; the graph is captured on first use
+int Propagator::replay(int dir, int flag, int it_start, int nrep) {
+  if(nrep <= 0) return KLRTM_OK;
+  hipGraphExec_t exec = nullptr;
+  for(auto &g : graphs)
+    if(g.dir == dir && g.flag == flag && g.w0 == w0) exec = g.exec;
+  if(!exec) {
+    const int G = graph_group();
+    float *s0 = w0, *s1 = w1;
+    const int q0 = iq0;
+    hipStream_t s = stream();
+    KL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    set_capturing(true);
+    int rc = KLRTM_OK;
+    for(int j = 0; j < G && !rc; j++) rc = dir > 0 ? fwd_captured(j, flag != 0) : bwd_captured(j, flag != 0);
+    set_capturing(false);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    w0 = s0, w1 = s1, iq0 = q0;  // G is even: the captured group leaves them as they were
+    if(!rc && e != hipSuccess) {
+      set_error("step capture failed: %s", hipGetErrorString(e));
+      rc = KLRTM_EDEVICE;
+    }
+    if(!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      set_error("graph instantiation failed");
+      rc = KLRTM_EDEVICE;
+    }
+    if(graph) (void)hipGraphDestroy(graph);
+    if(rc) return rc;
+    graphs.push_back({dir, flag, w0, exec});
+  }
+  int rc = launch_step_set(d_step, it_start);
+  if(rc) return rc;
+  for(int r = 0; r < nrep; r++) KL_HIP(hipGraphLaunch(exec, stream()));
+  graph_replays += nrep;
+  return KLRTM_OK;
 }
 
 // the post-loop steps of ImagingCondition::genImage for INVERSION2 (bgprtm/ImagingCondition.cpp:427-430)
@@ -524,20 +691,44 @@ int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, d
   return KLRTM_OK;
 }
 
+static int forward_one(Propagator &P, int it, int store_snapshots) {
+  const int k = std::max(1, P.c.snap_every);
+  int rc;
+  if((rc = P.step(it))) return rc;
+  if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f, P.src_mirror())) || (rc = P.surface())) return rc;
+  if(store_snapshots && it % k == 0 && (rc = P.store((it / k) % std::max(1, P.ntsav)))) return rc;
+  std::swap(P.w0, P.w1);
+  return KLRTM_OK;
+}
+
+static int backward_one(Propagator &P, int it, int image) {
+  const int k = std::max(1, P.c.snap_every);
+  int rc;
+  if((rc = P.step(it))) return rc;
+  if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f, P.rcv_mirror())) || (rc = P.surface())) return rc;
+  if(image && it >= 0 && it % k == 0 && (rc = P.image((it / k) % std::max(1, P.ntsav)))) return rc;
+  std::swap(P.w0, P.w1);
+  return KLRTM_OK;
+}
+
+// With captured sequences on, the steps up to the first group boundary and after the last whole group
+// run one by one; the whole groups in between are graph replays (same launches, same order).
 int propagator_forward_steps(Propagator &P, int it0, int nsteps, int store_snapshots) {
   KL_REQUIRE(!P.use_cache, KLRTM_ESTATE, "the stepping API keeps its snapshots in HBM (no snapshot cache)");
   int rc;
   if(store_snapshots && (rc = P.alloc_snapshots())) return rc;
-  const int k = std::max(1, P.c.snap_every);
-  for(int it = it0; it < it0 + nsteps; it++) {
-    if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.src, P.w0, it - 1, P.d_vel, 1.0f, P.src_mirror())) || (rc = P.surface())) return rc;
-    if(store_snapshots && it % k == 0) {
-      int s = (it / k) % std::max(1, P.ntsav);
-      if((rc = P.store(s))) return rc;
-    }
-    std::swap(P.w0, P.w1);
+  const int k = std::max(1, P.c.snap_every), end = it0 + nsteps;
+  int it = it0;
+  if(P.use_graph && it >= 0) {
+    if((rc = P.graph_ready())) return rc;
+    for(; it < end && it % k; it++)
+      if((rc = forward_one(P, it, store_snapshots))) return rc;
+    const int G = P.graph_group(), nrep = (end - it) / G;
+    if((rc = P.replay(1, store_snapshots != 0, it, nrep))) return rc;
+    it += nrep * G;
   }
+  for(; it < end; it++)
+    if((rc = forward_one(P, it, store_snapshots))) return rc;
   return KLRTM_OK;
 }
 
@@ -546,13 +737,18 @@ int propagator_backward_steps(Propagator &P, int it0, int nsteps, int image) {
   KL_REQUIRE(!P.use_cache, KLRTM_ESTATE, "the stepping API keeps its snapshots in HBM (no snapshot cache)");
   int rc;
   if(image && (rc = P.alloc_snapshots())) return rc;
-  const int k = std::max(1, P.c.snap_every);
-  for(int it = it0; it > it0 - nsteps; it--) {
-    if((rc = P.step(it))) return rc;
-    if((rc = receiver_apply(*P.rcv, P.w0, it + 1, P.d_vel, 1.0f, P.rcv_mirror())) || (rc = P.surface())) return rc;
-    if(image && it >= 0 && it % k == 0 && (rc = P.image((it / k) % std::max(1, P.ntsav)))) return rc;
-    std::swap(P.w0, P.w1);
+  const int k = std::max(1, P.c.snap_every), end = it0 - nsteps;
+  int it = it0;
+  if(P.use_graph && end >= -1) {
+    if((rc = P.graph_ready())) return rc;
+    for(; it > end && it % k != k - 1; it--)
+      if((rc = backward_one(P, it, image))) return rc;
+    const int G = P.graph_group(), nrep = (it - end) / G;
+    if((rc = P.replay(-1, image != 0, it, nrep))) return rc;
+    it -= nrep * G;
   }
+  for(; it > end; it--)
+    if((rc = backward_one(P, it, image))) return rc;
   return KLRTM_OK;
 }
 
@@ -576,10 +772,12 @@ int klrtm_propagator_destroy(klrtm_propagator *p) {
 int klrtm_propagator_set_model(klrtm_propagator *p, const float *vel, const float *eps, const float *del, const float *pjx,
                                const float *pjy) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   return propagator_set_model(*reinterpret_cast<Propagator *>(p), vel, eps, del, pjx, pjy);
 }
 int klrtm_propagator_set_rho(klrtm_propagator *p, const float *rho) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   Propagator &P = *reinterpret_cast<Propagator *>(p);
   P.use_rho = rho != nullptr;
   int rc;
@@ -591,10 +789,12 @@ int klrtm_propagator_set_rho(klrtm_propagator *p, const float *rho) {
 }
 int klrtm_propagator_set_q(klrtm_propagator *p, const float *invq, int order, const float *coeffs) {
   KL_REQUIRE(p && order >= 0, KLRTM_EINVAL, "bad argument");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   return propagator_set_q(*reinterpret_cast<Propagator *>(p), invq, order, coeffs);
 }
 int klrtm_propagator_set_ghost(klrtm_propagator *p, int iz_mirror, int source_side, int receiver_side) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   Propagator &P = *reinterpret_cast<Propagator *>(p);
   const bool any = source_side || receiver_side;
   KL_REQUIRE(!any || (iz_mirror >= 0 && 2 * iz_mirror < P.c.nz), KLRTM_EINVAL,
@@ -605,6 +805,7 @@ int klrtm_propagator_set_ghost(klrtm_propagator *p, int iz_mirror, int source_si
 }
 int klrtm_propagator_set_snapshot_cache(klrtm_propagator *p, const char *path, double mem_limit_gb, int keep) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   Propagator &P = *reinterpret_cast<Propagator *>(p);
   KL_REQUIRE(!P.d_snap, KLRTM_ESTATE, "snapshots are already allocated: set the cache before the first rtm call");
   KL_REQUIRE(mem_limit_gb >= 0, KLRTM_EINVAL, "negative memory limit");
@@ -625,12 +826,14 @@ int klrtm_propagator_snapshot_cache_info(klrtm_propagator *p, int *in_memory, si
 }
 int klrtm_propagator_set_source_wavelet(klrtm_propagator *p, const float *wav) {
   KL_REQUIRE(p && wav, KLRTM_EINVAL, "null argument");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   Propagator &P = *reinterpret_cast<Propagator *>(p);
   P.wavelet.assign(wav, wav + P.c.nt);
   return P.set_source_traces();
 }
 int klrtm_propagator_set_source_array(klrtm_propagator *p, int nsrc, const float *xyz, const float *delay_s) {
   KL_REQUIRE(p && xyz && delay_s && nsrc > 0, KLRTM_EINVAL, "bad source array");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   Propagator &P = *reinterpret_cast<Propagator *>(p);
   const klrtm_shot_config &c = P.c;
   Receiver *r = nullptr;
@@ -644,6 +847,7 @@ int klrtm_propagator_set_source_array(klrtm_propagator *p, int nsrc, const float
 }
 int klrtm_propagator_set_receivers(klrtm_propagator *p, const float *xyz) {
   KL_REQUIRE(p && xyz, KLRTM_EINVAL, "null argument");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   return propagator_set_receivers(*reinterpret_cast<Propagator *>(p), xyz);
 }
 int klrtm_propagator_model(klrtm_propagator *p, float *record) {
@@ -679,6 +883,7 @@ int klrtm_propagator_backward_steps(klrtm_propagator *p, int it0, int nsteps, in
 }
 int klrtm_propagator_set_record(klrtm_propagator *p, const float *record) {
   KL_REQUIRE(p && record, KLRTM_EINVAL, "null argument");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   Propagator &P = *reinterpret_cast<Propagator *>(p);
   KL_REQUIRE(P.rcv, KLRTM_ESTATE, "receivers not set");
   int rc = receiver_set_traces(*P.rcv, record);
@@ -707,11 +912,13 @@ int klrtm_propagator_images(klrtm_propagator *p, float **img, float **ill, doubl
 
 int klrtm_propagator_set_imaging(klrtm_propagator *p, int option, float kmin) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   return propagator_set_imaging(*reinterpret_cast<Propagator *>(p), option, kmin);
 }
 int klrtm_propagator_set_fk(klrtm_propagator *p, int enable, float vsurf, int trueamp, float z_depth, int deghost,
                             float flcut, float flpass, float zerofreq_threshold) {
   KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   Propagator &P = *reinterpret_cast<Propagator *>(p);
   if(!enable) {
     P.fk = false;
@@ -738,8 +945,24 @@ int klrtm_propagator_clear_images(klrtm_propagator *p) {
   return reinterpret_cast<Propagator *>(p)->clear_images();
 }
 
+int klrtm_propagator_set_graph(klrtm_propagator *p, int on) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  P.drop_graphs();
+  P.use_graph = on != 0;
+  return KLRTM_OK;
+}
+int klrtm_propagator_graph_stats(klrtm_propagator *p, int *ngraphs, long *replays) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  if(ngraphs) *ngraphs = (int)P.graphs.size();
+  if(replays) *replays = P.graph_replays;
+  return KLRTM_OK;
+}
+
 int klrtm_propagator_set_lcodec(klrtm_propagator *p, const klrtm_lcodec *c) {
   KL_REQUIRE(p && c, KLRTM_EINVAL, "null argument");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
   Propagator &P = *reinterpret_cast<Propagator *>(p);
   KL_REQUIRE(!P.d_snap || P.lc, KLRTM_ESTATE, "snapshots are already allocated for another format");
   const LCodec *lc = reinterpret_cast<const LCodec *>(c);

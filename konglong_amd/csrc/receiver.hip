@@ -146,11 +146,15 @@ struct GroupArgs {
 
 // wy[j][k] = (val_r*scaler)*ky_r[j] for the receiver r at CSR position k of
 // the group's nr: the y factor of every term, formed once per step.
+// The trace sample is t = step_index(ts) of traces [nt][nrt]; outside [0, nt) the factors are zero, so
+// the plane is zero and the apply kernels leave the wavefield untouched (as a skipped injection does).
 __global__ void inject_wy_kernel(int n, int nr, const int *__restrict__ bin_rcv, const float *__restrict__ cky,
-                                 const float *__restrict__ val, float scaler, float *__restrict__ wy) {
+                                 const float *__restrict__ traces, int nrt, int nt, const StepRef ts, float scaler,
+                                 float *__restrict__ wy) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if(e >= n) return;
-  wy[e] = (val[bin_rcv[e % nr]] * scaler) * cky[e];
+  const int t = step_index(ts);
+  wy[e] = t >= 0 && t < nt ? (traces[(size_t)t * nrt + bin_rcv[e % nr]] * scaler) * cky[e] : 0.f;
 }
 
 // T[iy][ix] = sum_r (val_r*ky_r)*kx_r over receivers covering (ix,iy).
@@ -395,12 +399,13 @@ int receiver_fk_spread(Receiver &R, float dx, float dy, float dt, int ntfft, flo
   if(!rc) rc = launch_fill(data, 0.f, ncell * ntfft);
   // apply_src_coefxy for every time sample (bgprtm/Receiver.cpp:509-529): the plane of each group
   for(int it = 0; it < nt && !rc; it++) {
-    const float *val = R.d_traces + (size_t)it * R.nr;
+    StepRef ts;
+    ts.off = it;
     for(auto &g : R.groups) {
       GroupArgs a = gargs(R, g);
       const int cells = g.bw * g.bh, nwy = g.nr * R.L;
-      hipLaunchKernelGGL(inject_wy_kernel, dim3(div_up(nwy, 256)), dim3(256), 0, stream(), nwy, g.nr, g.bin_rcv, g.cky, val,
-                         1.0f, R.d_wy);
+      hipLaunchKernelGGL(inject_wy_kernel, dim3(div_up(nwy, 256)), dim3(256), 0, stream(), nwy, g.nr, g.bin_rcv, g.cky,
+                         R.d_traces, R.nr, R.nt, ts, 1.0f, R.d_wy);
       hipLaunchKernelGGL(inject_plane_kernel<IG>, dim3(div_up((long)cells * IG, 256)), dim3(256), 0, stream(), a,
                          g.bin_start, g.corg, g.ckx, R.d_wy, R.d_plane);
       hipLaunchKernelGGL(fk_gather_kernel, dim3(div_up(cells, 256)), dim3(256), 0, stream(), R.d_plane, data, g.bx0, g.by0,
@@ -463,9 +468,19 @@ int receiver_fk_spread(Receiver &R, float dx, float dy, float dt, int ntfft, flo
   return rc;
 }
 
+int receiver_apply_at(Receiver &R, float *w0, const StepRef &ts, const float *vel, float scaler0, int iz_mirror);
+
 int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scaler0, int iz_mirror) {
   if(it < 0 || it > R.nt - 1) return KLRTM_OK;  // bgprtm/Receiver.cpp:1205
+  StepRef ts;
+  ts.off = it;
+  return receiver_apply_at(R, w0, ts, vel, scaler0, iz_mirror);
+}
+
+int receiver_apply_at(Receiver &R, float *w0, const StepRef &ts, const float *vel, float scaler0, int iz_mirror) {
   KL_REQUIRE(R.d_traces || R.d_fk, KLRTM_ESTATE, "receiver traces not set");
+  KL_REQUIRE(!ts.step || !R.d_fk, KLRTM_EINVAL, "the FK surface wavefield is injected at host-given time indices only");
+  const int it = ts.off;
   const float scaler = scaler0 * R.dt_dxyz;
   if(R.d_fk) {  // the FK surface wavefield (bgprtm/Receiver.cpp:1216-1225)
     LaunchTimer tm("inject");
@@ -478,14 +493,13 @@ int receiver_apply(Receiver &R, float *w0, int it, const float *vel, float scale
                          R.fk_lz, R.nx, R.ny, R.nz, iz_mirror, w0, vel);
     return check_launch("fk_apply_kernel");
   }
-  const float *val = R.d_traces + (size_t)it * R.nr;
   LaunchTimer tm("inject");
   for(auto &g : R.groups) {
     GroupArgs a = gargs(R, g);
     int cells = g.bw * g.bh;
     const int nw = g.nr * R.L;
-    hipLaunchKernelGGL(inject_wy_kernel, dim3(div_up(nw, 256)), dim3(256), 0, stream(), nw, g.nr, g.bin_rcv, g.cky, val,
-                       scaler, R.d_wy);
+    hipLaunchKernelGGL(inject_wy_kernel, dim3(div_up(nw, 256)), dim3(256), 0, stream(), nw, g.nr, g.bin_rcv, g.cky,
+                       R.d_traces, R.nr, R.nt, ts, scaler, R.d_wy);
     hipLaunchKernelGGL(inject_plane_kernel<IG>, dim3(div_up((long)cells * IG, 256)), dim3(256), 0, stream(), a,
                        g.bin_start, g.corg, g.ckx, R.d_wy, R.d_plane);
     long tot = (long)cells * g.lz;

@@ -17,6 +17,11 @@ void set_error(const char *fmt, ...) {
 }
 const char *get_error() { return g_err.c_str(); }
 
+// ---- stream capture: launches are recorded into a graph, so the launch timers stay off ----
+static thread_local bool t_capturing = false;
+void set_capturing(bool on) { t_capturing = on; }
+bool capturing() { return t_capturing; }
+
 // ---- streams ------------------------------------------------------------------
 static thread_local hipStream_t t_user_stream = nullptr;
 static thread_local bool t_user_stream_set = false;
@@ -76,6 +81,7 @@ static void drain_pending() {
 }
 
 LaunchTimer::LaunchTimer(const char *kernel_name) : name(kernel_name) {
+  if(t_capturing) return;
   std::lock_guard<std::mutex> lk(g_tmu);
   if(!g_timer_on) return;
   if(g_pending.size() > 4096) drain_pending();

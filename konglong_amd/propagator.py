@@ -139,6 +139,16 @@ class Propagator:
     def backward_steps(self, it0, nsteps, image=True):
         check(lib().klrtm_propagator_backward_steps(self.h, it0, nsteps, int(image)))
 
+    def set_graph(self, on=True):
+        """Run whole snapshot-interval groups of forward_steps / backward_steps as replays of captured
+        hipGraphs (a device step counter supplies the time index); results are unchanged."""
+        check(lib().klrtm_propagator_set_graph(self.h, int(on)))
+
+    def graph_stats(self):
+        g, r = C.c_int(), C.c_long()
+        check(lib().klrtm_propagator_graph_stats(self.h, C.byref(g), C.byref(r)))
+        return g.value, r.value
+
     def set_record(self, record):
         r = np.ascontiguousarray(record, np.float32)
         assert r.shape == (self.cfg.nrcv, self.cfg.nt)
