@@ -325,6 +325,80 @@ def batched_vs_plain(nshots, nt, batch, seed=505):
             "tiles": {f"{k[0]}x{k[1]}": list(v[0]) if v[0] else None for k, v in tuner.cache.items()}}
 
 
+def convae_vs_plain(train_models=8, train_steps=20000, batch=64, floor_frac=0.05, seed=606):
+    """Convolutional-autoencoder snapshots inside a whole 400^3 ISO FD8 RTM shot, against FP32, the
+    reference's COMPRESS_16 (the WaveFieldCompress fallback) and the linear learned codec.
+
+    Training (untimed in the shot rates): Adam (lr 1e-3, batches of `batch` 32^3 blocks, `train_steps`
+    steps) on 4 forward snapshots of one shot on each of `train_models` seeded random velocity models at
+    256^3 (dx 12.5 m, 1500-5000 m/s).  Then one whole shot of the iso-fd8-400 workload (2 s record,
+    receivers on every surface cell, snapshots every 4 steps, all kept in HBM) per snapshot format:
+    shots/hour, snapshot bytes against FP32, image NRMSE against the FP32-snapshot image (the NRMSE
+    against an FP64 oracle of the same kind of codec is measured by the autoencoder GPU test at 64^3)."""
+    from konglong_amd import convae, learned, models
+    from konglong_amd._lib import ENGINE_FD, ISO, SNAP_FP32, SNAP_C16, SNAP_LEARNED, SNAP_CONVAE, lib, check
+    from konglong_amd.propagator import Propagator, shot_config
+    w, eng, mdl, vel, eps, dl, dt = build_case("iso-fd8-400")
+    ny, nx, nz = w["shape"]
+    d, nb, fp, k = w["d"], w["nbnd"], w["fpeak"], w["snap_every"]
+    nt = int(round(w["record_s"] / dt))
+    t0 = time.perf_counter()
+    bank = convae.snapshot_bank((256, 256, 256), [seed + i for i in range(train_models)], d, dt, 1000, per_model=4,
+                                nbnd=nb, f_peak=fp, vmin=1500.0, vmax=5000.0, sigma=20.0)
+    t_bank = time.perf_counter() - t0
+    net = convae.ConvAutoencoder(seed)
+    t0 = time.perf_counter()
+    l0, l1 = net.fit(bank, train_steps, batch, 1e-3, floor_frac, seed)
+    _lib_sync()
+    t_train = time.perf_counter() - t0
+    bank.free()
+    print(f"[bench] autoencoder trained: loss {l0:.4f} -> {l1:.4f} in {t_train:.1f} s", file=sys.stderr, flush=True)
+    lin, _ = learned.train((256, 256, 256), [seed + 100, seed + 101], d, dt, 300, k, nl=32, nbnd=nb, f_peak=fp,
+                           vmin=1500.0, vmax=5000.0, sigma=20.0)
+    rcv = models.surface_receivers(nx, ny, nb + 2)
+    common = dict(nbnd=nb, vmax=float(vel.max()), src_xyz=((nx - 1) / 2, (ny - 1) / 2, nb + 2.0), f_peak=fp,
+                  spread=10, nrcv=rcv.shape[0], snap_every=k, tmute_frac=0.1, ic_fp32_only=1)
+    rec = None
+    res = {}
+    imgs = {}
+    for name, fmt in (("fp32", SNAP_FP32), ("compress16", SNAP_C16), ("linear", SNAP_LEARNED),
+                      ("autoencoder", SNAP_CONVAE)):
+        p = Propagator(shot_config(nx, ny, nz, d, dt, nt, ENGINE_FD, 8, ISO, snap_format=fmt, **common))
+        if fmt == SNAP_LEARNED:
+            check(lib().klrtm_propagator_set_lcodec(p.h, lin.h))
+        elif fmt == SNAP_CONVAE:
+            check(lib().klrtm_propagator_set_convae(p.h, net.h, floor_frac))
+        p.set_model(vel)
+        p.set_receivers(rcv)
+        if rec is None:
+            rec = p.model()
+        p.forward_steps(0, 1, True)  # allocates the snapshot store outside the timed shot
+        t0 = time.perf_counter()
+        imgs[name] = p.rtm(rec, want=("image",))["image"]
+        t_shot = time.perf_counter() - t0
+        code = {SNAP_FP32: 4 * nx * ny * nz, SNAP_C16: 2 * nx * ny * nz + 4 * ny,
+                SNAP_LEARNED: lin.code_bytes((ny, nx, nz)),
+                SNAP_CONVAE: convae.ConvAutoencoder.code_bytes((ny, nx, nz))}[fmt]
+        res[name] = {"shots_per_hour": round(3600.0 / t_shot, 2), "shot_s": round(t_shot, 3),
+                     "snapshot_bytes_vs_fp32": round(code / (4.0 * nx * ny * nz), 4)}
+        p.close()
+        print(f"[bench] convae leg: {name} shot {t_shot:.2f} s", file=sys.stderr, flush=True)
+    ref = imgs["fp32"].astype(np.float64)
+    for name in res:
+        res[name]["image_nrmse_vs_fp32"] = float(np.linalg.norm(imgs[name] - ref) / np.linalg.norm(ref))
+    net.close()
+    lin.close()
+    return {"workload": "iso-fd8-400 whole shot, autoencoder vs FP32 / COMPRESS_16 / linear snapshots",
+            "nt": nt, "snap_every": k, "train_models_256": train_models, "train_steps": train_steps, "batch": batch,
+            "train_s": round(t_train, 2), "bank_s": round(t_bank, 2), "loss_first": l0, "loss_last": l1,
+            "formats": res}
+
+
+def _lib_sync():
+    from konglong_amd import _lib
+    _lib.synchronize()
+
+
 def tti_full_shot(nt=3000, seed=303):
     """One whole TTI FD16 RTM shot on the 600 x 600 x 400 grid (dx 12.5 m, PML 40, 15 Hz, snapshots every
     3 steps), with learned snapshots held in HBM for the whole shot (163 GB for the 1000 snapshots; FP16
@@ -489,6 +563,8 @@ def main():
     ap.add_argument("--legs", default="iso-fd8-400,tti-fd16-600",
                     help="further workloads timed after the headline (comma list; empty: none)")
     ap.add_argument("--leg-steps", type=int, default=10)
+    ap.add_argument("--convae-steps", type=int, default=20000, help="autoencoder training steps (0: skip that leg)")
+    ap.add_argument("--convae-models", type=int, default=8, help="seeded 256^3 models the autoencoder trains on")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -524,6 +600,11 @@ def main():
                                                           "config", "kernels")}
             except Exception as e:  # the headline line is still printed
                 out["legs"][name] = {"error": f"{type(e).__name__}: {e}"}
+    if args.convae_steps > 0:
+        try:
+            out["learned_convae"] = convae_vs_plain(args.convae_models, args.convae_steps)
+        except Exception as e:  # the headline line is still printed
+            out["learned_convae"] = {"error": f"{type(e).__name__}: {e}"}
     if args.batched_shots > 0:
         try:
             out["batched"] = batched_vs_plain(args.batched_shots, args.batched_steps, args.batch)

@@ -36,7 +36,14 @@ enum { KLRTM_ENGINE_CFD = 4, KLRTM_ENGINE_FD = 7 };
 enum { KLRTM_ISO = 1, KLRTM_VTI = 2, KLRTM_TTI = 3 };
 /* snapshot storage formats (wf_compress, bgprtm/WaveFieldCompress.h:6, plus block-scaled half types
  * and the learned codec) */
-enum { KLRTM_SNAP_FP32 = 0, KLRTM_SNAP_C16 = 1, KLRTM_SNAP_FP16 = 16, KLRTM_SNAP_BF16 = 17, KLRTM_SNAP_LEARNED = 32 };
+enum {
+  KLRTM_SNAP_FP32 = 0,
+  KLRTM_SNAP_C16 = 1,
+  KLRTM_SNAP_FP16 = 16,
+  KLRTM_SNAP_BF16 = 17,
+  KLRTM_SNAP_LEARNED = 32,
+  KLRTM_SNAP_CONVAE = 33
+};
 
 /* ---- errors, device, memory ------------------------------------------------ */
 const char *klrtm_last_error(void);
@@ -274,6 +281,37 @@ int klrtm_lcodec_correlate(const klrtm_lcodec *c, const void *code, const float 
                            float taper_t, const float *taper_z, int ntaper_z, float *image, float *illum,
                            double *image64, double *illum64);
 
+/* ---- convolutional autoencoder snapshot codec (replaces WaveFieldCompress in the snapshot cache,
+ * bgprtm/WaveFieldCompress.cpp:25-279 as called from bgprtm/Propagator.cpp:1233-1243) ----
+ * A 32^3 block (order (ly*32+lx)*32+lz, cells outside the volume zero), divided by its scale
+ * s = rms(block) + floor_frac*rms(volume), is encoded by four 3x3x3 convolutions (1->16 stride 2,
+ * 16->32 stride 2, 32->32, 32->4, GELU between) to a 4 x 8^3 latent and decoded by four more (4->32,
+ * nearest x2 + 32->32, nearest x2 + 32->16, 16->1, GELU between); zero padding at every grid edge.
+ * Tensors are channels-last [block][(iy*D+ix)*D+iz][channel]; every convolution runs as an implicit
+ * GEMM on BF16 matrix cores with FP32 accumulation.  Parameters (klrtm_convae_param_count floats,
+ * host arrays): per layer the weights [cout][cin][27] (tap (dy+1)*9+(dx+1)*3+dz+1) then the biases.
+ * klrtm_convae_forward: latent [nb][512][4] and reconstruction [nb][32^3] (device, either may be NULL)
+ * of nb normalised blocks x [nb][32^3] (device).  klrtm_convae_loss_grad: the mean squared
+ * reconstruction error of x and its gradient (host [param_count]).  klrtm_convae_train: Adam (lr,
+ * beta 0.9 / 0.999) on batches of random blocks from a bank of nsnap device snapshots [nsnap][ny][nx][nz]
+ * (each block scaled by the rule above with its snapshot's rms); reports the first and last batch loss.
+ * A code (klrtm_convae_code_bytes) is the BF16 latents [nblk][2048] (256-byte aligned) then the FP32
+ * block scales [nblk], block index (by*ceil(nx/32) + bx)*ceil(nz/32) + bz. */
+typedef struct klrtm_convae klrtm_convae;
+int klrtm_convae_create(klrtm_convae **c, unsigned seed);
+int klrtm_convae_destroy(klrtm_convae *c);
+size_t klrtm_convae_param_count(void);
+int klrtm_convae_get_params(klrtm_convae *c, float *params);
+int klrtm_convae_set_params(klrtm_convae *c, const float *params);
+int klrtm_convae_forward(klrtm_convae *c, const float *x, int nb, float *latent, float *recon);
+int klrtm_convae_loss_grad(klrtm_convae *c, const float *x, int nb, float *grad, float *loss);
+int klrtm_convae_adam_step(klrtm_convae *c, float lr);
+int klrtm_convae_train(klrtm_convae *c, const float *bank, int nsnap, int nx, int ny, int nz, float floor_frac,
+                       int steps, int batch, float lr, unsigned seed, float *loss_first, float *loss_last);
+size_t klrtm_convae_code_bytes(int nx, int ny, int nz);
+int klrtm_convae_encode(klrtm_convae *c, const float *vol, int nx, int ny, int nz, float floor_frac, void *code);
+int klrtm_convae_decode(klrtm_convae *c, const void *code, float *vol, int nx, int ny, int nz);
+
 /* ---- ImagingCorrelator::crossone + ImagingCondition::calIllum (zero lag) ----- */
 /* image += S*R, illum += S*S with S = src*taper as in ImagingCondition::applyTape:
  * taper_t >= 0 multiplies every cell by taper_t (time taper wtaperT[it]); taper_t < 0
@@ -401,6 +439,11 @@ int klrtm_propagator_set_record(klrtm_propagator *p, const float *record_host);
 int klrtm_propagator_reset(klrtm_propagator *p);
 /* learned codec of KLRTM_SNAP_LEARNED snapshots (borrowed: it must outlive the propagator's use) */
 int klrtm_propagator_set_lcodec(klrtm_propagator *p, const klrtm_lcodec *c);
+/* convolutional autoencoder of KLRTM_SNAP_CONVAE snapshots (borrowed: it must outlive the propagator's
+ * use; several propagators may share one, each keeps its own work buffers); floor_frac is the block
+ * scale's share of the volume rms (klrtm_convae_encode).  The imaging step decodes the snapshot into a
+ * work volume and correlates it as FP32. */
+int klrtm_propagator_set_convae(klrtm_propagator *p, klrtm_convae *c, float floor_frac);
 /* Captured step sequences for the stepping API (the time loop of bgprtm/Propagator.cpp:1210-1255 as
  * hipGraph replays).  With on != 0, klrtm_propagator_forward_steps / backward_steps run every whole
  * group of G steps (G = snap_every, doubled when odd; forward groups start on the snapshot cadence,

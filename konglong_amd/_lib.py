@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libkonglong_rtm.so")
 KLRTM_OK = 0
 ENGINE_CFD, ENGINE_FD = 4, 7
 ISO, VTI, TTI = 1, 2, 3
-SNAP_FP32, SNAP_C16, SNAP_FP16, SNAP_BF16, SNAP_LEARNED = 0, 1, 16, 17, 32
+SNAP_FP32, SNAP_C16, SNAP_FP16, SNAP_BF16, SNAP_LEARNED, SNAP_CONVAE = 0, 1, 16, 17, 32, 33
 IC_IMAGE1, IC_INVERSION, IC_INVERSION2, IC_ALLIN3 = 0, 1, 2, 3
 
 _vp, _i, _f, _d, _sz, _l = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t, C.c_long
@@ -142,6 +142,19 @@ _SIGS = {
     "klrtm_propagator_set_record": (_i, [_vp, _pf]),
     "klrtm_propagator_set_lcodec": (_i, [_vp, _vp]),
     "klrtm_propagator_set_graph": (_i, [_vp, _i]),
+    "klrtm_propagator_set_convae": (_i, [_vp, _vp, C.c_float]),
+    "klrtm_convae_create": (_i, [C.POINTER(_vp), C.c_uint]),
+    "klrtm_convae_destroy": (_i, [_vp]),
+    "klrtm_convae_param_count": (_sz, []),
+    "klrtm_convae_get_params": (_i, [_vp, _vp]),
+    "klrtm_convae_set_params": (_i, [_vp, _vp]),
+    "klrtm_convae_forward": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "klrtm_convae_loss_grad": (_i, [_vp, _vp, _i, _vp, _pf]),
+    "klrtm_convae_adam_step": (_i, [_vp, C.c_float]),
+    "klrtm_convae_train": (_i, [_vp, _vp, _i, _i, _i, _i, C.c_float, _i, _i, C.c_float, C.c_uint, _pf, _pf]),
+    "klrtm_convae_code_bytes": (_sz, [_i, _i, _i]),
+    "klrtm_convae_encode": (_i, [_vp, _vp, _i, _i, _i, C.c_float, _vp]),
+    "klrtm_convae_decode": (_i, [_vp, _vp, _vp, _i, _i, _i]),
     "klrtm_propagator_graph_stats": (_i, [_vp, _pi, _pl]),
     "klrtm_propagator_kernel": (_i, [_vp, C.POINTER(_vp)]),
     "klrtm_propagator_images": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),

@@ -18,7 +18,7 @@ import time
 
 import numpy as np
 
-from ._lib import check, lib, ENGINE_FD, ISO, SNAP_FP32, SNAP_LEARNED
+from ._lib import check, lib, ENGINE_FD, ISO, SNAP_FP32, SNAP_LEARNED, SNAP_CONVAE
 from .propagator import Propagator, shot_config
 
 
@@ -119,7 +119,7 @@ class BatchRTM:
     shot_config keywords shared by all shots (d, dt, nt, engine, order, model, nbnd, ...)."""
 
     def __init__(self, out_shape, velocity, common, batch=4, snap_format=SNAP_FP32, lcodec=None, tuner=None,
-                 graph=False, snap_budget_gb=200.0):
+                 graph=False, snap_budget_gb=200.0, convae=None, cae_floor=0.05):
         from ._lib import DeviceArray
         self.nyF, self.nxF, self.nz = out_shape
         self.velocity = velocity
@@ -127,6 +127,7 @@ class BatchRTM:
         self.batch = max(1, int(batch))
         self.snap_format = snap_format
         self.lcodec = lcodec
+        self.convae, self.cae_floor = convae, cae_floor
         self.tuner = tuner
         self.graph = bool(graph)
         self.snap_budget = snap_budget_gb * 1e9
@@ -149,6 +150,8 @@ class BatchRTM:
         p = Propagator(cfg)
         if self.snap_format == SNAP_LEARNED:
             check(lib().klrtm_propagator_set_lcodec(p.h, self.lcodec.h))
+        elif self.snap_format == SNAP_CONVAE:
+            check(lib().klrtm_propagator_set_convae(p.h, self.convae.h, C.c_float(self.cae_floor)))
         p.set_model(vel)
         p.set_receivers(shot.rcv_xyz)
         return p
@@ -159,6 +162,8 @@ class BatchRTM:
         shape = (shot.ny, shot.nx, self.nz)
         if self.snap_format == SNAP_LEARNED:
             per = self.lcodec.code_bytes(shape)
+        elif self.snap_format == SNAP_CONVAE:
+            per = int(lib().klrtm_convae_code_bytes(shot.nx, shot.ny, self.nz))
         elif self.snap_format == SNAP_FP32:
             per = 4 * shot.nx * shot.ny * self.nz
         else:

@@ -29,7 +29,7 @@ __global__ void ic_fused_kernel(const IcArgs a, const void *__restrict__ snap, c
   const unsigned nxz = (unsigned)a.nx * (unsigned)a.nz;
   const int iy = blockIdx.y;
   const float taper_t = ic_taper_t(a);
-  if(a.step) {  // the snapshot of this imaging step, picked on the device
+  if(a.step && a.snap_base) {  // the snapshot of this imaging step, picked on the device
     const int itsav = (*a.step / a.k) % a.nslots;
     snap = a.snap_base + a.snap_stride * (size_t)itsav;
     if(a.scales_base) scales = a.scales_base + a.nscales * (size_t)itsav;

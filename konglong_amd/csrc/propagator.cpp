@@ -14,6 +14,7 @@
 #include "codec.h"
 #include "ic.h"
 #include "lcodec.h"
+#include "convae.h"
 #include "fd_kernels.h"
 #include "surface.h"
 #include "qatten.h"
@@ -44,6 +45,12 @@ struct Propagator {
   double *d_img64 = nullptr, *d_ill64 = nullptr;
   float *d_taper_z = nullptr;
   const LCodec *lc = nullptr;  // KLRTM_SNAP_LEARNED codec (borrowed)
+  // KLRTM_SNAP_CONVAE: the autoencoder (borrowed), this propagator's work buffers, the block-scale
+  // floor, and the volume a snapshot is decoded into for imaging
+  const ConvAE *cae = nullptr;
+  CaeWork *cae_work = nullptr;
+  float cae_floor = 0.05f;
+  float *d_dec = nullptr;
   // Q attenuation: raw 1/Q as uploaded, 1/Q after Vel2's correction, memory variables wq[2][order][n]
   float *d_invq_raw = nullptr, *d_invq = nullptr, *d_q[2] = {nullptr, nullptr};
   int q_order = 0, iq0 = 0;
@@ -140,6 +147,8 @@ void Propagator::drop_graphs() {
 
 Propagator::~Propagator() {
   drop_graphs();
+  delete cae_work;
+  (void)hipFree(d_dec);
   for(void *p : {(void *)d_step, (void *)d_stage, (void *)d_taper_tab}) (void)hipFree(p);
   if(cache) snapcache_destroy(cache);
   if(corr) correlator_destroy(corr);
@@ -178,6 +187,15 @@ int Propagator::alloc_snapshots() {
     snap_stride_bytes = n * sizeof(uint16_t);
     nscales = blockq_nscales(c.nx, c.ny, c.nz);
     break;
+  case KLRTM_SNAP_CONVAE: {
+    KL_REQUIRE(cae, KLRTM_ESTATE, "autoencoder snapshots need a codec (klrtm_propagator_set_convae)");
+    snap_stride_bytes = cae_code_bytes(c.nx, c.ny, c.nz);
+    if(!cae_work) cae_work = new CaeWork();
+    int rc = cae_reserve(*cae_work, CAE_CHUNK, false);
+    if(rc) return rc;
+    if(!d_dec && (rc = dmalloc(&d_dec, n))) return rc;
+    break;
+  }
   case KLRTM_SNAP_LEARNED:
     KL_REQUIRE(lc, KLRTM_ESTATE, "learned snapshots need a codec (klrtm_propagator_set_lcodec)");
     KL_REQUIRE(c.nx % 8 == 0 && c.ny % 8 == 0 && c.nz % 8 == 0, KLRTM_EINVAL,
@@ -262,6 +280,7 @@ int Propagator::store(int itsav) {
     break;
   case KLRTM_SNAP_C16: rc = c16_encode(w0, (uint16_t *)dst, c.nx, c.ny, c.nz, d_smax); break;
   case KLRTM_SNAP_LEARNED: rc = lcodec_encode(*lc, w0, dst, c.nx, c.ny, c.nz); break;
+  case KLRTM_SNAP_CONVAE: rc = cae_encode(*cae, *cae_work, w0, c.nx, c.ny, c.nz, cae_floor, dst); break;
   default: rc = blockq_encode(w0, (uint16_t *)dst, scales(itsav), c.nx, c.ny, c.nz, c.snap_format);
   }
   if(rc || !use_cache) return rc;
@@ -288,18 +307,24 @@ int Propagator::image(int itsav) {
   const char *snap = slot(itsav);
   double *i64 = c.ic_fp32_only ? nullptr : d_img64, *l64 = c.ic_fp32_only ? nullptr : d_ill64;
   const float *sc = nscales ? scales(itsav) : nullptr;
+  int fmt = c.snap_format;
+  if(fmt == KLRTM_SNAP_CONVAE) {  // decoded into a work volume, then imaged as FP32
+    int rc = cae_decode(*cae, *cae_work, snap, d_dec, c.nx, c.ny, c.nz);
+    if(rc) return rc;
+    snap = (const char *)d_dec, fmt = KLRTM_SNAP_FP32;
+  }
   if(ic_option != KLRTM_IC_IMAGE1) {
     // ImagingCondition::stackImageInterp with an FFT option: the tapered S and R (and the illumination),
     // then ImagingCorrelator::run (bgprtm/ImagingCondition.cpp:104-141); the FP64 image takes the same
     // terms, except under INVERSION2, whose post-loop cos term is FP32 only
     a.out_s = d_ics, a.out_r = d_icr;
-    int rc = launch_ic(c.snap_format, rq, a, snap, sc, w0, d_smax, nullptr, d_ill, nullptr, l64);
+    int rc = launch_ic(fmt, rq, a, snap, sc, w0, d_smax, nullptr, d_ill, nullptr, l64);
     if(rc) return rc;
     return correlator_run(*corr, d_img, d_fwi, d_raw, ic_option == KLRTM_IC_INVERSION2 ? nullptr : i64, d_ics, d_icr,
                           ic_option);
   }
-  if(c.snap_format == KLRTM_SNAP_LEARNED) return lcodec_image(*lc, a, snap, w0, d_img, d_ill, i64, l64);
-  return launch_ic(c.snap_format, rq, a, snap, sc, w0, d_smax, d_img, d_ill, i64, l64);
+  if(fmt == KLRTM_SNAP_LEARNED) return lcodec_image(*lc, a, snap, w0, d_img, d_ill, i64, l64);
+  return launch_ic(fmt, rq, a, snap, sc, w0, d_smax, d_img, d_ill, i64, l64);
 }
 
 // ---- captured step sequences ---------------------------------------------------------------------
@@ -337,6 +362,7 @@ int Propagator::store_captured() {
   switch(c.snap_format) {
   case KLRTM_SNAP_C16: rc = c16_encode(w0, (uint16_t *)d_stage, c.nx, c.ny, c.nz, d_smax); break;
   case KLRTM_SNAP_LEARNED: rc = lcodec_encode(*lc, w0, d_stage, c.nx, c.ny, c.nz); break;
+  case KLRTM_SNAP_CONVAE: rc = cae_encode(*cae, *cae_work, w0, c.nx, c.ny, c.nz, cae_floor, d_stage); break;
   default: rc = blockq_encode(w0, (uint16_t *)d_stage, stage_scales, c.nx, c.ny, c.nz, c.snap_format);
   }
   if(rc || (rc = launch_slot_copy(d_stage, d_snap, stride_w, stride_w, d_step, k, ntsav, 1))) return rc;
@@ -365,10 +391,13 @@ int Propagator::image_captured() {
   int rc;
   if(rq && (rc = c16_slice_max(w0, c.nx, c.ny, c.nz, d_smax))) return rc;
   double *i64 = c.ic_fp32_only ? nullptr : d_img64, *l64 = c.ic_fp32_only ? nullptr : d_ill64;
-  if(c.snap_format == KLRTM_SNAP_LEARNED) {
+  if(c.snap_format == KLRTM_SNAP_LEARNED || c.snap_format == KLRTM_SNAP_CONVAE) {
     const size_t stride_w = snap_stride_bytes / 4;
     if((rc = launch_slot_copy(d_stage, d_snap, stride_w, stride_w, d_step, a.k, ntsav, 0))) return rc;
-    return lcodec_image(*lc, a, d_stage, w0, d_img, d_ill, i64, l64);
+    if(c.snap_format == KLRTM_SNAP_LEARNED) return lcodec_image(*lc, a, d_stage, w0, d_img, d_ill, i64, l64);
+    if((rc = cae_decode(*cae, *cae_work, d_stage, d_dec, c.nx, c.ny, c.nz))) return rc;
+    a.snap_base = nullptr, a.scales_base = nullptr;  // the decoded volume, not a slot
+    return launch_ic(KLRTM_SNAP_FP32, false, a, d_dec, nullptr, w0, d_smax, d_img, d_ill, i64, l64);
   }
   return launch_ic(c.snap_format, rq, a, nullptr, nullptr, w0, d_smax, d_img, d_ill, i64, l64);
 }
@@ -957,6 +986,16 @@ int klrtm_propagator_graph_stats(klrtm_propagator *p, int *ngraphs, long *replay
   Propagator &P = *reinterpret_cast<Propagator *>(p);
   if(ngraphs) *ngraphs = (int)P.graphs.size();
   if(replays) *replays = P.graph_replays;
+  return KLRTM_OK;
+}
+
+int klrtm_propagator_set_convae(klrtm_propagator *p, klrtm_convae *c, float floor_frac) {
+  KL_REQUIRE(p && c && floor_frac >= 0.f, KLRTM_EINVAL, "bad argument");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  KL_REQUIRE(!P.d_snap || P.cae, KLRTM_ESTATE, "snapshots are already allocated for another format");
+  P.cae = reinterpret_cast<const ConvAE *>(c);
+  P.cae_floor = floor_frac;
   return KLRTM_OK;
 }
 

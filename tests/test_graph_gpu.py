@@ -10,8 +10,9 @@ import numpy as np
 import pytest
 
 from konglong_amd import models
-from konglong_amd._lib import (ENGINE_CFD, ENGINE_FD, ISO, VTI, SNAP_BF16, SNAP_C16, SNAP_FP32, SNAP_LEARNED, check,
-                               lib)
+from konglong_amd._lib import (ENGINE_CFD, ENGINE_FD, ISO, VTI, SNAP_BF16, SNAP_C16, SNAP_CONVAE, SNAP_FP32,
+                               SNAP_LEARNED, check, lib)
+from konglong_amd.convae import ConvAutoencoder
 from konglong_amd.learned import LearnedCompressor
 from konglong_amd.propagator import Propagator, shot_config
 from tests.helpers import smooth_field
@@ -27,8 +28,9 @@ CASES = [
     (ENGINE_FD, 8, ISO, SNAP_LEARNED, 3, 0),
     (ENGINE_FD, 8, ISO, SNAP_BF16, 5, 0),
     (ENGINE_CFD, 10, VTI, SNAP_C16, 2, 1),
+    (ENGINE_FD, 8, ISO, SNAP_CONVAE, 4, 0),
 ]
-IDS = ["iso-fp32-k4", "iso-learned-k3", "iso-bf16-k5", "vti-cfd-c16-k2-rq"]
+IDS = ["iso-fp32-k4", "iso-learned-k3", "iso-bf16-k5", "vti-cfd-c16-k2-rq", "iso-convae-k4"]
 
 
 def _images(p):
@@ -57,6 +59,8 @@ def test_captured_steps_match_eager(klrtm, case):
     if fmt == SNAP_LEARNED:
         Q, _ = np.linalg.qr(np.random.default_rng(5).standard_normal((512, 32)))
         codec = LearnedCompressor(Q.astype(np.float32))
+    elif fmt == SNAP_CONVAE:
+        codec = ConvAutoencoder(seed=3)  # untrained weights: the captured and eager runs must still agree
     cfg = shot_config(nx, ny, nz, D, DT, NT, engine, order, model, nbnd=NB, vmax=2500.0,
                       src_xyz=((nx - 1) / 2, (ny - 1) / 2, NB + 2.0), f_peak=25.0, nrcv=rcv.shape[0], snap_every=k,
                       snap_format=fmt, rec_through_codec=rq, tmute_frac=0.2)
@@ -79,8 +83,10 @@ def test_captured_steps_match_eager(klrtm, case):
     runs = []
     for graph in (False, True):
         p = Propagator(cfg)
-        if codec is not None:
+        if fmt == SNAP_LEARNED:
             check(lib().klrtm_propagator_set_lcodec(p.h, codec.h))
+        elif fmt == SNAP_CONVAE:
+            check(lib().klrtm_propagator_set_convae(p.h, codec.h, 0.05))
         p.set_model(vel, eps, dl)
         p.set_receivers(rcv)
         p.set_record(record)
