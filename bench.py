@@ -355,6 +355,39 @@ def convae_vs_plain(train_models=8, train_steps=20000, batch=64, floor_frac=0.05
     print(f"[bench] autoencoder trained: loss {l0:.4f} -> {l1:.4f} in {t_train:.1f} s", file=sys.stderr, flush=True)
     lin, _ = learned.train((256, 256, 256), [seed + 100, seed + 101], d, dt, 300, k, nl=32, nbnd=nb, f_peak=fp,
                            vmin=1500.0, vmax=5000.0, sigma=20.0)
+    # equal bytes: snapshot reconstruction error on held-out models, the autoencoder (4100 B per 32^3
+    # block) against the linear code without its int8 residual (nl = 32 BF16 values per 8^3 block:
+    # 4096 B per 32^3 of latents, residual scales zeroed) and with it
+    held = convae.snapshot_bank((256, 256, 256), [seed + 200, seed + 201], d, dt, 1000, per_model=2, nbnd=nb,
+                                f_peak=fp, vmin=1500.0, vmax=5000.0, sigma=20.0)
+    shp = (256, 256, 256)
+    nvol = 256 ** 3
+    from konglong_amd._lib import DeviceArray
+    cae_code = DeviceArray((convae.ConvAutoencoder.code_bytes(shp),), np.uint8)
+    lin_code = lin.new_code(shp)
+    snap, dec = DeviceArray(shp), DeviceArray(shp)
+    nblk8 = (256 // 8) ** 3
+    zsz = (nblk8 * 32 * 2 + 255) // 256 * 256
+    errs = {"autoencoder": [], "linear_no_residual": [], "linear": []}
+    for i in range(held.shape[0]):
+        check(lib().klrtm_memcpy_d2d(snap.ptr, held.ptr + i * nvol * 4, nvol * 4))
+        ref_v = snap.download().astype(np.float64)
+        nref = np.linalg.norm(ref_v)
+        net.encode(snap, cae_code, floor_frac)
+        net.decode(cae_code, dec)
+        errs["autoencoder"].append(float(np.linalg.norm(dec.download() - ref_v) / nref))
+        lin.encode(snap, lin_code)
+        lin.decode(lin_code, dec)
+        errs["linear"].append(float(np.linalg.norm(dec.download() - ref_v) / nref))
+        check(lib().klrtm_memset(lin_code.ptr + zsz, 0, nblk8 * 4))  # residual scales -> 0
+        lin.decode(lin_code, dec)
+        errs["linear_no_residual"].append(float(np.linalg.norm(dec.download() - ref_v) / nref))
+    held.free()
+    equal_bytes = {"snapshots": len(errs["linear"]),
+                   "bytes_vs_fp32": {"autoencoder": round(convae.ConvAutoencoder.code_bytes(shp) / (4.0 * nvol), 4),
+                                     "linear_no_residual": round(nblk8 * 32 * 2 / (4.0 * nvol), 4),
+                                     "linear": round(lin.code_bytes(shp) / (4.0 * nvol), 4)},
+                   "mean_rel_l2": {k_: round(float(np.mean(v)), 5) for k_, v in errs.items()}}
     rcv = models.surface_receivers(nx, ny, nb + 2)
     common = dict(nbnd=nb, vmax=float(vel.max()), src_xyz=((nx - 1) / 2, (ny - 1) / 2, nb + 2.0), f_peak=fp,
                   spread=10, nrcv=rcv.shape[0], snap_every=k, tmute_frac=0.1, ic_fp32_only=1)
@@ -390,8 +423,9 @@ def convae_vs_plain(train_models=8, train_steps=20000, batch=64, floor_frac=0.05
     lin.close()
     return {"workload": "iso-fd8-400 whole shot, autoencoder vs FP32 / COMPRESS_16 / linear snapshots",
             "nt": nt, "snap_every": k, "train_models_256": train_models, "train_steps": train_steps, "batch": batch,
-            "train_s": round(t_train, 2), "bank_s": round(t_bank, 2), "loss_first": l0, "loss_last": l1,
-            "formats": res}
+            "train_s": round(t_train, 2), "train_ms_per_step": round(t_train / max(train_steps, 1) * 1e3, 3),
+            "bank_s": round(t_bank, 2), "loss_first": l0, "loss_last": l1,
+            "snapshot_reconstruction_equal_bytes": equal_bytes, "formats": res}
 
 
 def _lib_sync():
@@ -563,7 +597,8 @@ def main():
     ap.add_argument("--legs", default="iso-fd8-400,tti-fd16-600",
                     help="further workloads timed after the headline (comma list; empty: none)")
     ap.add_argument("--leg-steps", type=int, default=10)
-    ap.add_argument("--convae-steps", type=int, default=20000, help="autoencoder training steps (0: skip that leg)")
+    ap.add_argument("--convae-steps", type=int, default=4000,
+                    help="autoencoder training steps (0: skip that leg; the full schedule is 20000)")
     ap.add_argument("--convae-models", type=int, default=8, help="seeded 256^3 models the autoencoder trains on")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))

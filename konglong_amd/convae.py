@@ -114,3 +114,40 @@ def train(shape, seeds, d, dt, nt, steps, batch=64, lr=1e-3, per_model=4, floor_
     info = dict(snapshots=bank.shape[0], steps=steps, batch=batch, loss_first=l0, loss_last=l1)
     bank.free()
     return net, info
+
+
+def _bench(n=400, train_steps=20, reps=3):
+    """Times encode and decode of one n^3 snapshot (smooth random field, untrained weights) and
+    `train_steps` Adam steps on 64-block batches of a small random bank; used for the codec's kernel
+    profile."""
+    import time
+    from ._lib import synchronize
+    rng = np.random.default_rng(0)
+    net = ConvAutoencoder(5)
+    bank = DeviceArray.from_host(rng.standard_normal((2, 96, 96, 96)).astype(np.float32))
+    synchronize()
+    t0 = time.perf_counter()
+    net.fit(bank, train_steps, 64, 1e-3)
+    synchronize()
+    res = {"train_ms_per_step": (time.perf_counter() - t0) / train_steps * 1e3}
+    bank.free()
+    shape = (n, n, n)
+    vol = DeviceArray.from_host(rng.standard_normal(shape).astype(np.float32))
+    code = DeviceArray((ConvAutoencoder.code_bytes(shape),), np.uint8)
+    out = DeviceArray(shape)
+    for name, f in (("encode_ms", lambda: net.encode(vol, code)), ("decode_ms", lambda: net.decode(code, out))):
+        f()
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        synchronize()
+        res[name] = (time.perf_counter() - t0) / reps * 1e3
+    net.close()
+    return res
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+    print(json.dumps({k: round(v, 4) for k, v in _bench(int(sys.argv[1]) if len(sys.argv) > 1 else 400).items()}))
