@@ -151,8 +151,10 @@ __global__ void __launch_bounds__(256) cae_conv_kernel(const float *__restrict__
   }
 }
 
-// dW[co][ci][tap] += sum_{b, p} dZ[b][p][co] * X[b][q(p, tap)][ci] over a chunk of 256 positions of one
-// block: M = output channels, N = 32 (tap, ci) pairs n = tap*CIN + ci, K = positions.
+// dW[co][ci][tap] += sum_{b, p} dZ[b][p][co] * X[b][q(p, tap)][ci] over a chunk of CAE_WG_CHUNK positions of
+// one block: M = output channels, N = 32 (tap, ci) pairs n = tap*CIN + ci, K = positions.  Long chunks keep
+// the atomic additions of the partial sums few (they all land on the layer's small dW).
+constexpr int CAE_WG_CHUNK = 4096;
 template<int CIN, int MODE>
 __global__ void __launch_bounds__(256) cae_wgrad_kernel(const float *__restrict__ X, const float *__restrict__ dZ,
                                                         int cout, int lgo, int lgg, float *__restrict__ dW) {
@@ -160,7 +162,7 @@ __global__ void __launch_bounds__(256) cae_wgrad_kernel(const float *__restrict_
   const int Dout = 1 << lgo, Dg = 1 << lgg;
   const long Pout = 1L << (3 * lgo), Pg = 1L << (3 * lgg);
   const int n0 = (blockIdx.x * 4 + wv) * 32;
-  const long pc = (long)blockIdx.y * 256;
+  const long pc = (long)blockIdx.y * CAE_WG_CHUNK;
   const long b = blockIdx.z;
   if(n0 >= CIN * 27 || pc >= Pout) return;
   const int half = lane >> 5, m = lane & 31;
@@ -169,7 +171,7 @@ __global__ void __launch_bounds__(256) cae_wgrad_kernel(const float *__restrict_
   const float *Xb = X + (size_t)b * Pg * CIN;
   const float *Zb = dZ + (size_t)b * Pout * cout;
   cae_f32x16 acc = {};
-  const long pend = pc + 256 < Pout ? pc + 256 : Pout;
+  const long pend = pc + CAE_WG_CHUNK < Pout ? pc + CAE_WG_CHUNK : Pout;
   for(long k0 = pc; k0 < pend; k0 += 16) {
     cae_bf16x8 a, bb;
 #pragma unroll
@@ -605,7 +607,7 @@ static int layer_wgrad(ConvAE &c, int l, const float *in, const float *dz, int n
   const int lgo = lg2(kDout[l]), lgg = lg2(kDin[l]);
   const long P = 1L << (3 * lgo);
   const int ntile = (kCin[l] * 27 + 31) / 32;
-  const dim3 grid((unsigned)((ntile + 3) / 4), (unsigned)((P + 255) / 256), (unsigned)nb);
+  const dim3 grid((unsigned)((ntile + 3) / 4), (unsigned)((P + CAE_WG_CHUNK - 1) / CAE_WG_CHUNK), (unsigned)nb);
   float *dW = c.d_grad + c.w_off[l];
   const int mode = kKind[l] == K_DOWN ? G_DOWN : kKind[l] == K_UP ? G_UP : G_SAME;
 #define CAE_W(CI, M) \
