@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(512) iso_step_kernel(const float *__restrict__
 // ---------------------------------------------------------------------------
 // Gradient + PML round 1 + scalar operator.  Writes px, py (= pr), pz.
 template<int R, int MODEL, bool RHO>
-__global__ void __launch_bounds__(1024) grad_kernel(const float *__restrict__ u, float *__restrict__ px,
+__global__ void __launch_bounds__(512) grad_kernel(const float *__restrict__ u, float *__restrict__ px,
                                                     float *__restrict__ py, float *__restrict__ pz,
                                                     const float *__restrict__ eps, const float *__restrict__ del,
                                                     const float *__restrict__ pjx, const float *__restrict__ pjy,
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(1024) grad_kernel(const float *__restrict__ u,
 // (+ update2nd in MODE_STEP).  Dx needs x-neighbours of px, Dz z-neighbours of
 // pz, Dy y-neighbours of py: two LDS tiles and one register queue.
 template<int R, int MODE, bool RHO>
-__global__ void __launch_bounds__(1024) div_kernel(const float *__restrict__ px, const float *__restrict__ py,
+__global__ void __launch_bounds__(512) div_kernel(const float *__restrict__ px, const float *__restrict__ py,
                                                    const float *__restrict__ pz, const float *__restrict__ w1,
                                                    float *__restrict__ w0, const float *__restrict__ vel,
                                                    const IsoArgs a) {
