@@ -28,10 +28,14 @@ enum ImageType { STACK = 0, OFCIG = 1, DTCIG = 2, ADCIG = 3, DECON = 4 };
 namespace konglong {
 // global_pars keys ImagingCondition reads (bgprtm/ImagingCondition.cpp:38-70, 151-198, 393-396): the
 // imaging option (LowcutFilter), the water-depth taper length (grid->getIDz(waterDepth)), minFreq / vMin
+// and the imaging grid doubling (ic_double_z/x/y with highTapeZ / highTapeXY, :61-67, 169-178; the reference's
+// default doubles when the FD dispersion exceeds 0.6, here the default is 1)
 struct ImagingParams {
   int LowcutFilter = KLRTM_IC_IMAGE1;
   int ntaperZ = 0;
   float minFreq = 1.0f, vMin = 1492.f;
+  int ic_double_z = 1, ic_double_x = 1, ic_double_y = 1;
+  float highTapeZ = 0.9f, highTapeXY = 0.7f;
 };
 inline ImagingParams &imaging_params() {
   static ImagingParams p;
@@ -44,8 +48,11 @@ inline ImagingParams &imaging_params() {
 // decoded with WaveFieldCompress::compression (with its edge taper), the source one tapered in time
 // (wtaperT over ntaperT = rint((t1 - t0)/dt) steps) or by the water-depth taper, the illumination (AMP)
 // accumulated and ImagingCorrelator::run applied with the LowcutFilter option; INVERSION2 then takes its
-// cos and sin terms.  Zero lag only (MaxTimeLag 0, STACK), nDbl = 1.  The image (and AMP, FWI, RAW) come
-// back on the host in the returned Grid and the accessors; with destImages and an OutputGrid the image is
+// cos and sin terms.  Zero lag only (MaxTimeLag 0, STACK).  On a doubled imaging grid (IMAGE1 or INVERSION) each
+// step high-cuts S and R (applyFFTV), accumulates the high-cut S^2, forms the half-sample copies and correlates
+// them into sub-images (stackImageInterp, :104-141), which are interleaved at the end, the illumination
+// averaged onto the doubled grid (:415-446).  The image (and AMP, FWI, RAW) come back on the host in the
+// returned Grid (the doubled grid's sizes and spacings) and the accessors; with destImages and an OutputGrid the image is
 // added into destImages[0] where its columns fall on output nodes.  Cone mute, illumination damping and
 // resampling are the separate ImagingPost / Lagrange calls.
 class ImagingCondition {
@@ -66,6 +73,8 @@ public:
     const size_t n = (size_t)nx * ny * nz, ns = WaveFieldCompress::nshort_volume(nz, nx, ny);
     const konglong::ImagingParams &P = konglong::imaging_params();
     const int opt = P.LowcutFilter;
+    const int dz = P.ic_double_z, dx = nx > 1 ? P.ic_double_x : 1, dy = ny > 1 ? P.ic_double_y : 1;
+    if(dz * dx * dy > 1) return genImageDouble(grid, wfComp, dz, dx, dy, destImages, destGrid);
     // ImagingCondition::calTapes (bgprtm/ImagingCondition.cpp:180-198)
     std::vector<float> wtaperT;
     const int ntaperT = (int)nearbyintf((src_->t1 - src_->t0) / src_->dt);
@@ -102,9 +111,72 @@ public:
     }
     if(opt == KLRTM_IC_INVERSION2) corr->costerm(img.get()), corr->sinterm(fwi.get(), img.get());
     image_ = img.host(), amp_ = amp.host(), fwi_ = fwi.host(), raw_ = raw.host();
+    return finish(grid, 1, 1, 1, destImages, destGrid);
+  }
+  const std::vector<float> &image() const { return image_; }
+  const std::vector<float> &illum() const { return amp_; }
+  const std::vector<float> &fwi() const { return fwi_; }
+  const std::vector<float> &raw() const { return raw_; }
+
+private:
+  std::shared_ptr<Grid> genImageDouble(Grid *grid, WaveFieldCompress *wfComp, int dz, int dx, int dy,
+                                       std::vector<float *> &destImages, OutputGrid *destGrid) {
+    const int nx = src_->nx, ny = src_->ny, nz = src_->nz, nt = src_->nt;
+    const size_t n = (size_t)nx * ny * nz, ns = WaveFieldCompress::nshort_volume(nz, nx, ny);
+    const konglong::ImagingParams &P = konglong::imaging_params();
+    const int opt = P.LowcutFilter;
+    if(opt != KLRTM_IC_IMAGE1 && opt != KLRTM_IC_INVERSION)
+      throw konglong::Error(KLRTM_EINVAL, "konglong: the doubled imaging grid takes IMAGE1 and INVERSION");
+    ImagingCorrelator corr(nx, ny, nz, src_->dx, src_->dy, src_->dz, nx * dx, ny * dy, nz * dz, P.minFreq / P.vMin);
+    konglong::check(klrtm_correlator_set_double(corr.handle(), dz, dx, dy, P.highTapeZ, P.highTapeXY));
+    const int nl = corr.nloop();
+    std::vector<float> wtaperT;
+    const int ntaperT = (int)nearbyintf((src_->t1 - src_->t0) / src_->dt);
+    for(int i = 0; i < ntaperT; i++) wtaperT.push_back(0.5f * (1.0f - cosf((float)M_PI * float(i) / float(ntaperT))));
+    std::vector<float> wtaperZ(std::max(P.ntaperZ, 0));
+    for(int i = 0; i < P.ntaperZ; i++) wtaperZ[i] = 0.5f * (1.0f - cosf((float)M_PI * float(i) / float(P.ntaperZ)));
+    konglong::DeviceBuffer<float> dtz(std::max<size_t>(1, wtaperZ.size()));
+    if(!wtaperZ.empty()) dtz.upload(wtaperZ.data());
+    konglong::DeviceBuffer<ushort> cs(ns), cr(ns);
+    konglong::DeviceBuffer<float> S(n * nl), R(n * nl), img(n * nl), amp(n), fwi(n * nl), amp2(n * nl);
+    std::vector<ushort> hs(ns), hr(ns);
+    std::vector<float *> sb(nl), rb(nl);
+    for(int i = 0; i < nl; i++) sb[i] = S.get() + i * n, rb[i] = R.get() + i * n;
+    for(int it = 0; it < nt; it++) {
+      const off_t off = (off_t)it * (off_t)(ns * sizeof(ushort));
+      if(src_->pread(hs.data(), ns * sizeof(ushort), off) < 0 || rec_->pread(hr.data(), ns * sizeof(ushort), off) < 0)
+        throw konglong::Error(KLRTM_EINVAL, "konglong: short read from a snapshot cache");
+      cs.upload(hs.data()), cr.upload(hr.data());
+      wfComp->uncompress(cs.get(), S.get(), nx, ny, nz);
+      wfComp->uncompress(cr.get(), R.get(), nx, ny, nz);
+      konglong::check(klrtm_apply_tape(S.get(), nx, ny, nz, it < ntaperT ? wtaperT[it] : -1.0f, dtz.get(),
+                                       it < ntaperT ? 0 : P.ntaperZ));
+      // applyFFTV, calIllum on the high-cut S, interp (IMAGE1) and run
+      konglong::check(klrtm_correlator_hightape(corr.handle(), S.get()));
+      konglong::check(klrtm_correlator_hightape(corr.handle(), R.get()));
+      konglong::check(klrtm_ic_correlate(S.get(), S.get(), nx, ny, nz, 1.0f, nullptr, 0, amp.get(), nullptr, nullptr,
+                                         nullptr));
+      if(opt == KLRTM_IC_IMAGE1) {
+        konglong::check(klrtm_correlator_interp(corr.handle(), S.get()));
+        konglong::check(klrtm_correlator_interp(corr.handle(), R.get()));
+      }
+      corr.run(nullptr, img.get(), fwi.get(), nullptr, sb.data(), rb.data(), opt);
+    }
+    corr.interleave(img.get(), sb.data(), rb.data());
+    corr.interleave(fwi.get(), sb.data(), rb.data());
+    konglong::check(klrtm_correlator_interp_illum(corr.handle(), amp.get(), amp2.get()));
+    image_ = img.host(), amp_ = amp2.host(), fwi_ = fwi.host(), raw_.assign(n * nl, 0.f);
+    return finish(grid, dz, dx, dy, destImages, destGrid);
+  }
+
+  // the returned grid, and the image added into destImages[0] on the output nodes
+  std::shared_ptr<Grid> finish(Grid *grid, int dz, int dx, int dy, std::vector<float *> &destImages,
+                               OutputGrid *destGrid) {
+    const int nx = src_->nx * dx, ny = src_->ny * dy, nz = src_->nz * dz;
     auto out = std::make_shared<Grid>();
     if(grid) *out = *grid;
     out->nx = nx, out->ny = ny, out->nz = nz, out->image = image_;
+    out->dx /= dx, out->dy /= dy, out->dz /= dz;
     if(destGrid && !destImages.empty() && destImages[0]) {
       // columns on output nodes (the grid's origin and spacing place them), depth sampling shared
       for(int iy = 0; iy < ny; iy++)
@@ -122,12 +194,7 @@ public:
     }
     return out;
   }
-  const std::vector<float> &image() const { return image_; }
-  const std::vector<float> &illum() const { return amp_; }
-  const std::vector<float> &fwi() const { return fwi_; }
-  const std::vector<float> &raw() const { return raw_; }
 
-private:
   CacheFile *src_, *rec_;
   float *vel_ = nullptr;
   std::vector<float> image_, amp_, fwi_, raw_;

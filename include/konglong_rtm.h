@@ -346,6 +346,32 @@ int klrtm_correlator_sinterm(klrtm_correlator *c, float *imgfull, const float *i
  * (bgprtm/ImagingCorrelator.cpp:195-283), and kfilter in place (bgprtm/ImagingCorrelator.cpp:285-306) */
 int klrtm_correlator_unit_derivative(klrtm_correlator *c, const float *in, int axis, float *out);
 int klrtm_correlator_kfilter(klrtm_correlator *c, float *wave);
+/* Doubled imaging grid (ic_double_z/x/y, bgprtm/ImagingCondition.cpp:61-67; ImagingCorrelator with
+ * nx2 = dx*nx, ny2 = dy*ny, nz2 = dz*nz, bgprtm/ImagingCorrelator.cpp:58-75): factors 1 or 2; 3D grids double
+ * all three axes, 2D grids z or z and x (the sub-image layouts of interleave3d / interleave2d).
+ * run() then high-cuts src and rec along the doubled axes (ImagingCondition::applyFFTV, high_z along z,
+ * high_xy along x and y, as fractions of the Nyquist wavenumber; defaults 0.9 and 0.7), forms their
+ * half-sample shifted copies (ImagingCorrelator::interp) before every correlation, and accumulates copy i
+ * of src times copy i of rec into sub-image i: image, img_fwi and img_raw hold nloop volumes
+ * (klrtm_correlator_nloop: 1, 2, 4 or 8), image64 must be NULL; options IMAGE1 and INVERSION.
+ * klrtm_correlator_interleave (ImagingCorrelator::interleave) writes the sub-images onto the doubled
+ * grid [ny*dy][nx*dx][nz*dz] (distinct output).  The reference's SHIFTHALF / HIGHTAPE filters are
+ * libFFTV's (unvendored): here zero-padded 1D FFTs along the axis, the shift exp(i pi k / L) on bin k of
+ * length L, the high cut 1 up to high*Nyquist and a cosine to 0 at Nyquist. */
+int klrtm_correlator_set_double(klrtm_correlator *c, int dz, int dx, int dy, float high_z, float high_xy);
+int klrtm_correlator_nloop(klrtm_correlator *c, int *nloop);
+int klrtm_correlator_interleave(klrtm_correlator *c, const float *sub, float *out);
+/* the pieces: ImagingCorrelator::interp (bgprtm/ImagingCorrelator.cpp:515-524) on buf (nloop volumes,
+ * copy 0 given, copies 1.. written), ImagingCondition::applyFFTV (bgprtm/ImagingCondition.cpp:169-178) in
+ * place, and ImagingCondition::interpIllum (bgprtm/ImagingCondition.cpp:759-810): an illumination volume
+ * onto the doubled grid (distinct output) */
+int klrtm_correlator_interp(klrtm_correlator *c, float *buf);
+/* ImagingCorrelator::run on the reference's buffer contract (bgprtm/ImagingCorrelator.cpp:355-414), for callers
+ * that run applyFFTV (and for IMAGE1 the interp) themselves as ImagingCondition::stackImageInterp does: src and
+ * rec hold nloop volumes; IMAGE1 correlates the given copies, INVERSION takes copy 0 and overwrites all copies */
+int klrtm_correlator_run_copies(klrtm_correlator *c, float *image, float *img_fwi, float *src, float *rec, int option);
+int klrtm_correlator_hightape(klrtm_correlator *c, float *wave);
+int klrtm_correlator_interp_illum(klrtm_correlator *c, const float *ill, float *out);
 
 /* ---- whole-shot driver (Propagator::rtm / Propagator::mod) ------------------ */
 typedef struct {
@@ -460,7 +486,8 @@ int klrtm_propagator_set_graph(klrtm_propagator *p, int on);
 int klrtm_propagator_graph_stats(klrtm_propagator *p, int *ngraphs, long *replays);
 /* the propagator's engine (for klrtm_kernel_set_tile) */
 int klrtm_propagator_kernel(klrtm_propagator *p, klrtm_kernel **k);
-/* device pointers of the shot's image accumulators (FP32 image, illumination; FP64 image, illumination) */
+/* device pointers of the shot's image accumulators (FP32 image, illumination; FP64 image, illumination); on a
+ * doubled imaging grid img holds the nloop sub-images (klrtm_propagator_set_ic_double) */
 int klrtm_propagator_images(klrtm_propagator *p, float **img, float **ill, double **img64, double **ill64);
 /* zero the image accumulators (the step API accumulates into them; klrtm_propagator_rtm clears them itself) */
 int klrtm_propagator_clear_images(klrtm_propagator *p);
@@ -479,6 +506,17 @@ int klrtm_propagator_set_fk(klrtm_propagator *p, int enable, float vsurf, int tr
                             float flcut, float flpass, float zerofreq_threshold);
 int klrtm_propagator_ic_images(klrtm_propagator *p, float **fwi, float **raw);
 int klrtm_propagator_finish_images(klrtm_propagator *p);
+/* imaging on a doubled grid (ic_double_z/x/y, highTapeZ, highTapeXY; bgprtm/ImagingCondition.cpp:61-67,
+ * 104-141, 415-446): factors 1 or 2 (3D: all three or none; 2D: z, or z and x), high-cut fractions as
+ * klrtm_correlator_set_double.  Each imaging step high-cuts the tapered S and R, adds the high-cut S^2 to
+ * the illumination and correlates the half-sample shifted copies into nloop sub-images (FP32 only: the FP64
+ * accumulators stay zero); klrtm_propagator_finish_images (klrtm_propagator_rtm calls it) interleaves the
+ * image and the FWI image onto the doubled grid and averages the illumination onto it.  ic_double_images
+ * gives those doubled-grid volumes [ny*dy][nx*dx][nz*dz] and nloop; klrtm_propagator_rtm then takes
+ * img = ill = NULL (its host copies are propagation-grid sized).  IMAGE1 and INVERSION, FP32, COMPRESS_16,
+ * block-scaled or autoencoder snapshots; not under a captured step sequence.  1 1 1 restores the default. */
+int klrtm_propagator_set_ic_double(klrtm_propagator *p, int dz, int dx, int dy, float high_z, float high_xy);
+int klrtm_propagator_ic_double_images(klrtm_propagator *p, float **img, float **fwi, float **ill, int *nloop);
 
 /* ---- Propagator::apply_symmetry (bgprtm/Propagator.cpp:1857-1918) --------------
  * Free-surface mirror in place on a device volume: above z index iz0 each column takes

@@ -121,6 +121,13 @@ _SIGS = {
     "klrtm_correlator_sinterm": (_i, [_vp, _vp, _vp]),
     "klrtm_correlator_unit_derivative": (_i, [_vp, _vp, _i, _vp]),
     "klrtm_correlator_kfilter": (_i, [_vp, _vp]),
+    "klrtm_correlator_set_double": (_i, [_vp, _i, _i, _i, C.c_float, C.c_float]),
+    "klrtm_correlator_nloop": (_i, [_vp, _pi]),
+    "klrtm_correlator_interleave": (_i, [_vp, _vp, _vp]),
+    "klrtm_correlator_interp": (_i, [_vp, _vp]),
+    "klrtm_correlator_run_copies": (_i, [_vp, _vp, _vp, _vp, _vp, _i]),
+    "klrtm_correlator_hightape": (_i, [_vp, _vp]),
+    "klrtm_correlator_interp_illum": (_i, [_vp, _vp, _vp]),
     "klrtm_propagator_create": (_i, [C.POINTER(_vp), C.POINTER(ShotConfig)]),
     "klrtm_propagator_destroy": (_i, [_vp]),
     "klrtm_propagator_set_model": (_i, [_vp, _pf, _pf, _pf, _pf, _pf]),
@@ -165,6 +172,8 @@ _SIGS = {
     "klrtm_receiver_fk_data": (_i, [_vp, C.POINTER(_vp)]),
     "klrtm_propagator_ic_images": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
     "klrtm_propagator_finish_images": (_i, [_vp]),
+    "klrtm_propagator_set_ic_double": (_i, [_vp, _i, _i, _i, _f, _f]),
+    "klrtm_propagator_ic_double_images": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), _pi]),
     "klrtm_stack_image": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _i, _i, _i, _i, _i]),
     "klrtm_apply_symmetry": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i]),
     "klrtm_illum_damping": (_i, [_vp, _i, _i, _i, _i, _i, _i, _f, _f, _pd]),

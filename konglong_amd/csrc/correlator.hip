@@ -15,6 +15,16 @@
 //   allin3: imgRAW += |S R|, then inversion1;
 //   costerm: image <- |k|^2 image (after the time loop of inversion2);
 //   sinterm: imgfull <- 2 imgfull - 0.5 imgcos.
+// Doubled imaging grids (ic_double_z/x/y = 2, bgprtm/ImagingCondition.cpp:61-67, 104-141, 169-178,
+// 415-431; bgprtm/ImagingCorrelator.cpp:58-75, 353-524): S and R are high-cut along every doubled axis
+// (HIGHTAPE: highTapeZ 0.9 along z, highTapeXY 0.7 along x and y), and before every correlation their
+// half-sample shifted copies are formed along the doubled axes in the reference's order (z, x, y, zx,
+// zy, xy, zxy from copy 0); copy i of S times copy i of R accumulates into sub-image i, and interleave
+// places sub-image i at its parity on the doubled grid.  Both filters are libFFTV's, which is not
+// vendored: here they are defined on zero-padded 1D FFTs along the axis (hipFFT, length the smallest
+// 2^a 3^b 5^c >= n + 16): the shift multiplies bin k of length L by exp(i pi k / L) (band-limited
+// interpolation at x + 1/2, the Nyquist bin dropping out), the high cut by 1 up to h times the Nyquist
+// wavenumber and a cosine to 0 at it.
 // The reference runs derive's three filters from one spectrum buffer that its
 // 3D, ny2 == ny set-up aliases with the x and y outputs (srcBufx = srcBufy =
 // srcBuff, bgprtm/ImagingCorrelator.cpp:42-51), so its lapy reads the x
@@ -24,7 +34,9 @@
 #pragma clang fp contract(off)
 #include "klrtm_internal.h"
 #include "ic.h"
+#include "timeshift.h"
 #include <hipfft/hipfft.h>
+#include <algorithm>
 #include <cmath>
 
 namespace klrtm {
@@ -39,11 +51,25 @@ struct Correlator {
   float *work = nullptr;     // two padded volumes: filtered spectra, then derivatives in space
   hipfftHandle r2c2 = 0, c2r2 = 0, r2c1 = 0, c2r1 = 0;  // batches of two and of one
   bool have[4] = {false, false, false, false};
+  // doubled grid: factors per axis (z, x, y), copies, the copies' shift bits, copy buffers of S and R
+  // (nloop volumes each), high-cut fractions, and the 1D line transforms per axis
+  int dbl[3] = {1, 1, 1};
+  int nloop = 1;
+  int bits[8] = {0};  // copy i: bit 0 z shift, bit 1 x shift, bit 2 y shift
+  int src_of[8] = {0};  // copy i is copy src_of[i] shifted along axis_of[i]
+  int axis_of[8] = {0};
+  float high_z = 0.9f, high_xy = 0.7f;
+  float *sbuf = nullptr, *rbuf = nullptr, *lines = nullptr;
+  int L[3] = {0, 0, 0};
+  hipfftHandle lr2c[3] = {0, 0, 0}, lc2r[3] = {0, 0, 0};
+  bool lhave[3] = {false, false, false};
   ~Correlator() {
     hipfftHandle *h[4] = {&r2c2, &c2r2, &r2c1, &c2r1};
     for(int i = 0; i < 4; i++)
       if(have[i]) (void)hipfftDestroy(*h[i]);
-    for(void *p : {(void *)d_k, (void *)spec, (void *)work})
+    for(int a = 0; a < 3; a++)
+      if(lhave[a]) (void)hipfftDestroy(lr2c[a]), (void)hipfftDestroy(lc2r[a]);
+    for(void *p : {(void *)d_k, (void *)spec, (void *)work, (void *)sbuf, (void *)rbuf, (void *)lines})
       if(p) (void)hipFree(p);
   }
 };
@@ -183,6 +209,109 @@ __global__ void corr_imgmath_kernel(float *__restrict__ full, const float *__res
     full[e] = 2.0f * full[e] - 0.5f * imp[e];
 }
 
+// Lines along axis a (0 z, 1 x, 2 y) of a volume [ny][nx][nz] copied into rows of Lp = L + 2 floats
+// (zero beyond n_a: the transform's padding); line index = the other two coordinates in memory order.
+__global__ void corr_lines_gather_kernel(const float *__restrict__ in, float *__restrict__ lines, int nx, int ny, int nz,
+                                         int axis, int L) {
+  const int na = axis == 0 ? nz : axis == 1 ? nx : ny;
+  const size_t nl = (size_t)nx * ny * nz / na, Lp = (size_t)L + 2, tot = nl * Lp;
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t l = e / Lp;
+    const int j = (int)(e - l * Lp);
+    float v = 0.f;
+    if(j < na) {
+      size_t idx;
+      if(axis == 0) idx = l * nz + j;
+      else if(axis == 1) idx = ((l / nz) * nx + j) * nz + l % nz;  // l = iy*nz + iz
+      else idx = ((size_t)j * nx) * nz + l;                        // l = ix*nz + iz
+      v = in[idx];
+    }
+    lines[e] = v;
+  }
+}
+__global__ void corr_lines_scatter_kernel(const float *__restrict__ lines, float *__restrict__ out, int nx, int ny, int nz,
+                                          int axis, int L, float scale) {
+  const int na = axis == 0 ? nz : axis == 1 ? nx : ny;
+  const size_t nl = (size_t)nx * ny * nz / na, Lp = (size_t)L + 2, tot = nl * (size_t)na;
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t l = e / na;
+    const int j = (int)(e - l * na);
+    size_t idx;
+    if(axis == 0) idx = l * nz + j;
+    else if(axis == 1) idx = ((l / nz) * nx + j) * nz + l % nz;
+    else idx = ((size_t)j * nx) * nz + l;
+    out[idx] = lines[l * Lp + j] * scale;
+  }
+}
+// per-bin factor of the line spectra: half-sample shift (mode 0) or high cut at fraction h (mode 1)
+__global__ void corr_lines_filter_kernel(float *__restrict__ lines, size_t nl, int L, int mode, float h) {
+  const int nb = L / 2 + 1;
+  const size_t tot = nl * nb, Lp = (size_t)L + 2;
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t l = e / nb;
+    const int k = (int)(e - l * nb);
+    float *c = lines + l * Lp + 2 * k;
+    if(mode == 0) {
+      float sn, cs;
+      sincospif((float)k / (float)L, &sn, &cs);
+      const float re = c[0] * cs - c[1] * sn, im = c[0] * sn + c[1] * cs;
+      c[0] = re, c[1] = im;
+    } else {
+      const float f = (float)k / (0.5f * L);
+      const float w = f <= h ? 1.f : 0.5f * (1.f + cospif((f - h) / (1.f - h)));
+      c[0] *= w, c[1] *= w;
+    }
+  }
+}
+// sub-images [nloop][ny][nx][nz] -> the doubled grid [ny*dy][nx*dx][nz*dz]: copy i at its shift parity
+__global__ void corr_interleave_kernel(const float *__restrict__ sub, float *__restrict__ out, int nx, int ny, int nz,
+                                       int dz, int dx, int dy, int nloop, const int4 bits03, const int4 bits47) {
+  const size_t n = (size_t)nx * ny * nz, tot = n * nloop;
+  const int b[8] = {bits03.x, bits03.y, bits03.z, bits03.w, bits47.x, bits47.y, bits47.z, bits47.w};
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n);
+    const size_t c = e - (size_t)i * n;
+    const int iz = (int)(c % nz);
+    const size_t t = c / nz;
+    const int ix = (int)(t % nx), iy = (int)(t / nx);
+    const int oz = iz * dz + (b[i] & 1), ox = ix * dx + ((b[i] >> 1) & 1), oy = iy * dy + ((b[i] >> 2) & 1);
+    out[((size_t)oy * nx * dx + ox) * ((size_t)nz * dz) + oz] = sub[e];
+  }
+}
+
+// ill += s * s (ImagingCondition::calIllum on the high-cut source wavefield, bgprtm/ImagingCondition.cpp:143-149)
+__global__ void corr_illum_kernel(const float *__restrict__ s, float *__restrict__ ill, size_t n) {
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    ill[i] += s[i] * s[i];
+}
+// the illumination onto the doubled grid by averages of the 2, 4 or 8 surrounding samples
+// (ImagingCondition::interpIllum, bgprtm/ImagingCondition.cpp:759-810): one thread per coarse point writes
+// its eight targets in the reference's order and sums in its order; on an undoubled axis the "second"
+// target coincides with the first and the later write stands, as there
+__global__ void corr_interp_illum_kernel(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, int nz,
+                                         int dz, int dx, int dy) {
+  const size_t n = (size_t)nx * ny * nz, nx2 = (size_t)nx * dx, nz2 = (size_t)nz * dz;
+  for(size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(e % nz);
+    const size_t t = e / nz;
+    const int ix = (int)(t % nx), iy = (int)(t / nx);
+    const int y1 = iy * dy, y2 = iy * dy + dy - 1, x1 = ix * dx, x2 = ix * dx + dx - 1, z1 = iz * dz, z2 = iz * dz + dz - 1;
+    const int yp = min(iy + 1, ny - 1), xp = min(ix + 1, nx - 1), zp = min(iz + 1, nz - 1);
+    auto at = [&](int y, int x, int z) { return in[((size_t)y * nx + x) * nz + z]; };
+    auto o = [&](int y, int x, int z) -> float & { return out[((size_t)y * nx2 + x) * nz2 + z]; };
+    const float c = at(iy, ix, iz);
+    o(y1, x1, z1) = c;
+    o(y2, x1, z1) = 0.5f * (c + at(yp, ix, iz));
+    o(y1, x2, z1) = 0.5f * (c + at(iy, xp, iz));
+    o(y1, x1, z2) = 0.5f * (c + at(iy, ix, zp));
+    o(y2, x1, z2) = 0.25f * (c + at(iy, ix, zp) + at(yp, ix, iz) + at(yp, ix, zp));
+    o(y1, x2, z2) = 0.25f * (c + at(iy, ix, zp) + at(iy, xp, iz) + at(iy, xp, zp));
+    o(y2, x2, z1) = 0.25f * (c + at(yp, ix, iz) + at(iy, xp, iz) + at(yp, xp, iz));
+    o(y2, x2, z2) = 0.125f * (c + at(yp, ix, iz) + at(iy, xp, iz) + at(iy, ix, zp) + at(yp, ix, zp) + at(yp, xp, iz) +
+                              at(iy, xp, zp) + at(yp, xp, zp));
+  }
+}
+
 static unsigned nblocks(size_t n) { return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 256 * 64)); }
 
 // assignW (bgprtm/ImagingCorrelator.cpp:141-150), dk = 2 pi / (d n) as the reference forms it
@@ -288,6 +417,175 @@ static int cross_plain(Correlator &C, const float *s, const float *r, float isig
   return check_launch("corr_cross_kernel");
 }
 
+// ---- doubled grids ---------------------------------------------------------------------------------
+
+static int line_plans(Correlator &C, int axis) {
+  if(C.lhave[axis]) return KLRTM_OK;
+  const int na = axis == 0 ? C.nz : axis == 1 ? C.nx : C.ny;
+  const int Lx = C.L[axis], nb = Lx / 2 + 1, Lp = Lx + 2;
+  const int nl = (int)(C.nxyz / na);
+  int n[1] = {Lx}, re[1] = {Lp}, ce[1] = {nb};
+  KL_FFTC(hipfftPlanMany(&C.lr2c[axis], 1, n, re, 1, Lp, ce, 1, nb, HIPFFT_R2C, nl));
+  KL_FFTC(hipfftPlanMany(&C.lc2r[axis], 1, n, ce, 1, nb, re, 1, Lp, HIPFFT_C2R, nl));
+  C.lhave[axis] = true;
+  return KLRTM_OK;
+}
+
+// out = the axis filter of in (mode 0: half-sample shift, 1: high cut at h); in == out allowed
+static int axis_filter(Correlator &C, const float *in, float *out, int axis, int mode, float h) {
+  int rc = line_plans(C, axis);
+  if(rc) return rc;
+  const int na = axis == 0 ? C.nz : axis == 1 ? C.nx : C.ny, Lx = C.L[axis];
+  const size_t nl = C.nxyz / na;
+  hipLaunchKernelGGL(corr_lines_gather_kernel, dim3(nblocks(nl * (Lx + 2))), dim3(256), 0, stream(), in, C.lines, C.nx,
+                     C.ny, C.nz, axis, Lx);
+  KL_FFTC(hipfftSetStream(C.lr2c[axis], stream()));
+  KL_FFTC(hipfftExecR2C(C.lr2c[axis], C.lines, (hipfftComplex *)C.lines));
+  hipLaunchKernelGGL(corr_lines_filter_kernel, dim3(nblocks(nl * (Lx / 2 + 1))), dim3(256), 0, stream(), C.lines, nl, Lx,
+                     mode, h);
+  KL_FFTC(hipfftSetStream(C.lc2r[axis], stream()));
+  KL_FFTC(hipfftExecC2R(C.lc2r[axis], (hipfftComplex *)C.lines, C.lines));
+  hipLaunchKernelGGL(corr_lines_scatter_kernel, dim3(nblocks(C.nxyz)), dim3(256), 0, stream(), C.lines, out, C.nx, C.ny,
+                     C.nz, axis, Lx, 1.0f / (float)Lx);
+  return check_launch("corr_lines_scatter_kernel");
+}
+
+// ImagingCondition::applyFFTV: high cut along the doubled axes (z, x, y), in place
+static int hightape(Correlator &C, float *v) {
+  int rc = KLRTM_OK;
+  for(int a = 0; a < 3 && !rc; a++)
+    if(C.dbl[a] > 1) rc = axis_filter(C, v, v, a, 1, a == 0 ? C.high_z : C.high_xy);
+  return rc;
+}
+
+// copy 0 of buf (nloop volumes) holds the field; copies 1.. are its half-sample shifts
+// (ImagingCorrelator::interp, bgprtm/ImagingCorrelator.cpp:515-524)
+static int interp(Correlator &C, float *buf) {
+  int rc = KLRTM_OK;
+  for(int i = 1; i < C.nloop && !rc; i++)
+    rc = axis_filter(C, buf + (size_t)C.src_of[i] * C.nxyz, buf + (size_t)i * C.nxyz, C.axis_of[i], 0, 0.f);
+  return rc;
+}
+
+// one signed correlation term per sub-image of the nloop-volume buffers sb, rb whose copy 0 holds the fields:
+// copies 1.. are formed in place, then sub-image i takes copy i of S times copy i of R (ImagingCorrelator's
+// interp + crossone loop, bgprtm/ImagingCorrelator.cpp:374-414)
+static int cross_copies(Correlator &C, float *sb, float *rb, float isign, float *image, float *img2) {
+  int rc;
+  if((rc = interp(C, sb)) || (rc = interp(C, rb))) return rc;
+  for(int i = 0; i < C.nloop && !rc; i++) {
+    const size_t o = (size_t)i * C.nxyz;
+    rc = cross_plain(C, sb + o, rb + o, isign, image ? image + o : nullptr, img2 ? img2 + o : nullptr, nullptr, nullptr);
+  }
+  return rc;
+}
+
+// the same for (copy-0) fields s and r held elsewhere: copied into the correlator's copy buffers first
+static int cross_term(Correlator &C, const float *s, const float *r, float isign, float *image, float *img2) {
+  KL_HIP(hipMemcpyAsync(C.sbuf, s, C.nxyz * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+  KL_HIP(hipMemcpyAsync(C.rbuf, r, C.nxyz * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+  return cross_copies(C, C.sbuf, C.rbuf, isign, image, img2);
+}
+
+// inversion1's derivative terms on the doubled grid from the spectra in C.spec: each H_a S, H_a R into copy
+// 0 of sb, rb (as buffPad2(-1) leaves them in srcBuf[0]), then cross_copies
+static int derivative_terms_dbl(Correlator &C, float *sb, float *rb, float *image, float *img2) {
+  int rc = plan(C, C.c2r2, C.have[1], HIPFFT_C2R, 2);
+  if(rc) return rc;
+  const size_t ncplx = C.nxy * ((C.nz + 2) / 2);
+  for(int axis = 0; axis < (C.ny > 1 ? 3 : 2); axis++) {
+    if(axis == 0)
+      hipLaunchKernelGGL(corr_unit_deriv_kernel<0>, dim3(nblocks(2 * ncplx)), dim3(256), 0, stream(), C.spec, C.work,
+                         C.d_k, C.nx, C.ny, C.nz, C.padded);
+    else if(axis == 1)
+      hipLaunchKernelGGL(corr_unit_deriv_kernel<1>, dim3(nblocks(2 * ncplx)), dim3(256), 0, stream(), C.spec, C.work,
+                         C.d_k, C.nx, C.ny, C.nz, C.padded);
+    else
+      hipLaunchKernelGGL(corr_unit_deriv_kernel<2>, dim3(nblocks(2 * ncplx)), dim3(256), 0, stream(), C.spec, C.work,
+                         C.d_k, C.nx, C.ny, C.nz, C.padded);
+    if((rc = check_launch("corr_unit_deriv_kernel")) || (rc = exec(C, C.c2r2, false, C.work))) return rc;
+    hipLaunchKernelGGL(corr_unpad_kernel, dim3(nblocks(C.nxyz)), dim3(256), 0, stream(), C.work, sb, C.nxy, C.nz,
+                       C.scalef);
+    hipLaunchKernelGGL(corr_unpad_kernel, dim3(nblocks(C.nxyz)), dim3(256), 0, stream(), C.work + C.padded, rb, C.nxy,
+                       C.nz, C.scalef);
+    if((rc = check_launch("corr_unpad_kernel")) || (rc = cross_copies(C, sb, rb, -1.0f, image, img2))) return rc;
+  }
+  return KLRTM_OK;
+}
+
+int correlator_set_double(Correlator &C, int dz, int dx, int dy, float high_z, float high_xy) {
+  KL_REQUIRE((dz == 1 || dz == 2) && (dx == 1 || dx == 2) && (dy == 1 || dy == 2), KLRTM_EINVAL,
+             "imaging grid doubling factors must be 1 or 2");
+  KL_REQUIRE(dz >= dx && dz >= dy, KLRTM_EINVAL, "a doubled x or y imaging grid needs a doubled z grid too");
+  KL_REQUIRE(dy == 1 || C.ny > 1, KLRTM_EINVAL, "no y doubling on a 2D grid");
+  // the reference's sub-image layouts: all axes doubled in 3D (interleave3d), z or z and x in 2D (interleave2d
+  // walks one x-z plane, bgprtm/ImagingCorrelator.cpp:447-474)
+  KL_REQUIRE(C.ny == 1 || (dz == dx && dz == dy), KLRTM_EINVAL,
+             "3D imaging grids double all three axes or none (ic_double_z/x/y %d/%d/%d)", dz, dx, dy);
+  KL_REQUIRE(high_z > 0.f && high_z <= 1.f && high_xy > 0.f && high_xy <= 1.f, KLRTM_EINVAL, "high-cut fractions in (0, 1]");
+  C.dbl[0] = dz, C.dbl[1] = dx, C.dbl[2] = dy;
+  C.high_z = high_z, C.high_xy = high_xy;
+  // copies in the reference's order: z, x, y, zx, zy, xy, zxy (those of doubled axes)
+  const int want[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+  int idx_of[8];  // copy index of each shift pattern (bits z | x<<1 | y<<2)
+  for(int b = 0; b < 8; b++) idx_of[b] = -1;
+  C.nloop = 1, C.bits[0] = 0, idx_of[0] = 0;
+  for(int w = 0; w < 7; w++) {
+    const int bz = want[w][0], bx = want[w][1], by = want[w][2];
+    if((bz && dz < 2) || (bx && dx < 2) || (by && dy < 2)) continue;
+    const int b = bz | (bx << 1) | (by << 2);
+    // derived from the copy without its last shifted axis (y, then x, then z), shifted along that axis
+    const int ax = by ? 2 : bx ? 1 : 0;
+    const int from = b & ~(1 << ax);
+    C.bits[C.nloop] = b, C.src_of[C.nloop] = idx_of[from], C.axis_of[C.nloop] = ax;
+    idx_of[b] = C.nloop++;
+  }
+  for(int a = 0; a < 3; a++) {
+    const int na = a == 0 ? C.nz : a == 1 ? C.nx : C.ny;
+    C.L[a] = C.dbl[a] > 1 ? fft_size(na + 16) : 0;
+  }
+  if(C.nloop > 1) {
+    (void)hipFree(C.sbuf), (void)hipFree(C.rbuf), (void)hipFree(C.lines);
+    C.sbuf = C.rbuf = C.lines = nullptr;
+    size_t lmax = 0;
+    for(int a = 0; a < 3; a++)
+      if(C.L[a]) lmax = std::max(lmax, C.nxyz / (a == 0 ? C.nz : a == 1 ? C.nx : C.ny) * (size_t)(C.L[a] + 2));
+    KL_HIP(hipMalloc(&C.sbuf, sizeof(float) * C.nxyz * C.nloop));
+    KL_HIP(hipMalloc(&C.rbuf, sizeof(float) * C.nxyz * C.nloop));
+    KL_HIP(hipMalloc(&C.lines, sizeof(float) * lmax));
+  }
+  return KLRTM_OK;
+}
+
+// ImagingCorrelator::interleave: the nloop sub-images of sub onto the doubled grid in out
+int correlator_interleave(Correlator &C, const float *sub, float *out) {
+  KL_REQUIRE(sub && out && sub != out, KLRTM_EINVAL, "interleave needs distinct input and output");
+  if(C.nloop == 1) {
+    KL_HIP(hipMemcpyAsync(out, sub, C.nxyz * sizeof(float), hipMemcpyDeviceToDevice, stream()));
+    return KLRTM_OK;
+  }
+  const int4 b03 = make_int4(C.bits[0], C.bits[1], C.bits[2], C.bits[3]);
+  const int4 b47 = make_int4(C.bits[4], C.bits[5], C.bits[6], C.bits[7]);
+  hipLaunchKernelGGL(corr_interleave_kernel, dim3(nblocks(C.nxyz * C.nloop)), dim3(256), 0, stream(), sub, out, C.nx, C.ny,
+                     C.nz, C.dbl[0], C.dbl[1], C.dbl[2], C.nloop, b03, b47);
+  return check_launch("corr_interleave_kernel");
+}
+
+int correlator_nloop(const Correlator &C) { return C.nloop; }
+
+int correlator_add_illum(Correlator &C, const float *s, float *ill) {
+  KL_REQUIRE(s && ill, KLRTM_EINVAL, "null illumination operand");
+  hipLaunchKernelGGL(corr_illum_kernel, dim3(nblocks(C.nxyz)), dim3(256), 0, stream(), s, ill, C.nxyz);
+  return check_launch("corr_illum_kernel");
+}
+
+int correlator_interp_illum(Correlator &C, const float *in, float *out) {
+  KL_REQUIRE(in && out && in != out, KLRTM_EINVAL, "illumination interpolation needs distinct input and output");
+  hipLaunchKernelGGL(corr_interp_illum_kernel, dim3(nblocks(C.nxyz)), dim3(256), 0, stream(), in, out, C.nx, C.ny, C.nz,
+                     C.dbl[0], C.dbl[1], C.dbl[2]);
+  return check_launch("corr_interp_illum_kernel");
+}
+
 // K applied in place to nv volumes (nv = 1: a, nv = 2: a and b)
 static int kfilter(Correlator &C, float *a, float *b) {
   int rc = forward_pair(C, a, b);
@@ -311,6 +609,18 @@ int correlator_run(Correlator &C, float *image, float *img_fwi, float *img_raw, 
   KL_REQUIRE(src && rec, KLRTM_EINVAL, "null wavefield");
   LaunchTimer tm("ic_correlator");
   int rc = KLRTM_OK;
+  if(C.nloop > 1) {
+    // doubled grid: images hold nloop sub-images; applyFFTV's high cut on S and R first
+    // (bgprtm/ImagingCondition.cpp:125-134)
+    KL_REQUIRE(!image64, KLRTM_EINVAL, "the doubled imaging grid accumulates FP32 images only");
+    KL_REQUIRE(option == KLRTM_IC_IMAGE1 || option == KLRTM_IC_INVERSION, KLRTM_EINVAL,
+               "the doubled imaging grid takes the IMAGE1 and INVERSION conditions");
+    if((rc = hightape(C, src)) || (rc = hightape(C, rec))) return rc;
+    if(option == KLRTM_IC_IMAGE1) return cross_term(C, src, rec, 1.0f, image, nullptr);
+    if((rc = forward_pair(C, src, rec))) return rc;  // derive (bgprtm/ImagingCorrelator.cpp:376)
+    if((rc = cross_term(C, src, rec, 1.0f, image, img_fwi))) return rc;
+    return derivative_terms_dbl(C, C.sbuf, C.rbuf, image, img_fwi);
+  }
   switch(option) {
   case KLRTM_IC_IMAGE1: return cross_plain(C, src, rec, 1.0f, image, nullptr, nullptr, image64);
   case KLRTM_IC_ALLIN3:
@@ -327,6 +637,26 @@ int correlator_run(Correlator &C, float *image, float *img_fwi, float *img_raw, 
     return cross_plain(C, src, rec, 1.0f, image, nullptr, nullptr, image64);
   default: set_error("unknown imaging option %d", option); return KLRTM_EINVAL;
   }
+}
+
+// ImagingCorrelator::run on the reference's buffers (after ImagingCondition's applyFFTV, and for IMAGE1 its
+// interp): src, rec hold nloop volumes; IMAGE1 correlates the given copies (imaging1,
+// bgprtm/ImagingCorrelator.cpp:363-372), INVERSION takes copy 0 and overwrites the copies (inversion1, :374-414)
+int correlator_run_copies(Correlator &C, float *image, float *img_fwi, float *src, float *rec, int option) {
+  KL_REQUIRE(src && rec, KLRTM_EINVAL, "null wavefield");
+  KL_REQUIRE(option == KLRTM_IC_IMAGE1 || option == KLRTM_IC_INVERSION, KLRTM_EINVAL,
+             "copies are correlated under IMAGE1 and INVERSION");
+  LaunchTimer tm("ic_correlator");
+  int rc = KLRTM_OK;
+  if(option == KLRTM_IC_IMAGE1) {
+    for(int i = 0; i < C.nloop && !rc; i++) {
+      const size_t o = (size_t)i * C.nxyz;
+      rc = cross_plain(C, src + o, rec + o, 1.0f, image ? image + o : nullptr, nullptr, nullptr, nullptr);
+    }
+    return rc;
+  }
+  if((rc = forward_pair(C, src, rec)) || (rc = cross_copies(C, src, rec, 1.0f, image, img_fwi))) return rc;
+  return derivative_terms_dbl(C, src, rec, image, img_fwi);
 }
 
 int correlator_costerm(Correlator &C, float *image) {
@@ -410,6 +740,35 @@ int klrtm_correlator_sinterm(klrtm_correlator *c, float *imgfull, const float *i
 int klrtm_correlator_unit_derivative(klrtm_correlator *c, const float *in, int axis, float *out) {
   KL_REQUIRE(c, KLRTM_EINVAL, "null correlator");
   return correlator_unit_derivative(*reinterpret_cast<Correlator *>(c), in, axis, out);
+}
+int klrtm_correlator_set_double(klrtm_correlator *c, int dz, int dx, int dy, float high_z, float high_xy) {
+  KL_REQUIRE(c, KLRTM_EINVAL, "null correlator");
+  return correlator_set_double(*reinterpret_cast<Correlator *>(c), dz, dx, dy, high_z, high_xy);
+}
+int klrtm_correlator_nloop(klrtm_correlator *c, int *nloop) {
+  KL_REQUIRE(c && nloop, KLRTM_EINVAL, "null argument");
+  *nloop = correlator_nloop(*reinterpret_cast<Correlator *>(c));
+  return KLRTM_OK;
+}
+int klrtm_correlator_interleave(klrtm_correlator *c, const float *sub, float *out) {
+  KL_REQUIRE(c, KLRTM_EINVAL, "null correlator");
+  return correlator_interleave(*reinterpret_cast<Correlator *>(c), sub, out);
+}
+int klrtm_correlator_run_copies(klrtm_correlator *c, float *image, float *img_fwi, float *src, float *rec, int option) {
+  KL_REQUIRE(c, KLRTM_EINVAL, "null correlator");
+  return correlator_run_copies(*reinterpret_cast<Correlator *>(c), image, img_fwi, src, rec, option);
+}
+int klrtm_correlator_interp(klrtm_correlator *c, float *buf) {
+  KL_REQUIRE(c && buf, KLRTM_EINVAL, "null argument");
+  return interp(*reinterpret_cast<Correlator *>(c), buf);
+}
+int klrtm_correlator_hightape(klrtm_correlator *c, float *wave) {
+  KL_REQUIRE(c && wave, KLRTM_EINVAL, "null argument");
+  return hightape(*reinterpret_cast<Correlator *>(c), wave);
+}
+int klrtm_correlator_interp_illum(klrtm_correlator *c, const float *ill, float *out) {
+  KL_REQUIRE(c, KLRTM_EINVAL, "null correlator");
+  return correlator_interp_illum(*reinterpret_cast<Correlator *>(c), ill, out);
 }
 int klrtm_correlator_kfilter(klrtm_correlator *c, float *wave) {
   KL_REQUIRE(c, KLRTM_EINVAL, "null correlator");

@@ -35,6 +35,13 @@ int correlator_create(Correlator **out, int nx, int ny, int nz, float dx, float 
 int correlator_costerm(Correlator &C, float *image);
 int correlator_sinterm(Correlator &C, float *imgfull, const float *imgcos);
 void correlator_destroy(Correlator *C);
+// doubled imaging grid (factors 1 or 2 per axis z, x, y; high-cut fractions of the Nyquist wavenumber)
+int correlator_set_double(Correlator &C, int dz, int dx, int dy, float high_z, float high_xy);
+int correlator_nloop(const Correlator &C);
+int correlator_interleave(Correlator &C, const float *sub, float *out);
+int correlator_add_illum(Correlator &C, const float *s, float *ill);
+int correlator_run_copies(Correlator &C, float *image, float *img_fwi, float *src, float *rec, int option);
+int correlator_interp_illum(Correlator &C, const float *in, float *out);
 int launch_ic(int fmt, bool rq, const IcArgs &a, const void *snap, const float *scales, const float *w, const unsigned *rmax,
               float *img, float *ill, double *img64, double *ill64);
 }  // namespace klrtm

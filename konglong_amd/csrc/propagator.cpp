@@ -91,6 +91,14 @@ struct Propagator {
   int ic_option = KLRTM_IC_IMAGE1;
   Correlator *corr = nullptr;
   float *d_fwi = nullptr, *d_raw = nullptr, *d_ics = nullptr, *d_icr = nullptr;
+  // doubled imaging grid (ic_double_z/x/y): d_img, d_fwi hold nloop sub-images; finish_images interleaves
+  // them into d_img2 / d_fwi2 on the doubled grid
+  int ic_dbl[3] = {1, 1, 1};
+  float ic_high_z = 0.9f, ic_high_xy = 0.7f;
+  int nloop = 1;
+  float *d_img2 = nullptr, *d_fwi2 = nullptr, *d_ill2 = nullptr;
+  size_t img_n = 0;  // floats held by d_img
+  float kmin_ic = 1.0f / 1492.0f;
   std::vector<float> wtaperT;
   std::vector<float> wavelet;    // the source wavelet (host, nt samples)
   std::vector<float> src_delay;  // source array: delay per source in seconds (empty: the single source)
@@ -152,7 +160,9 @@ Propagator::~Propagator() {
   for(void *p : {(void *)d_step, (void *)d_stage, (void *)d_taper_tab}) (void)hipFree(p);
   if(cache) snapcache_destroy(cache);
   if(corr) correlator_destroy(corr);
-  for(void *p : {(void *)d_fwi, (void *)d_raw, (void *)d_ics, (void *)d_icr}) (void)hipFree(p);
+  for(void *p : {(void *)d_fwi, (void *)d_raw, (void *)d_ics, (void *)d_icr, (void *)d_img2, (void *)d_fwi2,
+                 (void *)d_ill2})
+    (void)hipFree(p);
   delete eng;
   if(src) receiver_destroy(src);
   if(rcv) receiver_destroy(rcv);
@@ -238,9 +248,10 @@ int Propagator::surface() {
 int Propagator::clear_images() {
   // fill kernels rather than memset nodes, so that a captured step sequence stays replayable
   int rc;
-  if((rc = launch_fill(d_img, 0.f, n)) || (rc = launch_fill(d_ill, 0.f, n))) return rc;
+  const size_t ni = n * (size_t)nloop;
+  if((rc = launch_fill(d_img, 0.f, ni)) || (rc = launch_fill(d_ill, 0.f, n))) return rc;
   if((rc = launch_fill((float *)d_img64, 0.f, 2 * n)) || (rc = launch_fill((float *)d_ill64, 0.f, 2 * n))) return rc;
-  if(d_fwi && ((rc = launch_fill(d_fwi, 0.f, n)) || (rc = launch_fill(d_raw, 0.f, n)))) return rc;
+  if(d_fwi && ((rc = launch_fill(d_fwi, 0.f, ni)) || (rc = launch_fill(d_raw, 0.f, ni)))) return rc;
   return KLRTM_OK;
 }
 
@@ -313,15 +324,19 @@ int Propagator::image(int itsav) {
     if(rc) return rc;
     snap = (const char *)d_dec, fmt = KLRTM_SNAP_FP32;
   }
-  if(ic_option != KLRTM_IC_IMAGE1) {
-    // ImagingCondition::stackImageInterp with an FFT option: the tapered S and R (and the illumination),
-    // then ImagingCorrelator::run (bgprtm/ImagingCondition.cpp:104-141); the FP64 image takes the same
-    // terms, except under INVERSION2, whose post-loop cos term is FP32 only
+  if(ic_option != KLRTM_IC_IMAGE1 || nloop > 1) {
+    // ImagingCondition::stackImageInterp with an FFT option or a doubled imaging grid: the tapered S and R
+    // (and the illumination), then ImagingCorrelator::run (bgprtm/ImagingCondition.cpp:104-141); the FP64
+    // image takes the same terms, except under INVERSION2, whose post-loop cos term is FP32 only, and on
+    // the doubled grid, which is FP32
     a.out_s = d_ics, a.out_r = d_icr;
-    int rc = launch_ic(fmt, rq, a, snap, sc, w0, d_smax, nullptr, d_ill, nullptr, l64);
-    if(rc) return rc;
-    return correlator_run(*corr, d_img, d_fwi, d_raw, ic_option == KLRTM_IC_INVERSION2 ? nullptr : i64, d_ics, d_icr,
-                          ic_option);
+    // on the doubled grid the illumination takes the high-cut S, after the correlator's high cut
+    int rc = launch_ic(fmt, rq, a, snap, sc, w0, d_smax, nullptr, nloop > 1 ? nullptr : d_ill, nullptr,
+                       nloop > 1 ? nullptr : l64);
+    if(rc || (rc = correlator_run(*corr, d_img, d_fwi, d_raw, ic_option == KLRTM_IC_INVERSION2 || nloop > 1 ? nullptr : i64,
+                                  d_ics, d_icr, ic_option)))
+      return rc;
+    return nloop > 1 ? correlator_add_illum(*corr, d_ics, d_ill) : KLRTM_OK;
   }
   if(fmt == KLRTM_SNAP_LEARNED) return lcodec_image(*lc, a, snap, w0, d_img, d_ill, i64, l64);
   return launch_ic(fmt, rq, a, snap, sc, w0, d_smax, d_img, d_ill, i64, l64);
@@ -331,7 +346,8 @@ int Propagator::image(int itsav) {
 
 int Propagator::graph_ready() {
   KL_REQUIRE(!use_cache, KLRTM_ESTATE, "captured step sequences keep the snapshots in HBM (no snapshot cache)");
-  KL_REQUIRE(ic_option == KLRTM_IC_IMAGE1, KLRTM_ESTATE, "captured step sequences image with the zero-lag condition");
+  KL_REQUIRE(ic_option == KLRTM_IC_IMAGE1 && nloop == 1, KLRTM_ESTATE,
+             "captured step sequences image with the zero-lag condition on the propagation grid");
   KL_REQUIRE(!fk, KLRTM_ESTATE, "captured step sequences inject traces, not the FK surface wavefield");
   int rc;
   if((rc = alloc_snapshots())) return rc;
@@ -464,28 +480,70 @@ int Propagator::replay(int dir, int flag, int it_start, int nrep) {
   return KLRTM_OK;
 }
 
-// the post-loop steps of ImagingCondition::genImage for INVERSION2 (bgprtm/ImagingCondition.cpp:427-430)
+// the post-loop steps of ImagingCondition::genImage: INVERSION2's cos and sin terms, then the interleave of
+// doubled-grid sub-images (bgprtm/ImagingCondition.cpp:415-431)
 static int finish_images(Propagator &P) {
-  if(P.ic_option != KLRTM_IC_INVERSION2) return KLRTM_OK;
-  int rc = correlator_costerm(*P.corr, P.d_img);
-  return rc ? rc : correlator_sinterm(*P.corr, P.d_fwi, P.d_img);
+  int rc = KLRTM_OK;
+  if(P.ic_option == KLRTM_IC_INVERSION2 && ((rc = correlator_costerm(*P.corr, P.d_img)) ||
+                                            (rc = correlator_sinterm(*P.corr, P.d_fwi, P.d_img))))
+    return rc;
+  if(P.nloop > 1) {
+    if((rc = correlator_interleave(*P.corr, P.d_img, P.d_img2)) || (rc = correlator_interp_illum(*P.corr, P.d_ill, P.d_ill2)))
+      return rc;
+    if(P.ic_option != KLRTM_IC_IMAGE1) rc = correlator_interleave(*P.corr, P.d_fwi, P.d_fwi2);
+  }
+  return rc;
+}
+
+// d_img sized for ni floats (nloop sub-images on a doubled imaging grid)
+static int size_image(Propagator &P, size_t ni) {
+  if(P.img_n == ni) return KLRTM_OK;
+  (void)hipFree(P.d_img), P.d_img = nullptr, P.img_n = 0;
+  int rc = dmalloc(&P.d_img, ni);
+  if(!rc) P.img_n = ni;
+  return rc;
+}
+
+// (re)builds the correlator and the image buffers for the current option and imaging grid
+static int setup_correlator(Propagator &P) {
+  const bool dbl = P.ic_dbl[0] > 1 || P.ic_dbl[1] > 1 || P.ic_dbl[2] > 1;
+  if(P.corr) correlator_destroy(P.corr), P.corr = nullptr;
+  for(float **d : {&P.d_fwi, &P.d_raw, &P.d_img2, &P.d_fwi2, &P.d_ill2}) (void)hipFree(*d), *d = nullptr;
+  P.nloop = 1;
+  int rc;
+  if(P.ic_option == KLRTM_IC_IMAGE1 && !dbl) return (rc = size_image(P, P.n)) ? rc : P.clear_images();
+  KL_REQUIRE(P.c.snap_format != KLRTM_SNAP_LEARNED, KLRTM_EINVAL,
+             "the FFT imaging conditions and doubled imaging grids take FP32, COMPRESS_16, block-scaled or autoencoder "
+             "snapshots");
+  if((rc = correlator_create(&P.corr, P.c.nx, P.c.ny, P.c.nz, P.c.dx, P.c.dy, P.c.dz, P.kmin_ic))) return rc;
+  if(dbl && (rc = correlator_set_double(*P.corr, P.ic_dbl[0], P.ic_dbl[1], P.ic_dbl[2], P.ic_high_z, P.ic_high_xy)))
+    return rc;
+  P.nloop = correlator_nloop(*P.corr);
+  const size_t ni = P.n * (size_t)P.nloop;
+  if((rc = size_image(P, ni)) || (rc = dmalloc(&P.d_fwi, ni)) || (rc = dmalloc(&P.d_raw, ni))) return rc;
+  for(float **d : {&P.d_ics, &P.d_icr})
+    if(!*d && (rc = dmalloc(d, P.n))) return rc;
+  if(P.nloop > 1 && ((rc = dmalloc(&P.d_img2, ni)) || (rc = dmalloc(&P.d_fwi2, ni)) || (rc = dmalloc(&P.d_ill2, ni))))
+    return rc;
+  return P.clear_images();
 }
 
 int propagator_set_imaging(Propagator &P, int option, float kmin) {
   KL_REQUIRE(option >= KLRTM_IC_IMAGE1 && option <= KLRTM_IC_ALLIN3, KLRTM_EINVAL, "unknown imaging option %d", option);
-  if(option == KLRTM_IC_IMAGE1) {
-    P.ic_option = option;
-    return KLRTM_OK;
-  }
-  KL_REQUIRE(P.c.snap_format != KLRTM_SNAP_LEARNED, KLRTM_EINVAL,
-             "the FFT imaging conditions take FP32, COMPRESS_16 or block-scaled snapshots");
-  if(P.corr) correlator_destroy(P.corr), P.corr = nullptr;
-  int rc = correlator_create(&P.corr, P.c.nx, P.c.ny, P.c.nz, P.c.dx, P.c.dy, P.c.dz, kmin);
-  if(rc) return rc;
-  for(float **d : {&P.d_fwi, &P.d_raw, &P.d_ics, &P.d_icr})
-    if(!*d && (rc = dmalloc(d, P.n))) return rc;
-  P.ic_option = option;
-  return P.clear_images();
+  const int prev = P.ic_option;
+  const float prev_k = P.kmin_ic;
+  P.ic_option = option, P.kmin_ic = kmin;
+  int rc = setup_correlator(P);
+  if(rc) P.ic_option = prev, P.kmin_ic = prev_k;
+  return rc;
+}
+
+int propagator_set_ic_double(Propagator &P, int dz, int dx, int dy, float high_z, float high_xy) {
+  const int prev[3] = {P.ic_dbl[0], P.ic_dbl[1], P.ic_dbl[2]};
+  P.ic_dbl[0] = dz, P.ic_dbl[1] = dx, P.ic_dbl[2] = dy, P.ic_high_z = high_z, P.ic_high_xy = high_xy;
+  int rc = setup_correlator(P);
+  if(rc) P.ic_dbl[0] = prev[0], P.ic_dbl[1] = prev[1], P.ic_dbl[2] = prev[2];
+  return rc;
 }
 
 static float ricker(float t, float fp) {
@@ -527,6 +585,7 @@ int propagator_create(Propagator **out, const klrtm_shot_config *cfg) {
     delete P;
     return rc;
   }
+  P->img_n = P->n;
   hipError_t e = hipMalloc(&P->d_smax, sizeof(unsigned) * c.ny);
   if(e == hipSuccess) e = hipMalloc(&P->d_img64, sizeof(double) * P->n);
   if(e == hipSuccess) e = hipMalloc(&P->d_ill64, sizeof(double) * P->n);
@@ -673,6 +732,8 @@ int propagator_model(Propagator &P, float *record) {
 // forward with snapshots + backward with imaging (bgprtm/Propagator.cpp:1210-1255, 2226-2275)
 int propagator_rtm(Propagator &P, const float *record, float *img, float *ill, double *img64, double *ill64) {
   KL_REQUIRE(P.rcv, KLRTM_ESTATE, "receivers not set");
+  KL_REQUIRE(P.nloop == 1 || (!img && !ill), KLRTM_EINVAL,
+             "on a doubled imaging grid the images are read from klrtm_propagator_ic_double_images (img, ill NULL)");
   int rc;
   if((rc = P.alloc_snapshots())) return rc;
   const int k = std::max(1, P.c.snap_every), nt = P.c.nt;
@@ -956,6 +1017,20 @@ int klrtm_propagator_set_fk(klrtm_propagator *p, int enable, float vsurf, int tr
   KL_REQUIRE(vsurf > 0 && (trueamp || deghost), KLRTM_EINVAL, "FK needs vsurf > 0 and true amplitude or deghosting");
   P.fk = true, P.fk_vsurf = vsurf, P.fk_trueamp = trueamp, P.fk_z = z_depth, P.fk_deghost = deghost;
   P.fk_flcut = flcut, P.fk_flpass = flpass, P.fk_thr = zerofreq_threshold;
+  return KLRTM_OK;
+}
+int klrtm_propagator_set_ic_double(klrtm_propagator *p, int dz, int dx, int dy, float high_z, float high_xy) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  reinterpret_cast<Propagator *>(p)->drop_graphs();  // captured launches hold the old set-up
+  return propagator_set_ic_double(*reinterpret_cast<Propagator *>(p), dz, dx, dy, high_z, high_xy);
+}
+int klrtm_propagator_ic_double_images(klrtm_propagator *p, float **img, float **fwi, float **ill, int *nloop) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  if(img) *img = P.nloop > 1 ? P.d_img2 : P.d_img;
+  if(fwi) *fwi = P.nloop > 1 ? (P.ic_option == KLRTM_IC_IMAGE1 ? nullptr : P.d_fwi2) : P.d_fwi;
+  if(ill) *ill = P.nloop > 1 ? P.d_ill2 : P.d_ill;
+  if(nloop) *nloop = P.nloop;
   return KLRTM_OK;
 }
 int klrtm_propagator_ic_images(klrtm_propagator *p, float **fwi, float **raw) {

@@ -19,7 +19,7 @@ def correlate(src, rec, nx, ny, nz, taper_t=1.0, taper_z=None, ntaper_z=0, image
 
 class ImagingCorrelator:
     """ImagingCorrelator(nx, ny, nz, dx, dy, dz, nx2=nx, ny2=ny, nz2=nz, kmin) (bgprtm/ImagingCorrelator.h:20-33)
-    at nDbl = 1; device volumes (ny, nx, nz).  run() takes the option of ImagingCondition::getImagingOption;
+    (set_double for nDbl = 2); device volumes (ny, nx, nz).  run() takes the option of ImagingCondition::getImagingOption;
     src and rec are work buffers (INVERSION2 filters them in place, as the reference does)."""
 
     def __init__(self, nx, ny, nz, dx, dy, dz, kmin=1.0 / 1492.0):
@@ -42,6 +42,38 @@ class ImagingCorrelator:
 
     def kfilter(self, wave):
         check(lib().klrtm_correlator_kfilter(self.h, wave.ptr))
+
+    def set_double(self, dz=2, dx=2, dy=2, high_z=0.9, high_xy=0.7):
+        """The doubled imaging grid (nx2 = dx nx, ny2 = dy ny, nz2 = dz nz of the reference's constructor):
+        run() then fills nloop sub-images per image; interleave() puts them onto the doubled grid."""
+        check(lib().klrtm_correlator_set_double(self.h, int(dz), int(dx), int(dy), float(high_z), float(high_xy)))
+        self.dbl = (int(dz), int(dx), int(dy))
+
+    @property
+    def nloop(self):
+        n = C.c_int()
+        check(lib().klrtm_correlator_nloop(self.h, C.byref(n)))
+        return n.value
+
+    def interleave(self, sub, out):
+        check(lib().klrtm_correlator_interleave(self.h, sub.ptr, out.ptr))
+
+    def run_copies(self, src, rec, option, image=None, img_fwi=None):
+        """ImagingCorrelator::run on nloop-volume buffers (applyFFTV, and for IMAGE1 interp, done by the
+        caller)."""
+        check(lib().klrtm_correlator_run_copies(self.h, _p(image), _p(img_fwi), src.ptr, rec.ptr, int(option)))
+
+    def interp(self, buf):
+        """ImagingCorrelator::interp: copies 1.. of buf (nloop volumes) from copy 0."""
+        check(lib().klrtm_correlator_interp(self.h, buf.ptr))
+
+    def hightape(self, wave):
+        """ImagingCondition::applyFFTV in place."""
+        check(lib().klrtm_correlator_hightape(self.h, wave.ptr))
+
+    def interp_illum(self, ill, out):
+        """ImagingCondition::interpIllum onto the doubled grid."""
+        check(lib().klrtm_correlator_interp_illum(self.h, ill.ptr, out.ptr))
 
     def close(self):
         if self.h:

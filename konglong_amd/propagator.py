@@ -110,13 +110,40 @@ class Propagator:
         check(lib().klrtm_propagator_set_imaging(self.h, int(option), float(kmin)))
         self.ic_option = int(option)
 
+    def set_ic_double(self, dz=2, dx=2, dy=2, high_z=0.9, high_xy=0.7):
+        """Image on a grid doubled along z, x, y (ic_double_z/x/y with highTapeZ / highTapeXY): 3D grids
+        double all three axes, 2D grids z or z and x; (1, 1, 1) restores the propagation grid.  rtm() then
+        returns the image, FWI image and illumination on the doubled grid (FP32)."""
+        if self.shape[0] == 1:
+            dy = 1
+        check(lib().klrtm_propagator_set_ic_double(self.h, int(dz), int(dx), int(dy), float(high_z), float(high_xy)))
+        self.ic_dbl = (int(dz), int(dx), int(dy))
+
+    def double_images(self):
+        """Device pointers of the doubled-grid image, FWI image and illumination, their shape and nloop."""
+        img, fwi, ill, nl = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_int()
+        check(lib().klrtm_propagator_ic_double_images(self.h, C.byref(img), C.byref(fwi), C.byref(ill), C.byref(nl)))
+        dz, dx, dy = getattr(self, "ic_dbl", (1, 1, 1))
+        ny, nx, nz = self.shape
+        return dict(image=img.value, fwi=fwi.value, illum=ill.value, shape=(ny * dy, nx * dx, nz * dz), nloop=nl.value)
+
     def rtm(self, record=None, want=("image", "illum", "image64", "illum64")):
         n = self.shape
+        r = None if record is None else np.ascontiguousarray(record, np.float32)
+        if getattr(self, "ic_dbl", (1, 1, 1)) != (1, 1, 1):
+            check(lib().klrtm_propagator_rtm(self.h, fptr(r), None, None, None, None))
+            d = self.double_images()
+            out = {}
+            for name in ("image", "fwi", "illum"):
+                if d[name]:
+                    a = np.empty(d["shape"], np.float32)
+                    check(lib().klrtm_memcpy_d2h(a.ctypes.data, d[name], a.nbytes))
+                    out[name] = a
+            return out
         img = np.zeros(n, np.float32) if "image" in want else None
         ill = np.zeros(n, np.float32) if "illum" in want else None
         img64 = np.zeros(n, np.float64) if "image64" in want else None
         ill64 = np.zeros(n, np.float64) if "illum64" in want else None
-        r = None if record is None else np.ascontiguousarray(record, np.float32)
         check(lib().klrtm_propagator_rtm(self.h, fptr(r), fptr(img), fptr(ill), dptr(img64), dptr(ill64)))
         out = dict(image=img, illum=ill, image64=img64, illum64=ill64)
         if getattr(self, "ic_option", 0):   # the FWI and raw images of the FFT imaging conditions

@@ -14,6 +14,20 @@ bgprtm/ImagingCorrelator.cpp:
   * inversion1 (:374-414): image += S R - Hz S Hz R - Hx S Hx R - Hy S Hy R, imgFWI += S R + the three
     products; inversion2 (:416-431): imgFWI += S R, image += K S K R; imaging3 (:433-439): raw += |S R| then
     inversion1.
+Doubled imaging grids (ic_double_z/x/y):
+  * ImagingCondition::applyFFTV (ImagingCondition.cpp:169-178) and ImagingCorrelator::interp (:515-524)
+    call libFFTV's HIGHTAPE and SHIFTHALF filters; libFFTV is not part of the reference tree (parity
+    unpinned against it).  Restated as zero-padded 1D FFTs of length L = fft_size(n + 16) along the axis:
+    SHIFTHALF multiplies bin k by exp(i pi k / L), which gives f(x + 1/2) -- the position the interleave
+    gives the copy (index 2 i + 1 of the doubled axis); HIGHTAPE keeps bins up to h * Nyquist and tapers
+    with 0.5 (1 + cos(pi (f - h) / (1 - h))) to 0 at Nyquist (f = k / (L / 2));
+  * copies in the order z, x, y, zx, zy, xy, zxy of the doubled axes, each the shift of an earlier copy
+    along its last axis (ImagingCorrelator.cpp:58-75, 515-524);
+  * interleave3d / interleave2d (:440-513): copy i lands at (2 iy + by, 2 ix + bx, 2 iz + bz);
+  * imaging1 (:363-372) correlates the copies of the high-cut S and R per sub-image; inversion1 (:374-414)
+    forms the derivative spectra from the high-cut fields, then the copies of S, R, then of each H_a S,
+    H_a R (sign -1 in image, +1 in imgFWI);
+  * interpIllum (ImagingCondition.cpp:759-810): FP32 averages written in the reference's order.
 Volumes are (ny, nx, nz) arrays, z fastest, like the reference.  The reference's ny2 == ny set-up aliases
 its x/y derivative buffers with the spectrum (its lapy then reads the x derivative in space); the
 restatement follows the intended ny2 > ny path, where each filter reads the intact spectrum.
@@ -102,3 +116,131 @@ def run(option, s, r, d, kmin=0.0, image=None, img_fwi=None, img_raw=None):
             out[0] += kfilter(s, d, kmin) * kfilter(r, d, kmin)
         return tuple(out)
     raise ValueError(f"unknown imaging option {option}")
+
+
+# ---- doubled imaging grids -----------------------------------------------------------------------------
+
+_AX = {0: 2, 1: 1, 2: 0}   # axis code (0 z, 1 x, 2 y) -> array axis of a (ny, nx, nz) volume
+
+
+def fft_size(n):
+    """The smallest 2^a 3^b 5^c >= n."""
+    m = max(1, int(n))
+    while True:
+        k = m
+        for p in (2, 3, 5):
+            while k % p == 0:
+                k //= p
+        if k == 1:
+            return m
+        m += 1
+
+
+def line_filter(x, axis, mode, h=0.0):
+    """The zero-padded 1D filter along axis (0 z, 1 x, 2 y): mode 0 SHIFTHALF, mode 1 HIGHTAPE at h."""
+    x = np.asarray(x, np.float64)
+    ax = _AX[axis]
+    na = x.shape[ax]
+    L = fft_size(na + 16)
+    k = np.arange(L // 2 + 1, dtype=np.float64)
+    if mode == 0:
+        filt = np.exp(1j * np.pi * k / L)
+    else:
+        f = k / (0.5 * L)
+        filt = np.ones_like(f)
+        hi = f > h
+        filt[hi] = 0.5 * (1.0 + np.cos(np.pi * (f[hi] - h) / (1.0 - h)))
+    shape = [1, 1, 1]
+    shape[ax] = k.size
+    y = np.fft.irfft(np.fft.rfft(x, n=L, axis=ax) * filt.reshape(shape), n=L, axis=ax)
+    return np.take(y, np.arange(na), axis=ax)
+
+
+def double_copies(dz, dx, dy):
+    """[(bits, source copy, axis)] of the copies, copy 0 first: bits = z | x << 1 | y << 2."""
+    out, idx = [(0, -1, -1)], {0: 0}
+    for bz, bx, by in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1)):
+        if (bz and dz < 2) or (bx and dx < 2) or (by and dy < 2):
+            continue
+        b = bz | bx << 1 | by << 2
+        ax = 2 if by else 1 if bx else 0
+        idx[b] = len(out)
+        out.append((b, idx[b & ~(1 << ax)], ax))
+    return out
+
+
+def hightape(x, dbl, high_z=0.9, high_xy=0.7):
+    for a in range(3):
+        if dbl[a] > 1:
+            x = line_filter(x, a, 1, high_z if a == 0 else high_xy)
+    return x
+
+
+def interp(x, dbl):
+    """The copies [nloop] of x (copy 0 is x)."""
+    cps = [np.asarray(x, np.float64)]
+    for _, src, ax in double_copies(*dbl)[1:]:
+        cps.append(line_filter(cps[src], ax, 0))
+    return cps
+
+
+def interleave(sub, dbl):
+    """Sub-images [nloop][ny][nx][nz] -> the doubled grid."""
+    sub = np.asarray(sub)
+    dz, dx, dy = dbl
+    ny, nx, nz = sub.shape[1:]
+    out = np.zeros((ny * dy, nx * dx, nz * dz), sub.dtype)
+    for i, (b, _, _) in enumerate(double_copies(*dbl)):
+        out[(b >> 2) & 1::dy, (b >> 1) & 1::dx, b & 1::dz] = sub[i]
+    return out
+
+
+def interp_illum(ill, dbl):
+    """ImagingCondition::interpIllum in FP32, the eight writes per coarse point in the reference's order."""
+    a = np.asarray(ill, np.float32)
+    dz, dx, dy = dbl
+    ny, nx, nz = a.shape
+    out = np.zeros((ny * dy, nx * dx, nz * dz), np.float32)
+    c = a
+    yp = a[np.minimum(np.arange(ny) + 1, ny - 1)]
+    xp = a[:, np.minimum(np.arange(nx) + 1, nx - 1)]
+    zp = a[:, :, np.minimum(np.arange(nz) + 1, nz - 1)]
+    yxp = yp[:, np.minimum(np.arange(nx) + 1, nx - 1)]
+    yzp = yp[:, :, np.minimum(np.arange(nz) + 1, nz - 1)]
+    xzp = xp[:, :, np.minimum(np.arange(nz) + 1, nz - 1)]
+    yxzp = yxp[:, :, np.minimum(np.arange(nz) + 1, nz - 1)]
+    f = np.float32
+    vals = [
+        (0, 0, 0, c),
+        (1, 0, 0, f(0.5) * (c + yp)),
+        (0, 1, 0, f(0.5) * (c + xp)),
+        (0, 0, 1, f(0.5) * (c + zp)),
+        (1, 0, 1, f(0.25) * (c + zp + yp + yzp)),
+        (0, 1, 1, f(0.25) * (c + zp + xp + xzp)),
+        (1, 1, 0, f(0.25) * (c + yp + xp + yxp)),
+        (1, 1, 1, f(0.125) * (c + yp + xp + zp + yzp + yxp + xzp + yxzp)),
+    ]
+    for sy, sx, sz, v in vals:   # on an undoubled axis the second index equals the first; later writes stand
+        out[sy * (dy - 1)::dy, sx * (dx - 1)::dx, sz * (dz - 1)::dz] = v
+    return out
+
+
+def run_double(option, s, r, d, dbl, high=(0.9, 0.7), image=None, img_fwi=None):
+    """One ImagingCorrelator::run on the doubled grid after applyFFTV (IMAGE1 or INVERSION): returns the
+    high-cut S and the updated sub-image stacks (image, img_fwi), each [nloop][ny][nx][nz] FP64."""
+    s = hightape(s, dbl, *high)
+    r = hightape(r, dbl, *high)
+    out = [None if a is None else np.array(a, np.float64) for a in (image, img_fwi)]
+    terms = [(s, r, 1.0)]
+    if option == 1:
+        axes = (0, 1, 2) if s.shape[0] > 1 else (0, 1)
+        terms += [(unit_derivative(s, a, d), unit_derivative(r, a, d), -1.0) for a in axes]
+    elif option != 0:
+        raise ValueError("the doubled grid takes IMAGE1 and INVERSION")
+    for ts, tr, sign in terms:
+        for i, (cs, cr) in enumerate(zip(interp(ts, dbl), interp(tr, dbl))):
+            if out[0] is not None:
+                out[0][i] += sign * cs * cr
+            if out[1] is not None and option == 1:
+                out[1][i] += cs * cr
+    return s, out[0], out[1]

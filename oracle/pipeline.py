@@ -215,10 +215,14 @@ class Shot:
         # the reference's decoder applies the same edge taper to every codec
         return _edge_taper(snap.reshape(self.shape))
 
-    def rtm(self, record, ic_option=0, kmin=1.0 / 1492.0):
+    def rtm(self, record, ic_option=0, kmin=1.0 / 1492.0, dbl=None, high=(0.9, 0.7)):
         """Image and illumination of the record.  ic_option selects ImagingCondition::getImagingOption's
         imaging condition (bgprtm/ImagingCondition.cpp:151-160): 0 the zero-lag product (FP32 sums), 1-3 the
-        FFT conditions of oracle.correlator (FP64 sums), which also return the FWI and raw images."""
+        FFT conditions of oracle.correlator (FP64 sums), which also return the FWI and raw images.  dbl =
+        (dz, dx, dy) images on the doubled grid (ImagingCondition::stackImageInterp, :104-141, and the
+        post-loop interleave / interpIllum, :415-446): returns image, illumination, FWI image there."""
+        if dbl is not None and tuple(dbl) != (1, 1, 1):
+            return self._rtm_double(record, ic_option, tuple(dbl), high)
         k = self.snap_every
         ntsav = (self.nt + k - 1) // k
         snaps = [None] * ntsav
@@ -256,6 +260,38 @@ class Shot:
             acc[0] = correlator.costerm(acc[0], (self.d, self.d, self.d))
             acc[1] = correlator.sinterm(acc[1], acc[0])
         return acc[0], ill, acc[1], acc[2]
+
+
+    def _rtm_double(self, record, ic_option, dbl, high):
+        k = self.snap_every
+        ntsav = (self.nt + k - 1) // k
+        snaps = [None] * ntsav
+
+        def store(it, w0):
+            if it % k == 0 and it // k < ntsav:
+                snaps[it // k] = compress16(w0) if self.snap_format == 1 else w0.copy()
+        self.forward(store)
+        f32 = np.float32
+        ntaperT = int(np.rint(f32(self.tmute_frac) * (f32(self.nt) * f32(self.dt)) / (f32(k) * f32(self.dt))))
+        wtT = [np.float32(0.5) * (np.float32(1) - np.cos(np.float32(np.pi) * np.float32(i) / np.float32(ntaperT)))
+               for i in range(ntaperT)]
+        nloop = len(correlator.double_copies(*dbl))
+        acc = [np.zeros((nloop,) + self.shape), np.zeros((nloop,) + self.shape)]
+        ill = np.zeros(self.shape, np.float32)
+
+        def image(it, w0):
+            if it % k == 0 and it // k < ntsav:
+                s = it // k
+                S = self._decode(snaps[s])
+                R = uncompress16(compress16(w0), *self.shape) if self.snap_format == 1 else _edge_taper(w0)
+                if s < ntaperT:
+                    S = (S * wtT[s]).astype(np.float32)
+                Sh, acc[0], acc[1] = correlator.run_double(ic_option, S, R, (self.d, self.d, self.d), dbl, high, *acc)
+                Sh = Sh.astype(np.float32)
+                ill[...] += Sh * Sh
+        self.backward(record, image)
+        fwi = correlator.interleave(acc[1], dbl) if ic_option == 1 else None
+        return correlator.interleave(acc[0], dbl), correlator.interp_illum(ill, dbl), fwi
 
 
 class Shot64:

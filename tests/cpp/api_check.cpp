@@ -292,6 +292,24 @@ int main() {
     for(size_t i = 0; i < g2->image.size(); i++) fin = fin && std::isfinite(g2->image[i]), efwi += ic2.fwi()[i] * ic2.fwi()[i];
     report("ImagingCondition::genImage INVERSION", fin && efwi > 0, efwi);
     konglong::imaging_params().LowcutFilter = IMAGE1;
+    // the doubled imaging grid with the high cut at Nyquist (an identity filter): the even samples (copy 0)
+    // carry the propagation-grid image and illumination up to FP32 FFT round-off
+    konglong::ImagingParams &ip = konglong::imaging_params();
+    ip.ic_double_z = ip.ic_double_x = ip.ic_double_y = 2, ip.highTapeZ = ip.highTapeXY = 1.0f;
+    ImagingCondition ic3(&src, &rec, STACK);
+    std::shared_ptr<Grid> g3 = ic3.genImage(&g0, &wfc, 0.f, 0.f, 0.f, 0.f, 1e-3f);
+    std::vector<float> even(cn), even_amp(cn);
+    for(int iy = 0; iy < cy; iy++)
+      for(int ix = 0; ix < cx; ix++)
+        for(int iz = 0; iz < cz; iz++) {
+          const size_t o = ((size_t)(2 * iy) * (2 * cx) + 2 * ix) * (2 * cz) + 2 * iz;
+          even[((size_t)iy * cx + ix) * cz + iz] = g3->image[o], even_amp[((size_t)iy * cx + ix) * cz + iz] = ic3.illum()[o];
+        }
+    const bool shape_ok = g3->nx == 2 * cx && g3->ny == 2 * cy && g3->nz == 2 * cz && g3->dx == 5.f &&
+                          g3->image.size() == (size_t)8 * cn;
+    report("ImagingCondition::genImage doubled grid", shape_ok && rel_l2(even, wimg) < 1e-5 && rel_l2(even_amp, wamp) < 1e-5,
+           std::max(rel_l2(even, wimg), rel_l2(even_amp, wamp)));
+    ip.ic_double_z = ip.ic_double_x = ip.ic_double_y = 1, ip.highTapeZ = 0.9f, ip.highTapeXY = 0.7f;
   }
   // Propagator::apply_symmetry, computeDamping, Lagrange
   {
