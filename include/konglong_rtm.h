@@ -165,6 +165,11 @@ int klrtm_kernel_deriv1(klrtm_kernel *k, int axis, const float *in, float *out, 
  * default 256) and tx, ychunk are ignored.  Used by the autotuner. */
 int klrtm_kernel_set_tile(klrtm_kernel *k, int tz, int tx, int ychunk);
 int klrtm_kernel_get_tile(klrtm_kernel *k, int *tz, int *tx, int *ychunk);
+/* compact (CFD) engine, 3D step: run the independent line passes (the y and z gradient derivatives, then
+ * the x and z divergence derivatives) concurrently on a second stream forked from and joined to the
+ * library stream (on: the default; results are bit-identical either way).  No reference counterpart:
+ * the reference runs its passes one after another on the CPU. */
+int klrtm_kernel_set_overlap(klrtm_kernel *k, int on);
 
 /* CfdPlan(n, d) (bgprtm/Cfd_plan.cpp:26-262), host: bm[2][n*5] (diagonal + 4 upper
  * bands per row), bl[2][n*4] (elimination multipliers per row), am[2][60] (RHS

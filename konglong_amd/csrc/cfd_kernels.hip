@@ -895,7 +895,8 @@ void cfd_free(Engine &e) {
 
 template<int AXIS, int PASS, int MODEL>
 static int launch_line(Engine &e, int sg, int round, const float *in, float *tmp, float *pr, float *pz, float *pb,
-                       const float *w1, const char *name) {
+                       const float *w1, const char *name, hipStream_t st = nullptr) {
+  if(!st) st = stream();
   CfdPassArgs a;
   memset(&a, 0, sizeof(a));
   a.nx = e.nx, a.ny = e.ny, a.nz = e.nz;
@@ -906,15 +907,16 @@ static int launch_line(Engine &e, int sg, int round, const float *in, float *tmp
   a.pml = e.desc;
   // lanes per block: 256 by default, or the tile tz set for the compact engine (64, 128 or 256)
   const int bs = e.tz == 64 || e.tz == 128 ? e.tz : 256;
-  LaunchTimer tm(name);
-  hipLaunchKernelGGL((cfd_line_kernel<AXIS, PASS, MODEL>), dim3((unsigned)div_up(a.nlines, bs)), dim3(bs), 0, stream(),
-                     in, tmp, pr, pz, pb, e.vel, w1, e.eps, e.del, e.pjx, e.pjy, a);
+  LaunchTimer tm(name, st);
+  hipLaunchKernelGGL((cfd_line_kernel<AXIS, PASS, MODEL>), dim3((unsigned)div_up(a.nlines, bs)), dim3(bs), 0, st, in,
+                     tmp, pr, pz, pb, e.vel, w1, e.eps, e.del, e.pjx, e.pjy, a);
   return check_launch(name);
 }
 
 template<int PASS>
 static int launch_zline(Engine &e, int sg, int round, const float *in, float *tmp, float *pb, const float *w1,
-                        const char *name) {
+                        const char *name, hipStream_t st = nullptr) {
+  if(!st) st = stream();
   CfdPassArgs a;
   memset(&a, 0, sizeof(a));
   a.nx = e.nx, a.ny = e.ny, a.nz = e.nz;
@@ -923,29 +925,62 @@ static int launch_zline(Engine &e, int sg, int round, const float *in, float *tm
   a.round = round;
   a.has_pml = e.has_pml;
   a.pml = e.desc;
-  LaunchTimer tm(name);
-  hipLaunchKernelGGL((cfd_zline_kernel<PASS>), dim3((unsigned)div_up(a.nlines, 64)), dim3(64), 0, stream(), in, tmp, pb,
-                     e.vel, w1, a);
+  LaunchTimer tm(name, st);
+  hipLaunchKernelGGL((cfd_zline_kernel<PASS>), dim3((unsigned)div_up(a.nlines, 64)), dim3(64), 0, st, in, tmp, pb, e.vel,
+                     w1, a);
   return check_launch(name);
+}
+
+// the second stream of the compact engine's concurrent passes: fork (side waits for the caller's stream) and
+// join (the caller's stream waits for side); both streams at the default priority
+static int side_fork(Engine &e) {
+  if(!e.side) {
+    KL_HIP(hipStreamCreateWithFlags(&e.side, hipStreamNonBlocking));
+    KL_HIP(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
+    KL_HIP(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
+  }
+  KL_HIP(hipEventRecord(e.ev_fork, stream()));
+  KL_HIP(hipStreamWaitEvent(e.side, e.ev_fork, 0));
+  return KLRTM_OK;
+}
+static int side_join(Engine &e) {
+  KL_HIP(hipEventRecord(e.ev_join, e.side));
+  KL_HIP(hipStreamWaitEvent(stream(), e.ev_join, 0));
+  return KLRTM_OK;
 }
 
 template<int MODEL>
 static int cfd_run_m(Engine &e, int mode, const float *p0, float *pb, float *pr) {
   int rc;
   const bool y = e.ny > 1;
+  const bool step = mode == MODE_STEP;
   float *px = e.d_px, *pz = e.d_pz;
   // gradient (isign=+1, plan 1, PML round 1): y, z, then x with the scalar operator
-  if(y && (rc = launch_line<AXY, P_GRAD, MODEL>(e, 0, 0, p0, pr, pr, pz, pb, p0, "cfd_grad_y"))) return rc;
-  if((rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z"))) return rc;
+  if(y && e.overlap) {
+    // the y and z passes read p0 and write disjoint volumes (pr, pz) and PML state components, so they run
+    // concurrently: more lines in flight for the latency-bound sweeps
+    if((rc = side_fork(e)) || (rc = launch_line<AXY, P_GRAD, MODEL>(e, 0, 0, p0, pr, pr, pz, pb, p0, "cfd_grad_y", e.side)) ||
+       (rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z")) || (rc = side_join(e)))
+      return rc;
+  } else {
+    if(y && (rc = launch_line<AXY, P_GRAD, MODEL>(e, 0, 0, p0, pr, pr, pz, pb, p0, "cfd_grad_y"))) return rc;
+    if((rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z"))) return rc;
+  }
   if((rc = launch_line<AXX, P_GRAD_SCALAR, MODEL>(e, 0, 0, p0, px, pr, pz, pb, p0, "cfd_grad_x"))) return rc;
   // divergence (isign=-1, plan 2, PML round 2): x, z, y as bgprtm/KernelCPU.cpp:108-114
-  const bool step = mode == MODE_STEP;
   if(y && step) {
     // Dx and Dz stay in px and pz; the y pass applies all three accumulates and the leapfrog, so pb
     // is read and written once.  The y pass keeps its b' in pr, which the step leaves as work space
-    // (the un-fused kernel call leaves the scalar operator's y input there, as KernelCPU does).
-    if((rc = launch_line<AXX, P_GRAD, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x"))) return rc;
-    if((rc = launch_zline<P_GRAD>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z"))) return rc;
+    // (the un-fused kernel call leaves the scalar operator's y input there, as KernelCPU does).  The x and z
+    // passes transform px and pz in place, independent of each other.
+    if(e.overlap) {
+      if((rc = side_fork(e)) || (rc = launch_line<AXX, P_GRAD, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x", e.side)) ||
+         (rc = launch_zline<P_GRAD>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z")) || (rc = side_join(e)))
+        return rc;
+    } else {
+      if((rc = launch_line<AXX, P_GRAD, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x"))) return rc;
+      if((rc = launch_zline<P_GRAD>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z"))) return rc;
+    }
     return launch_line<AXY, P_DIV3_STEP, MODEL>(e, 1, 1, pr, pr, px, pz, pb, p0, "cfd_div_y");
   }
   if((rc = launch_line<AXX, P_DIV, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x"))) return rc;

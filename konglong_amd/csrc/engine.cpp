@@ -48,6 +48,9 @@ Engine::~Engine() {
   (void)hipFree(d_px);
   (void)hipFree(d_pz);
   cfd_free(*this);
+  if(ev_fork) (void)hipEventDestroy(ev_fork);
+  if(ev_join) (void)hipEventDestroy(ev_join);
+  if(side) (void)hipStreamDestroy(side);
 }
 
 size_t Engine::face_cells(int f) const {
@@ -354,6 +357,12 @@ int klrtm_kernel_deriv1(klrtm_kernel *kk, int axis, const float *in, float *out,
   IsoArgs a;
   k->fill_args(a, 1);
   return launch_deriv1(k->ne, axis, iacc, isign > 0 ? 0 : 1, in, out, k->vel, a);
+}
+int klrtm_kernel_set_overlap(klrtm_kernel *kk, int on) {
+  Engine *k = reinterpret_cast<Engine *>(kk);
+  KL_REQUIRE(k, KLRTM_EINVAL, "null engine");
+  k->overlap = on ? 1 : 0;
+  return KLRTM_OK;
 }
 int klrtm_kernel_set_tile(klrtm_kernel *kk, int tz, int tx, int ychunk) {
   Engine *k = reinterpret_cast<Engine *>(kk);

@@ -33,6 +33,11 @@ struct Engine {
   int tz = 0, tx = 0, ychunk = 0;
   // compact-FD plans (device)
   CfdDevPlan *cfd = nullptr;
+  // compact engine: run the independent line passes of a step (the y and z gradients, the x and z
+  // divergence derivatives) concurrently on a second stream forked from and joined to the caller's
+  int overlap = 1;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   ~Engine();
   size_t face_cells(int f) const;

@@ -48,7 +48,8 @@ __device__ __forceinline__ int step_index(const StepRef &r) { return r.step ? *r
 struct LaunchTimer {
   const char *name;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  explicit LaunchTimer(const char *kernel_name);
+  explicit LaunchTimer(const char *kernel_name, hipStream_t s = nullptr);  // s: the launch's stream if not stream()
+  hipStream_t st = nullptr;
   ~LaunchTimer();
 };
 int check_launch(const char *name);

@@ -80,19 +80,19 @@ static void drain_pending() {
   g_pending.clear();
 }
 
-LaunchTimer::LaunchTimer(const char *kernel_name) : name(kernel_name) {
+LaunchTimer::LaunchTimer(const char *kernel_name, hipStream_t s) : name(kernel_name), st(s ? s : stream()) {
   if(t_capturing) return;
   std::lock_guard<std::mutex> lk(g_tmu);
   if(!g_timer_on) return;
   if(g_pending.size() > 4096) drain_pending();
   e0 = get_event();
   e1 = get_event();
-  if(e0) (void)hipEventRecord(e0, stream());
+  if(e0) (void)hipEventRecord(e0, st);
 }
 LaunchTimer::~LaunchTimer() {
   if(!e0 || !e1) return;
   std::lock_guard<std::mutex> lk(g_tmu);
-  (void)hipEventRecord(e1, stream());
+  (void)hipEventRecord(e1, st);
   g_pending.push_back({name, e0, e1});
 }
 

@@ -49,6 +49,10 @@ class KernelCPU:
     def set_tile(self, tz=0, tx=0, ychunk=0):
         check(lib().klrtm_kernel_set_tile(self.h, tz, tx, ychunk))
 
+    def set_overlap(self, on=True):
+        """Compact engine: independent line passes of a step on two streams (bit-identical results)."""
+        check(lib().klrtm_kernel_set_overlap(self.h, int(on)))
+
     def get_tile(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         check(lib().klrtm_kernel_get_tile(self.h, C.byref(a), C.byref(b), C.byref(c)))
