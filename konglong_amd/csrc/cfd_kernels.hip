@@ -1080,6 +1080,7 @@ int cfd_deriv1(Engine &e, int axis, int iacc, int isign, const float *in, float 
 }
 
 int cfd_run(Engine &e, int model, int mode, const float *p0, float *pb, float *pr) {
+  LaunchTimer tm("cfd_step");  // the whole step on the caller's stream (the side stream's passes joined into it)
   if(e.rho) {
     if(model == KLRTM_VTI) return cfd_run_rho<KLRTM_VTI>(e, mode, p0, pb, pr);
     if(model == KLRTM_TTI) return cfd_run_rho<KLRTM_TTI>(e, mode, p0, pb, pr);

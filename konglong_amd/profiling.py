@@ -9,7 +9,9 @@ known bytes of a plain device-to-device copy in the same run (the largest
 __amd_rocclr_copyBuffer dispatch, `copy_bytes` read and written) to what the
 counter reports for it.  The per-step totals of the kernels whose names start
 with one of the group prefixes (each weighted by its launches per step) are written as `fetch_bytes_per_step` and
-`write_bytes_per_step` lines, which bench.py reads as the roofline traffic.
+`write_bytes_per_step` lines, which bench.py reads as the roofline traffic.  With a kernel trace in the stats
+directory, `step_kernel_us` is the time per step during which at least one group kernel runs (kernels on
+concurrent streams count once); the plain sum is `step_kernel_sum_us`.
 """
 import argparse
 import csv
@@ -30,6 +32,30 @@ def read_stats(d):
     for r in csv.DictReader(open(f)):
         out[_short(r["Name"])] = (int(r["Calls"]), float(r["AverageNs"]) / 1e3, float(r["Percentage"]))
     return out
+
+
+def read_busy(d, prefixes):
+    """Time (ns) during which at least one kernel of the group runs, from the kernel trace: the union of the
+    group's dispatch intervals, so that concurrent kernels count once (None without a trace)."""
+    fs = glob.glob(os.path.join(d, "*kernel_trace.csv"))
+    if not fs:
+        return None
+    iv = []
+    for r in csv.DictReader(open(fs[0])):
+        if any(_short(r["Kernel_Name"]).startswith(p) for p in prefixes):
+            iv.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    iv.sort()
+    busy, cur0, cur1 = 0, None, None
+    for a, b in iv:
+        if cur1 is None or a > cur1:
+            if cur1 is not None:
+                busy += cur1 - cur0
+            cur0, cur1 = a, b
+        else:
+            cur1 = max(cur1, b)
+    if cur1 is not None:
+        busy += cur1 - cur0
+    return busy
 
 
 def read_counter(d, name):
@@ -89,6 +115,10 @@ def main():
             tf += f * per
             tw += w * per
             tus += us * per
+    busy = read_busy(a.stats, prefixes)
+    if busy is not None:   # concurrent kernels of one step counted once
+        print(f"step_kernel_sum_us: {tus:.1f}")
+        tus = busy / 1e3 / steps
     print(f"step_kernel_us: {tus:.1f}")
     print(f"fetch_bytes_per_step: {tf:.0f}")
     print(f"write_bytes_per_step: {tw:.0f}")
