@@ -27,11 +27,12 @@ class Autotuner:
 
     Candidates are z x x tiles of at most 512 threads and y-chunk lengths; each is timed over a few
     forward steps with the library's kernel timers, and the fastest is kept per (grid, engine, order,
-    model).  The propagator's wavefields are reset afterwards."""
+    model).  For the compact engine a candidate is (lanes per line block, concurrent passes 0/1, 0).
+    The propagator's wavefields are reset afterwards."""
 
     TILES = ((64, 8), (64, 4), (32, 8), (128, 4))
     CFD_BLOCKS = (64, 128, 256)  # lanes per block of the compact engine's x/y line passes
-    CFD_TIMERS = (b"cfd_grad_y", b"cfd_grad_z", b"cfd_grad_x", b"cfd_div_x", b"cfd_div_z", b"cfd_div_y")
+    CFD_TIMERS = (b"cfd_step",)  # the step's span: its concurrent passes count once
     TILES_FIRST_DERIV = TILES + ((16, 16),)  # the gradient/divergence kernels also take 16 x 16
     YCHUNKS = (0, 16, 32, 64)   # 0: the engine's default chunking
 
@@ -52,7 +53,10 @@ class Autotuner:
         if key not in self.cache:
             self.cache[key] = self._search(p, k)
         best = self.cache[key][0]
-        if best is not None:
+        if best is not None and cfg.engine != ENGINE_FD:
+            check(lib().klrtm_kernel_set_tile(k, best[0], 0, 0))
+            check(lib().klrtm_kernel_set_overlap(k, best[1]))
+        elif best is not None:
             check(lib().klrtm_kernel_set_tile(k, best[0], best[1], best[2] if best[2] > 0 else -1))
         return best
 
@@ -79,12 +83,14 @@ class Autotuner:
         L = lib()
         table = {}
         p.reset()
-        if p.cfg.engine != ENGINE_FD:   # compact engine: lanes per block of the line passes
+        if p.cfg.engine != ENGINE_FD:   # compact engine: lanes per block of the line passes, concurrent passes
             for bs in self.CFD_BLOCKS:
-                check(L.klrtm_kernel_set_tile(k, bs, 0, 0))
-                ms = self._time(p, self.CFD_TIMERS)
-                if ms is not None:
-                    table[(bs, 0, 0)] = ms
+                for ov in (1, 0):
+                    check(L.klrtm_kernel_set_tile(k, bs, 0, 0))
+                    check(L.klrtm_kernel_set_overlap(k, ov))
+                    ms = self._time(p, self.CFD_TIMERS)
+                    if ms is not None:
+                        table[(bs, ov, 0)] = ms
         else:
             names = (b"iso_step",) if p.cfg.model == ISO else (b"fd_grad", b"fd_div_step")
             R = p.cfg.order // 2

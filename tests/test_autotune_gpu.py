@@ -40,7 +40,7 @@ def test_autotuner_picks_a_candidate_and_keeps_results(klrtm, case):
             assert best is not None and best == best2 and best in table
             assert table[best] == min(table.values())
             if engine == ENGINE_CFD:
-                assert set(table) == {(64, 0, 0), (128, 0, 0), (256, 0, 0)}
+                assert set(table) == {(bs, ov, 0) for bs in (64, 128, 256) for ov in (0, 1)}
             assert tuner.tune(p) == best            # cached
         recs.append(p.model())
         p.close()
