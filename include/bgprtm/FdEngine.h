@@ -13,7 +13,6 @@
 
 #include "konglong_gpu.h"
 
-class Grid;
 
 class FdEngine {
 public:
@@ -27,9 +26,10 @@ public:
   FdEngine &operator=(const FdEngine &) = delete;
 
   void setModel(float *vel) { konglong::check(klrtm_kernel_set_model(k_, vel, nullptr, nullptr, nullptr, nullptr, nullptr)); }
-  virtual void setJacob(Grid *, float *jacobx, float *jacoby, float *jacobz) {
-    if(jacobx || jacoby || jacobz)
-      throw konglong::Error(KLRTM_EINVAL, "konglong: irregular-grid jacobians are not supported");
+  // FdEngine_fd::setJacob (bgprtm/FdEngine_fd.cpp:92-131): IRREGULAR grids take per-iz z weights from zgrid
+  virtual void setJacob(Grid *grid, float *jacobx, float *jacoby, float *jacobz) {
+    (void)jacobx, (void)jacoby, (void)jacobz;
+    konglong::set_zgrid_of(k_, grid);
   }
   void setBoundary(int nxbnd1, int nxbnd2, int nybnd1, int nybnd2, int nzbnd1, int nzbnd2, float dt, float vmax) {
     konglong::check(klrtm_kernel_set_boundary(k_, nxbnd1, nxbnd2, nybnd1, nybnd2, nzbnd1, nzbnd2, dt, vmax));

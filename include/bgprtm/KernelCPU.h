@@ -10,7 +10,6 @@
 
 #include "konglong_gpu.h"
 
-class Grid;
 
 class KernelCPU {
 public:
@@ -29,11 +28,12 @@ public:
   void setModel(float *vel, float *rho, float *del, float *eps, float *pjx, float *pjy) {
     konglong::check(klrtm_kernel_set_model(k_, vel, rho, del, eps, pjx, pjy));
   }
-  // only the identity (regular grid) is supported
+  // bgprtm/KernelCPU.cpp:85-87 -> FdEngine_fd::setJacob (bgprtm/FdEngine_fd.cpp:92-131): a RECTANGLE grid (or
+  // none) keeps 1/dz; an IRREGULAR grid takes its per-iz z weights from grid->zgrid.  The x/y jacobians
+  // serve the pyramid grids, which are not built; the FD engine ignores jacobz, as the reference's does.
   void setJacob(Grid *grid, float *jacobx, float *jacoby, float *jacobz) {
-    if(jacobx || jacoby || jacobz)
-      throw konglong::Error(KLRTM_EINVAL, "konglong: irregular-grid jacobians are not supported");
-    (void)grid;
+    (void)jacobx, (void)jacoby, (void)jacobz;
+    konglong::set_zgrid_of(k_, grid);
   }
   void setBoundary(int nxbnd1, int nxbnd2, int nybnd1, int nybnd2, int nzbnd1, int nzbnd2, float dt, float vmax) {
     konglong::check(klrtm_kernel_set_boundary(k_, nxbnd1, nxbnd2, nybnd1, nybnd2, nzbnd1, nzbnd2, dt, vmax));

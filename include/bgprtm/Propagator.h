@@ -54,16 +54,7 @@ private:
 };
 #endif
 
-#ifndef KONGLONG_NO_GRID
-// bgprtm/Grid.h, rectangular grids only: sizes, spacings, origin (z0 is the depth of index 0) and, for the
-// grid Propagator::rtm returns, the shot image on it ((iy*nx + ix)*nz + iz)
-class Grid {
-public:
-  int nx = 0, ny = 0, nz = 0;
-  float dx = 0, dy = 0, dz = 0, x0 = 0, y0 = 0, z0 = 0;
-  std::vector<float> image;
-};
-#endif
+// Grid: konglong_gpu.h
 
 #ifndef KONGLONG_NO_OUTPUTGRID
 // bgprtm/OutputGrid.h

@@ -100,4 +100,32 @@ inline void synchronize() { check(klrtm_synchronize()); }
 
 }  // namespace konglong
 
+#ifndef KONGLONG_NO_GRID
+// bgprtm/Grid.h:9-11
+enum GRIDTYPE { RECTANGLE = 0, IRREGULAR, XPYRAMID, YPYRAMID, XYPYRAMID };
+// bgprtm/Grid.h: sizes, spacings, origin (z0 is the depth of index 0), the grid type with the depth of every
+// z index for IRREGULAR grids (zgrid), and, for the grid Propagator::rtm returns, the shot image on it
+// ((iy*nx + ix)*nz + iz)
+class Grid {
+public:
+  int nx = 0, ny = 0, nz = 0;
+  float dx = 0, dy = 0, dz = 0, x0 = 0, y0 = 0, z0 = 0;
+  int mytype = RECTANGLE;
+  std::vector<float> zgrid;
+  std::vector<float> image;
+};
+#endif
+
+namespace konglong {
+// the z weights of an FD engine for a grid (FdEngine_fd::setJacob, bgprtm/FdEngine_fd.cpp:92-131): none or
+// RECTANGLE keeps 1/dz, IRREGULAR takes grid->zgrid; the pyramid grids are not built
+template<class G> inline void set_zgrid_of(klrtm_kernel *k, const G *grid) {
+  if(grid && grid->mytype != RECTANGLE && grid->mytype != IRREGULAR)
+    throw Error(KLRTM_EINVAL, "konglong: pyramid grids are not supported");
+  const bool irr = grid && grid->mytype == IRREGULAR;
+  check(klrtm_kernel_set_zgrid(k, irr ? grid->zgrid.data() : nullptr));
+}
+inline void set_zgrid_of(klrtm_kernel *k, std::nullptr_t) { check(klrtm_kernel_set_zgrid(k, nullptr)); }
+}  // namespace konglong
+
 #endif  // KONGLONG_GPU_H_

@@ -165,6 +165,12 @@ int klrtm_kernel_deriv1(klrtm_kernel *k, int axis, const float *in, float *out, 
  * default 256) and tx, ychunk are ignored.  Used by the autotuner. */
 int klrtm_kernel_set_tile(klrtm_kernel *k, int tz, int tx, int ychunk);
 int klrtm_kernel_get_tile(klrtm_kernel *k, int *tz, int *tx, int *ychunk);
+/* FdEngine_fd::setJacob for an IRREGULAR grid (bgprtm/FdEngine_fd.cpp:92-131): zgrid[nz] holds the depth of
+ * every z index (increasing); the Laplacian and first-derivative z weights become per-iz, from
+ * dz/dtau and d2z/dtau2 of zgrid (libCommon's Taylor::derive1/derive2, not in the reference tree: central
+ * Taylor differences of order 2 min(ne, iz, nz-1-iz), one-sided at the ends).  NULL returns to the regular
+ * 1/dz.  FD engine only (the compact engine's jacobz scaling is not built). */
+int klrtm_kernel_set_zgrid(klrtm_kernel *k, const float *zgrid);
 /* compact (CFD) engine, 3D step: run the independent line passes (the y and z gradient derivatives, then
  * the x and z divergence derivatives) concurrently on a second stream forked from and joined to the
  * library stream (on: the default; results are bit-identical either way).  No reference counterpart:
@@ -489,6 +495,9 @@ int klrtm_propagator_set_convae(klrtm_propagator *p, klrtm_convae *c, float floo
  * klrtm_propagator_set_graph again.  graph_stats: graphs held and replays launched. */
 int klrtm_propagator_set_graph(klrtm_propagator *p, int on);
 int klrtm_propagator_graph_stats(klrtm_propagator *p, int *ngraphs, long *replays);
+/* an IRREGULAR z grid for the shot (klrtm_kernel_set_zgrid on its engine; NULL: regular).  Source and
+ * receiver z coordinates stay grid indices, as grid->getIDz gives them in the reference. */
+int klrtm_propagator_set_zgrid(klrtm_propagator *p, const float *zgrid);
 /* the propagator's engine (for klrtm_kernel_set_tile) */
 int klrtm_propagator_kernel(klrtm_propagator *p, klrtm_kernel **k);
 /* device pointers of the shot's image accumulators (FP32 image, illumination; FP64 image, illumination); on a

@@ -82,6 +82,8 @@ _SIGS = {
     "klrtm_kernel_step": (_i, [_vp, _i, _vp, _vp, _vp]),
     "klrtm_kernel_set_tile": (_i, [_vp, _i, _i, _i]),
     "klrtm_kernel_set_overlap": (_i, [_vp, _i]),
+    "klrtm_kernel_set_zgrid": (_i, [_vp, _vp]),
+    "klrtm_propagator_set_zgrid": (_i, [_vp, _vp]),
     "klrtm_kernel_get_tile": (_i, [_vp, _pi, _pi, _pi]),
     "klrtm_cfd_plan": (_i, [_i, _f, _pf, _pf, _pf]),
     "klrtm_spread_coeff": (_i, [_i, _pf, _f, _f, _f, _pi, _pi]),

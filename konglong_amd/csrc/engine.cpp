@@ -30,6 +30,71 @@ void taylor_coefs(int M, float *c1, float *c2) {
   c2[0] = float(-2.0 * s);
 }
 
+// FP64 Taylor weights of order 2M (the same closed form as taylor_coefs)
+static void taylor_coefs_d(int M, double *c1, double *c2) {
+  double s = 0;
+  c1[0] = 0;
+  for(int j = 1; j <= M; j++) {
+    double r = 1.0;
+    for(int k = 1; k <= j; k++) r *= double(M - j + k) / double(M + k);
+    const double sg = (j & 1) ? 1.0 : -1.0;
+    c1[j] = sg * r / j;
+    c2[j] = 2.0 * sg * r / (double(j) * j);
+    s += c2[j];
+  }
+  c2[0] = -2.0 * s;
+}
+
+// dz/dtau and d2z/dtau2 of the grid depths against the index (libCommon's Taylor::derive1 / derive2 with
+// d = 1, which the reference tree does not hold): central Taylor differences of order 2 min(ne, i, n-1-i),
+// first-order one-sided first derivatives and the neighbour's second derivative at the two ends
+static void zgrid_derivatives(const float *z, int n, int ne, float *zp, float *zpp) {
+  std::vector<double> c1(ne + 1), c2(ne + 1);
+  for(int i = 0; i < n; i++) {
+    const int nev = std::min(ne, std::min(i, n - 1 - i));
+    if(nev == 0) continue;
+    taylor_coefs_d(nev, c1.data(), c2.data());
+    double d1 = 0, d2 = c2[0] * z[i];
+    for(int j = 1; j <= nev; j++) d1 += c1[j] * ((double)z[i + j] - z[i - j]), d2 += c2[j] * ((double)z[i + j] + z[i - j]);
+    zp[i] = (float)d1, zpp[i] = (float)d2;
+  }
+  if(n > 1) {
+    zp[0] = (float)((double)z[1] - z[0]), zp[n - 1] = (float)((double)z[n - 1] - z[n - 2]);
+    zpp[0] = n > 2 ? zpp[1] : 0.f, zpp[n - 1] = n > 2 ? zpp[n - 2] : 0.f;
+  }
+}
+
+int Engine::set_zgrid(const float *zg) {
+  (void)hipFree(d_zc2), (void)hipFree(d_zc1);
+  d_zc2 = d_zc1 = nullptr;
+  if(!zg) return KLRTM_OK;
+  KL_REQUIRE(engine == KLRTM_ENGINE_FD, KLRTM_EINVAL,
+             "irregular z grids run on the FD engine (the compact engine's jacobz scaling is not built)");
+  KL_REQUIRE(nz > 2 * ne, KLRTM_EINVAL, "an irregular z grid needs nz > %d", 2 * ne);
+  std::vector<float> zp(nz), zpp(nz);
+  zgrid_derivatives(zg, nz, ne, zp.data(), zpp.data());
+  for(int iz = 0; iz < nz; iz++)
+    KL_REQUIRE(zp[iz] > 0.f, KLRTM_EINVAL, "z grid depths must increase (dz/dtau %g at iz %d)", (double)zp[iz], iz);
+  // FdEngine_fd::setJacob (bgprtm/FdEngine_fd.cpp:108-127) with const_ne: the order-2ne weights at every iz
+  std::vector<float> t2((size_t)nz * ZC2_STRIDE, 0.f), t1((size_t)nz * ZC1_STRIDE, 0.f);
+  for(int iz = 0; iz < nz; iz++) {
+    const float zpi = 1 / zp[iz], zpi2 = zpi * zpi, slope = -zpp[iz] * zpi2;
+    float *r2 = &t2[(size_t)iz * ZC2_STRIDE], *r1 = &t1[(size_t)iz * ZC1_STRIDE];
+    r2[0] = zpi2 * c2[0];
+    for(int j = 1; j <= ne; j++) {
+      const float c = zpi * c1[j], a = zpi2 * c2[j], b = slope * c;
+      r1[j] = c;
+      r2[j] = a - b;            // weight of u[iz - j]
+      r2[ZC_MAXR + j] = a + b;  // weight of u[iz + j]
+    }
+  }
+  KL_HIP(hipMalloc(&d_zc2, t2.size() * sizeof(float)));
+  KL_HIP(hipMalloc(&d_zc1, t1.size() * sizeof(float)));
+  KL_HIP(hipMemcpy(d_zc2, t2.data(), t2.size() * sizeof(float), hipMemcpyHostToDevice));
+  KL_HIP(hipMemcpy(d_zc1, t1.data(), t1.size() * sizeof(float), hipMemcpyHostToDevice));
+  return KLRTM_OK;
+}
+
 // PML profile of one face: bgprtm/PML.cpp:23-60 (non limit_slope branch), FP32.
 void pml_profile(int npml, float dx, float dt, float vmax, float pml_eps, float *coef, float *feoc) {
   float beta = dx / (dt * vmax);
@@ -47,6 +112,7 @@ Engine::~Engine() {
   (void)hipFree(d_state);
   (void)hipFree(d_px);
   (void)hipFree(d_pz);
+  (void)hipFree(d_zc2), (void)hipFree(d_zc1);
   cfd_free(*this);
   if(ev_fork) (void)hipEventDestroy(ev_fork);
   if(ev_join) (void)hipEventDestroy(ev_join);
@@ -199,10 +265,12 @@ void Engine::fill_args(IsoArgs &a, int deriv_order) {
     float zpi = 1 / dz, zpi2 = zpi * zpi;
     float dx2i = float(1. / (dx * dx)), dy2i = float(1. / (dy * dy));
     for(int m = 0; m <= ne; m++) a.cz[m] = zpi2 * c2[m], a.cx[m] = c2[m] * dx2i, a.cy[m] = c2[m] * dy2i;
+    a.zc2 = d_zc2;
   } else {
     float zpi = 1 / dz, dxi = 1 / dx, dyi = 1 / dy;
     for(int m = 0; m <= ne; m++) a.cz[m] = zpi * c1[m], a.cx[m] = c1[m] * dxi, a.cy[m] = c1[m] * dyi;
     a.rho = rho;
+    a.zc1 = d_zc1;
   }
 }
 
@@ -357,6 +425,11 @@ int klrtm_kernel_deriv1(klrtm_kernel *kk, int axis, const float *in, float *out,
   IsoArgs a;
   k->fill_args(a, 1);
   return launch_deriv1(k->ne, axis, iacc, isign > 0 ? 0 : 1, in, out, k->vel, a);
+}
+int klrtm_kernel_set_zgrid(klrtm_kernel *kk, const float *zgrid) {
+  Engine *k = reinterpret_cast<Engine *>(kk);
+  KL_REQUIRE(k, KLRTM_EINVAL, "null engine");
+  return k->set_zgrid(zgrid);
 }
 int klrtm_kernel_set_overlap(klrtm_kernel *kk, int on) {
   Engine *k = reinterpret_cast<Engine *>(kk);

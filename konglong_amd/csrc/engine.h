@@ -33,6 +33,10 @@ struct Engine {
   int tz = 0, tx = 0, ychunk = 0;
   // compact-FD plans (device)
   CfdDevPlan *cfd = nullptr;
+  // irregular z grid (FdEngine_fd::setJacob): per-iz weights of the z part on the device, null on a
+  // regular grid
+  float *d_zc2 = nullptr, *d_zc1 = nullptr;
+  int set_zgrid(const float *zgrid);
   // compact engine: run the independent line passes of a step (the y and z gradients, the x and z
   // divergence derivatives) concurrently on a second stream forked from and joined to the caller's
   int overlap = 1;

@@ -19,7 +19,12 @@ struct IsoArgs {
   float cz[9], cx[9], cy[9];  // [0..R]: 2nd-derivative (Laplacian) or 1st-derivative (grad/div) weights
   PmlDesc pml;
   const float *rho;  // density (first-derivative form only): gradient / rho, divergence terms * vel * rho
+  // irregular z grid (null on a regular one): per-iz weights of the z part, [iz][ZC2_STRIDE] for the
+  // Laplacian (0: centre, m: u[iz-m], ZC_MAXR+m: u[iz+m]) and [iz][ZC1_STRIDE] for the first derivative
+  // (m: weight of u[iz+m], minus it for u[iz-m])
+  const float *zc2, *zc1;
 };
+constexpr int ZC_MAXR = 8, ZC2_STRIDE = 2 * ZC_MAXR + 1, ZC1_STRIDE = ZC_MAXR + 1;
 
 int launch_iso(int R, int mode, const float *u, float *w0, const float *vel, float *pr, const IsoArgs &a);
 int launch_grad(int R, int model, const float *u, float *px, float *py, float *pz, const float *eps, const float *del,

@@ -154,6 +154,19 @@ __global__ void __launch_bounds__(512) iso_step_kernel(const float *__restrict__
   }
   float *srow0 = s + (c.tx + R) * S + c.tz + R;                // LDS tile of even planes
   float *srow1 = srow0 + (size_t)(a.tx + 2 * R) * S;            // and of odd planes
+  // irregular z grid: this block's per-iz weights of the z part, after the two tiles
+  // (FdEngine_fd::setJacob, bgprtm/FdEngine_fd.cpp:92-131)
+  const bool irr = a.zc2 != nullptr;
+  float *zct = s + 2 * (size_t)(a.tx + 2 * R) * S + c.tz * (2 * R + 1);
+  if(irr) {
+    if(c.tx == 0) {
+      const float *g = a.zc2 + (size_t)min(iz, nz - 1) * ZC2_STRIDE;
+      zct[0] = g[0];
+#pragma unroll
+      for(int m = 1; m <= R; m++) zct[m] = g[m], zct[R + m] = g[ZC_MAXR + m];
+    }
+    lds_barrier();
+  }
 
   auto round2 = [](float t0, float t1, float t2, float p, float &q1, float &q2) {
     float qi = t0 * q1 - t1 * p;
@@ -204,11 +217,20 @@ __global__ void __launch_bounds__(512) iso_step_kernel(const float *__restrict__
       lds_barrier();
       const float ctr = q[R];
       // z part, reference order m = -R..-1, 1..R (bgprtm/FdEngine_fd.cpp:223-236)
-      float lz = a.cz[0] * ctr;
+      float lz;
+      if(irr) {
+        lz = zct[0] * ctr;
 #pragma unroll
-      for(int m = R; m >= 1; m--) lz = fmaf(a.cz[m], srow[-m], lz);
+        for(int m = R; m >= 1; m--) lz = fmaf(zct[m], srow[-m], lz);
 #pragma unroll
-      for(int m = 1; m <= R; m++) lz = fmaf(a.cz[m], srow[m], lz);
+        for(int m = 1; m <= R; m++) lz = fmaf(zct[R + m], srow[m], lz);
+      } else {
+        lz = a.cz[0] * ctr;
+#pragma unroll
+        for(int m = R; m >= 1; m--) lz = fmaf(a.cz[m], srow[-m], lz);
+#pragma unroll
+        for(int m = 1; m <= R; m++) lz = fmaf(a.cz[m], srow[m], lz);
+      }
       // x part (bgprtm/FdEngine_fd.cpp:262-274)
       float lx = a.cx[0] * ctr;
 #pragma unroll
@@ -312,6 +334,17 @@ __global__ void __launch_bounds__(512) grad_kernel(const float *__restrict__ u, 
     q[k] = (c.act && yy >= 0 && yy < ny) ? u[(size_t)yy * nxz + c.col] : 0.f;
   }
   float *srow = s + (c.tx + R) * S + c.tz + R;
+  // irregular z grid: this block's per-iz first-derivative weights, after the tile
+  const bool irr = a.zc1 != nullptr;
+  float *zct = s + (size_t)(a.tx + 2 * R) * S + c.tz * (R + 1);
+  if(irr) {
+    if(c.tx == 0) {
+      const float *g = a.zc1 + (size_t)min(iz, nz - 1) * ZC1_STRIDE;
+#pragma unroll
+      for(int m = 1; m <= R; m++) zct[m] = g[m];
+    }
+    lds_barrier();
+  }
   float qn = 0.f;
   Halo hn = load_halo<R>(u, (size_t)c.y0 * nxz, c, a, true, true);
   if(c.act && c.y0 + R < ny) qn = u[(size_t)(c.y0 + R) * nxz + c.col];
@@ -343,10 +376,17 @@ __global__ void __launch_bounds__(512) grad_kernel(const float *__restrict__ u, 
     if(c.act) {
       // z: vz accumulates (-c_m) u[-m] for m=R..1, then c_m u[+m] (bgprtm/FdEngine_fd.cpp:394-403)
       float gz = 0.f;
+      if(irr) {
 #pragma unroll
-      for(int m = R; m >= 1; m--) gz = fmaf(-a.cz[m], srow[-m], gz);
+        for(int m = R; m >= 1; m--) gz = fmaf(-zct[m], srow[-m], gz);
 #pragma unroll
-      for(int m = 1; m <= R; m++) gz = fmaf(a.cz[m], srow[m], gz);
+        for(int m = 1; m <= R; m++) gz = fmaf(zct[m], srow[m], gz);
+      } else {
+#pragma unroll
+        for(int m = R; m >= 1; m--) gz = fmaf(-a.cz[m], srow[-m], gz);
+#pragma unroll
+        for(int m = 1; m <= R; m++) gz = fmaf(a.cz[m], srow[m], gz);
+      }
       // x: vx -= c u[-m]; vx += c u[+m] (bgprtm/FdEngine_fd.cpp:420-431)
       float gx = 0.f;
 #pragma unroll
@@ -412,6 +452,17 @@ __global__ void __launch_bounds__(512) div_kernel(const float *__restrict__ px, 
   }
   float *zrow = sz + c.tx * SZ + c.tz + R;
   float *xrow = sx + (c.tx + R) * TZ + c.tz;
+  // irregular z grid: this block's per-iz first-derivative weights, after the two tiles
+  const bool irr = a.zc1 != nullptr;
+  float *zct = sx + (size_t)(TX + 2 * R) * TZ + c.tz * (R + 1);
+  if(irr) {
+    if(c.tx == 0) {
+      const float *g = a.zc1 + (size_t)min(iz, nz - 1) * ZC1_STRIDE;
+#pragma unroll
+      for(int m = 1; m <= R; m++) zct[m] = g[m];
+    }
+    lds_barrier();
+  }
   auto fetch = [&](size_t p, float &qv, float &cz, float &cx, Halo &hz, Halo &hx, float &wv, float &vv, float &uv,
                    float &rv, int iyp) {
     hz = load_halo<R>(pz, p, c, a, true, false);
@@ -449,10 +500,17 @@ __global__ void __launch_bounds__(512) div_kernel(const float *__restrict__ px, 
         dxv = fmaf(a.cx[m], xrow[m * TZ], dxv);
       }
       float dzv = 0.f;
+      if(irr) {
 #pragma unroll
-      for(int m = R; m >= 1; m--) dzv = fmaf(-a.cz[m], zrow[-m], dzv);
+        for(int m = R; m >= 1; m--) dzv = fmaf(-zct[m], zrow[-m], dzv);
 #pragma unroll
-      for(int m = 1; m <= R; m++) dzv = fmaf(a.cz[m], zrow[m], dzv);
+        for(int m = 1; m <= R; m++) dzv = fmaf(zct[m], zrow[m], dzv);
+      } else {
+#pragma unroll
+        for(int m = R; m >= 1; m--) dzv = fmaf(-a.cz[m], zrow[-m], dzv);
+#pragma unroll
+        for(int m = 1; m <= R; m++) dzv = fmaf(a.cz[m], zrow[m], dzv);
+      }
       float dyv = 0.f;
       if(do_y) {
 #pragma unroll
@@ -500,7 +558,7 @@ __global__ void __launch_bounds__(256) fd_deriv1_kernel(const float *__restrict_
   const int iz = (int)(i % nz), ix = (int)((i / nz) % nx), iy = (int)(i / (long)nxz);
   const int pos = AXIS == 0 ? iz : AXIS == 1 ? ix : iy, np = AXIS == 0 ? nz : AXIS == 1 ? nx : ny;
   const size_t st = AXIS == 0 ? 1 : AXIS == 1 ? (size_t)nz : nxz;
-  const float *c = AXIS == 0 ? a.cz : AXIS == 1 ? a.cx : a.cy;
+  const float *c = AXIS == 0 ? (a.zc1 ? a.zc1 + (size_t)iz * ZC1_STRIDE : a.cz) : AXIS == 1 ? a.cx : a.cy;
   float d = 0.f;
   if(np > 1) {
 #pragma unroll
@@ -562,6 +620,7 @@ static int launch_iso_r(int mode, const float *u, float *w0, const float *vel, f
   grid_of(a);
   int nb = a.nbz * a.nbx * a.nby;
   size_t lds = 2 * sizeof(float) * (size_t)(a.tx + 2 * R) * (a.tz + 2 * R);  // two plane tiles
+  if(a.zc2) lds += sizeof(float) * (size_t)a.tz * (2 * R + 1);              // + the z weights per iz
   LaunchTimer tm(mode == MODE_STEP ? "iso_step" : mode == MODE_KERNEL ? "iso_kernel" : mode == MODE_PB ? "iso_pb"
                                                                                                 : "laplacian");
   if(mode == MODE_STEP) return launch_iso_rm<R, MODE_STEP>(u, w0, vel, pr, a, nb, lds);
@@ -587,6 +646,7 @@ static int launch_grad_rm(const float *u, float *px, float *py, float *pz, const
   grid_of(a);
   int nb = a.nbz * a.nbx * a.nby;
   size_t lds = sizeof(float) * (size_t)(a.tx + 2 * R) * (a.tz + 2 * R);
+  if(a.zc1) lds += sizeof(float) * (size_t)a.tz * (R + 1);
   LaunchTimer tm("fd_grad");
   if(a.rho)
     hipLaunchKernelGGL((grad_kernel<R, MODEL, true>), dim3(nb), dim3(a.tz * a.tx), lds, stream(), u, px, py, pz, eps, del,
@@ -621,6 +681,7 @@ static int launch_div_r(int mode, const float *px, const float *py, const float 
   grid_of(a);
   int nb = a.nbz * a.nbx * a.nby;
   size_t lds = sizeof(float) * ((size_t)a.tx * (a.tz + 2 * R) + (size_t)(a.tx + 2 * R) * a.tz);
+  if(a.zc1) lds += sizeof(float) * (size_t)a.tz * (R + 1);
   LaunchTimer tm(mode == MODE_STEP ? "fd_div_step" : "fd_div");
   const dim3 g(nb), b(a.tz * a.tx);
   if(mode == MODE_STEP && a.rho)

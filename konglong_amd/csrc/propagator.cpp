@@ -984,6 +984,12 @@ int klrtm_propagator_reset(klrtm_propagator *p) {
   return reinterpret_cast<Propagator *>(p)->reset();
 }
 
+int klrtm_propagator_set_zgrid(klrtm_propagator *p, const float *zgrid) {
+  KL_REQUIRE(p, KLRTM_EINVAL, "null propagator");
+  Propagator &P = *reinterpret_cast<Propagator *>(p);
+  P.drop_graphs();  // captured launches hold the old z weights
+  return P.eng->set_zgrid(zgrid);
+}
 int klrtm_propagator_kernel(klrtm_propagator *p, klrtm_kernel **k) {
   KL_REQUIRE(p && k, KLRTM_EINVAL, "null argument");
   *k = reinterpret_cast<klrtm_kernel *>(reinterpret_cast<Propagator *>(p)->eng);

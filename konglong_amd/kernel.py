@@ -5,6 +5,8 @@ the arithmetic runs in the HIP kernels of libkonglong_rtm.so.
 """
 import ctypes as C
 
+import numpy as np
+
 from ._lib import lib, check, ENGINE_FD, ENGINE_CFD, ISO, VTI, TTI  # noqa: F401
 
 
@@ -48,6 +50,15 @@ class KernelCPU:
 
     def set_tile(self, tz=0, tx=0, ychunk=0):
         check(lib().klrtm_kernel_set_tile(self.h, tz, tx, ychunk))
+
+    def set_zgrid(self, zgrid=None):
+        """IRREGULAR grid: the depth of every z index (FdEngine_fd::setJacob); None returns to 1/dz."""
+        if zgrid is None:
+            check(lib().klrtm_kernel_set_zgrid(self.h, None))
+            return
+        z = np.ascontiguousarray(zgrid, np.float32)
+        assert z.shape == (self.nz,)
+        check(lib().klrtm_kernel_set_zgrid(self.h, z.ctypes.data))
 
     def set_overlap(self, on=True):
         """Compact engine: independent line passes of a step on two streams (bit-identical results)."""

@@ -110,6 +110,13 @@ class Propagator:
         check(lib().klrtm_propagator_set_imaging(self.h, int(option), float(kmin)))
         self.ic_option = int(option)
 
+    def set_zgrid(self, zgrid=None):
+        """IRREGULAR grid: the depth of every z index (FD engine); None returns to the regular 1/dz."""
+        z = None if zgrid is None else np.ascontiguousarray(zgrid, np.float32)
+        if z is not None:
+            assert z.shape == (self.shape[2],)
+        check(lib().klrtm_propagator_set_zgrid(self.h, None if z is None else z.ctypes.data))
+
     def set_ic_double(self, dz=2, dx=2, dy=2, high_z=0.9, high_xy=0.7):
         """Image on a grid doubled along z, x, y (ic_double_z/x/y with highTapeZ / highTapeXY): 3D grids
         double all three axes, 2D grids z or z and x; (1, 1, 1) restores the propagation grid.  rtm() then

@@ -42,6 +42,7 @@ def lib(path=None):
             getattr(L, f"orc_engine_set_threads_{s}").argtypes = [C.c_void_p, C.c_int]
             getattr(L, f"orc_engine_set_model_{s}").argtypes = [C.c_void_p] + [C.c_void_p] * 5
             getattr(L, f"orc_engine_set_rho_{s}").argtypes = [C.c_void_p, C.c_void_p]
+            getattr(L, f"orc_engine_set_zgrid_{s}").argtypes = [C.c_void_p, C.c_void_p]
             getattr(L, f"orc_engine_set_boundary_{s}").argtypes = [C.c_void_p, _pi, T, T, T, C.c_int, T, C.c_int, C.c_int]
             getattr(L, f"orc_engine_kernel_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
             getattr(L, f"orc_engine_step_{s}").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -136,6 +137,12 @@ class Engine:
 
     def clean_pml(self):
         getattr(lib(), f"orc_engine_clean_pml_{self.s}")(self.h)
+
+    def set_zgrid(self, zgrid=None):
+        """IRREGULAR grid (FdEngine_fd::setJacob): the depth of every z index, None for a regular grid."""
+        self._zgrid = None if zgrid is None else np.ascontiguousarray(zgrid, np.float32)
+        getattr(lib(), f"orc_engine_set_zgrid_{self.s}")(self.h, None if self._zgrid is None else
+                                                         self._zgrid.ctypes.data)
 
     def kernel(self, model, p0, pb, pr):
         getattr(lib(), f"orc_engine_kernel_{self.s}")(self.h, model, p0.ctypes.data, pb.ctypes.data, pr.ctypes.data)

@@ -380,11 +380,51 @@ template<class T> struct Engine {
     }
   }
 
+  // --- IRREGULAR z grid: FdEngine_fd::setJacob (bgprtm/FdEngine_fd.cpp:92-131) with const_ne -----------
+  // zp, zpp = d zgrid / d tau, d2 zgrid / d tau2 (libCommon's Taylor::derive1/derive2 with d = 1; libCommon is
+  // not in the reference tree, so this restates the build's choice: central Taylor differences of order
+  // 2 min(ne, i, n-1-i) in FP64, one-sided first differences and the neighbour's second derivative at the
+  // ends).  Per iz: zpi = 1/zp, slope = -zpp zpi^2, first-derivative weight zpi c1_j, Laplacian weights
+  // zpi^2 c2_j -/+ slope zpi c1_j for u[iz-j] / u[iz+j].
+  bool irr = false;
+  std::vector<T> zc2, zc1;  // [nz][2ne+1]: c0, u[iz-j] (j = 1..ne), u[iz+j]; [nz][ne+1]: c_j
+  void set_zgrid(const float *zg) {
+    irr = zg != nullptr;
+    if(!irr) return;
+    std::vector<float> zp(nz), zpp(nz);
+    for(int i = 0; i < nz; i++) {
+      const int nev = std::min(ne, std::min(i, nz - 1 - i));
+      if(nev == 0) continue;
+      std::vector<double> a1(nev + 1), a2(nev + 1);
+      taylor_coef1(nev, a1.data());
+      taylor_coef2(nev, a2.data());
+      double d1 = 0, d2 = a2[0] * zg[i];
+      for(int j = 1; j <= nev; j++)
+        d1 += a1[j] * ((double)zg[i + j] - zg[i - j]), d2 += a2[j] * ((double)zg[i + j] + zg[i - j]);
+      zp[i] = (float)d1, zpp[i] = (float)d2;
+    }
+    zp[0] = (float)((double)zg[1] - zg[0]), zp[nz - 1] = (float)((double)zg[nz - 1] - zg[nz - 2]);
+    zpp[0] = zpp[1], zpp[nz - 1] = zpp[nz - 2];
+    const int S2 = 2 * ne + 1, S1 = ne + 1;
+    zc2.assign((size_t)nz * S2, T(0)), zc1.assign((size_t)nz * S1, T(0));
+    for(int iz = 0; iz < nz; iz++) {
+      const T zi = T(1) / T(zp[iz]), zi2 = zi * zi, slope = -T(zpp[iz]) * zi2;
+      zc2[(size_t)iz * S2] = zi2 * c2[0];
+      for(int j = 1; j <= ne; j++) {
+        const T c = zi * c1[j], a = zi2 * c2[j], b = slope * c;
+        zc1[(size_t)iz * S1 + j] = c;
+        zc2[(size_t)iz * S2 + j] = a - b;
+        zc2[(size_t)iz * S2 + ne + j] = a + b;
+      }
+    }
+  }
+
   // --- Laplacian with zero halo + PML: bgprtm/FdEngine_fd.cpp:204-325 -------
   void laplacian(const T *in, T *out) {
     const T cz0 = zpi2 * c2[0];
     std::vector<T> czm(ne + 1);
     for(int j = 1; j <= ne; j++) czm[j] = zpi2 * c2[j];
+    const int S2 = 2 * ne + 1;
     bool do_y = ny > 2;
 #pragma omp parallel for num_threads(nthreads) collapse(2) schedule(static)
     for(int iy = 0; iy < ny; iy++)
@@ -393,12 +433,22 @@ template<class T> struct Engine {
         bufc.resize(nz);
         const T *u = in + (size_t)iy * nxz + (size_t)ix * nz;
         T *v = out + (size_t)iy * nxz + (size_t)ix * nz;
-        for(int iz = 0; iz < nz; iz++) v[iz] = cz0 * u[iz];
-        for(int m = -ne; m <= ne; m++) {
-          if(m == 0) continue;
-          T c = czm[m < 0 ? -m : m];
-          int iz0 = std::max(0, -m), iz1 = std::min(nz, nz - m);
-          for(int iz = iz0; iz < iz1; iz++) v[iz] += c * u[iz + m];
+        if(irr) {  // per-iz weights (bgprtm/FdEngine_fd.cpp:224-236)
+          for(int iz = 0; iz < nz; iz++) v[iz] = zc2[(size_t)iz * S2] * u[iz];
+          for(int m = -ne; m <= ne; m++) {
+            if(m == 0) continue;
+            const int k = m < 0 ? -m : ne + m;
+            int iz0 = std::max(0, -m), iz1 = std::min(nz, nz - m);
+            for(int iz = iz0; iz < iz1; iz++) v[iz] += zc2[(size_t)iz * S2 + k] * u[iz + m];
+          }
+        } else {
+          for(int iz = 0; iz < nz; iz++) v[iz] = cz0 * u[iz];
+          for(int m = -ne; m <= ne; m++) {
+            if(m == 0) continue;
+            T c = czm[m < 0 ? -m : m];
+            int iz0 = std::max(0, -m), iz1 = std::min(nz, nz - m);
+            for(int iz = iz0; iz < iz1; iz++) v[iz] += c * u[iz + m];
+          }
         }
         {
           T c = c2[0] * dx2i;
@@ -450,8 +500,16 @@ template<class T> struct Engine {
     if(!iacc) std::fill(out, out + nz, T(0));
     for(int m = -ne; m <= ne; m++) {
       if(m == 0) continue;
-      T c = (m < 0) ? -(zpi * c1[-m]) : zpi * c1[m];
       int iz0 = std::max(0, -m), iz1 = std::min(nz, nz - m);
+      if(irr) {  // per-iz weights (bgprtm/FdEngine_fd.cpp:396-402)
+        const int j = m < 0 ? -m : m;
+        for(int iz = iz0; iz < iz1; iz++) {
+          const T c = zc1[(size_t)iz * (ne + 1) + j];
+          vz[iz] += (m < 0 ? -c : c) * u[iz + m];
+        }
+        continue;
+      }
+      T c = (m < 0) ? -(zpi * c1[-m]) : zpi * c1[m];
       for(int iz = iz0; iz < iz1; iz++) vz[iz] += c * u[iz + m];
     }
     pml1_zbnd(vz, ix, iy, isign);
@@ -1034,6 +1092,7 @@ void orc_cfd_derive_d(int n, double d, int sgn, const double *x, double *b, int 
     e->setBoundary(nb, dt, vmax, pml_eps, dimPML);                                                                    \
   }                                                                                                                   \
   void orc_engine_clean_pml_##S(void *h) { ((Engine<T> *)h)->cleanPML(); }                                            \
+  void orc_engine_set_zgrid_##S(void *h, const float *zg) { ((Engine<T> *)h)->set_zgrid(zg); }                        \
   void orc_engine_set_rho_##S(void *h, const T *rho) { ((Engine<T> *)h)->rho = rho; }                                  \
   void orc_engine_kernel_##S(void *h, int model, const T *p0, T *pb, T *pr) {                                         \
     ((Engine<T> *)h)->kernel(model, p0, pb, pr);                                                                      \

@@ -37,6 +37,7 @@ void orc_compress16(const float *in, uint16_t *out, int nx, int ny, int nz);
 void orc_uncompress16(const uint16_t *in, float *out, int nx, int ny, int nz);
 void orc_compress_pair(const float *in, uint16_t *out, int nx, int ny, int nz, int mode);
 void orc_pml_round(float g, float *p, float *q, int n);
+void orc_engine_set_zgrid_f(void *h, const float *zg);
 void orc_apply_symmetry(float *wf0, int nx, int ny, int nz, int iz0, int sym, int tprlen, int taper);
 void orc_illum_damping(float *illum, int nx, int ny, int nz, int smooth, int sxy, int sz, float ampDamping,
                        float dampWeight, double *average_out);
@@ -102,6 +103,38 @@ int main() {
     std::vector<float> got = dr.host();
     double e = rel_l2(got, want);
     report("FdEngine_fd::laplacian", e < 1e-5, e);
+    orc_engine_destroy_f(o);
+  }
+  // KernelCPU::setJacob with an IRREGULAR grid: the per-iz z weights against the oracle's (FD8 ISO step,
+  // 1e-5), and a RECTANGLE grid with jacobian pointers (as Propagator passes them) keeps the regular weights
+  {
+    Grid g;
+    g.nx = nx, g.ny = ny, g.nz = nz, g.dx = g.dy = g.dz = d, g.mytype = IRREGULAR;
+    g.zgrid.resize(nz);
+    for(int iz = 0; iz < nz; iz++) g.zgrid[iz] = 10.f * iz + 0.12f * iz * iz;
+    std::vector<float> jac(nz, 1.f);
+    KernelCPU k(nx, ny, nz, d, d, d, FD::FD, 1);
+    k.setModel(dv.get(), nullptr, nullptr, nullptr, nullptr, nullptr);
+    k.setJacob(&g, jac.data(), jac.data(), jac.data());
+    k.setBoundary(nb[0], nb[1], nb[2], nb[3], nb[4], nb[5], dt, vmax);
+    void *o = orc_engine_create_f(nx, ny, nz, d, d, d, FD::FD, 8);
+    orc_engine_set_model_f(o, vel.data(), nullptr, nullptr, nullptr, nullptr);
+    orc_engine_set_boundary_f(o, nb, dt, vmax, 1e-3f, 1, 0.3f, 1, 3);
+    orc_engine_set_zgrid_f(o, g.zgrid.data());
+    std::vector<float> opb = pb, opr(n, 0.f);
+    konglong::DeviceBuffer<float> b2(n), r2(n);
+    b2.upload(pb.data());
+    k.ISO(d0.get(), b2.get(), r2.get());
+    orc_engine_kernel_f(o, ISO, p0.data(), opb.data(), opr.data());
+    std::vector<float> got = b2.host();
+    double e = rel_l2(got, opb);
+    g.mytype = RECTANGLE;
+    k.setJacob(&g, jac.data(), jac.data(), jac.data());
+    k.cleanPMLMemory();
+    b2.upload(pb.data());
+    k.ISO(d0.get(), b2.get(), r2.get());
+    std::vector<float> reg = b2.host();
+    report("KernelCPU::setJacob IRREGULAR / RECTANGLE", e < 1e-5 && rel_l2(reg, opb) > 1e-3, e);
     orc_engine_destroy_f(o);
   }
   // WaveFieldCompress COMPRESS_16P codes

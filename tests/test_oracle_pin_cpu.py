@@ -472,3 +472,58 @@ def test_snapshot_cadence_and_time_taper():
     A = np.column_stack([contrib[s].ravel() for s in range(1, ntaper)])
     w, *_ = np.linalg.lstsq(A, (img_t - img).ravel(), rcond=None)
     assert np.allclose(w + 1, np.sin(np.pi * np.arange(1, ntaper) / (2 * ntaper)) ** 2, atol=1e-9)
+
+
+# ---- IRREGULAR z grid (FdEngine_fd::setJacob) ------------------------------------------------------------
+
+def _stretch(nz, a=10.0, b=0.15):
+    """Depths z(tau) = a tau + b tau^2 (FP32): the Taylor differences of z are exact in the interior."""
+    tau = np.arange(nz, dtype=np.float64)
+    return (a * tau + b * tau * tau).astype(np.float32)
+
+
+@pytest.mark.parametrize("order", [8, 16])
+def test_irregular_z_derivatives_exact_on_quadratic_stretch(order):
+    """On z(tau) quadratic, f = z^2 is a quartic in tau, which the order-8/16 Taylor stencils differentiate
+    exactly: the z Laplacian is 2 and the first derivative of f = z is 1 away from the grid's ends.  A
+    wrong sign of the slope term or a missing 1/zp^2 fails by O(1)."""
+    from oracle import Engine
+    ny, nx, nz = 1, 17, 48
+    R = order // 2
+    e = Engine(nx, ny, nz, 10.0, 10.0, 10.0, 7, order, np.float64)
+    e.set_model(np.ones((ny, nx, nz)))
+    e.set_boundary([0] * 6, 1e-3, 3000.0)
+    zg = _stretch(nz)
+    e.set_zgrid(zg)
+    z = zg.astype(np.float64)
+    f = np.broadcast_to(z * z, (ny, nx, nz)).copy()
+    out = np.zeros_like(f)
+    e.laplacian(f, out)
+    assert np.abs(out[0, nx // 2, R:nz - R] - 2.0).max() < 1e-6
+    g = np.broadcast_to(z, (ny, nx, nz)).copy()
+    d = np.zeros_like(g)
+    e.deriv1(0, g, d, 0, 1)
+    assert np.abs(d[0, nx // 2, R:nz - R] - 1.0).max() < 1e-6
+    e.set_zgrid(None)   # the regular 1/dz metric misses the stretch
+    e.laplacian(f, out)
+    assert np.abs(out[0, nx // 2, R:nz - R] - 2.0).max() > 1.0
+
+
+def test_uniform_zgrid_reproduces_the_regular_grid():
+    """zgrid = dz * iz through the per-iz weights agrees with the regular 1/dz weights to rounding."""
+    from oracle import Engine
+    ny, nx, nz = 6, 17, 40
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((ny, nx, nz))
+    outs = []
+    for zg in (None, (10.0 * np.arange(nz)).astype(np.float32)):
+        e = Engine(nx, ny, nz, 10.0, 10.0, 10.0, 7, 8, np.float64)
+        e.set_model(np.ones((ny, nx, nz)))
+        e.set_boundary([0] * 6, 1e-3, 3000.0)
+        e.set_zgrid(zg)
+        lap, d1 = np.zeros_like(u), np.zeros_like(u)
+        e.laplacian(u, lap)
+        e.deriv1(0, u, d1, 0, 1)
+        outs.append((lap, d1))
+    for a, b in zip(*outs):
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()
