@@ -169,7 +169,8 @@ int klrtm_kernel_get_tile(klrtm_kernel *k, int *tz, int *tx, int *ychunk);
  * every z index (increasing); the Laplacian and first-derivative z weights become per-iz, from
  * dz/dtau and d2z/dtau2 of zgrid (libCommon's Taylor::derive1/derive2, not in the reference tree: central
  * Taylor differences of order 2 min(ne, iz, nz-1-iz), one-sided at the ends).  NULL returns to the regular
- * 1/dz.  FD engine only (the compact engine's jacobz scaling is not built). */
+ * 1/dz.  The compact engine instead scales its z derivative by jacobz = 1 / (zgrid' / dz) with the order-8
+ * derivative (Grid::setupJacob, bgprtm/Grid.cpp:654-666; FdEngine_cfd::dz1_2D, bgprtm/FdEngine_cfd.cpp:71-95). */
 int klrtm_kernel_set_zgrid(klrtm_kernel *k, const float *zgrid);
 /* compact (CFD) engine, 3D step: run the independent line passes (the y and z gradient derivatives, then
  * the x and z divergence derivatives) concurrently on a second stream forked from and joined to the

@@ -38,6 +38,10 @@ struct CfdPassArgs {
   int round;                  // PML round (0: gradient, 1: divergence)
   int has_pml;
   PmlDesc pml;
+  // IRREGULAR z grid (FdEngine_cfd::dz1_2D, bgprtm/FdEngine_cfd.cpp:71-95): jacobz[iz] of the z derivative,
+  // or null.  z passes: the stored derivative is D*jz, the accumulates pb - ((D*vel)*rho)*jz; the y pass
+  // of the fused step takes it for its Dz term
+  const float *jz;
 };
 
 // ---------------------------------------------------------------------------
@@ -100,6 +104,8 @@ struct LineSweep {
   float *tmp, *pr, *pz, *pb;
   const float *vel, *w1, *eps, *del, *pjx, *pjy;
   const float *prl, *pjyl;  // pr, pjy for loads (another valid stream in 2D, where the value is unused)
+  bool jzl;                 // IRREGULAR z grid: the Dz term of P_DIV3_STEP takes this lane's jacobz
+  float jzv;
   size_t base, s;
   const cfloat *U;
   int n, ny;
@@ -199,7 +205,7 @@ struct LineSweep {
     } else {
       // the x, z, y accumulates of bgprtm/KernelCPU.cpp:108-114 in their order
       float w = B.o[0][j] - B.o[2][j] * B.o[1][j];
-      w = w - B.o[3][j] * B.o[1][j];
+      w = jzl ? w - (B.o[3][j] * B.o[1][j]) * jzv : w - B.o[3][j] * B.o[1][j];
       w = w - g * B.o[1][j];
       pb[c] = (B.o[4][j] + B.o[4][j]) - w;
     }
@@ -349,6 +355,7 @@ __global__ void __launch_bounds__(256) cfd_line_kernel(const float *in, float *t
   L.vel = vel, L.w1 = w1, L.eps = eps, L.del = del, L.pjx = pjx, L.pjy = pjy;
   L.prl = ny > 1 ? pr : pz, L.pjyl = ny > 1 ? pjy : pjx;
   L.base = base, L.s = s, L.U = as_const(a.bm), L.n = n, L.ny = ny;
+  L.jzl = a.jz != nullptr, L.jzv = a.jz ? a.jz[cz] : 1.f;
   L.Y0 = L.Y1 = L.Y2 = L.Y3 = 0.f;
   if(!a.has_pml) {
     L.template segment<false, false>(n - 1, 0);
@@ -625,6 +632,7 @@ __global__ void __launch_bounds__(64) cfd_zline_kernel(const float *in, float *t
     float *row = buf + lane * ZSP;
     const int i0 = c * ZC, i1 = min(n, i0 + ZC);
     const int z = i0 + kl;
+    const float jzv = a.jz && z < n ? a.jz[z] : 1.f;
     const bool pml = a.has_pml && (cross_any || i0 < zlo || i1 > zhi);
     // own z-layer slot of this lane's sample
     float oa = 0.f, ob = 0.f, od = 1.f;
@@ -685,11 +693,11 @@ __global__ void __launch_bounds__(64) cfd_zline_kernel(const float *in, float *t
           buf_st(qi, rs_q[2], cy.off == OOR || z >= n ? OOR : cy.off + (uint32_t)z * 4u, 0);
         }
         if(PASS == P_GRAD) {
-          buf_st(g, rs_tmp, vc, soff(r));
+          buf_st(a.jz ? g * jzv : g, rs_tmp, vc, soff(r));
         } else if(PASS == P_DIV) {
-          buf_st(G.o[0][rr] - g * G.o[1][rr], rs_pb, vc, soff(r));
+          buf_st(a.jz ? G.o[0][rr] - (g * G.o[1][rr]) * jzv : G.o[0][rr] - g * G.o[1][rr], rs_pb, vc, soff(r));
         } else {
-          const float w = G.o[0][rr] - g * G.o[1][rr];
+          const float w = a.jz ? G.o[0][rr] - (g * G.o[1][rr]) * jzv : G.o[0][rr] - g * G.o[1][rr];
           buf_st((G.o[2][rr] + G.o[2][rr]) - w, rs_pb, vc, soff(r));
         }
       }
@@ -895,7 +903,7 @@ void cfd_free(Engine &e) {
 
 template<int AXIS, int PASS, int MODEL>
 static int launch_line(Engine &e, int sg, int round, const float *in, float *tmp, float *pr, float *pz, float *pb,
-                       const float *w1, const char *name, hipStream_t st = nullptr) {
+                       const float *w1, const char *name, hipStream_t st = nullptr, bool jzf = false) {
   if(!st) st = stream();
   CfdPassArgs a;
   memset(&a, 0, sizeof(a));
@@ -905,6 +913,7 @@ static int launch_line(Engine &e, int sg, int round, const float *in, float *tmp
   a.round = round;
   a.has_pml = e.has_pml;
   a.pml = e.desc;
+  a.jz = jzf ? e.d_jz : nullptr;
   // lanes per block: 256 by default, or the tile tz set for the compact engine (64, 128 or 256)
   const int bs = e.tz == 64 || e.tz == 128 ? e.tz : 256;
   LaunchTimer tm(name, st);
@@ -915,7 +924,7 @@ static int launch_line(Engine &e, int sg, int round, const float *in, float *tmp
 
 template<int PASS>
 static int launch_zline(Engine &e, int sg, int round, const float *in, float *tmp, float *pb, const float *w1,
-                        const char *name, hipStream_t st = nullptr) {
+                        const char *name, hipStream_t st = nullptr, bool jzf = false) {
   if(!st) st = stream();
   CfdPassArgs a;
   memset(&a, 0, sizeof(a));
@@ -925,6 +934,7 @@ static int launch_zline(Engine &e, int sg, int round, const float *in, float *tm
   a.round = round;
   a.has_pml = e.has_pml;
   a.pml = e.desc;
+  a.jz = jzf ? e.d_jz : nullptr;
   LaunchTimer tm(name, st);
   hipLaunchKernelGGL((cfd_zline_kernel<PASS>), dim3((unsigned)div_up(a.nlines, 64)), dim3(64), 0, st, in, tmp, pb, e.vel,
                      w1, a);
@@ -960,11 +970,11 @@ static int cfd_run_m(Engine &e, int mode, const float *p0, float *pb, float *pr)
     // the y and z passes read p0 and write disjoint volumes (pr, pz) and PML state components, so they run
     // concurrently: more lines in flight for the latency-bound sweeps
     if((rc = side_fork(e)) || (rc = launch_line<AXY, P_GRAD, MODEL>(e, 0, 0, p0, pr, pr, pz, pb, p0, "cfd_grad_y", e.side)) ||
-       (rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z")) || (rc = side_join(e)))
+       (rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z", nullptr, true)) || (rc = side_join(e)))
       return rc;
   } else {
     if(y && (rc = launch_line<AXY, P_GRAD, MODEL>(e, 0, 0, p0, pr, pr, pz, pb, p0, "cfd_grad_y"))) return rc;
-    if((rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z"))) return rc;
+    if((rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z", nullptr, true))) return rc;
   }
   if((rc = launch_line<AXX, P_GRAD_SCALAR, MODEL>(e, 0, 0, p0, px, pr, pz, pb, p0, "cfd_grad_x"))) return rc;
   // divergence (isign=-1, plan 2, PML round 2): x, z, y as bgprtm/KernelCPU.cpp:108-114
@@ -981,15 +991,15 @@ static int cfd_run_m(Engine &e, int mode, const float *p0, float *pb, float *pr)
       if((rc = launch_line<AXX, P_GRAD, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x"))) return rc;
       if((rc = launch_zline<P_GRAD>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z"))) return rc;
     }
-    return launch_line<AXY, P_DIV3_STEP, MODEL>(e, 1, 1, pr, pr, px, pz, pb, p0, "cfd_div_y");
+    return launch_line<AXY, P_DIV3_STEP, MODEL>(e, 1, 1, pr, pr, px, pz, pb, p0, "cfd_div_y", nullptr, true);
   }
   if((rc = launch_line<AXX, P_DIV, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x"))) return rc;
   if(y) {
-    if((rc = launch_zline<P_DIV>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z"))) return rc;
+    if((rc = launch_zline<P_DIV>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z", nullptr, true))) return rc;
     return launch_line<AXY, P_DIV, MODEL>(e, 1, 1, pr, px, pr, pz, pb, p0, "cfd_div_y");
   }
-  if(step) return launch_zline<P_DIV_STEP>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z");
-  return launch_zline<P_DIV>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z");
+  if(step) return launch_zline<P_DIV_STEP>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z", nullptr, true);
+  return launch_zline<P_DIV>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z", nullptr, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,13 +1026,14 @@ __global__ void cfd_rho_grad_kernel(float *__restrict__ px, float *__restrict__ 
 // the x and z accumulates and/or the y accumulate of the derivatives given, + leapfrog if w1
 __global__ void cfd_rho_div_kernel(float *__restrict__ pb, const float *__restrict__ dx, const float *__restrict__ dz,
                                    const float *__restrict__ dy, const float *__restrict__ vel,
-                                   const float *__restrict__ rho, const float *__restrict__ w1, size_t n) {
+                                   const float *__restrict__ rho, const float *__restrict__ w1, size_t n,
+                                   const float *__restrict__ jz, int nz) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
   const float v = vel[i], r = rho[i];
   float w = pb[i];
   if(dx) w = w - dx[i] * v * r;
-  if(dz) w = w - dz[i] * v * r;
+  if(dz) w = jz ? w - dz[i] * v * r * jz[i % nz] : w - dz[i] * v * r;
   if(dy) w = w - dy[i] * (v * r);
   if(w1) w = (w1[i] + w1[i]) - w;
   pb[i] = w;
@@ -1036,7 +1047,7 @@ static int cfd_run_rho(Engine &e, int mode, const float *p0, float *pb, float *p
   const size_t n = (size_t)e.nx * e.ny * e.nz;
   const dim3 g((unsigned)div_up((long)n, 256)), b(256);
   if(y && (rc = launch_line<AXY, P_GRAD, MODEL>(e, 0, 0, p0, pr, pr, pz, pb, p0, "cfd_grad_y"))) return rc;
-  if((rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z"))) return rc;
+  if((rc = launch_zline<P_GRAD>(e, 0, 0, p0, pz, pb, p0, "cfd_grad_z", nullptr, true))) return rc;
   if((rc = launch_line<AXX, P_GRAD, MODEL>(e, 0, 0, p0, px, pr, pz, pb, p0, "cfd_grad_x"))) return rc;
   hipLaunchKernelGGL((cfd_rho_grad_kernel<MODEL>), g, b, 0, stream(), px, y ? pr : nullptr, pz, e.rho, e.eps, e.del, e.pjx,
                      e.pjy, n);
@@ -1045,10 +1056,11 @@ static int cfd_run_rho(Engine &e, int mode, const float *p0, float *pb, float *p
   if((rc = launch_line<AXX, P_GRAD, MODEL>(e, 1, 1, px, px, pr, pz, pb, p0, "cfd_div_x"))) return rc;
   if((rc = launch_zline<P_GRAD>(e, 1, 1, pz, pz, pb, p0, "cfd_div_z"))) return rc;
   const float *lf = mode == MODE_STEP ? p0 : nullptr;
-  hipLaunchKernelGGL(cfd_rho_div_kernel, g, b, 0, stream(), pb, px, pz, nullptr, e.vel, e.rho, y ? nullptr : lf, n);
+  hipLaunchKernelGGL(cfd_rho_div_kernel, g, b, 0, stream(), pb, px, pz, nullptr, e.vel, e.rho, y ? nullptr : lf, n, e.d_jz,
+                     e.nz);
   if((rc = check_launch("cfd_rho_div_kernel")) || !y) return rc;
   if((rc = launch_line<AXY, P_GRAD, MODEL>(e, 1, 1, pr, px, pr, pz, pb, p0, "cfd_div_y"))) return rc;
-  hipLaunchKernelGGL(cfd_rho_div_kernel, g, b, 0, stream(), pb, nullptr, nullptr, px, e.vel, e.rho, lf, n);
+  hipLaunchKernelGGL(cfd_rho_div_kernel, g, b, 0, stream(), pb, nullptr, nullptr, px, e.vel, e.rho, lf, n, nullptr, e.nz);
   return check_launch("cfd_rho_div_kernel");
 }
 
@@ -1058,18 +1070,19 @@ static int cfd_run_rho(Engine &e, int mode, const float *p0, float *pb, float *p
 int cfd_deriv1(Engine &e, int axis, int iacc, int isign, const float *in, float *out, float *tmp) {
   const int sg = isign > 0 ? 0 : 1, rnd = sg;
   float *t = iacc ? tmp : out;
-  if(iacc && e.rho) {  // out -= vel*rho*D(in) in the axis' operation order
-    int rc = cfd_deriv1(e, axis, 0, isign, in, t, nullptr);
+  if(iacc && e.rho) {  // out -= vel*rho*D(in) in the axis' operation order (jacobz last, on the raw derivative)
+    int rc = axis == 0 ? launch_zline<P_GRAD>(e, sg, rnd, in, t, nullptr, nullptr, "cfd_deriv1_z")
+                       : cfd_deriv1(e, axis, 0, isign, in, t, nullptr);
     if(rc) return rc;
     const size_t n = (size_t)e.nx * e.ny * e.nz;
     hipLaunchKernelGGL(cfd_rho_div_kernel, dim3((unsigned)div_up((long)n, 256)), dim3(256), 0, stream(), out,
                        axis == 1 ? t : nullptr, axis == 0 ? t : nullptr, axis == 2 ? t : nullptr, e.vel, e.rho, nullptr,
-                       n);
+                       n, axis == 0 ? e.d_jz : nullptr, e.nz);
     return check_launch("cfd_rho_div_kernel");
   }
   if(axis == 0) {
-    if(iacc) return launch_zline<P_DIV>(e, sg, rnd, in, t, out, nullptr, "cfd_deriv1_z");
-    return launch_zline<P_GRAD>(e, sg, rnd, in, t, nullptr, nullptr, "cfd_deriv1_z");
+    if(iacc) return launch_zline<P_DIV>(e, sg, rnd, in, t, out, nullptr, "cfd_deriv1_z", nullptr, true);
+    return launch_zline<P_GRAD>(e, sg, rnd, in, t, nullptr, nullptr, "cfd_deriv1_z", nullptr, true);
   }
   if(axis == 1) {
     if(iacc) return launch_line<AXX, P_DIV, KLRTM_ISO>(e, sg, rnd, in, t, nullptr, nullptr, out, nullptr, "cfd_deriv1_x");

@@ -65,11 +65,22 @@ static void zgrid_derivatives(const float *z, int n, int ne, float *zp, float *z
 }
 
 int Engine::set_zgrid(const float *zg) {
-  (void)hipFree(d_zc2), (void)hipFree(d_zc1);
-  d_zc2 = d_zc1 = nullptr;
+  (void)hipFree(d_zc2), (void)hipFree(d_zc1), (void)hipFree(d_jz);
+  d_zc2 = d_zc1 = d_jz = nullptr;
   if(!zg) return KLRTM_OK;
-  KL_REQUIRE(engine == KLRTM_ENGINE_FD, KLRTM_EINVAL,
-             "irregular z grids run on the FD engine (the compact engine's jacobz scaling is not built)");
+  KL_REQUIRE(nz > 8, KLRTM_EINVAL, "an irregular z grid needs nz > 8");
+  for(int iz = 1; iz < nz; iz++)
+    KL_REQUIRE(zg[iz] > zg[iz - 1], KLRTM_EINVAL, "z grid depths must increase (iz %d)", iz);
+  if(engine == KLRTM_ENGINE_CFD) {
+    // Grid::setupJacob (bgprtm/Grid.cpp:654-666): jacobz = 1 / (Taylor::derive1(zgrid, d = dz, 4)), which the
+    // compact engine's z derivative takes (bgprtm/FdEngine_cfd.cpp:79, 91)
+    std::vector<float> zp(nz), zpp(nz), jz(nz);
+    zgrid_derivatives(zg, nz, 4, zp.data(), zpp.data());
+    for(int iz = 0; iz < nz; iz++) jz[iz] = 1 / (zp[iz] / dz);
+    KL_HIP(hipMalloc(&d_jz, nz * sizeof(float)));
+    KL_HIP(hipMemcpy(d_jz, jz.data(), nz * sizeof(float), hipMemcpyHostToDevice));
+    return KLRTM_OK;
+  }
   KL_REQUIRE(nz > 2 * ne, KLRTM_EINVAL, "an irregular z grid needs nz > %d", 2 * ne);
   std::vector<float> zp(nz), zpp(nz);
   zgrid_derivatives(zg, nz, ne, zp.data(), zpp.data());
@@ -112,7 +123,7 @@ Engine::~Engine() {
   (void)hipFree(d_state);
   (void)hipFree(d_px);
   (void)hipFree(d_pz);
-  (void)hipFree(d_zc2), (void)hipFree(d_zc1);
+  (void)hipFree(d_zc2), (void)hipFree(d_zc1), (void)hipFree(d_jz);
   cfd_free(*this);
   if(ev_fork) (void)hipEventDestroy(ev_fork);
   if(ev_join) (void)hipEventDestroy(ev_join);

@@ -36,6 +36,7 @@ struct Engine {
   // irregular z grid (FdEngine_fd::setJacob): per-iz weights of the z part on the device, null on a
   // regular grid
   float *d_zc2 = nullptr, *d_zc1 = nullptr;
+  float *d_jz = nullptr;  // compact engine: jacobz[iz] (Grid::setupJacob)
   int set_zgrid(const float *zgrid);
   // compact engine: run the independent line passes of a step (the y and z gradients, the x and z
   // divergence derivatives) concurrently on a second stream forked from and joined to the caller's

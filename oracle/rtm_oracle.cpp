@@ -386,12 +386,13 @@ template<class T> struct Engine {
   // 2 min(ne, i, n-1-i) in FP64, one-sided first differences and the neighbour's second derivative at the
   // ends).  Per iz: zpi = 1/zp, slope = -zpp zpi^2, first-derivative weight zpi c1_j, Laplacian weights
   // zpi^2 c2_j -/+ slope zpi c1_j for u[iz-j] / u[iz+j].
+  // The compact engine instead scales its z derivative by jacobz = 1 / (zgrid' / dz) with the order-8
+  // derivative (Grid::setupJacob, bgprtm/Grid.cpp:654-666; FdEngine_cfd::dz1_2D, bgprtm/FdEngine_cfd.cpp:71-95).
   bool irr = false;
   std::vector<T> zc2, zc1;  // [nz][2ne+1]: c0, u[iz-j] (j = 1..ne), u[iz+j]; [nz][ne+1]: c_j
-  void set_zgrid(const float *zg) {
-    irr = zg != nullptr;
-    if(!irr) return;
-    std::vector<float> zp(nz), zpp(nz);
+  std::vector<T> jacobz;    // compact engine
+  static void zgrid_d(const float *zg, int nz, int ne, std::vector<float> &zp, std::vector<float> &zpp) {
+    zp.assign(nz, 0.f), zpp.assign(nz, 0.f);
     for(int i = 0; i < nz; i++) {
       const int nev = std::min(ne, std::min(i, nz - 1 - i));
       if(nev == 0) continue;
@@ -405,6 +406,21 @@ template<class T> struct Engine {
     }
     zp[0] = (float)((double)zg[1] - zg[0]), zp[nz - 1] = (float)((double)zg[nz - 1] - zg[nz - 2]);
     zpp[0] = zpp[1], zpp[nz - 1] = zpp[nz - 2];
+  }
+  void set_zgrid(const float *zg) {
+    irr = zg != nullptr;
+    jacobz.clear();
+    if(!irr) return;
+    if(engine == ENGINE_CFD) {
+      std::vector<float> zp, zpp;
+      zgrid_d(zg, nz, 4, zp, zpp);
+      jacobz.resize(nz);
+      for(int iz = 0; iz < nz; iz++) jacobz[iz] = T(1) / (T(zp[iz]) / dz);
+      irr = false;  // the FD weights stay regular
+      return;
+    }
+    std::vector<float> zp, zpp;
+    zgrid_d(zg, nz, ne, zp, zpp);
     const int S2 = 2 * ne + 1, S1 = ne + 1;
     zc2.assign((size_t)nz * S2, T(0)), zc1.assign((size_t)nz * S1, T(0));
     for(int iz = 0; iz < nz; iz++) {
@@ -580,9 +596,16 @@ template<class T> struct Engine {
     T *dst = iacc ? w.data() : out;
     for(int ix = 0; ix < nx; ix++) cfd[AX_Z].derive(sg, in + (size_t)ix * nz, dst + (size_t)ix * nz, 1);
     for(int ix = 0; ix < nx; ix++) pml1_zbnd(dst + (size_t)ix * nz, ix, iy, isign);
+    const T *jz = jacobz.empty() ? nullptr : jacobz.data();
+    if(!iacc && jz)
+      for(size_t i = 0; i < nxz; i++) dst[i] *= jz[i % nz];
     if(iacc) {
       const T *rs = rho_of(vslice);
-      for(size_t i = 0; i < nxz; i++) out[i] = rs ? out[i] - dst[i] * vslice[i] * rs[i] : out[i] - dst[i] * vslice[i];
+      if(jz)  // pout*vel*(rho)*jacobz, left to right (bgprtm/FdEngine_cfd.cpp:91)
+        for(size_t i = 0; i < nxz; i++)
+          out[i] = rs ? out[i] - dst[i] * vslice[i] * rs[i] * jz[i % nz] : out[i] - dst[i] * vslice[i] * jz[i % nz];
+      else
+        for(size_t i = 0; i < nxz; i++) out[i] = rs ? out[i] - dst[i] * vslice[i] * rs[i] : out[i] - dst[i] * vslice[i];
     }
   }
   // x-derivative of one y-slice: bgprtm/FdEngine_cfd.cpp:83-105

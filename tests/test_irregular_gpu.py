@@ -1,7 +1,7 @@
 """IRREGULAR z grids (FdEngine_fd::setJacob, bgprtm/FdEngine_fd.cpp:92-131) on the GPU against the FP32
-oracle: FD steps of every model with PML, the z first derivative, and a whole ISO shot; the compact engine
-refuses them.  Tolerance 1e-4 relative L2 after several steps (the FD kernels contract multiply-adds), as the
-regular-grid step tests."""
+oracle: FD steps of every model with PML, the compact engine's jacobz scaling, the z first derivative, and a
+whole ISO shot.  Tolerance 1e-4 relative L2 after several steps (the FD kernels contract multiply-adds), as the
+regular-grid step tests, and bit for bit on the compact engine."""
 import numpy as np
 import pytest
 
@@ -85,7 +85,48 @@ def test_irregular_step_parity(klrtm, case):
     k.close()
 
 
-def test_irregular_z_derivative_and_compact_refusal(klrtm):
+CFD_CASES = [(ISO, False), (VTI, False), (VTI, True)]
+
+
+@pytest.mark.parametrize("case", CFD_CASES, ids=["CFD10-ISO", "CFD10-VTI", "CFD10-VTI-rho"])
+def test_irregular_compact_step_bit_exact(klrtm, case):
+    """The compact engine scales its z derivative by jacobz (FdEngine_cfd::dz1_2D): steps bit-identical to the
+    oracle, as on the regular grid; a uniform zgrid leaves the regular step unchanged bit for bit."""
+    model, with_rho = case
+    print(f"[irregular-z compact] model {model} rho {with_rho}", flush=True)
+    m, eps, dl, pjx, pjy, w1, w0 = _inputs(model, seed=8)
+    rho = smooth_field((NY, NX, NZ), 1.0, 2.5, 77) if with_rho else None
+    zg = _zgrid(NZ)
+    outs = []
+    for z in (zg, (D * np.arange(NZ)).astype(np.float32), None):
+        o = oracle.Engine(NX, NY, NZ, D, D, D, ENGINE_CFD, 10, np.float32)
+        o.set_model(m, eps, dl)
+        o.set_rho(rho)
+        o.set_boundary(NB, DT, 2500.0, dim_pml=3)
+        o.set_zgrid(z)
+        k = KernelCPU(NX, NY, NZ, D, D, D, ENGINE_CFD, 10)
+        arrs = [DeviceArray.from_host(a) if a is not None else None for a in (m, rho, dl, eps)]
+        k.setModel(arrs[0], arrs[1], arrs[2], arrs[3])
+        k.setBoundary(*NB, DT, 2500.0)
+        k.set_zgrid(z)
+        ow1, ow0, owr = w1.copy(), w0.copy(), np.zeros_like(w1)
+        g1, g0, gr = DeviceArray.from_host(w1), DeviceArray.from_host(w0), DeviceArray((NY, NX, NZ))
+        for _ in range(8):
+            o.step(model, ow1, ow0, owr)
+            ow0, ow1 = ow1, ow0
+            k.step(model, g1, g0, gr)
+            g0, g1 = g1, g0
+        got = g1.download()
+        print(f"  zgrid {'stretched' if z is zg else 'uniform' if z is not None else 'none'}: "
+              f"rel_l2 vs oracle {rel_l2(got, ow1):.3e}", flush=True)
+        assert np.array_equal(got.view(np.uint32), ow1.view(np.uint32))
+        outs.append(got)
+        k.close()
+    assert np.array_equal(outs[1].view(np.uint32), outs[2].view(np.uint32))
+    assert rel_l2(outs[0], outs[2]) > 1e-2
+
+
+def test_irregular_z_derivatives(klrtm):
     print("[irregular-z deriv1]", flush=True)
     zg = _zgrid(NZ)
     rng = np.random.default_rng(9)
@@ -107,13 +148,28 @@ def test_irregular_z_derivative_and_compact_refusal(klrtm):
     check(lib().klrtm_kernel_deriv1(k.h, 0, du.ptr, dout.ptr, 0, 1))
     assert rel_l2(dout.download(), want) <= 1e-5
     k.close()
-    c = KernelCPU(NX, NY, NZ, D, D, D, ENGINE_CFD, 10)
-    with pytest.raises(KlrtmError):
+    # the compact engine's z derivative, plain and accumulating (with and without density)
+    for rho in (None, smooth_field((NY, NX, NZ), 1.0, 2.0, 78)):
+        oc = oracle.Engine(NX, NY, NZ, D, D, D, ENGINE_CFD, 10, np.float32)
+        oc.set_model(m)
+        oc.set_rho(rho)
+        oc.set_boundary(NB, DT, 2500.0, dim_pml=3)
+        oc.set_zgrid(zg)
+        c = KernelCPU(NX, NY, NZ, D, D, D, ENGINE_CFD, 10)
+        drho = DeviceArray.from_host(rho) if rho is not None else None
+        c.setModel(dm, drho)
+        c.setBoundary(*NB, DT, 2500.0)
         c.set_zgrid(zg)
+        for iacc in (0, 1):
+            want = np.full_like(u, 0.25)
+            oc.deriv1(0, u, want, iacc, -1)
+            dout.upload(np.full_like(u, 0.25))
+            check(lib().klrtm_kernel_deriv1(c.h, 0, du.ptr, dout.ptr, iacc, -1))
+            assert np.array_equal(dout.download().view(np.uint32), want.view(np.uint32)), (iacc, rho is not None)
+        c.close()
     with pytest.raises(KlrtmError):   # depths must increase
         k2 = KernelCPU(NX, NY, NZ, D, D, D, ENGINE_FD, 8)
         k2.set_zgrid(zg[::-1].copy())
-    c.close()
 
 
 def test_irregular_shot_matches_oracle(klrtm):
