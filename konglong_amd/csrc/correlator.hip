@@ -263,6 +263,57 @@ __global__ void corr_lines_filter_kernel(float *__restrict__ lines, size_t nl, i
     }
   }
 }
+// x and y lines through 32 x 32 LDS tiles: the volume side is read / written along the line index (z fastest,
+// contiguous) and the padded rows along the sample index, so both sides are coalesced.  Tile (lt, jt) covers
+// lines 32 lt.. and samples 32 jt..; a block is 32 x 8 lanes, each moving four elements.
+__device__ __forceinline__ size_t xline_index(int axis, size_t l, int j, int nx, int nz, size_t nl) {
+  return axis == 1 ? ((l / nz) * nx + j) * (size_t)nz + l % nz : (size_t)j * nl + l;
+}
+__global__ void __launch_bounds__(256) corr_xlines_gather_kernel(const float *__restrict__ in, float *__restrict__ lines,
+                                                                 int nx, int ny, int nz, int axis, int L) {
+  __shared__ float t[32][33];
+  const int na = axis == 1 ? nx : ny, Lp = L + 2;
+  const size_t nl = (size_t)nx * ny * nz / na;
+  const size_t l0 = (size_t)blockIdx.x * 32;
+  const int j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for(int r = 0; r < 4; r++) {  // read: lanes along l, rows along j
+    const int j = j0 + ty + 8 * r;
+    const size_t l = l0 + tx;
+    t[ty + 8 * r][tx] = (l < nl && j < na) ? in[xline_index(axis, l, j, nx, nz, nl)] : 0.f;
+  }
+  __syncthreads();
+#pragma unroll
+  for(int r = 0; r < 4; r++) {  // write: lanes along j, rows along l
+    const size_t l = l0 + ty + 8 * r;
+    const int j = j0 + tx;
+    if(l < nl && j < Lp) lines[l * Lp + j] = t[tx][ty + 8 * r];
+  }
+}
+__global__ void __launch_bounds__(256) corr_xlines_scatter_kernel(const float *__restrict__ lines, float *__restrict__ out,
+                                                                  int nx, int ny, int nz, int axis, int L, float scale) {
+  __shared__ float t[32][33];
+  const int na = axis == 1 ? nx : ny, Lp = L + 2;
+  const size_t nl = (size_t)nx * ny * nz / na;
+  const size_t l0 = (size_t)blockIdx.x * 32;
+  const int j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for(int r = 0; r < 4; r++) {  // read: lanes along j
+    const size_t l = l0 + ty + 8 * r;
+    const int j = j0 + tx;
+    t[ty + 8 * r][tx] = (l < nl && j < na) ? lines[l * Lp + j] * scale : 0.f;
+  }
+  __syncthreads();
+#pragma unroll
+  for(int r = 0; r < 4; r++) {  // write: lanes along l
+    const int j = j0 + ty + 8 * r;
+    const size_t l = l0 + tx;
+    if(l < nl && j < na) out[xline_index(axis, l, j, nx, nz, nl)] = t[tx][ty + 8 * r];
+  }
+}
+
 // sub-images [nloop][ny][nx][nz] -> the doubled grid [ny*dy][nx*dx][nz*dz]: copy i at its shift parity
 __global__ void corr_interleave_kernel(const float *__restrict__ sub, float *__restrict__ out, int nx, int ny, int nz,
                                        int dz, int dx, int dy, int nloop, const int4 bits03, const int4 bits47) {
@@ -419,6 +470,7 @@ static int cross_plain(Correlator &C, const float *s, const float *r, float isig
 
 // ---- doubled grids ---------------------------------------------------------------------------------
 
+// in place, one padded row of L + 2 floats per line
 static int line_plans(Correlator &C, int axis) {
   if(C.lhave[axis]) return KLRTM_OK;
   const int na = axis == 0 ? C.nz : axis == 1 ? C.nx : C.ny;
@@ -437,16 +489,25 @@ static int axis_filter(Correlator &C, const float *in, float *out, int axis, int
   if(rc) return rc;
   const int na = axis == 0 ? C.nz : axis == 1 ? C.nx : C.ny, Lx = C.L[axis];
   const size_t nl = C.nxyz / na;
-  hipLaunchKernelGGL(corr_lines_gather_kernel, dim3(nblocks(nl * (Lx + 2))), dim3(256), 0, stream(), in, C.lines, C.nx,
-                     C.ny, C.nz, axis, Lx);
+  const unsigned lt = (unsigned)((nl + 31) / 32);
+  if(axis > 0)
+    hipLaunchKernelGGL(corr_xlines_gather_kernel, dim3(lt, (unsigned)((Lx + 2 + 31) / 32)), dim3(256), 0, stream(), in,
+                       C.lines, C.nx, C.ny, C.nz, axis, Lx);
+  else
+    hipLaunchKernelGGL(corr_lines_gather_kernel, dim3(nblocks(nl * (Lx + 2))), dim3(256), 0, stream(), in, C.lines, C.nx,
+                       C.ny, C.nz, axis, Lx);
   KL_FFTC(hipfftSetStream(C.lr2c[axis], stream()));
   KL_FFTC(hipfftExecR2C(C.lr2c[axis], C.lines, (hipfftComplex *)C.lines));
   hipLaunchKernelGGL(corr_lines_filter_kernel, dim3(nblocks(nl * (Lx / 2 + 1))), dim3(256), 0, stream(), C.lines, nl, Lx,
                      mode, h);
   KL_FFTC(hipfftSetStream(C.lc2r[axis], stream()));
   KL_FFTC(hipfftExecC2R(C.lc2r[axis], (hipfftComplex *)C.lines, C.lines));
-  hipLaunchKernelGGL(corr_lines_scatter_kernel, dim3(nblocks(C.nxyz)), dim3(256), 0, stream(), C.lines, out, C.nx, C.ny,
-                     C.nz, axis, Lx, 1.0f / (float)Lx);
+  if(axis > 0)
+    hipLaunchKernelGGL(corr_xlines_scatter_kernel, dim3(lt, (unsigned)((na + 31) / 32)), dim3(256), 0, stream(), C.lines,
+                       out, C.nx, C.ny, C.nz, axis, Lx, 1.0f / (float)Lx);
+  else
+    hipLaunchKernelGGL(corr_lines_scatter_kernel, dim3(nblocks(C.nxyz)), dim3(256), 0, stream(), C.lines, out, C.nx,
+                       C.ny, C.nz, axis, Lx, 1.0f / (float)Lx);
   return check_launch("corr_lines_scatter_kernel");
 }
 
