@@ -515,6 +515,8 @@ static int setup_correlator(Propagator &P) {
   KL_REQUIRE(P.c.snap_format != KLRTM_SNAP_LEARNED, KLRTM_EINVAL,
              "the FFT imaging conditions and doubled imaging grids take FP32, COMPRESS_16, block-scaled or autoencoder "
              "snapshots");
+  KL_REQUIRE(!dbl || P.ic_option == KLRTM_IC_IMAGE1 || P.ic_option == KLRTM_IC_INVERSION, KLRTM_EINVAL,
+             "the doubled imaging grid takes the IMAGE1 and INVERSION conditions");
   if((rc = correlator_create(&P.corr, P.c.nx, P.c.ny, P.c.nz, P.c.dx, P.c.dy, P.c.dz, P.kmin_ic))) return rc;
   if(dbl && (rc = correlator_set_double(*P.corr, P.ic_dbl[0], P.ic_dbl[1], P.ic_dbl[2], P.ic_high_z, P.ic_high_xy)))
     return rc;

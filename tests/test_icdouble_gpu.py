@@ -130,6 +130,13 @@ def test_rtm_on_doubled_grid_matches_oracle(klrtm, option):
     p.set_ic_double(1, 1, 1)
     plain = p.rtm(rec, want=("image", "illum"))
     assert plain["image"].shape == p.shape
+    p.set_imaging(IC_INVERSION2)   # refused on a doubled grid at set-up, whichever comes first
+    with pytest.raises(KlrtmError):
+        p.set_ic_double(2, 2, 2)
+    p.set_imaging(option)
+    p.set_ic_double(2, 2, 2)
+    with pytest.raises(KlrtmError):
+        p.set_imaging(IC_INVERSION2)
     p.close()
 
 
