@@ -507,16 +507,16 @@ static int size_image(Propagator &P, size_t ni) {
 // (re)builds the correlator and the image buffers for the current option and imaging grid
 static int setup_correlator(Propagator &P) {
   const bool dbl = P.ic_dbl[0] > 1 || P.ic_dbl[1] > 1 || P.ic_dbl[2] > 1;
+  KL_REQUIRE((P.ic_option == KLRTM_IC_IMAGE1 && !dbl) || P.c.snap_format != KLRTM_SNAP_LEARNED, KLRTM_EINVAL,
+             "the FFT imaging conditions and doubled imaging grids take FP32, COMPRESS_16, block-scaled or autoencoder "
+             "snapshots");
+  KL_REQUIRE(!dbl || P.ic_option == KLRTM_IC_IMAGE1 || P.ic_option == KLRTM_IC_INVERSION, KLRTM_EINVAL,
+             "the doubled imaging grid takes the IMAGE1 and INVERSION conditions");
   if(P.corr) correlator_destroy(P.corr), P.corr = nullptr;
   for(float **d : {&P.d_fwi, &P.d_raw, &P.d_img2, &P.d_fwi2, &P.d_ill2}) (void)hipFree(*d), *d = nullptr;
   P.nloop = 1;
   int rc;
   if(P.ic_option == KLRTM_IC_IMAGE1 && !dbl) return (rc = size_image(P, P.n)) ? rc : P.clear_images();
-  KL_REQUIRE(P.c.snap_format != KLRTM_SNAP_LEARNED, KLRTM_EINVAL,
-             "the FFT imaging conditions and doubled imaging grids take FP32, COMPRESS_16, block-scaled or autoencoder "
-             "snapshots");
-  KL_REQUIRE(!dbl || P.ic_option == KLRTM_IC_IMAGE1 || P.ic_option == KLRTM_IC_INVERSION, KLRTM_EINVAL,
-             "the doubled imaging grid takes the IMAGE1 and INVERSION conditions");
   if((rc = correlator_create(&P.corr, P.c.nx, P.c.ny, P.c.nz, P.c.dx, P.c.dy, P.c.dz, P.kmin_ic))) return rc;
   if(dbl && (rc = correlator_set_double(*P.corr, P.ic_dbl[0], P.ic_dbl[1], P.ic_dbl[2], P.ic_high_z, P.ic_high_xy)))
     return rc;
@@ -536,15 +536,23 @@ int propagator_set_imaging(Propagator &P, int option, float kmin) {
   const float prev_k = P.kmin_ic;
   P.ic_option = option, P.kmin_ic = kmin;
   int rc = setup_correlator(P);
-  if(rc) P.ic_option = prev, P.kmin_ic = prev_k;
+  if(rc) {  // back to the previous set-up (the message of the failure stands)
+    P.ic_option = prev, P.kmin_ic = prev_k;
+    (void)setup_correlator(P);
+  }
   return rc;
 }
 
 int propagator_set_ic_double(Propagator &P, int dz, int dx, int dy, float high_z, float high_xy) {
   const int prev[3] = {P.ic_dbl[0], P.ic_dbl[1], P.ic_dbl[2]};
+  const float prev_h[2] = {P.ic_high_z, P.ic_high_xy};
   P.ic_dbl[0] = dz, P.ic_dbl[1] = dx, P.ic_dbl[2] = dy, P.ic_high_z = high_z, P.ic_high_xy = high_xy;
   int rc = setup_correlator(P);
-  if(rc) P.ic_dbl[0] = prev[0], P.ic_dbl[1] = prev[1], P.ic_dbl[2] = prev[2];
+  if(rc) {  // back to the previous set-up (the message of the failure stands)
+    P.ic_dbl[0] = prev[0], P.ic_dbl[1] = prev[1], P.ic_dbl[2] = prev[2];
+    P.ic_high_z = prev_h[0], P.ic_high_xy = prev_h[1];
+    (void)setup_correlator(P);
+  }
   return rc;
 }
 
