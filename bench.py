@@ -8,7 +8,7 @@ its cadence) and a backward step of the receiver wavefield (kernel + leapfrog
 the snapshot cadence).  Inputs are resident in HBM when the timed region
 starts.  `value` counts both propagations: 2 * nx*ny*nz cell updates per step.
 
-Default workload (configs[1]-style): VTI pseudo-acoustic RTM with the
+Default workload: VTI pseudo-acoustic RTM with the
 10th-order compact FD engine on a 400^3 grid, dx = 12.5 m, 32-cell PML,
 20 Hz Ricker, 400x400 surface receivers, FP32 snapshots every 4 steps.
 """
