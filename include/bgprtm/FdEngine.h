@@ -16,7 +16,7 @@
 
 class FdEngine {
 public:
-  FdEngine(int nx, int ny, int nz, float dx, float dy, float dz, int engine, int order) {
+  FdEngine(int nx, int ny, int nz, float dx, float dy, float dz, int engine, int order) : nz_(nz) {
     konglong::check(klrtm_kernel_create(&k_, nx, ny, nz, dx, dy, dz, engine, order));
     const konglong::Params &p = konglong::params();
     konglong::check(klrtm_kernel_set_pml_options(k_, p.pml_eps, p.pml3d, p.pml3d_scaler, p.top_pml_flag));
@@ -26,10 +26,11 @@ public:
   FdEngine &operator=(const FdEngine &) = delete;
 
   void setModel(float *vel) { konglong::check(klrtm_kernel_set_model(k_, vel, nullptr, nullptr, nullptr, nullptr, nullptr)); }
-  // FdEngine_fd::setJacob (bgprtm/FdEngine_fd.cpp:92-131): IRREGULAR grids take per-iz z weights from zgrid
+  // FdEngine_fd::setJacob (bgprtm/FdEngine_fd.cpp:92-131): IRREGULAR and the pyramid grids take per-iz z
+  // weights from zgrid
   virtual void setJacob(Grid *grid, float *jacobx, float *jacoby, float *jacobz) {
     (void)jacobx, (void)jacoby, (void)jacobz;
-    konglong::set_zgrid_of(k_, grid);
+    konglong::set_zgrid_of(k_, grid, nz_);
   }
   void setBoundary(int nxbnd1, int nxbnd2, int nybnd1, int nybnd2, int nzbnd1, int nzbnd2, float dt, float vmax) {
     konglong::check(klrtm_kernel_set_boundary(k_, nxbnd1, nxbnd2, nybnd1, nybnd2, nzbnd1, nzbnd2, dt, vmax));
@@ -48,6 +49,7 @@ public:
   klrtm_kernel *handle() const { return k_; }
 
 protected:
+  int nz_;
   klrtm_kernel *k_ = nullptr;
 };
 

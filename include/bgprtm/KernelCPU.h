@@ -29,11 +29,12 @@ public:
     konglong::check(klrtm_kernel_set_model(k_, vel, rho, del, eps, pjx, pjy));
   }
   // bgprtm/KernelCPU.cpp:85-87 -> FdEngine_fd::setJacob (bgprtm/FdEngine_fd.cpp:92-131): a RECTANGLE grid (or
-  // none) keeps 1/dz; an IRREGULAR grid takes its per-iz z weights from grid->zgrid.  The x/y jacobians
-  // serve the pyramid grids, which are not built; the FD engine ignores jacobz, as the reference's does.
+  // none) keeps 1/dz; IRREGULAR and the pyramid grids take their per-iz z weights from grid->zgrid (the
+  // compact engine its jacobz, derived from zgrid as Grid::setupJacob does).  The engines use neither
+  // x/y jacobian, as the reference's do not.
   void setJacob(Grid *grid, float *jacobx, float *jacoby, float *jacobz) {
     (void)jacobx, (void)jacoby, (void)jacobz;
-    konglong::set_zgrid_of(k_, grid);
+    konglong::set_zgrid_of(k_, grid, nz_);
   }
   void setBoundary(int nxbnd1, int nxbnd2, int nybnd1, int nybnd2, int nzbnd1, int nzbnd2, float dt, float vmax) {
     konglong::check(klrtm_kernel_set_boundary(k_, nxbnd1, nxbnd2, nybnd1, nybnd2, nzbnd1, nzbnd2, dt, vmax));

@@ -117,15 +117,22 @@ public:
 #endif
 
 namespace konglong {
-// the z weights of an FD engine for a grid (FdEngine_fd::setJacob, bgprtm/FdEngine_fd.cpp:92-131): none or
-// RECTANGLE keeps 1/dz, IRREGULAR takes grid->zgrid; the pyramid grids are not built
-template<class G> inline void set_zgrid_of(klrtm_kernel *k, const G *grid) {
-  if(grid && grid->mytype != RECTANGLE && grid->mytype != IRREGULAR)
-    throw Error(KLRTM_EINVAL, "konglong: pyramid grids are not supported");
-  const bool irr = grid && grid->mytype == IRREGULAR;
+// the z weights of an engine for a grid (FdEngine::setJacob, bgprtm/FdEngine.cpp:98-106; FdEngine_fd::setJacob,
+// bgprtm/FdEngine_fd.cpp:92-131): none or RECTANGLE keeps 1/dz; every other type (IRREGULAR and the pyramid
+// grids) takes the depth of each z index from grid->zgrid, as the reference's engines do, which keep the x/y
+// jacobians of the pyramid grids but use them in neither FdEngine_fd nor FdEngine_cfd.  nz is the engine's:
+// the grid must have the same (the reference asserts it)
+template<class G> inline void set_zgrid_of(klrtm_kernel *k, const G *grid, int nz) {
+  if(!grid) {
+    check(klrtm_kernel_set_zgrid(k, nullptr));
+    return;
+  }
+  if(grid->nz != nz) throw Error(KLRTM_EINVAL, "konglong: the grid's nz does not match the engine's");
+  const bool irr = grid->mytype != RECTANGLE;
+  if(irr && (int)grid->zgrid.size() < nz) throw Error(KLRTM_EINVAL, "konglong: grid->zgrid holds fewer than nz depths");
   check(klrtm_kernel_set_zgrid(k, irr ? grid->zgrid.data() : nullptr));
 }
-inline void set_zgrid_of(klrtm_kernel *k, std::nullptr_t) { check(klrtm_kernel_set_zgrid(k, nullptr)); }
+inline void set_zgrid_of(klrtm_kernel *k, std::nullptr_t, int) { check(klrtm_kernel_set_zgrid(k, nullptr)); }
 }  // namespace konglong
 
 #endif  // KONGLONG_GPU_H_
