@@ -584,13 +584,15 @@ int correlator_set_double(Correlator &C, int dz, int dx, int dy, float high_z, f
   KL_REQUIRE(C.ny == 1 || (dz == dx && dz == dy), KLRTM_EINVAL,
              "3D imaging grids double all three axes or none (ic_double_z/x/y %d/%d/%d)", dz, dx, dy);
   KL_REQUIRE(high_z > 0.f && high_z <= 1.f && high_xy > 0.f && high_xy <= 1.f, KLRTM_EINVAL, "high-cut fractions in (0, 1]");
-  C.dbl[0] = dz, C.dbl[1] = dx, C.dbl[2] = dy;
-  C.high_z = high_z, C.high_xy = high_xy;
+  // the new layout is built in locals and committed only once its buffers exist, so a failed allocation
+  // leaves the correlator as it was
+  const int dbl[3] = {dz, dx, dy};
+  int nloop = 1, bits[8] = {0}, src_of[8] = {0}, axis_of[8] = {0}, L[3];
   // copies in the reference's order: z, x, y, zx, zy, xy, zxy (those of doubled axes)
   const int want[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
   int idx_of[8];  // copy index of each shift pattern (bits z | x<<1 | y<<2)
   for(int b = 0; b < 8; b++) idx_of[b] = -1;
-  C.nloop = 1, C.bits[0] = 0, idx_of[0] = 0;
+  idx_of[0] = 0;
   for(int w = 0; w < 7; w++) {
     const int bz = want[w][0], bx = want[w][1], by = want[w][2];
     if((bz && dz < 2) || (bx && dx < 2) || (by && dy < 2)) continue;
@@ -598,23 +600,33 @@ int correlator_set_double(Correlator &C, int dz, int dx, int dy, float high_z, f
     // derived from the copy without its last shifted axis (y, then x, then z), shifted along that axis
     const int ax = by ? 2 : bx ? 1 : 0;
     const int from = b & ~(1 << ax);
-    C.bits[C.nloop] = b, C.src_of[C.nloop] = idx_of[from], C.axis_of[C.nloop] = ax;
-    idx_of[b] = C.nloop++;
+    bits[nloop] = b, src_of[nloop] = idx_of[from], axis_of[nloop] = ax;
+    idx_of[b] = nloop++;
   }
   for(int a = 0; a < 3; a++) {
     const int na = a == 0 ? C.nz : a == 1 ? C.nx : C.ny;
-    C.L[a] = C.dbl[a] > 1 ? fft_size(na + 16) : 0;
+    L[a] = dbl[a] > 1 ? fft_size(na + 16) : 0;
   }
-  if(C.nloop > 1) {
-    (void)hipFree(C.sbuf), (void)hipFree(C.rbuf), (void)hipFree(C.lines);
-    C.sbuf = C.rbuf = C.lines = nullptr;
+  float *sb = nullptr, *rb = nullptr, *ln = nullptr;
+  if(nloop > 1) {
     size_t lmax = 0;
     for(int a = 0; a < 3; a++)
-      if(C.L[a]) lmax = std::max(lmax, C.nxyz / (a == 0 ? C.nz : a == 1 ? C.nx : C.ny) * (size_t)(C.L[a] + 2));
-    KL_HIP(hipMalloc(&C.sbuf, sizeof(float) * C.nxyz * C.nloop));
-    KL_HIP(hipMalloc(&C.rbuf, sizeof(float) * C.nxyz * C.nloop));
-    KL_HIP(hipMalloc(&C.lines, sizeof(float) * lmax));
+      if(L[a]) lmax = std::max(lmax, C.nxyz / (a == 0 ? C.nz : a == 1 ? C.nx : C.ny) * (size_t)(L[a] + 2));
+    hipError_t e = hipMalloc(&sb, sizeof(float) * C.nxyz * nloop);
+    if(e == hipSuccess) e = hipMalloc(&rb, sizeof(float) * C.nxyz * nloop);
+    if(e == hipSuccess) e = hipMalloc(&ln, sizeof(float) * lmax);
+    if(e != hipSuccess) {
+      (void)hipFree(sb), (void)hipFree(rb), (void)hipFree(ln);
+      set_error("doubled imaging grid: copy buffers of %d volumes: %s", nloop, hipGetErrorString(e));
+      return KLRTM_EDEVICE;
+    }
   }
+  (void)hipFree(C.sbuf), (void)hipFree(C.rbuf), (void)hipFree(C.lines);
+  C.sbuf = sb, C.rbuf = rb, C.lines = ln;
+  for(int a = 0; a < 3; a++) C.dbl[a] = dbl[a], C.L[a] = L[a];
+  C.nloop = nloop;
+  for(int i = 0; i < 8; i++) C.bits[i] = bits[i], C.src_of[i] = src_of[i], C.axis_of[i] = axis_of[i];
+  C.high_z = high_z, C.high_xy = high_xy;
   return KLRTM_OK;
 }
 
