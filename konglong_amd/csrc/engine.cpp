@@ -65,44 +65,56 @@ static void zgrid_derivatives(const float *z, int n, int ne, float *zp, float *z
 }
 
 int Engine::set_zgrid(const float *zg) {
-  (void)hipFree(d_zc2), (void)hipFree(d_zc1), (void)hipFree(d_jz);
-  d_zc2 = d_zc1 = d_jz = nullptr;
-  if(!zg) return KLRTM_OK;
-  KL_REQUIRE(nz > 8, KLRTM_EINVAL, "an irregular z grid needs nz > 8");
-  for(int iz = 1; iz < nz; iz++)
-    KL_REQUIRE(zg[iz] > zg[iz - 1], KLRTM_EINVAL, "z grid depths must increase (iz %d)", iz);
-  if(engine == KLRTM_ENGINE_CFD) {
-    // Grid::setupJacob (bgprtm/Grid.cpp:654-666): jacobz = 1 / (Taylor::derive1(zgrid, d = dz, 4)), which the
-    // compact engine's z derivative takes (bgprtm/FdEngine_cfd.cpp:79, 91)
-    std::vector<float> zp(nz), zpp(nz), jz(nz);
-    zgrid_derivatives(zg, nz, 4, zp.data(), zpp.data());
-    for(int iz = 0; iz < nz; iz++) jz[iz] = 1 / (zp[iz] / dz);
-    KL_HIP(hipMalloc(&d_jz, nz * sizeof(float)));
-    KL_HIP(hipMemcpy(d_jz, jz.data(), nz * sizeof(float), hipMemcpyHostToDevice));
-    return KLRTM_OK;
-  }
-  KL_REQUIRE(nz > 2 * ne, KLRTM_EINVAL, "an irregular z grid needs nz > %d", 2 * ne);
-  std::vector<float> zp(nz), zpp(nz);
-  zgrid_derivatives(zg, nz, ne, zp.data(), zpp.data());
-  for(int iz = 0; iz < nz; iz++)
-    KL_REQUIRE(zp[iz] > 0.f, KLRTM_EINVAL, "z grid depths must increase (dz/dtau %g at iz %d)", (double)zp[iz], iz);
-  // FdEngine_fd::setJacob (bgprtm/FdEngine_fd.cpp:108-127) with const_ne: the order-2ne weights at every iz
-  std::vector<float> t2((size_t)nz * ZC2_STRIDE, 0.f), t1((size_t)nz * ZC1_STRIDE, 0.f);
-  for(int iz = 0; iz < nz; iz++) {
-    const float zpi = 1 / zp[iz], zpi2 = zpi * zpi, slope = -zpp[iz] * zpi2;
-    float *r2 = &t2[(size_t)iz * ZC2_STRIDE], *r1 = &t1[(size_t)iz * ZC1_STRIDE];
-    r2[0] = zpi2 * c2[0];
-    for(int j = 1; j <= ne; j++) {
-      const float c = zpi * c1[j], a = zpi2 * c2[j], b = slope * c;
-      r1[j] = c;
-      r2[j] = a - b;            // weight of u[iz - j]
-      r2[ZC_MAXR + j] = a + b;  // weight of u[iz + j]
+  // the new tables are checked, built and uploaded first and replace the old ones together, so a rejected
+  // grid or a failed allocation leaves the engine as it was
+  float *n2 = nullptr, *n1 = nullptr, *njz = nullptr;
+  auto upload = [](float *&dst, const std::vector<float> &h) {
+    hipError_t e = hipMalloc(&dst, h.size() * sizeof(float));
+    if(e == hipSuccess) e = hipMemcpy(dst, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+    return e;
+  };
+  if(zg) {
+    KL_REQUIRE(nz > 8, KLRTM_EINVAL, "an irregular z grid needs nz > 8");
+    for(int iz = 1; iz < nz; iz++)
+      KL_REQUIRE(zg[iz] > zg[iz - 1], KLRTM_EINVAL, "z grid depths must increase (iz %d)", iz);
+    hipError_t e = hipSuccess;
+    if(engine == KLRTM_ENGINE_CFD) {
+      // Grid::setupJacob (bgprtm/Grid.cpp:654-666): jacobz = 1 / (Taylor::derive1(zgrid, d = dz, 4)), which the
+      // compact engine's z derivative takes (bgprtm/FdEngine_cfd.cpp:79, 91)
+      std::vector<float> zp(nz), zpp(nz), jz(nz);
+      zgrid_derivatives(zg, nz, 4, zp.data(), zpp.data());
+      for(int iz = 0; iz < nz; iz++) jz[iz] = 1 / (zp[iz] / dz);
+      e = upload(njz, jz);
+    } else {
+      KL_REQUIRE(nz > 2 * ne, KLRTM_EINVAL, "an irregular z grid needs nz > %d", 2 * ne);
+      std::vector<float> zp(nz), zpp(nz);
+      zgrid_derivatives(zg, nz, ne, zp.data(), zpp.data());
+      for(int iz = 0; iz < nz; iz++)
+        KL_REQUIRE(zp[iz] > 0.f, KLRTM_EINVAL, "z grid depths must increase (dz/dtau %g at iz %d)", (double)zp[iz], iz);
+      // FdEngine_fd::setJacob (bgprtm/FdEngine_fd.cpp:108-127) with const_ne: the order-2ne weights at every iz
+      std::vector<float> t2((size_t)nz * ZC2_STRIDE, 0.f), t1((size_t)nz * ZC1_STRIDE, 0.f);
+      for(int iz = 0; iz < nz; iz++) {
+        const float zpi = 1 / zp[iz], zpi2 = zpi * zpi, slope = -zpp[iz] * zpi2;
+        float *r2 = &t2[(size_t)iz * ZC2_STRIDE], *r1 = &t1[(size_t)iz * ZC1_STRIDE];
+        r2[0] = zpi2 * c2[0];
+        for(int j = 1; j <= ne; j++) {
+          const float c = zpi * c1[j], a = zpi2 * c2[j], b = slope * c;
+          r1[j] = c;
+          r2[j] = a - b;            // weight of u[iz - j]
+          r2[ZC_MAXR + j] = a + b;  // weight of u[iz + j]
+        }
+      }
+      e = upload(n2, t2);
+      if(e == hipSuccess) e = upload(n1, t1);
+    }
+    if(e != hipSuccess) {
+      (void)hipFree(n2), (void)hipFree(n1), (void)hipFree(njz);
+      set_error("irregular z grid tables: %s", hipGetErrorString(e));
+      return KLRTM_EDEVICE;
     }
   }
-  KL_HIP(hipMalloc(&d_zc2, t2.size() * sizeof(float)));
-  KL_HIP(hipMalloc(&d_zc1, t1.size() * sizeof(float)));
-  KL_HIP(hipMemcpy(d_zc2, t2.data(), t2.size() * sizeof(float), hipMemcpyHostToDevice));
-  KL_HIP(hipMemcpy(d_zc1, t1.data(), t1.size() * sizeof(float), hipMemcpyHostToDevice));
+  (void)hipFree(d_zc2), (void)hipFree(d_zc1), (void)hipFree(d_jz);
+  d_zc2 = n2, d_zc1 = n1, d_jz = njz;
   return KLRTM_OK;
 }
 
