@@ -225,6 +225,12 @@ def declared_symbols():
     return list(_SIGS)
 
 
+def require(cond, what):
+    """Argument check that stays on under python -O: the C ABI reads and writes through the pointers it gets."""
+    if not cond:
+        raise ValueError(what)
+
+
 def check(rc):
     if rc != KLRTM_OK:
         msg = lib().klrtm_last_error()
@@ -235,19 +241,19 @@ def fptr(a):
     """float* of a contiguous float32 numpy array (or None)."""
     if a is None:
         return None
-    assert a.dtype == np.float32 and a.flags.c_contiguous
+    require(a.dtype == np.float32 and a.flags.c_contiguous, "needs a contiguous float32 array")
     return a.ctypes.data_as(_pf)
 
 
 def dptr(a):
     if a is None:
         return None
-    assert a.dtype == np.float64 and a.flags.c_contiguous
+    require(a.dtype == np.float64 and a.flags.c_contiguous, "needs a contiguous float64 array")
     return a.ctypes.data_as(_pd)
 
 
 def iptr(a):
-    assert a.dtype == np.int32 and a.flags.c_contiguous
+    require(a.dtype == np.int32 and a.flags.c_contiguous, "needs a contiguous int32 array")
     return a.ctypes.data_as(_pi)
 
 
@@ -279,7 +285,7 @@ class DeviceArray:
 
     def upload(self, a):
         a = np.ascontiguousarray(a, dtype=self.dtype)
-        assert a.nbytes == self.nbytes
+        require(a.nbytes == self.nbytes, f"upload of {a.nbytes} bytes into a device array of {self.nbytes}")
         check(lib().klrtm_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes))
 
     def download(self):

@@ -9,7 +9,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import DeviceArray, check, lib, ENGINE_FD, ISO, SNAP_FP32
+from ._lib import DeviceArray, check, lib, require, ENGINE_FD, ISO, SNAP_FP32
 from .learned import random_velocity
 from .propagator import Propagator, shot_config
 
@@ -30,7 +30,7 @@ class ConvAutoencoder:
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, np.float32)
-        assert p.size == self.n
+        require(p.size == self.n, f"the autoencoder has {self.n} parameters, got {p.size}")
         check(lib().klrtm_convae_set_params(self.h, p.ctypes.data))
 
     @staticmethod

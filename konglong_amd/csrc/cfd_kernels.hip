@@ -944,10 +944,20 @@ static int launch_zline(Engine &e, int sg, int round, const float *in, float *tm
 // the second stream of the compact engine's concurrent passes: fork (side waits for the caller's stream) and
 // join (the caller's stream waits for side); both streams at the default priority
 static int side_fork(Engine &e) {
-  if(!e.side) {
-    KL_HIP(hipStreamCreateWithFlags(&e.side, hipStreamNonBlocking));
-    KL_HIP(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
-    KL_HIP(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
+  if(!e.side) {  // created together: the engine holds all three or none
+    hipStream_t s = nullptr;
+    hipEvent_t f = nullptr, j = nullptr;
+    hipError_t err = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if(err == hipSuccess) err = hipEventCreateWithFlags(&f, hipEventDisableTiming);
+    if(err == hipSuccess) err = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+    if(err != hipSuccess) {
+      if(j) (void)hipEventDestroy(j);
+      if(f) (void)hipEventDestroy(f);
+      if(s) (void)hipStreamDestroy(s);
+      set_error("side stream of the concurrent compact passes: %s", hipGetErrorString(err));
+      return KLRTM_EDEVICE;
+    }
+    e.side = s, e.ev_fork = f, e.ev_join = j;
   }
   KL_HIP(hipEventRecord(e.ev_fork, stream()));
   KL_HIP(hipStreamWaitEvent(e.side, e.ev_fork, 0));

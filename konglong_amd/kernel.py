@@ -57,7 +57,8 @@ class KernelCPU:
             check(lib().klrtm_kernel_set_zgrid(self.h, None))
             return
         z = np.ascontiguousarray(zgrid, np.float32)
-        assert z.shape == (self.nz,)
+        if z.shape != (self.nz,):
+            raise ValueError(f"zgrid needs {self.nz} depths, got shape {z.shape}")
         check(lib().klrtm_kernel_set_zgrid(self.h, z.ctypes.data))
 
     def set_overlap(self, on=True):

@@ -19,7 +19,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import DeviceArray, check, lib, ENGINE_FD, ISO, SNAP_FP32
+from ._lib import DeviceArray, check, lib, require, ENGINE_FD, ISO, SNAP_FP32
 from .propagator import Propagator, shot_config
 
 BLOCK = 8
@@ -73,7 +73,7 @@ class LearnedCompressor:
 
     def __init__(self, W):
         W = np.ascontiguousarray(W, np.float32)
-        assert W.shape[0] == FEATURES and W.shape[1] in (32, 64, 128, 256), W.shape
+        require(W.shape[0] == FEATURES and W.shape[1] in (32, 64, 128, 256), f"code matrix of shape {W.shape}")
         self.W = W
         self.nl = W.shape[1]
         h = C.c_void_p()

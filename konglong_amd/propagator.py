@@ -3,7 +3,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import lib, check, fptr, dptr, ShotConfig, ENGINE_FD, ISO, SNAP_FP32
+from ._lib import lib, check, require, fptr, dptr, ShotConfig, ENGINE_FD, ISO, SNAP_FP32
 
 
 def shot_config(nx, ny, nz, d, dt, nt, engine=ENGINE_FD, order=8, model=ISO, nbnd=20, vmax=None, src_xyz=None,
@@ -51,7 +51,7 @@ class Propagator:
         """Density (Model::useRho): the gradient is divided by rho and each divergence term scaled by
         vel*rho; ISO takes the first-derivative form.  None switches it off."""
         r = None if rho is None else np.ascontiguousarray(rho, np.float32)
-        assert r is None or r.shape == self.shape
+        require(r is None or r.shape == self.shape, "density volume of the propagator's shape")
         check(lib().klrtm_propagator_set_rho(self.h, fptr(r)))
 
     def set_q(self, invq, coeffs):
@@ -62,7 +62,7 @@ class Propagator:
             return
         q = np.ascontiguousarray(invq, np.float32)
         cf = np.ascontiguousarray(coeffs, np.float32)
-        assert q.shape == self.shape and cf.size % 2 == 1
+        require(q.shape == self.shape and cf.size % 2 == 1, "1/Q volume of the propagator's shape, 2 order + 1 coefficients")
         check(lib().klrtm_propagator_set_q(self.h, fptr(q), (cf.size - 1) // 2, fptr(cf)))
 
     def set_ghost(self, iz_mirror, source_side=True, receiver_side=True):
@@ -86,17 +86,17 @@ class Propagator:
         its delay in seconds (FFT time shift)."""
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
         dl = np.ascontiguousarray(delays_s, np.float32).reshape(-1)
-        assert dl.size == xyz.shape[0]
+        require(dl.size == xyz.shape[0], "one delay per source")
         check(lib().klrtm_propagator_set_source_array(self.h, xyz.shape[0], fptr(xyz), fptr(dl)))
 
     def set_source_wavelet(self, wavelet):
         w = np.ascontiguousarray(wavelet, np.float32)
-        assert w.shape == (self.cfg.nt,)
+        require(w.shape == (self.cfg.nt,), f"wavelet of {self.cfg.nt} samples")
         check(lib().klrtm_propagator_set_source_wavelet(self.h, fptr(w)))
 
     def set_receivers(self, xyz):
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
-        assert xyz.shape[0] == self.cfg.nrcv
+        require(xyz.shape[0] == self.cfg.nrcv, f"{self.cfg.nrcv} receiver positions")
         check(lib().klrtm_propagator_set_receivers(self.h, fptr(xyz)))
 
     def model(self, want_record=True):
@@ -111,10 +111,10 @@ class Propagator:
         self.ic_option = int(option)
 
     def set_zgrid(self, zgrid=None):
-        """IRREGULAR grid: the depth of every z index (FD engine); None returns to the regular 1/dz."""
+        """IRREGULAR grid: the depth of every z index (both engines); None returns to the regular 1/dz."""
         z = None if zgrid is None else np.ascontiguousarray(zgrid, np.float32)
-        if z is not None:
-            assert z.shape == (self.shape[2],)
+        if z is not None and z.shape != (self.shape[2],):
+            raise ValueError(f"zgrid needs {self.shape[2]} depths, got shape {z.shape}")
         check(lib().klrtm_propagator_set_zgrid(self.h, None if z is None else z.ctypes.data))
 
     def set_ic_double(self, dz=2, dx=2, dy=2, high_z=0.9, high_xy=0.7):
@@ -185,7 +185,7 @@ class Propagator:
 
     def set_record(self, record):
         r = np.ascontiguousarray(record, np.float32)
-        assert r.shape == (self.cfg.nrcv, self.cfg.nt)
+        require(r.shape == (self.cfg.nrcv, self.cfg.nt), f"record of shape ({self.cfg.nrcv}, {self.cfg.nt})")
         check(lib().klrtm_propagator_set_record(self.h, fptr(r)))
 
     def reset(self):

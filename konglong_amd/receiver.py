@@ -3,7 +3,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import lib, check, fptr, iptr
+from ._lib import lib, check, require, fptr, iptr
 
 
 def spread_coeff(n, x, k1, k2, nlimit):
@@ -26,7 +26,7 @@ class Receiver:
 
     def set_traces(self, traces):
         t = np.ascontiguousarray(traces, np.float32)
-        assert t.shape == (self.nr, self.nt)
+        require(t.shape == (self.nr, self.nt), f"traces of shape ({self.nr}, {self.nt})")
         check(lib().klrtm_receiver_set_traces(self.h, fptr(t)))
 
     def apply(self, w0, it, vel, scaler0=1.0, iz_mirror=-1):
