@@ -102,6 +102,17 @@ struct RcvGroup {
   int2 *corg;            // device, kernel origins (x, y) in CSR order
   float *ckx, *cky;      // device, [L][nr] taps in CSR order (zero past each kernel's length)
   int nr;
+  // separable form, for groups whose receivers lie on a few rows sharing one y kernel each: the rows'
+  // x spreads first, then the y spread of the rows (row receivers sorted by x origin, rows by y origin)
+  bool sep;
+  int nrows;
+  int *rbin;   // device, nrows*(bw+1): CSR over x origins within each row
+  int *rorg;   // device, x origin of each row receiver
+  int *rrid;   // device, receiver id of each row receiver
+  float *rkx;  // device, [L][nr] x taps of the row receivers
+  int *cbin;   // device, bh+1: CSR of the rows over y origins
+  int *roy;    // device, y origin of each row
+  float *rky;  // device, [L][nrows] y taps of the rows
 };
 
 struct Receiver {
@@ -118,7 +129,8 @@ struct Receiver {
   float *d_record = nullptr;  // [nt][nr]
   float *d_plane = nullptr;   // scratch plane (largest group bbox)
   float *d_wy = nullptr;      // scratch y factors of the injection terms (largest group, [L][nr])
-  size_t plane_cells = 0;
+  float *d_rows = nullptr;    // scratch x spreads of the rows of a separable group (largest, nrows*bw)
+  size_t plane_cells = 0, row_cells = 0;
   // FK pre-processing (spread_sdomain): the surface wavefield [nt][ny][nx] injected with receiver 0's
   // z kernel instead of the traces
   float *d_fk = nullptr, *d_fk_kz = nullptr;
@@ -130,10 +142,12 @@ Receiver::~Receiver() {
   for(auto &g : groups) {
     (void)hipFree(g.kz), (void)hipFree(g.bin_start), (void)hipFree(g.bin_rcv);
     (void)hipFree(g.corg), (void)hipFree(g.ckx), (void)hipFree(g.cky);
+    (void)hipFree(g.rbin), (void)hipFree(g.rorg), (void)hipFree(g.rrid), (void)hipFree(g.rkx);
+    (void)hipFree(g.cbin), (void)hipFree(g.roy), (void)hipFree(g.rky);
   }
   (void)hipFree(d_kx), (void)hipFree(d_ky), (void)hipFree(d_lx), (void)hipFree(d_ly);
   (void)hipFree(d_offx), (void)hipFree(d_offy), (void)hipFree(d_traces), (void)hipFree(d_record), (void)hipFree(d_plane);
-  (void)hipFree(d_wy), (void)hipFree(d_fk), (void)hipFree(d_fk_kz);
+  (void)hipFree(d_wy), (void)hipFree(d_rows), (void)hipFree(d_fk), (void)hipFree(d_fk_kz);
 }
 
 // ---------------------------------------------------------------------------
@@ -200,6 +214,47 @@ __global__ void __launch_bounds__(256)
   if(act && sub == 0) plane[c] = T;
 }
 constexpr int IG = 4;
+
+// Separable form of the same plane.  v[k] = val*scaler of row receiver k (zero outside the record).
+__global__ void inject_val_kernel(int n, const int *__restrict__ rrid, const float *__restrict__ traces, int nrt, int nt,
+                                  const StepRef ts, float scaler, float *__restrict__ v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if(k >= n) return;
+  const int t = step_index(ts);
+  v[k] = t >= 0 && t < nt ? traces[(size_t)t * nrt + rrid[k]] * scaler : 0.f;
+}
+// X[j][px] = sum over receivers k of row j whose x kernel covers the cell of v[k]*kx_k (receivers in x-origin
+// order; an origin bin is clamped into the box, so each term is kept only where its tap index is in range)
+__global__ void inject_row_kernel(const GroupArgs a, int nrows, const int *__restrict__ rbin, const int *__restrict__ rorg,
+                                  const float *__restrict__ rkx, const float *__restrict__ v, float *__restrict__ X) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= (long)nrows * a.bw) return;
+  const int j = (int)(i / a.bw), px = (int)(i % a.bw), jx = a.bx0 + px, L = a.L;
+  const int *b = rbin + (size_t)j * (a.bw + 1);
+  const int k0 = b[max(0, px - a.lxmax + 1)], k1 = b[px + 1];
+  float s = 0.f;
+  for(int k = k0; k < k1; k++) {
+    const int ex = jx - rorg[k];
+    const float term = v[k] * rkx[(size_t)min(max(ex, 0), L - 1) * a.nr + k];
+    s += ex >= 0 && ex < L ? term : 0.f;
+  }
+  X[i] = s;
+}
+// T[py][px] = sum over rows j whose y kernel covers the cell of ky_j*X[j][px] (rows in y-origin order)
+__global__ void inject_col_kernel(const GroupArgs a, int nrows, const int *__restrict__ cbin, const int *__restrict__ roy,
+                                  const float *__restrict__ rky, const float *__restrict__ X, float *__restrict__ plane) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if(c >= a.bw * a.bh) return;
+  const int px = c % a.bw, py = c / a.bw, jy = a.by0 + py, L = a.L;
+  const int j0 = cbin[max(0, py - a.lymax + 1)], j1 = cbin[py + 1];
+  float s = 0.f;
+  for(int j = j0; j < j1; j++) {
+    const int ey = jy - roy[j];
+    const float term = rky[(size_t)min(max(ey, 0), L - 1) * nrows + j] * X[(size_t)j * a.bw + px];
+    s += ey >= 0 && ey < L ? term : 0.f;
+  }
+  plane[c] = s;
+}
 
 // w0[col, offz+k] += T*kz[k]*vel
 __global__ void inject_apply_kernel(const GroupArgs a, const float *__restrict__ plane, const float *__restrict__ kz,
@@ -497,11 +552,21 @@ int receiver_apply_at(Receiver &R, float *w0, const StepRef &ts, const float *ve
   for(auto &g : R.groups) {
     GroupArgs a = gargs(R, g);
     int cells = g.bw * g.bh;
-    const int nw = g.nr * R.L;
-    hipLaunchKernelGGL(inject_wy_kernel, dim3(div_up(nw, 256)), dim3(256), 0, stream(), nw, g.nr, g.bin_rcv, g.cky,
-                       R.d_traces, R.nr, R.nt, ts, scaler, R.d_wy);
-    hipLaunchKernelGGL(inject_plane_kernel<IG>, dim3(div_up((long)cells * IG, 256)), dim3(256), 0, stream(), a,
-                       g.bin_start, g.corg, g.ckx, R.d_wy, R.d_plane);
+    if(g.sep) {  // rows' x spreads, then their y spread (about 2 * 2 * spread terms per cell)
+      hipLaunchKernelGGL(inject_val_kernel, dim3(div_up(g.nr, 256)), dim3(256), 0, stream(), g.nr, g.rrid, R.d_traces,
+                         R.nr, R.nt, ts, scaler, R.d_wy);
+      const long nrow = (long)g.nrows * g.bw;
+      hipLaunchKernelGGL(inject_row_kernel, dim3(div_up(nrow, 256)), dim3(256), 0, stream(), a, g.nrows, g.rbin, g.rorg,
+                         g.rkx, R.d_wy, R.d_rows);
+      hipLaunchKernelGGL(inject_col_kernel, dim3(div_up(cells, 256)), dim3(256), 0, stream(), a, g.nrows, g.cbin, g.roy,
+                         g.rky, R.d_rows, R.d_plane);
+    } else {
+      const int nw = g.nr * R.L;
+      hipLaunchKernelGGL(inject_wy_kernel, dim3(div_up(nw, 256)), dim3(256), 0, stream(), nw, g.nr, g.bin_rcv, g.cky,
+                         R.d_traces, R.nr, R.nt, ts, scaler, R.d_wy);
+      hipLaunchKernelGGL(inject_plane_kernel<IG>, dim3(div_up((long)cells * IG, 256)), dim3(256), 0, stream(), a,
+                         g.bin_start, g.corg, g.ckx, R.d_wy, R.d_plane);
+    }
     long tot = (long)cells * g.lz;
     hipLaunchKernelGGL(inject_apply_kernel, dim3(div_up(tot, 256)), dim3(256), 0, stream(), a, R.d_plane, g.kz, w0, vel);
     if(iz_mirror >= 0) {
@@ -635,22 +700,63 @@ int receiver_create(Receiver **out, int nx, int ny, int nz, int nr, const float 
         cky[(size_t)j * ids.size() + k] = R->ky[(size_t)r * R->L + j];
       }
     }
+    // separable form: rows of receivers sharing one y kernel (origin, length and taps), used when the
+    // group has at least 64 receivers and at least 4 per row on average
+    std::map<std::vector<uint32_t>, std::vector<int>> by_y;
+    for(int r : ids) {
+      std::vector<uint32_t> key(2 + R->L);
+      key[0] = (uint32_t)R->offy[r], key[1] = (uint32_t)R->ly[r];
+      memcpy(key.data() + 2, &R->ky[(size_t)r * R->L], sizeof(float) * R->L);
+      by_y[key].push_back(r);
+    }
+    g.nrows = (int)by_y.size();
+    g.sep = g.nr >= 64 && 4 * g.nrows <= g.nr;
+    std::vector<int> rbin, rorg, rrid, cbin, roy;
+    std::vector<float> rkx, rky;
+    if(g.sep) {
+      std::vector<std::vector<int> *> rows;
+      for(auto &kv : by_y) rows.push_back(&kv.second);
+      std::stable_sort(rows.begin(), rows.end(),
+                       [&](const std::vector<int> *u, const std::vector<int> *w) { return R->offy[u->front()] < R->offy[w->front()]; });
+      const int nrows = g.nrows, L = R->L;
+      rbin.assign((size_t)nrows * (g.bw + 1), 0), roy.assign(nrows, 0), rky.assign((size_t)L * nrows, 0.f);
+      cbin.assign(g.bh + 1, 0);
+      for(int j = 0; j < nrows; j++) {
+        std::vector<int> &row = *rows[j];
+        std::stable_sort(row.begin(), row.end(), [&](int u, int w) { return R->offx[u] < R->offx[w]; });
+        const int base = (int)rrid.size();
+        int *b = &rbin[(size_t)j * (g.bw + 1)];
+        for(int r : row) b[std::min(std::max(R->offx[r] - g.bx0, 0), g.bw - 1) + 1]++, rrid.push_back(r), rorg.push_back(R->offx[r]);
+        b[0] = base;
+        for(int q = 1; q <= g.bw; q++) b[q] += b[q - 1];
+        roy[j] = R->offy[row.front()];
+        for(int t = 0; t < L; t++) rky[(size_t)t * nrows + j] = R->ky[(size_t)row.front() * L + t];
+        cbin[std::min(std::max(roy[j] - g.by0, 0), g.bh - 1) + 1]++;
+      }
+      for(int q = 1; q <= g.bh; q++) cbin[q] += cbin[q - 1];
+      rkx.assign((size_t)L * g.nr, 0.f);
+      for(int k = 0; k < g.nr; k++)
+        for(int t = 0; t < L; t++) rkx[(size_t)t * g.nr + k] = R->kx[(size_t)rrid[k] * L + t];
+    }
     R->groups.push_back(g);  // owned by R from here (freed with it on any error below)
     RcvGroup &G = R->groups.back();
     if((rc = upload(&G.kz, kzv)) || (rc = upload(&G.bin_start, cnt)) || (rc = upload(&G.bin_rcv, rid)) ||
        (rc = upload(&G.corg, corg)) || (rc = upload(&G.ckx, ckx)) || (rc = upload(&G.cky, cky)))
       break;
+    if(G.sep && ((rc = upload(&G.rbin, rbin)) || (rc = upload(&G.rorg, rorg)) || (rc = upload(&G.rrid, rrid)) ||
+                 (rc = upload(&G.rkx, rkx)) || (rc = upload(&G.cbin, cbin)) || (rc = upload(&G.roy, roy)) ||
+                 (rc = upload(&G.rky, rky))))
+      break;
     R->plane_cells = std::max(R->plane_cells, (size_t)g.bw * g.bh);
+    if(G.sep) R->row_cells = std::max(R->row_cells, (size_t)g.nrows * g.bw);
   }
   if(!rc) {
-    if((rc = upload(&R->d_kx, R->kx)) || (rc = upload(&R->d_ky, R->ky)) || (rc = upload(&R->d_lx, R->lx)) ||
-       (rc = upload(&R->d_ly, R->ly)) || (rc = upload(&R->d_offx, R->offx)) || (rc = upload(&R->d_offy, R->offy))) {
-    }
-  }
-  if(!rc) {
+    (void)((rc = upload(&R->d_kx, R->kx)) || (rc = upload(&R->d_ky, R->ky)) || (rc = upload(&R->d_lx, R->lx)) ||
+           (rc = upload(&R->d_ly, R->ly)) || (rc = upload(&R->d_offx, R->offx)) || (rc = upload(&R->d_offy, R->offy)));
   }
   if(!rc) {
     hipError_t e = hipMalloc(&R->d_plane, std::max<size_t>(1, R->plane_cells) * sizeof(float));
+    if(e == hipSuccess && R->row_cells) e = hipMalloc(&R->d_rows, R->row_cells * sizeof(float));
     size_t nwy = 1;
     for(auto &g : R->groups) nwy = std::max(nwy, (size_t)g.nr * R->L);
     if(e == hipSuccess) e = hipMalloc(&R->d_wy, nwy * sizeof(float));
