@@ -109,6 +109,7 @@ struct RcvGroup {
   int *rbin;   // device, nrows*(bw+1): CSR over x origins within each row
   int *rorg;   // device, x origin of each row receiver
   int *rrid;   // device, receiver id of each row receiver
+  int *rrow;   // device, row of each row receiver
   float *rkx;  // device, [L][nr] x taps of the row receivers
   int *cbin;   // device, bh+1: CSR of the rows over y origins
   int *roy;    // device, y origin of each row
@@ -142,7 +143,7 @@ Receiver::~Receiver() {
   for(auto &g : groups) {
     (void)hipFree(g.kz), (void)hipFree(g.bin_start), (void)hipFree(g.bin_rcv);
     (void)hipFree(g.corg), (void)hipFree(g.ckx), (void)hipFree(g.cky);
-    (void)hipFree(g.rbin), (void)hipFree(g.rorg), (void)hipFree(g.rrid), (void)hipFree(g.rkx);
+    (void)hipFree(g.rbin), (void)hipFree(g.rorg), (void)hipFree(g.rrid), (void)hipFree(g.rrow), (void)hipFree(g.rkx);
     (void)hipFree(g.cbin), (void)hipFree(g.roy), (void)hipFree(g.rky);
   }
   (void)hipFree(d_kx), (void)hipFree(d_ky), (void)hipFree(d_lx), (void)hipFree(d_ly);
@@ -323,6 +324,40 @@ __global__ void extract_rcv_kernel(const GroupArgs a, const int *__restrict__ id
     for(int jx = 0; jx < lx[r]; jx++) val += (cy * kx[(size_t)r * a.L + jx]) * row[jx];
   }
   rec_row[r] = val;
+}
+
+// Separable extraction (groups with the row structure of the separable injection): the y taps of each row
+// over the plane first, Z[j][x] = sum_t ky_j[t]*P[oy_j + t][x], then rec_k = sum_t kx_k[t]*Z[row k][ox_k + t]
+__global__ void extract_ycol_kernel(const GroupArgs a, int nrows, const int *__restrict__ roy, const float *__restrict__ rky,
+                                    const float *__restrict__ plane, float *__restrict__ Z) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= (long)nrows * a.bw) return;
+  const int j = (int)(i / a.bw), px = (int)(i % a.bw), L = a.L;
+  const int y0 = roy[j] - a.by0;
+  float s = 0.f;
+  for(int t = 0; t < L; t++) {
+    const int py = y0 + t;
+    const bool ok = py >= 0 && py < a.bh;
+    const float term = rky[(size_t)t * nrows + j] * plane[(size_t)min(max(py, 0), a.bh - 1) * a.bw + px];
+    s += ok ? term : 0.f;
+  }
+  Z[i] = s;
+}
+__global__ void extract_xrow_kernel(const GroupArgs a, const int *__restrict__ rrid, const int *__restrict__ rrow,
+                                    const int *__restrict__ rorg, const float *__restrict__ rkx, const float *__restrict__ Z,
+                                    float *__restrict__ rec_row) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if(k >= a.nr) return;
+  const int L = a.L, x0 = rorg[k] - a.bx0;
+  const float *z = Z + (size_t)rrow[k] * a.bw;
+  float s = 0.f;
+  for(int t = 0; t < L; t++) {
+    const int px = x0 + t;
+    const bool ok = px >= 0 && px < a.bw;
+    const float term = rkx[(size_t)t * a.nr + k] * z[min(max(px, 0), a.bw - 1)];
+    s += ok ? term : 0.f;
+  }
+  rec_row[rrid[k]] = s;
 }
 
 static GroupArgs gargs(const Receiver &R, const RcvGroup &g) {
@@ -587,8 +622,16 @@ int receiver_extract(Receiver &R, const float *w, int jt, int iz_mirror) {
     a.izm = iz_mirror;
     int cells = g.bw * g.bh;
     hipLaunchKernelGGL(extract_plane_kernel, dim3(div_up(cells, 256)), dim3(256), 0, stream(), a, g.kz, w, R.d_plane);
-    hipLaunchKernelGGL(extract_rcv_kernel, dim3(div_up(g.nr, 256)), dim3(256), 0, stream(), a, g.bin_rcv, R.d_kx, R.d_ky,
-                       R.d_lx, R.d_ly, R.d_offx, R.d_offy, R.d_plane, row);
+    if(g.sep) {  // the rows' y taps, then each receiver's x taps (about 2 * 2 * spread terms per receiver)
+      const long nrow = (long)g.nrows * g.bw;
+      hipLaunchKernelGGL(extract_ycol_kernel, dim3(div_up(nrow, 256)), dim3(256), 0, stream(), a, g.nrows, g.roy, g.rky,
+                         R.d_plane, R.d_rows);
+      hipLaunchKernelGGL(extract_xrow_kernel, dim3(div_up(g.nr, 256)), dim3(256), 0, stream(), a, g.rrid, g.rrow, g.rorg,
+                         g.rkx, R.d_rows, row);
+    } else {
+      hipLaunchKernelGGL(extract_rcv_kernel, dim3(div_up(g.nr, 256)), dim3(256), 0, stream(), a, g.bin_rcv, R.d_kx, R.d_ky,
+                         R.d_lx, R.d_ly, R.d_offx, R.d_offy, R.d_plane, row);
+    }
   }
   return check_launch("extract");
 }
@@ -711,7 +754,7 @@ int receiver_create(Receiver **out, int nx, int ny, int nz, int nr, const float 
     }
     g.nrows = (int)by_y.size();
     g.sep = g.nr >= 64 && 4 * g.nrows <= g.nr;
-    std::vector<int> rbin, rorg, rrid, cbin, roy;
+    std::vector<int> rbin, rorg, rrid, rrow, cbin, roy;
     std::vector<float> rkx, rky;
     if(g.sep) {
       std::vector<std::vector<int> *> rows;
@@ -726,7 +769,9 @@ int receiver_create(Receiver **out, int nx, int ny, int nz, int nr, const float 
         std::stable_sort(row.begin(), row.end(), [&](int u, int w) { return R->offx[u] < R->offx[w]; });
         const int base = (int)rrid.size();
         int *b = &rbin[(size_t)j * (g.bw + 1)];
-        for(int r : row) b[std::min(std::max(R->offx[r] - g.bx0, 0), g.bw - 1) + 1]++, rrid.push_back(r), rorg.push_back(R->offx[r]);
+        for(int r : row)
+          b[std::min(std::max(R->offx[r] - g.bx0, 0), g.bw - 1) + 1]++, rrid.push_back(r), rorg.push_back(R->offx[r]),
+              rrow.push_back(j);
         b[0] = base;
         for(int q = 1; q <= g.bw; q++) b[q] += b[q - 1];
         roy[j] = R->offy[row.front()];
@@ -744,6 +789,7 @@ int receiver_create(Receiver **out, int nx, int ny, int nz, int nr, const float 
        (rc = upload(&G.corg, corg)) || (rc = upload(&G.ckx, ckx)) || (rc = upload(&G.cky, cky)))
       break;
     if(G.sep && ((rc = upload(&G.rbin, rbin)) || (rc = upload(&G.rorg, rorg)) || (rc = upload(&G.rrid, rrid)) ||
+                 (rc = upload(&G.rrow, rrow)) ||
                  (rc = upload(&G.rkx, rkx)) || (rc = upload(&G.cbin, cbin)) || (rc = upload(&G.roy, roy)) ||
                  (rc = upload(&G.rky, rky))))
       break;
